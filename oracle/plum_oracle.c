@@ -1,0 +1,958 @@
+/*
+ * plum_oracle.c -- FP64 CPU restatement of the reference trial-move energy
+ * path.  TEST INFRASTRUCTURE ONLY: loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+ * product.  It follows the reference algorithm literally (pairwise
+ * multi-image real space, pairwise full-grid k-space, per-pair old/new
+ * differences, the same early exits and the dipole lag), so that the HIP
+ * engine, which regroups the same sums (structure factor, cell lists), is
+ * checked against the reference's own arithmetic.
+ *
+ * Pinning of each part (the checks that would catch a dropped term, a wrong
+ * sign/index or a transposed operand), all in tests/test_oracle.py:
+ *   cutoff derivation ........ the four reference example logs (r_c, k_c^2,
+ *                               rounded cell counts) -> tests/golden/
+ *   LJ / WCA ................. closed forms at chosen r, continuity at the
+ *                               cutoff, sigma/epsilon mixing
+ *   real space ............... brute-force image sum on tiny boxes
+ *   k space .................. pairwise == |S(k)|^2 identity; Madelung
+ *                               constants (NaCl, CsCl) from the full Ewald
+ *                               total, alpha-independence
+ *   self, dipole, spring ..... closed forms
+ *   6-3 wall + FENE .......... closed forms incl. the L/R branches
+ *   dE bookkeeping ........... dE == E_scratch(after) - E_scratch(before)
+ *                               for every term except the dipole lag, and the
+ *                               lag itself against its two-step definition
+ *   mt19937 .................. C++ [rand.predef] known answer (10000th
+ *                               output of the default-seeded engine)
+ *   move generators .......... invariants (bond lengths, rigid rotation,
+ *                               translation); the crankshaft rotation is a
+ *                               Rodrigues restatement of Eigen's AngleAxis
+ *                               (Eigen is absent) -> bit-level parity of
+ *                               proposals is unpinned, geometry is pinned.
+ */
+#include "plum_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* utilities/constants.h:11-62 */
+static const double kVeryLargeEnergy = 1E+8;
+static const double kPi = 3.14159265359;
+static const double kEwaldCutoff = 1E-7;
+static const double k213 = 1.25992104989;
+static const double k216 = 1.12246204831;
+static const int kDiCorrection = 3;
+
+struct orc_sys {
+  orc_params p;
+  orc_ewald_info e;
+  int n, n_mols;
+  double *cur, *trial;     /* 3n interleaved */
+  double *q;
+  int *type, *mol_of, *mol_start;
+  unsigned char *moved;
+  /* k tables (potential_ewald_coul.cc:78-115) */
+  int ceto[3];
+  double *kx, *ky, *kz, *k2, *ek2;
+  /* running totals and pending differences (force_field.cc:436-451) */
+  double E_lj, E_real, E_kspace, E_self, E_dipole, E_bond, E_wall;
+  double cur_dipl, trial_dipl;
+  double pend[ORC_NTERMS];
+  int pend_valid;
+  double *bond_cur;        /* per-molecule spring energy cache */
+};
+
+/* ------------------------------------------------------------------ */
+/* geometry (misc.cc:43-56, bead.cc:162-176)                           */
+/* ------------------------------------------------------------------ */
+
+/* GetDistVector: vector from b1 to b2 on TRIAL coordinates, minimum image
+   with round() in the first npbc dimensions (misc.cc:43-56). */
+static void get_dist_vector(const double* b1, const double* b2,
+                            const double box[3], int npbc, double d[3]) {
+  for (int i = 0; i < 3; i++) {
+    double di = b2[i] - b1[i];
+    if (i < npbc) di -= box[i] * round(di / box[i]);
+    d[i] = di;
+  }
+}
+
+/* Bead::BBDist on trial coordinates (bead.cc:162-176). */
+static double bb_dist(const double* a, const double* b, const double box[3],
+                      int npbc) {
+  double dist = 0;
+  for (int i = 0; i < 3; i++) {
+    double di = a[i] - b[i];
+    if (i < npbc) {
+      di -= box[i] * round(di / box[i]);
+      if (fabs(di) > 0.5 * box[i]) di = box[i] - fabs(di);
+    }
+    dist += di * di;
+  }
+  return sqrt(dist);
+}
+
+/* ------------------------------------------------------------------ */
+/* Ewald set-up (potential_ewald_coul.cc:29-115)                        */
+/* ------------------------------------------------------------------ */
+void orc_derive_cutoffs(const double box_in[3], double lB, double alpha,
+                        int dipole, orc_ewald_info* out) {
+  double box[3] = {box_in[0], box_in[1], box_in[2]};
+  /* potential_ewald_coul.cc:43-49 -- slab padding */
+  if (dipole) {
+    double min_padding = 150;
+    if (box[2] * kDiCorrection > min_padding) box[2] = box[2] * kDiCorrection;
+    else box[2] += min_padding;
+  }
+  for (int i = 0; i < 3; i++) out->box_e[i] = box[i];
+  out->vol_e = box[0] * box[1] * box[2];
+  /* :56-61 real-space cutoff, one unit length at a time */
+  double real_cutoff = 1;
+  while (0.5 * lB * 1 * 1 * erfc(sqrt(alpha) * real_cutoff) / real_cutoff >
+         kEwaldCutoff)
+    real_cutoff += 1;
+  out->real_cutoff = real_cutoff;
+  for (int i = 0; i < 3; i++) out->real_cell[i] = (int)ceil(real_cutoff / box[i]);
+  /* :66-75 reciprocal cutoff, alpha at a time (accumulated by addition) */
+  double repl_cutoff = alpha;
+  double min_box_vol = pow(box[0], 3);
+  while (lB * 1 * 1 / (2 * kPi * min_box_vol) * (4 * kPi * kPi) *
+             exp(-repl_cutoff / (4 * alpha)) / repl_cutoff >
+         kEwaldCutoff)
+    repl_cutoff += alpha;
+  out->repl_cutoff = repl_cutoff;
+  for (int i = 0; i < 3; i++)
+    out->repl_cell[i] = (int)ceil(sqrt(repl_cutoff) * box[i] / (2 * kPi));
+  out->n_k_full = -1;
+}
+
+static void build_k_tables(orc_sys* s) {
+  const double* box = s->e.box_e;
+  const int* rc = s->e.repl_cell;
+  for (int d = 0; d < 3; d++) s->ceto[d] = 2 * rc[d] + 1;
+  int n3 = s->ceto[0] * s->ceto[1] * s->ceto[2];
+  s->kx = malloc(sizeof(double) * s->ceto[0]);
+  s->ky = malloc(sizeof(double) * s->ceto[1]);
+  s->kz = malloc(sizeof(double) * s->ceto[2]);
+  s->k2 = malloc(sizeof(double) * n3);
+  s->ek2 = malloc(sizeof(double) * n3);
+  /* potential_ewald_coul.cc:89-96: l * 2*kPi / L with an int product first */
+  for (int l = -rc[0]; l <= rc[0]; l++) s->kx[l + rc[0]] = (double)(l * 2) * kPi / box[0];
+  for (int l = -rc[1]; l <= rc[1]; l++) s->ky[l + rc[1]] = (double)(l * 2) * kPi / box[1];
+  for (int l = -rc[2]; l <= rc[2]; l++) s->kz[l + rc[2]] = (double)(l * 2) * kPi / box[2];
+  int nk = 0;
+  /* :102-115 */
+  for (int ix = 0; ix < s->ceto[0]; ix++)
+    for (int iy = 0; iy < s->ceto[1]; iy++)
+      for (int iz = 0; iz < s->ceto[2]; iz++) {
+        int idx = s->ceto[1] * s->ceto[2] * ix + s->ceto[2] * iy + iz;
+        s->k2[idx] = s->kx[ix] * s->kx[ix] + s->ky[iy] * s->ky[iy] + s->kz[iz] * s->kz[iz];
+        s->ek2[idx] = exp(-s->k2[idx] / (4 * s->p.alpha)) / s->k2[idx];
+        if (s->k2[idx] > 0 && s->k2[idx] <= s->e.repl_cutoff) nk++;
+      }
+  s->e.n_k_full = nk;
+}
+
+/* ------------------------------------------------------------------ */
+/* pair / bead energies                                                */
+/* ------------------------------------------------------------------ */
+
+/* PotentialTruncatedLJ::PairEnergy (potential_truncated_lj.cc:49-85) */
+static double lj_pair_xyz(const orc_sys* s, const double* a, int ta,
+                          const double* b, int tb) {
+  double energy = 0;
+  double r = bb_dist(a, b, s->p.box, s->p.npbc);
+  double sigma = (s->p.lj_sigma[ta] + s->p.lj_sigma[tb]) / 2;
+  double epsilon = sqrt(s->p.lj_eps[ta] * s->p.lj_eps[tb]);
+  double r6 = 0;
+  if (r <= 0) r6 = -1;
+  if (r6 == -1) {
+    energy = kVeryLargeEnergy;
+  } else if (s->p.lj_cutoff < 0) {
+    double r6_ref = pow(1.0 / k216, 6);
+    double energy_ref = 4 * epsilon * (r6_ref * r6_ref - r6_ref);
+    if (r < k216 * sigma) {
+      r6 = pow(sigma / r, 6);
+      energy = 4 * epsilon * (r6 * r6 - r6) - energy_ref;
+    }
+  } else if (r < s->p.lj_cutoff) {
+    double r6_ref = pow(sigma / s->p.lj_cutoff, 6);
+    double energy_ref = 4 * epsilon * (r6_ref * r6_ref - r6_ref);
+    r6 = pow(sigma / r, 6);
+    energy = 4 * epsilon * (r6 * r6 - r6) - energy_ref;
+  }
+  return energy;
+}
+
+/* PotentialEwaldCoul::PairEnergyReal (potential_ewald_coul.cc:134-164) */
+static double real_pair_xyz(const orc_sys* s, const double* a, double qa,
+                            const double* b, double qb) {
+  double energy = 0;
+  if (qa * qb == 0) return energy;
+  double dist[3];
+  get_dist_vector(a, b, s->e.box_e, s->p.npbc, dist);
+  double prefactor = s->p.lB * qa * qb;
+  const int* c = s->e.real_cell;
+  const double* L = s->e.box_e;
+  double sa = sqrt(s->p.alpha);
+  for (int i = -c[0]; i <= c[0]; i++)
+    for (int j = -c[1]; j <= c[1]; j++)
+      for (int k = -c[2]; k <= c[2]; k++) {
+        double rv0 = dist[0] + i * L[0];
+        double rv1 = dist[1] + j * L[1];
+        double rv2 = dist[2] + k * L[2];
+        double r = sqrt(rv0 * rv0 + rv1 * rv1 + rv2 * rv2);
+        if (r > 0 && r <= s->e.real_cutoff) energy += prefactor * erfc(sa * r) / r;
+      }
+  return energy;
+}
+
+/* PotentialEwaldCoul::PairEnergyRepl (potential_ewald_coul.cc:198-225) */
+static double repl_pair_xyz(const orc_sys* s, const double* a, double qa,
+                            const double* b, double qb) {
+  double energy = 0;
+  if (qa * qb == 0) return energy;
+  double r[3];
+  get_dist_vector(a, b, s->e.box_e, s->p.npbc, r);
+  double prefactor = s->p.lB * qa * qb / (kPi * s->e.vol_e) * (4 * kPi * kPi);
+  for (int lx = 0; lx < s->ceto[0]; lx++)
+    for (int ly = 0; ly < s->ceto[1]; ly++)
+      for (int lz = 0; lz < s->ceto[2]; lz++) {
+        int idx = s->ceto[1] * s->ceto[2] * lx + s->ceto[2] * ly + lz;
+        if (s->k2[idx] > 0 && s->k2[idx] <= s->e.repl_cutoff)
+          energy += prefactor * s->ek2[idx] *
+                    cos(s->kx[lx] * r[0] + s->ky[ly] * r[1] + s->kz[lz] * r[2]);
+      }
+  return energy;
+}
+
+/* PotentialEwaldCoul::SelfEnergy (potential_ewald_coul.cc:253-257) */
+static double self_energy(const orc_sys* s, double q) {
+  return -s->p.lB * sqrt(s->p.alpha / kPi) * q * q;
+}
+
+/* PotentialEwaldCoul::DipoleE (potential_ewald_coul.cc:415-428): CURRENT z. */
+static double dipole_energy_current(const orc_sys* s) {
+  double Mz = 0;
+  for (int i = 0; i < s->n; i++) Mz += s->q[i] * s->cur[3 * i + 2];
+  return s->p.lB * 2 * kPi / (s->e.vol_e / 1.0) * Mz * Mz;
+}
+
+/* PotentialSpring::MoleculeEnergy on trial coordinates, no PBC
+   (potential_spring.cc:22-47). */
+static double spring_molecule(const orc_sys* s, int m) {
+  double e = 0;
+  static const double nobox[3] = {0, 0, 0};
+  for (int i = s->mol_start[m]; i < s->mol_start[m + 1] - 1; i++) {
+    double r = bb_dist(&s->trial[3 * i], &s->trial[3 * (i + 1)], nobox, 0);
+    e += 0.5 * s->p.k_bond * (r - s->p.r0) * (r - s->p.r0);
+  }
+  return e;
+}
+
+/* PotentialTruncatedLJWall::BeadEnergy on trial z
+   (potential_truncated_lj_wall.cc:46-134). box is the unpadded box. */
+static double wall_bead_z(const orc_sys* s, double z, int t) {
+  double energy = 0;
+  const double Lz = s->p.box[2];
+  double sigma = s->p.wall_sigma[t];
+  double epsilon = s->p.wall_eps[t];
+  int kind = s->p.type_kind[t];
+  double R0 = 3 * k213 * sigma;
+  double K = 1;
+  double m_cut = s->p.wall_cut;
+  if (epsilon == 0) {
+    energy = 0;
+  } else if (z <= 0 || z >= Lz) {
+    return kVeryLargeEnergy;
+  } else {
+    if (m_cut < 0 && kind != ORC_KIND_R && kind != ORC_KIND_L) {
+      double r3_ref = pow(1.0 / k213, 3);
+      double energy_ref = 2.59807621135 * epsilon * (r3_ref * r3_ref - r3_ref);
+      if (z < k213 * sigma) {
+        double r3 = pow(sigma / z, 3);
+        energy += 2.59807621135 * epsilon * (r3 * r3 - r3) - energy_ref;
+      }
+      if (Lz - z < k213 * sigma) {
+        double r3 = pow(sigma / (Lz - z), 3);
+        energy += 2.59807621135 * epsilon * (r3 * r3 - r3) - energy_ref;
+      }
+    } else if (kind == ORC_KIND_L) {
+      double r3 = pow(sigma / z, 3);
+      energy += 2.59807621135 * epsilon * (r3 * r3 - r3);
+      energy += -0.5 * K * R0 * R0 * log(1 - pow(z / R0, 2));
+      if (Lz - z < k213 * sigma) {
+        double r3b = pow(sigma / (Lz - z), 3);
+        energy += 2.59807621135 * epsilon * (r3b * r3b - r3b);
+      } else {
+        double r3b = pow(1.0 / k213, 3);
+        energy += 2.59807621135 * epsilon * (r3b * r3b - r3b);
+      }
+    } else if (kind == ORC_KIND_R) {
+      double r3 = pow(sigma / (Lz - z), 3);
+      energy += 2.59807621135 * epsilon * (r3 * r3 - r3);
+      energy += -0.5 * K * R0 * R0 * log(1 - pow((Lz - z) / R0, 2));
+      if (z < k213 * sigma) {
+        double r3b = pow(sigma / z, 3);
+        energy += 2.59807621135 * epsilon * (r3b * r3b - r3b);
+      } else {
+        double r3b = pow(1.0 / k213, 3);
+        energy += 2.59807621135 * epsilon * (r3b * r3b - r3b);
+      }
+    } else {
+      double r3_ref = pow(1.0 / m_cut, 3);
+      double energy_ref = 2.59807621135 * epsilon * (r3_ref * r3_ref - r3_ref);
+      if (z < m_cut) {
+        double r3 = pow(sigma / z, 3);
+        energy += 2.59807621135 * epsilon * (r3 * r3 - r3) - energy_ref;
+      }
+      if (Lz - z < m_cut) {
+        double r3 = pow(sigma / (Lz - z), 3);
+        energy += 2.59807621135 * epsilon * (r3 * r3 - r3) - energy_ref;
+      }
+    }
+  }
+  return energy;
+}
+
+/* ------------------------------------------------------------------ */
+/* construction                                                        */
+/* ------------------------------------------------------------------ */
+orc_sys* orc_create(const orc_params* p, int n, const double* xyz,
+                    const double* q, const int* type, int n_mols,
+                    const int* mol_start) {
+  orc_sys* s = calloc(1, sizeof(orc_sys));
+  s->p = *p;
+  s->n = n;
+  s->n_mols = n_mols;
+  s->cur = malloc(sizeof(double) * 3 * (n ? n : 1));
+  s->trial = malloc(sizeof(double) * 3 * (n ? n : 1));
+  memcpy(s->cur, xyz, sizeof(double) * 3 * n);
+  memcpy(s->trial, xyz, sizeof(double) * 3 * n);
+  s->q = malloc(sizeof(double) * (n ? n : 1));
+  memcpy(s->q, q, sizeof(double) * n);
+  s->type = malloc(sizeof(int) * (n ? n : 1));
+  memcpy(s->type, type, sizeof(int) * n);
+  s->mol_start = malloc(sizeof(int) * (n_mols + 1));
+  memcpy(s->mol_start, mol_start, sizeof(int) * (n_mols + 1));
+  s->mol_of = malloc(sizeof(int) * (n ? n : 1));
+  for (int m = 0; m < n_mols; m++)
+    for (int i = mol_start[m]; i < mol_start[m + 1]; i++) s->mol_of[i] = m;
+  s->moved = calloc(n ? n : 1, 1);
+  s->bond_cur = calloc(n_mols ? n_mols : 1, sizeof(double));
+  if (p->use_ewald) {
+    orc_derive_cutoffs(p->box, p->lB, p->alpha, p->dipole, &s->e);
+    build_k_tables(s);
+  } else {
+    for (int i = 0; i < 3; i++) s->e.box_e[i] = p->box[i];
+    s->e.vol_e = p->box[0] * p->box[1] * p->box[2];
+  }
+  /* ForceField::InitializeEnergy -> running totals (force_field.cc:365-381) */
+  double t[ORC_NTERMS];
+  orc_energy_full(s, t);
+  s->E_lj = t[ORC_T_LJ];
+  s->E_real = t[ORC_T_REAL];
+  s->E_kspace = t[ORC_T_KSPACE];
+  s->E_self = t[ORC_T_SELF];
+  s->E_dipole = t[ORC_T_DIPOLE];
+  s->E_bond = t[ORC_T_BOND];
+  s->E_wall = t[ORC_T_WALL];
+  s->cur_dipl = s->trial_dipl = t[ORC_T_DIPOLE];
+  for (int m = 0; m < n_mols; m++) s->bond_cur[m] = spring_molecule(s, m);
+  return s;
+}
+
+void orc_destroy(orc_sys* s) {
+  if (!s) return;
+  free(s->cur); free(s->trial); free(s->q); free(s->type); free(s->mol_start);
+  free(s->mol_of); free(s->moved); free(s->bond_cur);
+  free(s->kx); free(s->ky); free(s->kz); free(s->k2); free(s->ek2);
+  free(s);
+}
+
+void orc_get_ewald_info(const orc_sys* s, orc_ewald_info* out) { *out = s->e; }
+int orc_n_beads(const orc_sys* s) { return s->n; }
+void orc_get_positions(const orc_sys* s, double* xyz) {
+  memcpy(xyz, s->cur, sizeof(double) * 3 * s->n);
+}
+
+double orc_pair_lj(orc_sys* s, int i, int j) {
+  return lj_pair_xyz(s, &s->trial[3 * i], s->type[i], &s->trial[3 * j], s->type[j]);
+}
+double orc_pair_real(orc_sys* s, int i, int j) {
+  return real_pair_xyz(s, &s->trial[3 * i], s->q[i], &s->trial[3 * j], s->q[j]);
+}
+double orc_pair_repl(orc_sys* s, int i, int j) {
+  return repl_pair_xyz(s, &s->trial[3 * i], s->q[i], &s->trial[3 * j], s->q[j]);
+}
+double orc_wall_bead(orc_sys* s, int i) {
+  return wall_bead_z(s, s->trial[3 * i + 2], s->type[i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* full energy (ForceField::InitializeEnergy, force_field.cc:365-381)  */
+/* ------------------------------------------------------------------ */
+void orc_energy_full(orc_sys* s, double t[ORC_NTERMS]) {
+  for (int k = 0; k < ORC_NTERMS; k++) t[k] = 0;
+  const int* ms = s->mol_start;
+  /* PotentialPair::EnergyInitialization (potential_pair.cc:55-100):
+     intramolecular j<k, then intermolecular. */
+  if (s->p.use_pair) {
+    double e = 0;
+    for (int m = 0; m < s->n_mols; m++)
+      for (int j = ms[m]; j < ms[m + 1] - 1; j++)
+        for (int k = j + 1; k < ms[m + 1]; k++) e += orc_pair_lj(s, j, k);
+    for (int m1 = 0; m1 < s->n_mols - 1; m1++)
+      for (int m2 = m1 + 1; m2 < s->n_mols; m2++)
+        for (int k = ms[m1]; k < ms[m1 + 1]; k++)
+          for (int l = ms[m2]; l < ms[m2 + 1]; l++) e += orc_pair_lj(s, k, l);
+    t[ORC_T_LJ] = e;
+  }
+  /* PotentialEwald::EnergyInitialization (potential_ewald.cc:176-230) */
+  if (s->p.use_ewald) {
+    double er = 0, ek = 0, es = 0;
+    for (int m1 = 0; m1 < s->n_mols; m1++)
+      for (int m2 = m1; m2 < s->n_mols; m2++)
+        for (int k = ms[m1]; k < ms[m1 + 1]; k++)
+          for (int l = ms[m2]; l < ms[m2 + 1]; l++) {
+            if ((m2 == m1 && l >= k) || m2 > m1) {
+              double r = orc_pair_real(s, k, l);
+              double g = orc_pair_repl(s, k, l);
+              if (m2 == m1 && l == k) { r *= 0.5; g *= 0.5; }
+              er += r;
+              ek += g;
+            }
+          }
+    for (int i = 0; i < s->n; i++) es += self_energy(s, s->q[i]);
+    t[ORC_T_REAL] = er;
+    t[ORC_T_KSPACE] = ek;
+    t[ORC_T_SELF] = es;
+    if (s->p.dipole) t[ORC_T_DIPOLE] = dipole_energy_current(s);
+  }
+  /* PotentialBond::EnergyInitialization (potential_bond.cc:16-27) */
+  if (s->p.use_bond) {
+    double e = 0;
+    for (int m = 0; m < s->n_mols; m++) e += spring_molecule(s, m);
+    t[ORC_T_BOND] = e;
+  }
+  /* PotentialExternal::EnergyInitialization (potential_external.cc:48-61) */
+  if (s->p.use_ext) {
+    double e = 0;
+    for (int i = 0; i < s->n; i++) e += orc_wall_bead(s, i);
+    t[ORC_T_WALL] = e;
+  }
+}
+
+/* The same k-space total regrouped as (2 pi lB / V) sum_k ek2 |S(k)|^2
+   over the full grid.  Used only to pin the pairwise form above. */
+double orc_kspace_total_sk(orc_sys* s) {
+  if (!s->p.use_ewald) return 0;
+  double pref = s->p.lB / (kPi * s->e.vol_e) * (4 * kPi * kPi);
+  double tot = 0;
+  for (int ix = 0; ix < s->ceto[0]; ix++)
+    for (int iy = 0; iy < s->ceto[1]; iy++)
+      for (int iz = 0; iz < s->ceto[2]; iz++) {
+        int idx = s->ceto[1] * s->ceto[2] * ix + s->ceto[2] * iy + iz;
+        if (!(s->k2[idx] > 0 && s->k2[idx] <= s->e.repl_cutoff)) continue;
+        double sr = 0, si = 0;
+        for (int j = 0; j < s->n; j++) {
+          const double* r = &s->trial[3 * j];
+          double ph = s->kx[ix] * r[0] + s->ky[iy] * r[1] + s->kz[iz] * r[2];
+          sr += s->q[j] * cos(ph);
+          si += s->q[j] * sin(ph);
+        }
+        tot += s->ek2[idx] * (sr * sr + si * si);
+      }
+  return 0.5 * pref * tot;
+}
+
+/* ------------------------------------------------------------------ */
+/* trial moves and energy differences                                  */
+/* ------------------------------------------------------------------ */
+void orc_set_trial(orc_sys* s, int mol, int n_moved, const int* idx,
+                   const double* txyz) {
+  (void)mol;
+  for (int k = 0; k < n_moved; k++) {
+    int i = idx[k];
+    s->moved[i] = 1;
+    for (int d = 0; d < 3; d++) s->trial[3 * i + d] = txyz[3 * k + d];
+  }
+}
+
+int orc_get_moved(const orc_sys* s, int mol, int* idx, double* txyz) {
+  int n = 0;
+  for (int i = s->mol_start[mol]; i < s->mol_start[mol + 1]; i++)
+    if (s->moved[i]) {
+      if (idx) idx[n] = i;
+      if (txyz) for (int d = 0; d < 3; d++) txyz[3 * n + d] = s->trial[3 * i + d];
+      n++;
+    }
+  return n;
+}
+
+/* LJ pair on an explicit coordinate set: cur (c=0) or trial (c=1). */
+static double lj_ij(orc_sys* s, int i, int ci, int j, int cj) {
+  const double* a = ci ? &s->trial[3 * i] : &s->cur[3 * i];
+  const double* b = cj ? &s->trial[3 * j] : &s->cur[3 * j];
+  return lj_pair_xyz(s, a, s->type[i], b, s->type[j]);
+}
+static double real_ij(orc_sys* s, int i, int ci, int j, int cj) {
+  const double* a = ci ? &s->trial[3 * i] : &s->cur[3 * i];
+  const double* b = cj ? &s->trial[3 * j] : &s->cur[3 * j];
+  return real_pair_xyz(s, a, s->q[i], b, s->q[j]);
+}
+static double repl_ij(orc_sys* s, int i, int ci, int j, int cj) {
+  const double* a = ci ? &s->trial[3 * i] : &s->cur[3 * i];
+  const double* b = cj ? &s->trial[3 * j] : &s->cur[3 * j];
+  return repl_pair_xyz(s, a, s->q[i], b, s->q[j]);
+}
+
+/* PotentialPair::EnergyDifference (potential_pair.cc:152-201).  The cached
+   old pair energy equals the pair function on current coordinates (it was
+   computed from them and the beads have not moved since), so it is
+   recomputed here rather than stored. */
+static double pair_delta(orc_sys* s, int mol) {
+  double dE = 0;
+  const int a = s->mol_start[mol], b = s->mol_start[mol + 1];
+  for (int i = a; i < b - 1; i++)
+    for (int j = i + 1; j < b; j++)
+      if (s->moved[i] || s->moved[j]) {
+        double new_e = lj_ij(s, i, 1, j, 1);
+        dE += (new_e - lj_ij(s, i, 0, j, 0));
+      }
+  for (int m = 0; m < s->n_mols; m++) {
+    if (m == mol) continue;
+    for (int j = s->mol_start[m]; j < s->mol_start[m + 1]; j++)
+      for (int k = a; k < b; k++)
+        if (s->moved[k]) {
+          double new_e = lj_ij(s, k, 1, j, 1);
+          dE += (new_e - lj_ij(s, k, 0, j, 0));
+        }
+  }
+  return dE;
+}
+
+/* PotentialExternal::EnergyDifference (potential_external.cc:100-117) */
+static double ext_delta(orc_sys* s, int mol) {
+  double dE = 0;
+  for (int i = s->mol_start[mol]; i < s->mol_start[mol + 1]; i++)
+    if (s->moved[i]) {
+      double eNew = wall_bead_z(s, s->trial[3 * i + 2], s->type[i]);
+      if (eNew >= kVeryLargeEnergy) return kVeryLargeEnergy;
+      dE += eNew - wall_bead_z(s, s->cur[3 * i + 2], s->type[i]);
+    }
+  return dE;
+}
+
+/* PotentialEwald::EnergyDifference (potential_ewald.cc:429-535).  Returns the
+   total in the reference's summation order and the per-term split. */
+static double ewald_delta(orc_sys* s, int mol, double* d_real, double* d_repl,
+                          double* d_self, double* d_dip) {
+  double dE = 0, dr = 0, dk = 0, ds = 0;
+  const int a = s->mol_start[mol], b = s->mol_start[mol + 1];
+  for (int i = a; i < b; i++)
+    for (int j = i; j < b; j++)
+      if (s->moved[i] || s->moved[j]) {
+        double nr = real_ij(s, i, 1, j, 1), nk = repl_ij(s, i, 1, j, 1);
+        double orl = real_ij(s, i, 0, j, 0), ok = repl_ij(s, i, 0, j, 0);
+        if (j == i) { nr *= 0.5; nk *= 0.5; orl *= 0.5; ok *= 0.5; }
+        dE += (nr + nk - (orl + ok));
+        dr += nr - orl;
+        dk += nk - ok;
+      }
+  for (int m = 0; m < s->n_mols; m++) {
+    if (m == mol) continue;
+    for (int j = s->mol_start[m]; j < s->mol_start[m + 1]; j++)
+      for (int k = a; k < b; k++)
+        if (s->moved[k]) {
+          double nr = real_ij(s, k, 1, j, 1), nk = repl_ij(s, k, 1, j, 1);
+          double orl = real_ij(s, k, 0, j, 0), ok = repl_ij(s, k, 0, j, 0);
+          dE += (nr + nk - (orl + ok));
+          dr += nr - orl;
+          dk += nk - ok;
+        }
+  }
+  for (int i = a; i < b; i++)
+    if (s->moved[i]) {
+      double e = self_energy(s, s->q[i]) - self_energy(s, s->q[i]);
+      dE += e;
+      ds += e;
+    }
+  double dd = 0;
+  if (s->p.dipole) {
+    /* :528-531 -- evaluated on CURRENT coordinates (the lag) */
+    s->trial_dipl = dipole_energy_current(s);
+    dd = s->trial_dipl - s->cur_dipl;
+    dE += dd;
+  }
+  *d_real = dr; *d_repl = dk; *d_self = ds; *d_dip = dd;
+  return dE;
+}
+
+/* ForceField::EnergyDifference (force_field.cc:407-434) */
+double orc_energy_difference(orc_sys* s, int mol, double t[ORC_NTERMS]) {
+  for (int k = 0; k < ORC_NTERMS; k++) t[k] = 0;
+  double dE = 0;
+  s->pend_valid = 1;
+  if (s->p.use_pair) {
+    t[ORC_T_LJ] = pair_delta(s, mol);
+    dE += t[ORC_T_LJ];
+    if (dE >= kVeryLargeEnergy) goto done;
+  }
+  if (s->p.use_ext) {
+    t[ORC_T_WALL] = ext_delta(s, mol);
+    dE += t[ORC_T_WALL];
+    if (dE >= kVeryLargeEnergy) goto done;
+  }
+  if (s->p.use_ewald) {
+    double dr, dk, ds, dd;
+    dE += ewald_delta(s, mol, &dr, &dk, &ds, &dd);
+    t[ORC_T_REAL] = dr; t[ORC_T_KSPACE] = dk; t[ORC_T_SELF] = ds; t[ORC_T_DIPOLE] = dd;
+  }
+  if (s->p.use_bond) {
+    /* PotentialSpring::EnergyDifference (potential_spring.cc:63-80) */
+    if (s->mol_start[mol + 1] - s->mol_start[mol] > 1) {
+      double te = spring_molecule(s, mol);
+      t[ORC_T_BOND] = te - s->bond_cur[mol];
+      dE += t[ORC_T_BOND];
+    }
+  }
+done:
+  for (int k = 0; k < ORC_NTERMS; k++) s->pend[k] = t[k];
+  return dE;
+}
+
+/* ForceField::FinalizeEnergies (force_field.cc:436-451) followed by the
+   commit/rollback and UnsetMoved of Simulation::TranslationalMove
+   (simulation.cc:334-351). */
+void orc_finalize(orc_sys* s, int mol, int accept) {
+  const int a = s->mol_start[mol], b = s->mol_start[mol + 1];
+  if (accept && s->pend_valid) {
+    s->E_lj += s->pend[ORC_T_LJ];
+    s->E_real += s->pend[ORC_T_REAL];
+    s->E_kspace += s->pend[ORC_T_KSPACE];
+    s->E_self += s->pend[ORC_T_SELF];
+    s->E_dipole += s->pend[ORC_T_DIPOLE];
+    s->E_bond += s->pend[ORC_T_BOND];
+    s->E_wall += s->pend[ORC_T_WALL];
+  }
+  /* potential_ewald.cc:614-622 */
+  if (s->p.dipole) {
+    if (accept) s->cur_dipl = s->trial_dipl;
+    else s->trial_dipl = s->cur_dipl;
+  }
+  for (int i = a; i < b; i++) {
+    for (int d = 0; d < 3; d++) {
+      if (accept) s->cur[3 * i + d] = s->trial[3 * i + d];
+      else s->trial[3 * i + d] = s->cur[3 * i + d];
+    }
+    s->moved[i] = 0;
+  }
+  /* potential_bond.cc:69-82: the trial molecule energy becomes current */
+  if (accept && s->p.use_bond && b - a > 1) s->bond_cur[mol] = spring_molecule(s, mol);
+  s->pend_valid = 0;
+}
+
+void orc_running_totals(const orc_sys* s, double t[ORC_NTERMS]) {
+  t[ORC_T_LJ] = s->E_lj; t[ORC_T_REAL] = s->E_real; t[ORC_T_KSPACE] = s->E_kspace;
+  t[ORC_T_SELF] = s->E_self; t[ORC_T_DIPOLE] = s->E_dipole; t[ORC_T_BOND] = s->E_bond;
+  t[ORC_T_WALL] = s->E_wall;
+}
+
+/* ------------------------------------------------------------------ */
+/* std::mt19937 (C++ [rand.predef]; Matsumoto & Nishimura 1998)         */
+/* ------------------------------------------------------------------ */
+void orc_mt_seed(orc_mt19937* r, uint32_t seed) {
+  r->mt[0] = seed;
+  for (int i = 1; i < 624; i++)
+    r->mt[i] = 1812433253u * (r->mt[i - 1] ^ (r->mt[i - 1] >> 30)) + (uint32_t)i;
+  r->mti = 624;
+}
+
+uint32_t orc_mt_next(orc_mt19937* r) {
+  if (r->mti >= 624) {
+    for (int i = 0; i < 624; i++) {
+      uint32_t y = (r->mt[i] & 0x80000000u) | (r->mt[(i + 1) % 624] & 0x7fffffffu);
+      uint32_t v = r->mt[(i + 397) % 624] ^ (y >> 1);
+      if (y & 1u) v ^= 0x9908b0dfu;
+      r->mt[i] = v;
+    }
+    r->mti = 0;
+  }
+  uint32_t y = r->mt[r->mti++];
+  y ^= (y >> 11);
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= (y >> 18);
+  return y;
+}
+
+/* (double)rand_gen()/rand_gen.max() */
+static double u01(orc_mt19937* r) { return (double)orc_mt_next(r) / 4294967295.0; }
+
+/* randSphere (misc.cc:95-109), Marsaglia 1972 */
+void orc_rand_sphere(orc_mt19937* r, double vec[3]) {
+  double rand_square = 2, r1 = 0, r2 = 0;
+  while (rand_square > 1) {
+    r1 = 1 - 2 * u01(r);
+    r2 = 1 - 2 * u01(r);
+    rand_square = r1 * r1 + r2 * r2;
+  }
+  double ranh = 2 * sqrt(1 - rand_square);
+  vec[0] = r1 * ranh;
+  vec[1] = r2 * ranh;
+  vec[2] = (1 - 2 * rand_square);
+}
+
+/* ------------------------------------------------------------------ */
+/* move generators (molecule.cc:103-312)                               */
+/* ------------------------------------------------------------------ */
+static int count_moved(const orc_sys* s, int mol) {
+  int n = 0;
+  for (int i = s->mol_start[mol]; i < s->mol_start[mol + 1]; i++) n += s->moved[i];
+  return n;
+}
+
+/* Molecule::BeadTranslate (molecule.cc:103-133) */
+int orc_move_bead_translate(orc_sys* s, int mol, double move_size, orc_mt19937* r) {
+  double vec[3];
+  orc_rand_sphere(r, vec);
+  double vec_len = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+  if (vec_len > 0) vec_len = 3 * move_size / vec_len;
+  int b0 = s->mol_start[mol];
+  s->moved[b0] = 1;
+  for (int i = 0; i < 3; i++) s->trial[3 * b0 + i] = s->cur[3 * b0 + i] + vec_len * vec[i];
+  return count_moved(s, mol);
+}
+
+/* Molecule::COMTranslate (molecule.cc:135-151) */
+int orc_move_com(orc_sys* s, int mol, double move_size, orc_mt19937* r) {
+  double v0 = 0.5 * move_size * u01(r);
+  double v1 = 0.5 * move_size * u01(r);
+  double v2 = 0.5 * move_size * u01(r);
+  double vec[3] = {v0, v1, v2};
+  if (orc_mt_next(r) % 2 == 0) vec[0] = -vec[0];
+  if (orc_mt_next(r) % 2 == 0) vec[1] = -vec[1];
+  if (orc_mt_next(r) % 2 == 0) vec[2] = -vec[2];
+  for (int i = s->mol_start[mol]; i < s->mol_start[mol + 1]; i++) {
+    for (int j = 0; j < 3; j++) s->trial[3 * i + j] = s->cur[3 * i + j] + vec[j];
+    s->moved[i] = 1;
+  }
+  return count_moved(s, mol);
+}
+
+/* Molecule::Pivot (molecule.cc:153-237) */
+int orc_move_pivot(orc_sys* s, int mol, double move_size, orc_mt19937* r,
+                   double eq_bond_len, int vary_bond) {
+  const int base = s->mol_start[mol];
+  const int len = s->mol_start[mol + 1] - base;
+  double* T = &s->trial[3 * base];
+  int pivot = (int)floor(len * u01(r));
+  if (pivot == len) pivot--;
+  double msr = move_size * u01(r);
+  for (int i = pivot + 1; i < len; i++) {
+    double vec[3];
+    orc_rand_sphere(r, vec);
+    double x = T[3 * i + 0] + msr * vec[0];
+    double y = T[3 * i + 1] + msr * vec[1];
+    double z = T[3 * i + 2] + msr * vec[2];
+    double dx = x - T[3 * (i - 1) + 0], dy = y - T[3 * (i - 1) + 1], dz = z - T[3 * (i - 1) + 2];
+    double bond_len = eq_bond_len;
+    if (vary_bond) bond_len += (eq_bond_len / 5.0) * (u01(r) - 0.5);
+    double norm = bond_len / sqrt(dx * dx + dy * dy + dz * dz);
+    double cx = norm * dx + T[3 * (i - 1) + 0];
+    double cy = norm * dy + T[3 * (i - 1) + 1];
+    double cz = norm * dz + T[3 * (i - 1) + 2];
+    double mx = cx - T[3 * i + 0], my = cy - T[3 * i + 1], mz = cz - T[3 * i + 2];
+    for (int j = i; j < len; j++) {
+      T[3 * j + 0] = T[3 * j + 0] + mx;
+      T[3 * j + 1] = T[3 * j + 1] + my;
+      T[3 * j + 2] = T[3 * j + 2] + mz;
+    }
+  }
+  for (int i = pivot - 1; i >= 0; i--) {
+    double vec[3];
+    orc_rand_sphere(r, vec);
+    double x = T[3 * i + 0] + msr * vec[0];
+    double y = T[3 * i + 1] + msr * vec[1];
+    double z = T[3 * i + 2] + msr * vec[2];
+    double dx = x - T[3 * (i + 1) + 0], dy = y - T[3 * (i + 1) + 1], dz = z - T[3 * (i + 1) + 2];
+    double bond_len = eq_bond_len;
+    if (vary_bond) bond_len += (eq_bond_len / 5.0) * (u01(r) - 0.5);
+    double norm = bond_len / sqrt(dx * dx + dy * dy + dz * dz);
+    double cx = norm * dx + T[3 * (i + 1) + 0];
+    double cy = norm * dy + T[3 * (i + 1) + 1];
+    double cz = norm * dz + T[3 * (i + 1) + 2];
+    double mx = cx - T[3 * i + 0], my = cy - T[3 * i + 1], mz = cz - T[3 * i + 2];
+    for (int j = i; j >= 0; j--) {
+      T[3 * j + 0] = T[3 * j + 0] + mx;
+      T[3 * j + 1] = T[3 * j + 1] + my;
+      T[3 * j + 2] = T[3 * j + 2] + mz;
+    }
+  }
+  for (int i = 0; i < len; i++) s->moved[base + i] = 1;
+  return count_moved(s, mol);
+}
+
+/* Molecule::Crankshaft (molecule.cc:239-265).  Eigen's AngleAxisd rotation
+   is restated as the Rodrigues matrix R = c I + s [a]x + (1-c) a a^T. */
+int orc_move_crankshaft(orc_sys* s, int mol, double move_size, orc_mt19937* r) {
+  const int base = s->mol_start[mol];
+  const int len = s->mol_start[mol + 1] - base;
+  int first = (int)floor((len - 2) * u01(r));
+  int last = (int)floor((len - first) * u01(r)) + first;
+  if (last >= len) last = len - 1;  /* out-of-range guard (p = 2^-32) */
+  const double* C = &s->cur[3 * base];
+  double* T = &s->trial[3 * base];
+  double ax[3];
+  for (int i = 0; i < 3; i++) ax[i] = C[3 * last + i] - C[3 * first + i];
+  double nn = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2];
+  if (nn > 0) {
+    double inv = sqrt(nn);
+    for (int i = 0; i < 3; i++) ax[i] /= inv;
+  }
+  double angle = move_size * (u01(r) * 2 * M_PI - M_PI);
+  double sn = sin(angle), c = cos(angle);
+  double R[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) R[i][j] = (1 - c) * ax[i] * ax[j] + (i == j ? c : 0);
+  R[0][1] -= sn * ax[2]; R[1][0] += sn * ax[2];
+  R[0][2] += sn * ax[1]; R[2][0] -= sn * ax[1];
+  R[1][2] -= sn * ax[0]; R[2][1] += sn * ax[0];
+  for (int i = first + 1; i < last; i++) {
+    s->moved[base + i] = 1;
+    double v[3];
+    for (int k = 0; k < 3; k++) v[k] = C[3 * i + k] - C[3 * first + k];
+    for (int k = 0; k < 3; k++)
+      T[3 * i + k] = R[k][0] * v[0] + R[k][1] * v[1] + R[k][2] * v[2] + C[3 * first + k];
+  }
+  return count_moved(s, mol);
+}
+
+/* Molecule::RandomReptation (molecule.cc:267-312) */
+int orc_move_reptation(orc_sys* s, int mol, orc_mt19937* r, double eq_bond_len,
+                       int vary_bond) {
+  const int base = s->mol_start[mol];
+  const int len = s->mol_start[mol + 1] - base;
+  if (eq_bond_len < 0) return -1;
+  double bond_len = eq_bond_len;
+  if (vary_bond) bond_len += (eq_bond_len / 5.0) * (u01(r) - 0.5);
+  int direction = 1, begin = 0, end = len - 1;
+  if (orc_mt_next(r) % 2 == 0) { direction = -1; begin = len - 1; end = 0; }
+  const double* C = &s->cur[3 * base];
+  double* T = &s->trial[3 * base];
+  for (int i = begin; i != end; i += direction)
+    for (int j = 0; j < 3; j++) T[3 * i + j] = C[3 * (i + direction) + j];
+  double vec[3];
+  orc_rand_sphere(r, vec);
+  double vec_len = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+  for (int i = 0; i < 3; i++) T[3 * end + i] = C[3 * end + i] + bond_len * vec[i] / vec_len;
+  for (int i = 0; i < len; i++) s->moved[base + i] = 1;
+  return count_moved(s, mol);
+}
+
+/* ------------------------------------------------------------------ */
+/* Simulation::Run step + TranslationalMove (simulation.cc:216-355)     */
+/* ------------------------------------------------------------------ */
+void orc_mc_step(orc_sys* s, orc_mt19937* r, const double move_prob[5],
+                 double move_size, int phantom, int grafted, double bond_len,
+                 int vary_bond, orc_step_info* info) {
+  memset(info, 0, sizeof(*info));
+  info->mol = -1;
+  (void)orc_mt_next(r);  /* simulation.cc:221, the per-step draw */
+  int n_mol = s->n_mols;
+  /* Simulation::UpdateMolCounts (simulation.cc:1066-1094) */
+  int n_chain = 0, n_cion = 0, n_aion = 0, n_nion = 0;
+  for (int i = phantom; i < n_mol; i++) {
+    int sz = s->mol_start[i + 1] - s->mol_start[i];
+    double q0 = s->q[s->mol_start[i]];
+    if (sz > 1) n_chain++;
+    else if (q0 > 0) n_cion++;
+    else if (q0 < 0) n_aion++;
+    else n_nion++;
+  }
+  n_chain -= grafted;
+  if (n_mol - phantom <= 0) return;
+  int chain_id = -1, ion_id = -1;
+  int n_ions = n_cion + n_aion + n_nion;
+  int which_chain = (int)floor(u01(r) * (grafted + n_chain));
+  int which_ion = (int)floor(u01(r) * n_ions);
+  if (which_chain == grafted + n_chain) which_chain--;
+  if (which_ion == n_ions) which_ion--;
+  int cc = -1, ic = -1;
+  for (int i = phantom; i < n_mol; i++) {
+    int sz = s->mol_start[i + 1] - s->mol_start[i];
+    if (sz > 1) cc++; else ic++;
+    if (cc == which_chain) { chain_id = i; which_chain = -2; }
+    if (ic == which_ion) { ion_id = i; which_ion = -2; }
+  }
+  int move_type = 0;
+  double rn = u01(r);
+  double cur_type = move_prob[0];
+  while (cur_type < rn) { move_type++; cur_type += move_prob[move_type]; }
+  if (move_type >= 5) move_type = 4;
+  int mol_id = -1;
+  if (move_type == 0 && n_ions > 0) {
+    mol_id = ion_id;
+    orc_move_bead_translate(s, mol_id, move_size, r);
+  } else if (grafted + n_chain > 0) {
+    mol_id = chain_id;
+    switch (move_type) {
+      case 1: orc_move_com(s, mol_id, move_size, r); break;
+      case 2: orc_move_pivot(s, mol_id, move_size, r, bond_len, vary_bond); break;
+      case 3: orc_move_crankshaft(s, mol_id, move_size, r); break;
+      case 4: orc_move_reptation(s, mol_id, r, bond_len, vary_bond); break;
+    }
+  }
+  if ((move_type == 0 && n_ions > 0) || (move_type > 0 && grafted + n_chain > 0)) {
+    info->attempted = 1;
+    info->move_type = move_type;
+    info->mol = mol_id;
+    info->n_moved = count_moved(s, mol_id);
+    double dE = orc_energy_difference(s, mol_id, info->terms);
+    int accept = 0;
+    if (dE >= kVeryLargeEnergy) accept = 0;
+    else accept = (u01(r) < exp(-s->p.beta * dE));
+    info->dE = dE;
+    info->accepted = accept;
+    orc_finalize(s, mol_id, accept);
+  }
+}
+
+/* Simulation::RadiusOfGyration (simulation.cc:406-434) over all chains. */
+double orc_radius_of_gyration(const orc_sys* s) {
+  static const double nobox[3] = {0, 0, 0};
+  double rg = 0;
+  int nc = 0;
+  for (int m = 0; m < s->n_mols; m++) {
+    int a = s->mol_start[m], b = s->mol_start[m + 1], len = b - a;
+    if (len <= 1) continue;
+    double rm = 0;
+    for (int j = a; j < b - 1; j++)
+      for (int k = j + 1; k < b; k++) {
+        double r = bb_dist(&s->cur[3 * j], &s->cur[3 * k], nobox, 0);
+        rm += r * r;
+      }
+    rm /= (len * len);
+    rg += rm;
+    nc++;
+  }
+  return nc ? rg / nc : 0;
+}
+
+/* Simulation::EndToEndDistance (simulation.cc:483-501): mean squared. */
+double orc_end_to_end(const orc_sys* s) {
+  static const double nobox[3] = {0, 0, 0};
+  double e = 0;
+  int nc = 0;
+  for (int m = 0; m < s->n_mols; m++) {
+    int a = s->mol_start[m], b = s->mol_start[m + 1];
+    if (b - a <= 1) continue;
+    double d = bb_dist(&s->cur[3 * a], &s->cur[3 * (b - 1)], nobox, 0);
+    e += d * d;
+    nc++;
+  }
+  return nc ? e / nc : 0;
+}

@@ -1,0 +1,139 @@
+/*
+ * plum_oracle.h -- CPU FP64 restatement of the reference trial-move energy
+ * path (nuwapi/Plum), used ONLY as the test oracle.
+ *
+ * This is test infrastructure: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker.  The
+ * product (plum_amd/) never links or calls it.
+ *
+ * Every function restates a reference routine; the citation is on the
+ * definition in plum_oracle.c.  Pinning (what checks each function besides
+ * itself) is listed in the header comment of plum_oracle.c.
+ */
+#ifndef PLUM_ORACLE_H_
+#define PLUM_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_TYPES 16
+
+/* Bead kinds for the wall potential (symbol "L", "R" or anything else). */
+enum { ORC_KIND_PLAIN = 0, ORC_KIND_L = 1, ORC_KIND_R = 2 };
+
+/* Per-term energy record.  Order is fixed and shared with the product ABI. */
+enum {
+  ORC_T_LJ = 0, ORC_T_REAL = 1, ORC_T_KSPACE = 2, ORC_T_SELF = 3,
+  ORC_T_DIPOLE = 4, ORC_T_BOND = 5, ORC_T_WALL = 6, ORC_NTERMS = 7
+};
+
+typedef struct {
+  double box[3];      /* simulation box (unpadded) */
+  int npbc;           /* 2 (slit) or 3 (bulk) */
+  double beta;
+  int use_pair, use_ewald, use_bond, use_ext;
+  /* TruncatedLJ */
+  double lj_cutoff;   /* < 0 : WCA */
+  int n_types;
+  double lj_sigma[ORC_MAX_TYPES], lj_eps[ORC_MAX_TYPES];
+  /* Ewald Coul */
+  double lB, dielectric, alpha;
+  int dipole;
+  /* Spring */
+  double k_bond, r0;
+  /* TruncatedLJWall */
+  double wall_cut, wall_sigma_w, wall_eps_w;
+  double wall_sigma[ORC_MAX_TYPES], wall_eps[ORC_MAX_TYPES];
+  int type_kind[ORC_MAX_TYPES];
+} orc_params;
+
+typedef struct {
+  double box_e[3];    /* Ewald box (z padded when dipole correction is on) */
+  double vol_e;
+  double real_cutoff;
+  int real_cell[3];
+  double repl_cutoff;
+  int repl_cell[3];
+  int n_k_full;       /* grid points with 0 < k2 <= repl_cutoff */
+} orc_ewald_info;
+
+typedef struct orc_sys orc_sys;
+
+/* Creation copies the arrays.  xyz is interleaved (x0,y0,z0,x1,...).
+   mol_start has n_mols+1 entries (beads of molecule m are
+   mol_start[m] .. mol_start[m+1]-1, in file order). */
+orc_sys* orc_create(const orc_params* p, int n_beads, const double* xyz,
+                    const double* q, const int* type, int n_mols,
+                    const int* mol_start);
+void orc_destroy(orc_sys* s);
+void orc_get_ewald_info(const orc_sys* s, orc_ewald_info* out);
+
+/* Standalone cutoff derivation (potential_ewald_coul.cc:43-75). */
+void orc_derive_cutoffs(const double box_in[3], double lB, double alpha,
+                        int dipole, orc_ewald_info* out);
+
+/* Full (from-scratch) energy of the current configuration, per term. */
+void orc_energy_full(orc_sys* s, double terms[ORC_NTERMS]);
+/* k-space total in the structure-factor form (same sum, regrouped). */
+double orc_kspace_total_sk(orc_sys* s);
+/* Reference pair functions on the CURRENT coordinates of beads i, j. */
+double orc_pair_lj(orc_sys* s, int i, int j);
+double orc_pair_real(orc_sys* s, int i, int j);
+double orc_pair_repl(orc_sys* s, int i, int j);
+double orc_wall_bead(orc_sys* s, int i);
+
+/* Trial move set-up: moved beads of molecule `mol` get trial coordinates. */
+void orc_set_trial(orc_sys* s, int mol, int n_moved, const int* idx,
+                   const double* trial_xyz);
+/* ForceField::EnergyDifference with its early exits; per-term dE into
+   terms[], total returned. */
+double orc_energy_difference(orc_sys* s, int mol, double terms[ORC_NTERMS]);
+/* ForceField::FinalizeEnergies + Simulation commit/rollback + UnsetMoved. */
+void orc_finalize(orc_sys* s, int mol, int accept);
+/* Running totals (incremental, like Tot*Energy). */
+void orc_running_totals(const orc_sys* s, double terms[ORC_NTERMS]);
+void orc_get_positions(const orc_sys* s, double* xyz);
+int orc_n_beads(const orc_sys* s);
+
+/* ---- std::mt19937 and the reference move generators ---------------- */
+typedef struct { uint32_t mt[624]; int mti; } orc_mt19937;
+void orc_mt_seed(orc_mt19937* r, uint32_t seed);
+uint32_t orc_mt_next(orc_mt19937* r);
+void orc_rand_sphere(orc_mt19937* r, double vec[3]);
+
+/* Move generators: write trial coordinates of molecule `mol` into the
+   system (all beads of the molecule; moved flags set as the reference does).
+   Return the number of moved beads. */
+int orc_move_bead_translate(orc_sys* s, int mol, double move_size, orc_mt19937* r);
+int orc_move_com(orc_sys* s, int mol, double move_size, orc_mt19937* r);
+int orc_move_pivot(orc_sys* s, int mol, double move_size, orc_mt19937* r,
+                   double eq_bond_len, int vary_bond);
+int orc_move_crankshaft(orc_sys* s, int mol, double move_size, orc_mt19937* r);
+int orc_move_reptation(orc_sys* s, int mol, orc_mt19937* r, double eq_bond_len,
+                       int vary_bond);
+/* Export the moved list / trial coordinates of the pending move. */
+int orc_get_moved(const orc_sys* s, int mol, int* idx, double* trial_xyz);
+
+typedef struct {
+  int attempted, move_type, mol, accepted, n_moved;
+  double dE;
+  double terms[ORC_NTERMS];
+} orc_step_info;
+
+/* One Simulation::Run() step without GC: the per-step draw, then
+   Simulation::TranslationalMove. */
+void orc_mc_step(orc_sys* s, orc_mt19937* r, const double move_prob[5],
+                 double move_size, int phantom, int grafted, double bond_len,
+                 int vary_bond, orc_step_info* info);
+
+/* Observables (simulation.cc:406-501). */
+double orc_radius_of_gyration(const orc_sys* s);
+double orc_end_to_end(const orc_sys* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
