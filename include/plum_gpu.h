@@ -1,0 +1,180 @@
+/*
+ * plum_gpu.h -- C ABI of the MI355X trial-move energy engine.
+ *
+ * Drop-in boundary for the energy calls of nuwapi/Plum's ForceField
+ * (src/force_field/force_field.h:207-294).  Plain C: opaque context, plain
+ * pointers and sizes, status-code returns (0 = ok, < 0 = error, message via
+ * plum_last_error), no exceptions across the boundary, one HIP stream per
+ * context.  Inputs stay caller-owned; device buffers are context-owned.
+ *
+ * Units and conventions are the reference's: kT = 1 energy unit, lengths in
+ * sigma, `alpha` is the Ewald kappa^2 (sigma^-2), the Ewald cutoffs are
+ * derived from alpha exactly as potential_ewald_coul.cc:43-75 does, the
+ * energy difference is evaluated on trial coordinates and returned per term,
+ * and the 1e8 sentinel of kVeryLargeEnergy marks a forbidden move.
+ */
+#ifndef PLUM_GPU_H_
+#define PLUM_GPU_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLUM_MAX_TYPES 16
+
+typedef struct plum_ctx plum_ctx;
+
+enum plum_status {
+  PLUM_OK = 0,
+  PLUM_E_ARG = -1,          /* invalid argument or input */
+  PLUM_E_HIP = -2,          /* HIP runtime error (no device, launch failure) */
+  PLUM_E_NOMEM = -3,        /* device allocation failed */
+  PLUM_E_STATE = -4,        /* call out of order (e.g. finalize without dE) */
+  PLUM_E_CAPACITY = -5,     /* a fixed-capacity structure overflowed */
+  PLUM_E_UNSUPPORTED = -6,  /* combination the engine does not implement */
+  PLUM_E_PARSE = -7         /* malformed run.in / crd / top input */
+};
+
+/* Arithmetic of the pair sums. */
+enum plum_precision {
+  PLUM_FP64 = 0,       /* FP64 everywhere: parity path */
+  PLUM_FP32_PAIR = 1   /* FP32 pair math (real space, LJ, k-space phases),
+                          FP64 accumulation, FP64 S(k), FP64 acceptance */
+};
+
+/* Force-field parameters: the values run.in carries for the potentials used
+   by ForceField::Initialize (force_field.cc:46-195).  Bead types are indices
+   into the per-type tables; the caller maps bead symbols to indices. */
+typedef struct {
+  double box[3];        /* simulation box, unpadded (ReadTop) */
+  int npbc;             /* 2 (slit, z walls) or 3 (bulk) */
+  double beta;
+  int use_pair, use_ewald, use_bond, use_ext;
+  /* TruncatedLJ (potential_truncated_lj.cc:20-47) */
+  double lj_cutoff;     /* < 0 selects the WCA form */
+  int n_types;
+  double lj_sigma[PLUM_MAX_TYPES], lj_eps[PLUM_MAX_TYPES];
+  /* Ewald Coul (potential_ewald_coul.cc:33-36) */
+  double lB, dielectric, alpha;
+  int dipole_correction;
+  /* Spring (potential_spring.cc:16) */
+  double k_bond, r0;
+  /* TruncatedLJWall (potential_truncated_lj_wall.cc:13-44) */
+  double wall_cut, wall_sigma_w, wall_eps_w;
+  double wall_sigma[PLUM_MAX_TYPES], wall_eps[PLUM_MAX_TYPES];
+  int type_kind[PLUM_MAX_TYPES];   /* 0 plain, 1 grafted "L", 2 grafted "R" */
+  /* engine options (no reference counterpart) */
+  int precision;        /* enum plum_precision */
+  int dipole_exact;     /* 0: reference dipole lag (default); 1: exact dE */
+  int cell_div;         /* cells per search radius and dimension; 0 = auto */
+} plum_params;
+
+/* Per-term energies; layout shared with the CPU oracle's term order. */
+typedef struct {
+  double lj, real, kspace, self, dipole, bond, wall;
+} plum_terms;
+
+typedef struct {
+  double box_e[3];      /* Ewald box (z padded with the dipole correction) */
+  double vol_e;
+  double real_cutoff;
+  int real_cell[3];
+  double repl_cutoff;
+  int repl_cell[3];
+  int n_k_full;         /* k vectors with 0 < k^2 <= repl_cutoff */
+  int n_k_half;         /* stored half space (k and -k paired) */
+  int n_cells[3];       /* real-space cell grid */
+  int cell_cap;         /* slots per cell */
+  int n_beads, n_mols, max_mol_len;
+} plum_info;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Replaces ForceField::Initialize + InitializeEnergy
+   (force_field.cc:46-363, :365-405) for the energy state: uploads the
+   configuration (xyz interleaved x0,y0,z0,x1,..; molecule m owns beads
+   mol_start[m] .. mol_start[m+1]-1), derives the Ewald cutoffs, builds the
+   k-vector table, S(k), the cell list and all running totals on the GPU. */
+int plum_create(const plum_params* params, int n_beads, const double* xyz,
+                const double* q, const int* type, int n_mols,
+                const int* mol_start, plum_ctx** out);
+int plum_destroy(plum_ctx* ctx);
+
+/* Derived Ewald cutoffs and engine sizes. */
+int plum_get_info(const plum_ctx* ctx, plum_info* out);
+
+/* From-scratch energy of the current configuration, per term, computed on
+   the GPU.  Replaces the EnergyInitialization pass of each potential
+   (potential_pair.cc:55-100, potential_ewald.cc:176-230,
+   potential_bond.cc:16-27, potential_external.cc:48-61). */
+int plum_energy_full(plum_ctx* ctx, plum_terms* out);
+
+/* Replaces ForceField::EnergyDifference (force_field.cc:407-434): molecule
+   `mol` moves; its beads idx[0..n_moved) get trial coordinates trial_xyz
+   (interleaved).  Returns the per-term dE and the total with the
+   reference's early exits (LJ >= 1e8, then wall >= 1e8, stop adding). */
+int plum_energy_difference(plum_ctx* ctx, int mol, int n_moved, const int* idx,
+                           const double* trial_xyz, plum_terms* dE,
+                           double* total);
+
+/* Replaces ForceField::FinalizeEnergies (force_field.cc:436-451) together
+   with Simulation's commit/rollback of the moved molecule
+   (simulation.cc:336-351). */
+int plum_finalize(plum_ctx* ctx, int accept);
+
+/* Running totals: ForceField::Tot{Pair,Ewald,Bond,Ext}Energy
+   (force_field.cc:1201-1219), split per term. */
+int plum_totals(plum_ctx* ctx, plum_terms* out);
+
+/* Rebuilds S(k), the cell list and the dipole moment from scratch and
+   resets the running totals to the from-scratch energy; `drift` (may be
+   NULL) receives running - scratch before the reset.  No reference
+   counterpart: the reference keeps incremental totals only. */
+int plum_resync(plum_ctx* ctx, plum_terms* drift);
+
+/* Current (committed) coordinates, interleaved, n_beads*3 doubles. */
+int plum_get_positions(plum_ctx* ctx, double* xyz);
+
+/* Message of the last failing call on this thread. */
+const char* plum_last_error(void);
+
+/* ---- GPU-resident Monte Carlo ----------------------------------------- */
+
+/* Move-mix and run options of the device-resident trial-move loop.  The
+   proposal generators restate Molecule::{BeadTranslate, COMTranslate, Pivot,
+   Crankshaft, RandomReptation} (molecule.cc:103-312) and the selection of
+   Simulation::TranslationalMove (simulation.cc:241-355), drawing from a
+   counter-based Philox4x32-10 stream instead of mt19937. */
+typedef struct {
+  double move_prob[5];  /* bead, COM, pivot, crankshaft, reptation */
+  double move_size;
+  int phantom, grafted;
+  double bond_len;      /* EqBondLen or rigid bond length */
+  int vary_bond;        /* +-10% bond length in pivot / reptation */
+  unsigned long long seed;
+  int resync_every;     /* from-scratch resync period in moves (0 = never) */
+} plum_mc_params;
+
+typedef struct {
+  long long attempted[5], accepted[5];
+  long long moves;
+  double seconds;       /* wall time of the run (host clock, synchronized) */
+  plum_terms totals;    /* running totals at the end */
+} plum_mc_stats;
+
+/* Runs n_moves trial moves entirely on the GPU (proposal, dE, Metropolis,
+   commit) on the context's stream. */
+int plum_mc_run(plum_ctx* ctx, const plum_mc_params* mc, long long n_moves,
+                plum_mc_stats* stats);
+
+/* Average device duration (microseconds, HIP events on the context stream)
+   of each engine kernel over the last plum_mc_run with profiling enabled.
+   names/avg_us/count arrays of capacity `cap`; returns the number filled. */
+int plum_set_profiling(plum_ctx* ctx, int on);
+int plum_kernel_times(plum_ctx* ctx, int cap, const char** names,
+                      double* avg_us, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""ctypes binding of the C ABI in include/plum_gpu.h (libplum_gpu.so).
+
+This is the product path: every call goes to the HIP engine.  There is no
+CPU fallback; if the shared library or a GPU is missing, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libplum_gpu.so")
+MAX_TYPES = 16
+TERMS = ("lj", "real", "kspace", "self", "dipole", "bond", "wall")
+
+PLUM_FP64 = 0
+PLUM_FP32_PAIR = 1
+
+# every symbol include/plum_gpu.h declares
+ABI_SYMBOLS = (
+    "plum_create", "plum_destroy", "plum_get_info", "plum_energy_full",
+    "plum_energy_difference", "plum_finalize", "plum_totals", "plum_resync",
+    "plum_get_positions", "plum_last_error", "plum_mc_run", "plum_set_profiling",
+    "plum_kernel_times",
+)
+
+
+class PlumError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("box", C.c_double * 3), ("npbc", C.c_int), ("beta", C.c_double),
+        ("use_pair", C.c_int), ("use_ewald", C.c_int), ("use_bond", C.c_int),
+        ("use_ext", C.c_int),
+        ("lj_cutoff", C.c_double), ("n_types", C.c_int),
+        ("lj_sigma", C.c_double * MAX_TYPES), ("lj_eps", C.c_double * MAX_TYPES),
+        ("lB", C.c_double), ("dielectric", C.c_double), ("alpha", C.c_double),
+        ("dipole_correction", C.c_int),
+        ("k_bond", C.c_double), ("r0", C.c_double),
+        ("wall_cut", C.c_double), ("wall_sigma_w", C.c_double), ("wall_eps_w", C.c_double),
+        ("wall_sigma", C.c_double * MAX_TYPES), ("wall_eps", C.c_double * MAX_TYPES),
+        ("type_kind", C.c_int * MAX_TYPES),
+        ("precision", C.c_int), ("dipole_exact", C.c_int), ("cell_div", C.c_int),
+    ]
+
+
+class Terms(C.Structure):
+    _fields_ = [(t, C.c_double) for t in TERMS]
+
+    def as_dict(self):
+        return {t: getattr(self, t) for t in TERMS}
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("box_e", C.c_double * 3), ("vol_e", C.c_double), ("real_cutoff", C.c_double),
+        ("real_cell", C.c_int * 3), ("repl_cutoff", C.c_double), ("repl_cell", C.c_int * 3),
+        ("n_k_full", C.c_int), ("n_k_half", C.c_int), ("n_cells", C.c_int * 3),
+        ("cell_cap", C.c_int), ("n_beads", C.c_int), ("n_mols", C.c_int),
+        ("max_mol_len", C.c_int),
+    ]
+
+
+class MCParams(C.Structure):
+    _fields_ = [
+        ("move_prob", C.c_double * 5), ("move_size", C.c_double), ("phantom", C.c_int),
+        ("grafted", C.c_int), ("bond_len", C.c_double), ("vary_bond", C.c_int),
+        ("seed", C.c_ulonglong), ("resync_every", C.c_int),
+    ]
+
+
+class MCStats(C.Structure):
+    _fields_ = [
+        ("attempted", C.c_longlong * 5), ("accepted", C.c_longlong * 5),
+        ("moves", C.c_longlong), ("seconds", C.c_double), ("totals", Terms),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libplum_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PlumError("libplum_gpu.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.plum_create.argtypes = [C.POINTER(Params), C.c_int, dp, dp, ip, C.c_int, ip,
+                                  C.POINTER(vp)]
+        L.plum_destroy.argtypes = [vp]
+        L.plum_get_info.argtypes = [vp, C.POINTER(Info)]
+        L.plum_energy_full.argtypes = [vp, C.POINTER(Terms)]
+        L.plum_energy_difference.argtypes = [vp, C.c_int, C.c_int, ip, dp, C.POINTER(Terms), dp]
+        L.plum_finalize.argtypes = [vp, C.c_int]
+        L.plum_totals.argtypes = [vp, C.POINTER(Terms)]
+        L.plum_resync.argtypes = [vp, C.POINTER(Terms)]
+        L.plum_get_positions.argtypes = [vp, dp]
+        L.plum_last_error.restype = C.c_char_p
+        L.plum_mc_run.argtypes = [vp, C.POINTER(MCParams), C.c_longlong, C.POINTER(MCStats)]
+        L.plum_set_profiling.argtypes = [vp, C.c_int]
+        L.plum_kernel_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), dp,
+                                        C.POINTER(C.c_longlong)]
+        for f in ABI_SYMBOLS:
+            if f != "plum_last_error":
+                getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PlumError("plum error %d: %s" % (rc, lib().plum_last_error().decode()))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def make_params(sysdef, precision=PLUM_FP64, dipole_exact=0, cell_div=0):
+    p = Params()
+    for i in range(3):
+        p.box[i] = sysdef["box"][i]
+    p.npbc = sysdef["npbc"]
+    p.beta = sysdef.get("beta", 1.0)
+    p.use_pair = int(sysdef.get("use_pair", 1))
+    p.use_ewald = int(sysdef.get("use_ewald", 1))
+    p.use_bond = int(sysdef.get("use_bond", 1))
+    p.use_ext = int(sysdef.get("use_ext", 0))
+    p.lj_cutoff = sysdef.get("lj_cutoff", 2.5)
+    types = sysdef["types"]
+    p.n_types = len(types)
+    for t, td in enumerate(types):
+        p.lj_sigma[t] = td.get("sigma", 0.0)
+        p.lj_eps[t] = td.get("eps", 0.0)
+        p.wall_sigma[t] = td.get("wall_sigma", 0.0)
+        p.wall_eps[t] = td.get("wall_eps", 0.0)
+        p.type_kind[t] = {"L": 1, "R": 2}.get(td["name"], 0)
+    p.lB = sysdef.get("lB", 1.0)
+    p.dielectric = sysdef.get("dielectric", 80.0)
+    p.alpha = sysdef.get("alpha", 0.05)
+    p.dipole_correction = int(sysdef.get("dipole", 0))
+    p.k_bond = sysdef.get("k_bond", 400.0)
+    p.r0 = sysdef.get("r0", 1.0)
+    p.wall_cut = sysdef.get("wall_cut", -1.0)
+    p.wall_sigma_w = sysdef.get("wall_sigma_w", 1.0)
+    p.wall_eps_w = sysdef.get("wall_eps_w", 1.0)
+    p.precision = precision
+    p.dipole_exact = dipole_exact
+    p.cell_div = cell_div
+    return p
+
+
+class Engine:
+    """One system on the GPU, driven through the C ABI."""
+
+    def __init__(self, sysdef, precision=PLUM_FP64, dipole_exact=0, cell_div=0):
+        L = lib()
+        self.params = make_params(sysdef, precision, dipole_exact, cell_div)
+        xyz = np.ascontiguousarray(sysdef["xyz"], dtype=np.float64).reshape(-1)
+        q = np.ascontiguousarray(sysdef["q"], dtype=np.float64)
+        ty = np.ascontiguousarray(sysdef["type"], dtype=np.int32)
+        ms = np.ascontiguousarray(sysdef["mol_start"], dtype=np.int32)
+        self.n = len(q)
+        self.mol_start = ms
+        h = C.c_void_p()
+        _check(L.plum_create(C.byref(self.params), self.n, _dp(xyz), _dp(q), _ip(ty),
+                             len(ms) - 1, _ip(ms), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().plum_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = Info()
+        _check(lib().plum_get_info(self.h, C.byref(i)))
+        return i
+
+    def energy_full(self):
+        t = Terms()
+        _check(lib().plum_energy_full(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def delta(self, mol, idx, trial_xyz):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        tr = np.ascontiguousarray(trial_xyz, dtype=np.float64).reshape(-1)
+        t = Terms()
+        tot = C.c_double()
+        _check(lib().plum_energy_difference(self.h, mol, len(idx), _ip(idx), _dp(tr),
+                                            C.byref(t), C.byref(tot)))
+        return tot.value, t.as_dict()
+
+    def finalize(self, accept):
+        _check(lib().plum_finalize(self.h, int(accept)))
+
+    def totals(self):
+        t = Terms()
+        _check(lib().plum_totals(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def resync(self):
+        t = Terms()
+        _check(lib().plum_resync(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def positions(self):
+        out = np.zeros(3 * self.n)
+        _check(lib().plum_get_positions(self.h, _dp(out)))
+        return out.reshape(-1, 3)
+
+    def mc_run(self, n_moves, move_prob, move_size=1.0, seed=1, phantom=0, grafted=0,
+               bond_len=1.0, vary_bond=True, resync_every=0):
+        mp = MCParams()
+        for i in range(5):
+            mp.move_prob[i] = move_prob[i]
+        mp.move_size = move_size
+        mp.phantom = phantom
+        mp.grafted = grafted
+        mp.bond_len = bond_len
+        mp.vary_bond = int(vary_bond)
+        mp.seed = seed
+        mp.resync_every = resync_every
+        st = MCStats()
+        _check(lib().plum_mc_run(self.h, C.byref(mp), n_moves, C.byref(st)))
+        return st
+
+    def set_profiling(self, on):
+        _check(lib().plum_set_profiling(self.h, int(on)))
+
+    def kernel_times(self, cap=32):
+        names = (C.c_char_p * cap)()
+        avg = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        n = lib().plum_kernel_times(self.h, cap, names, _dp(avg),
+                                    cnt.ctypes.data_as(C.POINTER(C.c_longlong)))
+        if n < 0:
+            _check(n)
+        return {names[i].decode(): (avg[i], int(cnt[i])) for i in range(n)}
