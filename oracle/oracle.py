@@ -73,6 +73,8 @@ def lib():
         ip = C.POINTER(C.c_int)
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.POINTER(OrcParams), C.c_int, dp, dp, ip, C.c_int, ip]
+        L.orc_create_ex.restype = C.c_void_p
+        L.orc_create_ex.argtypes = [C.POINTER(OrcParams), C.c_int, dp, dp, ip, C.c_int, ip, C.c_int]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_get_ewald_info.argtypes = [C.c_void_p, C.POINTER(OrcEwaldInfo)]
         L.orc_derive_cutoffs.argtypes = [dp, C.c_double, C.c_double, C.c_int,
@@ -106,6 +108,8 @@ def lib():
                                          C.c_double, C.c_int]
         L.orc_mc_step.argtypes = [C.c_void_p, C.POINTER(OrcMT), dp, C.c_double, C.c_int,
                                   C.c_int, C.c_double, C.c_int, C.POINTER(OrcStepInfo)]
+        L.orc_bead_energy_sample.argtypes = [C.c_void_p, dp, C.c_double, C.c_int, C.c_int]
+        L.orc_bead_energy_sample.restype = C.c_double
         L.orc_radius_of_gyration.argtypes = [C.c_void_p]
         L.orc_radius_of_gyration.restype = C.c_double
         L.orc_end_to_end.argtypes = [C.c_void_p]
@@ -164,7 +168,7 @@ def derive_cutoffs(box, lB, alpha, dipole):
 class Oracle:
     """One reference-semantics system on the CPU."""
 
-    def __init__(self, sysdef):
+    def __init__(self, sysdef, init_energy=True):
         L = lib()
         self.sysdef = sysdef
         self.params = make_params(sysdef)
@@ -175,8 +179,8 @@ class Oracle:
         self.n = len(q)
         self.n_mols = len(ms) - 1
         self.mol_start = ms
-        self.h = L.orc_create(C.byref(self.params), self.n, _dp(xyz), _dp(q), _ip(ty),
-                              self.n_mols, _ip(ms))
+        self.h = L.orc_create_ex(C.byref(self.params), self.n, _dp(xyz), _dp(q), _ip(ty),
+                                 self.n_mols, _ip(ms), int(init_energy))
 
     def close(self):
         if self.h:
@@ -237,6 +241,10 @@ class Oracle:
         t = np.zeros(3 * n)
         k = lib().orc_get_moved(self.h, mol, _ip(idx), _dp(t))
         return idx[:k].copy(), t[:3 * k].reshape(-1, 3).copy()
+
+    def bead_energy_sample(self, xyz, q, t, n_partners):
+        p = np.ascontiguousarray(xyz, dtype=np.float64)
+        return lib().orc_bead_energy_sample(self.h, _dp(p), float(q), int(t), int(n_partners))
 
     def rg(self):
         return lib().orc_radius_of_gyration(self.h)

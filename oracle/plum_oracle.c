@@ -323,6 +323,12 @@ static double wall_bead_z(const orc_sys* s, double z, int t) {
 orc_sys* orc_create(const orc_params* p, int n, const double* xyz,
                     const double* q, const int* type, int n_mols,
                     const int* mol_start) {
+  return orc_create_ex(p, n, xyz, q, type, n_mols, mol_start, 1);
+}
+
+orc_sys* orc_create_ex(const orc_params* p, int n, const double* xyz,
+                       const double* q, const int* type, int n_mols,
+                       const int* mol_start, int init_energy) {
   orc_sys* s = calloc(1, sizeof(orc_sys));
   s->p = *p;
   s->n = n;
@@ -350,8 +356,8 @@ orc_sys* orc_create(const orc_params* p, int n, const double* xyz,
     s->e.vol_e = p->box[0] * p->box[1] * p->box[2];
   }
   /* ForceField::InitializeEnergy -> running totals (force_field.cc:365-381) */
-  double t[ORC_NTERMS];
-  orc_energy_full(s, t);
+  double t[ORC_NTERMS] = {0, 0, 0, 0, 0, 0, 0};
+  if (init_energy) orc_energy_full(s, t);
   s->E_lj = t[ORC_T_LJ];
   s->E_real = t[ORC_T_REAL];
   s->E_kspace = t[ORC_T_KSPACE];
@@ -919,6 +925,21 @@ void orc_mc_step(orc_sys* s, orc_mt19937* r, const double move_prob[5],
     info->accepted = accept;
     orc_finalize(s, mol_id, accept);
   }
+}
+
+double orc_bead_energy_sample(orc_sys* s, const double xyz[3], double q, int t,
+                              int n_partners) {
+  double e = 0;
+  if (n_partners > s->n) n_partners = s->n;
+  for (int j = 0; j < n_partners; j++) {
+    const double* b = &s->trial[3 * j];
+    if (s->p.use_pair) e += lj_pair_xyz(s, xyz, t, b, s->type[j]);
+    if (s->p.use_ewald) {
+      e += real_pair_xyz(s, xyz, q, b, s->q[j]);
+      e += repl_pair_xyz(s, xyz, q, b, s->q[j]);
+    }
+  }
+  return e;
 }
 
 /* Simulation::RadiusOfGyration (simulation.cc:406-434) over all chains. */

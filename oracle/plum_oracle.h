@@ -68,6 +68,12 @@ typedef struct orc_sys orc_sys;
 orc_sys* orc_create(const orc_params* p, int n_beads, const double* xyz,
                     const double* q, const int* type, int n_mols,
                     const int* mol_start);
+/* As orc_create; init_energy = 0 skips the from-scratch energy pass (and so
+   leaves the running totals at zero) for large systems that are only
+   sampled. */
+orc_sys* orc_create_ex(const orc_params* p, int n_beads, const double* xyz,
+                       const double* q, const int* type, int n_mols,
+                       const int* mol_start, int init_energy);
 void orc_destroy(orc_sys* s);
 void orc_get_ewald_info(const orc_sys* s, orc_ewald_info* out);
 
@@ -128,6 +134,15 @@ typedef struct {
 void orc_mc_step(orc_sys* s, orc_mt19937* r, const double move_prob[5],
                  double move_size, int phantom, int grafted, double bond_len,
                  int vary_bond, orc_step_info* info);
+
+/* Pairwise energy (LJ + real + k space, reference pair functions) of a bead
+   of charge q and type t at position xyz against beads [0, n_partners):
+   the inner loop of PotentialEwald::EnergyDifference for one moved bead
+   (potential_ewald.cc:479-514) and of PotentialPair::EnergyDifference
+   (potential_pair.cc:266-282).  Used to time the reference algorithm on a
+   bounded sample. */
+double orc_bead_energy_sample(orc_sys* s, const double xyz[3], double q, int t,
+                              int n_partners);
 
 /* Observables (simulation.cc:406-501). */
 double orc_radius_of_gyration(const orc_sys* s);

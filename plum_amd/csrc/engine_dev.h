@@ -49,6 +49,7 @@ struct Geo {
   double wall_r3_k213;  // pow(1/k213, 3), host-evaluated
   double wall_r3_cut;   // pow(1/wall_cut, 3), host-evaluated
   int dipole, dipole_exact;
+  double fp64_below;    // FP32 path: pairs closer than this use FP64 math
 };
 
 // LJ pair table entry for (ti, tj): sigma, epsilon, cutoff distance (strict
@@ -56,6 +57,12 @@ struct Geo {
 // expressions (potential_truncated_lj.cc:49-85).
 struct LJPair {
   double sigma, eps, cut, shift;
+};
+
+// FP32 copy for the reduced-precision pair path: sigma^2, 4 eps, cut^2, shift.
+struct LJPair32 {
+  float sigma2, eps4, shift;
+  double cut;
 };
 
 struct WallType {
