@@ -157,14 +157,13 @@ def main():
     mp = sd["move_prob"]
     n_moved = mp[0] * 1 + (mp[1] + mp[2] + mp[4]) * clen + mp[3] * max(clen - 2, 0) / 3.0
     cand = real_space_work(sd, info)
-    dom = max(kt.items(), key=lambda kv: kv[1][0])
-    # algorithmic work per launch of the two wide kernels
-    work = {
-        "k_real_delta": ("mfma", FLOP_PER_PAIR * 2 * n_moved * cand / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"),
-        "k_kspace_delta": ("hbm", 40.0 * info.n_k_half / 1e9, PEAK_HBM_GBS, "GB/s"),
-    }
-    rl_name = "k_real_delta" if "k_real_delta" in kt else dom[0]
-    bound, per_launch, peak, unit = work.get(rl_name, ("hbm", 0.0, PEAK_HBM_GBS, "GB/s"))
+    # algorithmic work of one k_compute launch (one trial move): the k-space
+    # update 16 n N_k + 8 N_k FLOP plus the real-space pair sum 45 FLOP per
+    # candidate pair-image, old and new position of every moved bead
+    kflop = 16.0 * n_moved * info.n_k_half + 8.0 * info.n_k_half
+    rflop = FLOP_PER_PAIR * 2 * n_moved * cand
+    rl_name = "k_compute"
+    per_launch = (kflop + rflop) / 1e12
     avg_us = kt[rl_name][0]
     achieved = per_launch / (avg_us * 1e-6) if avg_us > 0 else 0.0
 
@@ -179,11 +178,14 @@ def main():
                    "particles": n, "moves_per_step": a.moves_per_step},
         "accept_ratio": acc_ratio,
         "kernel_avg_us": {k: round(v[0], 3) for k, v in kt.items()},
-        "roofline": {"bound": bound, "kernel": rl_name, "achieved": achieved, "peak": peak,
-                     "unit": unit, "frac": achieved / peak if peak else 0.0, "traffic": None,
+        "roofline": {"bound": "mfma", "pipe": "FP64 VALU (dense FP64 peak)", "kernel": rl_name,
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "work_per_launch": per_launch,
-                     "basis": "%.1f moved beads x 2 positions x %.0f candidate pair-images x "
-                              "%.0f FLOP" % (n_moved, cand, FLOP_PER_PAIR)},
+                     "basis": "k-space 16 x %.1f moved beads x %d k + 8 x %d k; real space "
+                              "%.0f FLOP x 2 positions x %.1f moved beads x %.0f candidate "
+                              "pair-images" % (n_moved, info.n_k_half, info.n_k_half,
+                                               FLOP_PER_PAIR, n_moved, cand)},
     }
     if not a.no_fp32:
         e32 = E.Engine(sd, precision=E.PLUM_FP32_PAIR)

@@ -1,0 +1,50 @@
+/*
+ * plum_host.h -- C ABI of the host layer (libplum_host.so): the run.in
+ * reader and a Monte Carlo driver with the reference's move semantics, on top
+ * of the GPU engine of plum_gpu.h.  For callers that hand over a reference
+ * input deck instead of building arrays themselves.
+ */
+#ifndef PLUM_HOST_H_
+#define PLUM_HOST_H_
+
+#include "plum_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct plum_sim plum_sim;
+
+/* What a run.in (plus its crd/top files) describes.  Replaces the input half
+   of Simulation::Simulation (simulation.cc:23-201) and of
+   ForceField::Initialize (force_field.cc:46-195); no GPU is touched. */
+typedef struct {
+  int n_beads, n_mols, n_types;
+  int npbc, use_pair, use_ewald, use_bond, use_ext, use_gc, dipole_correction;
+  double box[3];
+  double beta, move_size, move_prob[5];
+  long long steps;
+  double lj_cutoff, lB, alpha, k_bond, r0, rigid_bond, wall_cut;
+  int phantom, grafted;
+} plum_runin_summary;
+
+int plum_runin_parse(const char* run_in_path, plum_runin_summary* out);
+
+/* Simulation built from a run.in path (crd/top resolved next to it), seeded
+   explicitly (the reference seeds from the clock, simulation.cc:133-136). */
+int plum_sim_create(const char* run_in_path, unsigned seed, plum_sim** out);
+/* Simulation::Run without grand-canonical moves and without output files
+   (simulation.cc:216-355): `steps` trial moves on the host RNG stream, each
+   energy difference evaluated on the GPU. */
+int plum_sim_run(plum_sim* sim, long long steps, long long attempted[5], long long accepted[5]);
+int plum_sim_energy(plum_sim* sim, plum_terms* running, plum_terms* from_scratch);
+int plum_sim_positions(plum_sim* sim, double* xyz, int n_beads);
+/* Simulation::RadiusOfGyration / EndToEndDistance (simulation.cc:406-501). */
+int plum_sim_observables(plum_sim* sim, double* rg2, double* ree2);
+int plum_sim_destroy(plum_sim* sim);
+const char* plum_host_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -39,6 +39,7 @@ struct plum_ctx {
   int nchunk = 1;        // bead chunks of the S(k) build
   int chunk = 512;
   int nrealb = 1;        // blocks of the full real-space kernel
+  int nsplit = 1;        // blocks per real-space query
   double* epart_k = nullptr;
   std::vector<int> mol_start;
   std::vector<int> type_h;
@@ -254,35 +255,20 @@ int nmm_for(const plum_ctx* c, int nm) {
   return std::max(1, std::min(c->nmmb, (npair + kBlock - 1) / kBlock));
 }
 
-// Phase tables, k-space and real-space kernels of one trial move, once the
-// moved list exists on the device.  nm is the exact count (host path) or the
-// capacity (device loop, the kernels read the real count on the device).
-void launch_delta(plum_ctx* c, int nm, std::vector<hipEvent_t>* ev, size_t* e) {
+// k-space and real-space work of one trial move in one launch, once the
+// moved list (and, for the k space, the phase tables) exists on the device.
+// nm is the exact count (host path) or the capacity (device loop; the
+// kernels read the real count on the device).
+void launch_compute(plum_ctx* c, int nm) {
   const Geo& g = c->g;
-  const int fp32 = c->p.precision == PLUM_FP32_PAIR;
-  auto mark = [&]() {
-    if (ev) (void)hipEventRecord((*ev)[(*e)++], c->st);
-  };
-  if (g.use_ewald && g.n_kh > 0) {
-    const int tw = nm * 2 * std::max(1, g.tab_w);
-    const int tb = std::max(1, std::min(2048, (tw + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_tables, dim3(tb), dim3(kBlock), 0, c->st, g, c->d);
-    mark();
-    hipLaunchKernelGGL(k_kspace_delta, dim3(c->nkb), dim3(kBlock), 0, c->st, g, c->d);
-    mark();
-  } else {
-    mark();
-    mark();
-  }
+  const int nkb = (g.use_ewald && g.n_kh > 0) ? c->nkb : 0;
   const int nmm = nmm_for(c, nm);
-  const int nreal = 2 * nm;
-  if (nreal + nmm > 0) {
-    if (fp32)
-      hipLaunchKernelGGL(k_real_delta<1>, dim3(nreal + nmm), dim3(kBlock), 0, c->st, g, c->d, nreal, nmm);
-    else
-      hipLaunchKernelGGL(k_real_delta<0>, dim3(nreal + nmm), dim3(kBlock), 0, c->st, g, c->d, nreal, nmm);
-  }
-  mark();
+  const int nreal = 2 * nm * c->nsplit;
+  const int grid = nkb + nreal + nmm;
+  if (c->p.precision == PLUM_FP32_PAIR)
+    hipLaunchKernelGGL(k_compute<1>, dim3(grid), dim3(kBlock), 0, c->st, g, c->d, nkb, nreal, nmm, c->nsplit);
+  else
+    hipLaunchKernelGGL(k_compute<0>, dim3(grid), dim3(kBlock), 0, c->st, g, c->d, nkb, nreal, nmm, c->nsplit);
 }
 
 }  // namespace
@@ -459,19 +445,26 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   ALLOC(d.bond_cur, n_mols);
   ALLOC(d.mv_old, maxm); ALLOC(d.mv_new, maxm); ALLOC(d.mv_type, maxm);
   ALLOC(d.ctl, 1);
-  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, 2 * maxm); ALLOC(d.part_mm, c->nmmb);
+  {
+    // split each real-space query so a thread sees ~1.5 candidate slots
+    const double rho = n / info.vol_e;
+    double cand = rho;
+    for (int k = 0; k < 3; k++) cand *= 2 * g.rs + g.h[k];
+    c->nsplit = std::max(1, std::min(8, (int)std::lround(cand / 384.0)));
+  }
+  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, 2 * maxm * c->nsplit); ALLOC(d.part_mm, c->nmmb);
   ALLOC(d.sk_part, (size_t)c->nchunk * std::max(1, g.n_kh));
   ALLOC(c->epart_k, c->nkb_full);
   ALLOC(d.chain_ids, n_mols); ALLOC(d.ion_ids, n_mols);
-  // move record: header (16 B) + idx[maxm] + pad + xyz[3 maxm], contiguous
-  c->off_xyz = 16 + sizeof(int) * maxm + ((sizeof(int) * maxm) % 8);
+  // move record: header + idx[maxm] + pad + xyz[3 maxm], contiguous
+  c->off_xyz = sizeof(MoveRec) + sizeof(int) * maxm + ((sizeof(int) * maxm) % 8);
   c->in_bytes = c->off_xyz + sizeof(double) * 3 * maxm;
   {
     char* din = nullptr;
     ALLOC(din, c->in_bytes);
     c->d_in = din;
     d.rec = reinterpret_cast<MoveRec*>(din);
-    d.mv_idx = reinterpret_cast<int*>(din + 16);
+    d.mv_idx = reinterpret_cast<int*>(din + sizeof(MoveRec));
     d.mv_xyz = reinterpret_cast<double*>(din + c->off_xyz);
   }
 #undef ALLOC
@@ -551,19 +544,34 @@ int plum_energy_difference(plum_ctx* c, int mol, int n_moved, const int* idx, co
   r->mol = mol;
   r->move_type = -1;
   r->valid = 1;
-  std::memcpy(c->h_in + 16, idx, sizeof(int) * n_moved);
+  int lo = b, hi = a - 1;
+  for (int k = 0; k < n_moved; k++) {
+    lo = std::min(lo, idx[k]);
+    hi = std::max(hi, idx[k]);
+  }
+  r->contig = (n_moved > 0 && hi - lo + 1 == n_moved) ? 1 : 0;
+  r->lo = lo;
+  r->hi = hi;
+  std::memcpy(c->h_in + sizeof(MoveRec), idx, sizeof(int) * n_moved);
   std::memcpy(c->h_in + c->off_xyz, trial_xyz, sizeof(double) * 3 * n_moved);
-  HIPCHK(hipMemcpyAsync(c->d_in, c->h_in, 16 + sizeof(int) * n_moved, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_in, c->h_in, sizeof(MoveRec) + sizeof(int) * n_moved, hipMemcpyHostToDevice,
+                        c->st));
   if (n_moved > 0)
     HIPCHK(hipMemcpyAsync(c->d_in + c->off_xyz, c->h_in + c->off_xyz, sizeof(double) * 3 * n_moved,
                           hipMemcpyHostToDevice, c->st));
   const Geo& g = c->g;
   hipLaunchKernelGGL(k_host_moves, dim3(std::max(1, (n_moved + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->st,
                      g, c->d);
-  launch_delta(c, n_moved, nullptr, nullptr);
+  if (g.use_ewald && g.n_kh > 0 && n_moved > 0) {
+    const int tw = n_moved * 2 * std::max(1, g.tab_w);
+    hipLaunchKernelGGL(k_tables, dim3(std::max(1, std::min(2048, (tw + kBlock - 1) / kBlock))), dim3(kBlock), 0,
+                       c->st, g, c->d);
+  }
+  launch_compute(c, n_moved);
   McDev none{};
   hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(kBlock), 0, c->st, g, c->d,
-                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * n_moved, nmm_for(c, n_moved), 0, none);
+                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * n_moved * c->nsplit, nmm_for(c, n_moved), 0,
+                     none);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -659,22 +667,16 @@ int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, 
 // ---------------------------------------------------------------- device loop
 namespace {
 
-constexpr int kDevKernels = 5;
-const char* kDevKernelNames[kDevKernels] = {"k_propose", "k_tables", "k_kspace_delta", "k_real_delta",
-                                            "k_epilogue"};
+constexpr int kDevKernels = 2;
+const char* kDevKernelNames[kDevKernels] = {"k_compute", "k_serial"};
 
-// Kernel sequence of one device-resident trial move.  With `ev` (profiling),
-// kernel k is bracketed by events k and k+1 on the context stream.
-void launch_device_move(plum_ctx* c, const McDev& mc, std::vector<hipEvent_t>* ev) {
+// One k_serial launch (epilogue of the pending move and/or proposal of the
+// next); with `ev` the launch is bracketed by events.
+void launch_serial(plum_ctx* c, const McDev& mc, int do_epi, int do_prop) {
   const Geo& g = c->g;
-  size_t e = 0;
-  if (ev) (void)hipEventRecord((*ev)[e++], c->st);
-  hipLaunchKernelGGL(k_propose, dim3(1), dim3(kBlock), 0, c->st, g, c->d, mc);
-  if (ev) (void)hipEventRecord((*ev)[e++], c->st);
-  launch_delta(c, c->maxm, ev, &e);
-  hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(kBlock), 0, c->st, g, c->d,
-                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * c->maxm, nmm_for(c, c->maxm), 1, mc);
-  if (ev) (void)hipEventRecord((*ev)[e++], c->st);
+  hipLaunchKernelGGL(k_serial, dim3(1), dim3(kBlock), 0, c->st, g, c->d, mc,
+                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * c->maxm * c->nsplit, nmm_for(c, c->maxm),
+                     do_epi, do_prop);
 }
 
 void flush_sk(plum_ctx* c) {
@@ -726,7 +728,7 @@ extern "C" int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_m
   const int batch = 256;
   std::vector<hipEvent_t> ev;
   if (c->profiling) {
-    ev.resize((size_t)(kDevKernels + 1) * batch);
+    ev.resize((size_t)3 * batch);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     c->kname.assign(kDevKernelNames, kDevKernelNames + kDevKernels);
     c->ksum_us.assign(kDevKernels, 0.0);
@@ -734,23 +736,22 @@ extern "C" int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_m
   }
   const auto t0 = std::chrono::steady_clock::now();
   const long long sk0 = c->h_ctl->skipped;
-  long long issued = 0;     // moves launched, not counting skipped ones
+  long long issued = 0;     // moves completed, not counting skipped ones
   long long since_resync = 0;
   int guard = 0;
   while (issued < n_moves) {
     const long long todo = std::min<long long>(batch, n_moves - issued);
+    // pipeline: propose; (compute, epilogue+propose) x (todo-1); compute, epilogue
+    launch_serial(c, mc, 0, 1);
     for (long long i = 0; i < todo; i++) {
-      if (c->profiling) {
-        std::vector<hipEvent_t> sub(ev.begin() + (kDevKernels + 1) * i,
-                                    ev.begin() + (kDevKernels + 1) * (i + 1));
-        launch_device_move(c, mc, &sub);
-      } else {
-        launch_device_move(c, mc, nullptr);
-      }
-      if (((issued + i + 1) % compact_every) == 0)
-        hipLaunchKernelGGL(k_compact, dim3((c->g.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, c->st, c->g,
-                           c->d);
+      const bool last = (i == todo - 1);
+      if (c->profiling) (void)hipEventRecord(ev[3 * i], c->st);
+      launch_compute(c, c->maxm);
+      if (c->profiling) (void)hipEventRecord(ev[3 * i + 1], c->st);
+      launch_serial(c, mc, 1, last ? 0 : 1);
+      if (c->profiling) (void)hipEventRecord(ev[3 * i + 2], c->st);
     }
+    hipLaunchKernelGGL(k_compact, dim3((c->g.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, c->st, c->g, c->d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -758,8 +759,7 @@ extern "C" int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_m
       for (long long i = 0; i < todo; i++)
         for (int k = 0; k < kDevKernels; k++) {
           float ms = 0;
-          if (hipEventElapsedTime(&ms, ev[(kDevKernels + 1) * i + k], ev[(kDevKernels + 1) * i + k + 1]) ==
-              hipSuccess) {
+          if (hipEventElapsedTime(&ms, ev[3 * i + k], ev[3 * i + k + 1]) == hipSuccess) {
             c->ksum_us[k] += 1000.0 * ms;
             c->kcount[k] += 1;
           }

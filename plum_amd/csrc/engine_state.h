@@ -12,11 +12,15 @@ struct MoveRec {
   int mol;
   int move_type;   // 0 bead, 1 COM, 2 pivot, 3 crankshaft, 4 reptation, -1 external
   int valid;       // 0: no move attempted (e.g. no chain to move)
+  int contig;      // moved beads are exactly the index range [lo, hi]
+  int lo, hi;
+  int pad;
 };
 
 // Control block: scalars of the energy state and the run.
 struct Ctl {
   double Mz;            // sum q z over current coordinates
+  double dMz;           // sum q (z' - z) of the pending move
   double cur_dipl;      // the reference's current_dipl_E
   double trial_dipl;    // the reference's trial_dipl_E
   double bond_trial;    // spring energy of the moved molecule (trial)

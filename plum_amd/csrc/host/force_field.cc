@@ -1,0 +1,181 @@
+#include "force_field.h"
+
+#include <sstream>
+#include <stdexcept>
+
+namespace plum {
+
+ForceField::~ForceField() {
+  if (ctx_) plum_destroy(ctx_);
+}
+
+void ForceField::check(int rc, const char* what) {
+  if (rc != PLUM_OK)
+    throw std::runtime_error(std::string("ForceField::") + what + ": " + plum_last_error());
+}
+
+int ForceField::TypeOf(const std::string& symbol) const {
+  auto it = types_.find(symbol);
+  return it == types_.end() ? -1 : it->second;
+}
+
+double ForceField::EqBondLen() const {
+  if (!r_.use_bond) throw std::runtime_error("Illegal request of bond length (no bond potential)");
+  return r_.r0;
+}
+
+void ForceField::Initialize(double beta, int npbc, double box_l[3], std::vector<Molecule>& mols, int phantom,
+                            int coion, int grafted, int grafted_counterion, std::istream& in) {
+  RunIn r;
+  r.beta = beta;
+  r.npbc = npbc;
+  r.phantom = phantom;
+  r.coion = coion;
+  r.grafted = grafted;
+  r.grafted_counterion = grafted_counterion;
+  std::string err;
+  if (!read_ff_block(in, r, err)) throw std::runtime_error(err);
+  Initialize(r, box_l, mols);
+}
+
+void ForceField::build_params(plum_params& p, const double box_l[3]) {
+  p = plum_params{};
+  for (int k = 0; k < 3; k++) p.box[k] = box_l[k];
+  p.npbc = r_.npbc;
+  p.beta = r_.beta;
+  p.use_pair = r_.use_pair;
+  p.use_ewald = r_.use_ewald;
+  p.use_bond = r_.use_bond;
+  p.use_ext = r_.use_ext;
+  p.lj_cutoff = r_.lj_cutoff;
+  p.n_types = (int)type_names_.size();
+  // per-type tables; symbols missing from a table read 0, like the
+  // reference's map<string,double>::operator[]
+  for (size_t t = 0; t < type_names_.size(); t++) {
+    const std::string& s = type_names_[t];
+    for (const TypeParam& tp : r_.lj_types)
+      if (tp.symbol == s) { p.lj_sigma[t] = tp.sigma; p.lj_eps[t] = tp.eps; }
+    for (const TypeParam& tp : r_.wall_types)
+      if (tp.symbol == s) { p.wall_sigma[t] = tp.sigma; p.wall_eps[t] = tp.eps; }
+    p.type_kind[t] = s == "L" ? 1 : (s == "R" ? 2 : 0);
+  }
+  p.lB = r_.lB;
+  p.dielectric = r_.dielectric;
+  p.alpha = r_.alpha;
+  p.dipole_correction = r_.dipole_correction;
+  p.k_bond = r_.k_bond;
+  p.r0 = r_.r0;
+  p.wall_cut = r_.wall_cut;
+  p.wall_sigma_w = r_.wall_sigma;
+  p.wall_eps_w = r_.wall_eps;
+  p.precision = PLUM_FP64;
+}
+
+void ForceField::Initialize(const RunIn& r, double box_l[3], std::vector<Molecule>& mols) {
+  r_ = r;
+  if (r_.use_gc) throw std::runtime_error("grand-canonical moves are not handled by this ForceField");
+  types_.clear();
+  type_names_.clear();
+  auto add = [&](const std::string& s) {
+    if (types_.count(s)) return;
+    if ((int)type_names_.size() >= PLUM_MAX_TYPES) throw std::runtime_error("too many bead types");
+    types_[s] = (int)type_names_.size();
+    type_names_.push_back(s);
+  };
+  for (const TypeParam& t : r_.lj_types) add(t.symbol);
+  for (const TypeParam& t : r_.wall_types) add(t.symbol);
+  for (Molecule& m : mols)
+    for (Bead& b : m.bds) {
+      add(b.Symbol());
+      b.SetType(types_[b.Symbol()]);
+    }
+  offset_.assign(mols.size() + 1, 0);
+  for (size_t m = 0; m < mols.size(); m++) offset_[m + 1] = offset_[m] + mols[m].Size();
+  for (int k = 0; k < 3; k++) box_[k] = box_l[k];
+  InitializeEnergy(mols);
+  UpdateMolCounts(mols);
+}
+
+void ForceField::InitializeEnergy(std::vector<Molecule>& mols) {
+  plum_params p;
+  double box[3];
+  const int n = offset_.back();
+  std::vector<double> xyz(3 * (size_t)n), q(n);
+  std::vector<int> type(n), ms(mols.size() + 1);
+  for (size_t m = 0; m < mols.size(); m++) {
+    ms[m] = offset_[m];
+    for (int k = 0; k < mols[m].Size(); k++) {
+      const Bead& b = mols[m].bds[k];
+      const int i = offset_[m] + k;
+      for (int e = 0; e < 3; e++) xyz[3 * (size_t)i + e] = b.GetCrd(0, e);
+      q[i] = b.Charge();
+      type[i] = b.Type();
+    }
+  }
+  ms[mols.size()] = n;
+  box[0] = box_[0];
+  box[1] = box_[1];
+  box[2] = box_[2];
+  build_params(p, box);
+  if (ctx_) {
+    plum_destroy(ctx_);
+    ctx_ = nullptr;
+  }
+  check(plum_create(&p, n, xyz.data(), q.data(), type.data(), (int)mols.size(), ms.data(), &ctx_),
+        "InitializeEnergy");
+}
+
+double ForceField::EnergyDifference(std::vector<Molecule>& mols, int moved_mol) {
+  const Molecule& mol = mols.at(moved_mol);
+  std::vector<int> idx;
+  std::vector<double> tr;
+  idx.reserve(mol.Size());
+  tr.reserve(3 * (size_t)mol.Size());
+  for (int k = 0; k < mol.Size(); k++) {
+    const Bead& b = mol.bds[k];
+    if (!b.GetMoved()) continue;
+    idx.push_back(offset_[moved_mol] + k);
+    for (int e = 0; e < 3; e++) tr.push_back(b.GetCrd(1, e));
+  }
+  double total = 0;
+  check(plum_energy_difference(ctx_, moved_mol, (int)idx.size(), idx.data(), tr.data(), &last_dE_, &total),
+        "EnergyDifference");
+  return total;
+}
+
+void ForceField::FinalizeEnergies(std::vector<Molecule>&, bool accept, int) {
+  check(plum_finalize(ctx_, accept ? 1 : 0), "FinalizeEnergies");
+}
+
+plum_terms ForceField::Totals() {
+  plum_terms t;
+  check(plum_totals(ctx_, &t), "Totals");
+  return t;
+}
+
+plum_terms ForceField::FullEnergy() {
+  plum_terms t;
+  check(plum_energy_full(ctx_, &t), "FullEnergy");
+  return t;
+}
+
+double ForceField::TotPairEnergy() { return Totals().lj; }
+double ForceField::TotEwaldEnergy() {
+  const plum_terms t = Totals();
+  return t.real + t.kspace + t.self + t.dipole;
+}
+double ForceField::TotBondEnergy() { return Totals().bond; }
+double ForceField::TotExtEnergy() { return Totals().wall; }
+
+// force_field.cc:1258-1277
+void ForceField::UpdateMolCounts(std::vector<Molecule>& mols) {
+  n_chain_ = n_cion_ = n_aion_ = 0;
+  for (int i = r_.phantom; i < (int)mols.size(); i++) {
+    if (mols[i].Size() > 1) n_chain_++;
+    else if (mols[i].bds[0].Charge() >= 0) n_cion_++;
+    else n_aion_++;
+  }
+  n_chain_ -= r_.grafted;
+}
+
+}  // namespace plum

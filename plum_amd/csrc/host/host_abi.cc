@@ -1,0 +1,152 @@
+// extern "C" surface of the host layer (include/plum_host.h).
+#include <fstream>
+#include <stdexcept>
+#include <string>
+
+#include "plum_host.h"
+#include "simulation.h"
+
+struct plum_sim {
+  plum::Simulation* s = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_host_err;
+
+int fail(int code, const std::string& m) {
+  g_host_err = m;
+  return code;
+}
+
+std::string dir_of(const std::string& path) {
+  const size_t p = path.find_last_of('/');
+  return p == std::string::npos ? std::string(".") : path.substr(0, p);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* plum_host_last_error(void) { return g_host_err.c_str(); }
+
+int plum_runin_parse(const char* path, plum_runin_summary* out) {
+  if (!path || !out) return fail(PLUM_E_ARG, "null argument");
+  std::ifstream in(path);
+  if (!in) return fail(PLUM_E_ARG, std::string("cannot open ") + path);
+  plum::RunIn r;
+  std::string err;
+  if (!plum::read_sim_block(in, r, err)) return fail(PLUM_E_PARSE, err);
+  if (!plum::read_ff_block(in, r, err)) return fail(PLUM_E_PARSE, err);
+  std::vector<plum::Molecule> mols;
+  int n_bead = 0, n_mol = 0, ids = 0;
+  double box[3];
+  const std::string dir = dir_of(path);
+  if (!plum::read_top(dir + "/" + r.top_file, n_bead, n_mol, box, mols, err)) return fail(PLUM_E_PARSE, err);
+  if (!plum::read_crd(dir + "/" + r.crd_file, mols, ids, err)) return fail(PLUM_E_PARSE, err);
+  *out = plum_runin_summary{};
+  out->n_beads = ids;
+  out->n_mols = n_mol;
+  std::vector<std::string> syms;
+  auto add = [&](const std::string& s) {
+    for (auto& t : syms)
+      if (t == s) return;
+    syms.push_back(s);
+  };
+  for (auto& t : r.lj_types) add(t.symbol);
+  for (auto& t : r.wall_types) add(t.symbol);
+  for (auto& m : mols)
+    for (auto& b : m.bds) add(b.Symbol());
+  out->n_types = (int)syms.size();
+  out->npbc = r.npbc;
+  out->use_pair = r.use_pair;
+  out->use_ewald = r.use_ewald;
+  out->use_bond = r.use_bond;
+  out->use_ext = r.use_ext;
+  out->use_gc = r.use_gc;
+  out->dipole_correction = r.dipole_correction;
+  for (int k = 0; k < 3; k++) out->box[k] = box[k];
+  out->beta = r.beta;
+  out->move_size = r.move_size;
+  for (int k = 0; k < 5; k++) out->move_prob[k] = r.move_prob[k];
+  out->steps = r.steps;
+  out->lj_cutoff = r.lj_cutoff;
+  out->lB = r.lB;
+  out->alpha = r.alpha;
+  out->k_bond = r.k_bond;
+  out->r0 = r.r0;
+  out->rigid_bond = r.rigid_bond;
+  out->wall_cut = r.wall_cut;
+  out->phantom = r.phantom;
+  out->grafted = r.grafted;
+  return PLUM_OK;
+}
+
+int plum_sim_create(const char* path, unsigned seed, plum_sim** out) {
+  if (!path || !out) return fail(PLUM_E_ARG, "null argument");
+  *out = nullptr;
+  std::ifstream in(path);
+  if (!in) return fail(PLUM_E_ARG, std::string("cannot open ") + path);
+  plum_sim* s = new plum_sim();
+  try {
+    s->s = new plum::Simulation(in, seed, dir_of(path));
+  } catch (const std::exception& e) {
+    delete s;
+    return fail(PLUM_E_PARSE, e.what());
+  }
+  *out = s;
+  return PLUM_OK;
+}
+
+int plum_sim_run(plum_sim* s, long long steps, long long attempted[5], long long accepted[5]) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  try {
+    s->s->Run(steps);
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_HIP, e.what());
+  }
+  for (int k = 0; k < 5; k++) {
+    if (attempted) attempted[k] = s->s->attempted[k];
+    if (accepted) accepted[k] = s->s->accepted[k];
+  }
+  return PLUM_OK;
+}
+
+int plum_sim_energy(plum_sim* s, plum_terms* running, plum_terms* full) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  try {
+    if (running) *running = s->s->ff.Totals();
+    if (full) *full = s->s->ff.FullEnergy();
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_HIP, e.what());
+  }
+  return PLUM_OK;
+}
+
+int plum_sim_positions(plum_sim* s, double* xyz, int n) {
+  if (!s || !s->s || !xyz) return fail(PLUM_E_ARG, "null argument");
+  int i = 0;
+  for (auto& m : s->s->mols)
+    for (auto& b : m.bds) {
+      if (i >= n) return fail(PLUM_E_ARG, "buffer too small");
+      for (int k = 0; k < 3; k++) xyz[3 * i + k] = b.GetCrd(0, k);
+      i++;
+    }
+  return PLUM_OK;
+}
+
+int plum_sim_observables(plum_sim* s, double* rg2, double* ree2) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  if (rg2) *rg2 = s->s->RadiusOfGyration();
+  if (ree2) *ree2 = s->s->EndToEndDistance();
+  return PLUM_OK;
+}
+
+int plum_sim_destroy(plum_sim* s) {
+  if (!s) return PLUM_OK;
+  delete s->s;
+  delete s;
+  return PLUM_OK;
+}
+
+}  // extern "C"

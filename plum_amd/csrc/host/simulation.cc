@@ -1,0 +1,274 @@
+#include "simulation.h"
+
+#include <cmath>
+#include <sstream>
+#include <stdexcept>
+
+namespace plum {
+
+namespace {
+
+// (double)rand_gen()/rand_gen.max(), the reference's uniform draw
+double u01(std::mt19937& g) { return (double)g() / g.max(); }
+
+// randSphere (misc.cc:95-109), Marsaglia 1972
+void rand_sphere(double v[3], std::mt19937& g) {
+  double s = 2, r1 = 0, r2 = 0;
+  while (s > 1) {
+    r1 = 1 - 2 * u01(g);
+    r2 = 1 - 2 * u01(g);
+    s = r1 * r1 + r2 * r2;
+  }
+  const double h = 2 * std::sqrt(1 - s);
+  v[0] = r1 * h;
+  v[1] = r2 * h;
+  v[2] = 1 - 2 * s;
+}
+
+std::string join(const std::string& dir, const std::string& f) {
+  if (dir.empty() || (!f.empty() && f[0] == '/')) return f;
+  return dir + "/" + f;
+}
+
+}  // namespace
+
+// Molecule::BeadTranslate (molecule.cc:103-133): 3 move_size along a random
+// direction from the current position of the first bead.
+void BeadTranslate(Molecule& m, double move_size, std::mt19937& g) {
+  double v[3];
+  rand_sphere(v, g);
+  double vl = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (vl > 0) vl = 3 * move_size / vl;
+  m.bds[0].SetMoved();
+  for (int i = 0; i < 3; i++) m.bds[0].SetCrd(1, i, m.bds[0].GetCrd(0, i) + vl * v[i]);
+}
+
+// Molecule::COMTranslate (molecule.cc:135-151)
+void COMTranslate(Molecule& m, double move_size, std::mt19937& g) {
+  double v[3];
+  for (int k = 0; k < 3; k++) v[k] = 0.5 * move_size * u01(g);
+  for (int k = 0; k < 3; k++)
+    if (g() % 2 == 0) v[k] = -v[k];
+  for (Bead& b : m.bds) {
+    for (int k = 0; k < 3; k++) b.SetCrd(1, k, b.GetCrd(0, k) + v[k]);
+    b.SetMoved();
+  }
+}
+
+// Molecule::Pivot (molecule.cc:153-237): every bond beyond the pivot gets a
+// random kick of size move_size*U, is renormalised to the (varied) bond
+// length, and the chain beyond it follows.
+void Pivot(Molecule& m, double move_size, std::mt19937& g, double eq, bool vary) {
+  const int len = m.Size();
+  int pivot = (int)std::floor(len * u01(g));
+  if (pivot == len) pivot--;
+  const double msr = move_size * u01(g);
+  auto step = [&](int i, int nb, int dir) {
+    double v[3];
+    rand_sphere(v, g);
+    double x[3], d[3];
+    for (int k = 0; k < 3; k++) {
+      x[k] = m.bds[i].GetCrd(1, k) + msr * v[k];
+      d[k] = x[k] - m.bds[nb].GetCrd(1, k);
+    }
+    double bl = eq;
+    if (vary) bl += (eq / 5.0) * (u01(g) - 0.5);
+    const double norm = bl / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double mv[3];
+    for (int k = 0; k < 3; k++) mv[k] = norm * d[k] + m.bds[nb].GetCrd(1, k) - m.bds[i].GetCrd(1, k);
+    for (int j = i; j >= 0 && j < len; j += dir)
+      for (int k = 0; k < 3; k++) m.bds[j].SetCrd(1, k, m.bds[j].GetCrd(1, k) + mv[k]);
+  };
+  for (int i = pivot + 1; i < len; i++) step(i, i - 1, 1);
+  for (int i = pivot - 1; i >= 0; i--) step(i, i + 1, -1);
+  for (Bead& b : m.bds) b.SetMoved();
+}
+
+// Molecule::Crankshaft (molecule.cc:239-265); the axis-angle rotation is the
+// Rodrigues matrix c I + s [a]x + (1-c) a a^T.
+void Crankshaft(Molecule& m, double move_size, std::mt19937& g) {
+  const int len = m.Size();
+  const int first = (int)std::floor((len - 2) * u01(g));
+  int last = (int)std::floor((len - first) * u01(g)) + first;
+  if (last >= len) last = len - 1;  // the reference indexes past the end when mt() == max
+  double ax[3];
+  for (int i = 0; i < 3; i++) ax[i] = m.bds[last].GetCrd(0, i) - m.bds[first].GetCrd(0, i);
+  const double nn = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2];
+  if (nn > 0) {
+    const double inv = std::sqrt(nn);
+    for (int i = 0; i < 3; i++) ax[i] /= inv;
+  }
+  const double angle = move_size * (u01(g) * 2 * M_PI - M_PI);
+  const double sn = std::sin(angle), c = std::cos(angle);
+  double R[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) R[i][j] = (1 - c) * ax[i] * ax[j] + (i == j ? c : 0);
+  R[0][1] -= sn * ax[2]; R[1][0] += sn * ax[2];
+  R[0][2] += sn * ax[1]; R[2][0] -= sn * ax[1];
+  R[1][2] -= sn * ax[0]; R[2][1] += sn * ax[0];
+  for (int i = first + 1; i < last; i++) {
+    m.bds[i].SetMoved();
+    double v[3];
+    for (int k = 0; k < 3; k++) v[k] = m.bds[i].GetCrd(0, k) - m.bds[first].GetCrd(0, k);
+    for (int k = 0; k < 3; k++)
+      m.bds[i].SetCrd(1, k, R[k][0] * v[0] + R[k][1] * v[1] + R[k][2] * v[2] + m.bds[first].GetCrd(0, k));
+  }
+}
+
+// Molecule::RandomReptation (molecule.cc:267-312)
+void RandomReptation(Molecule& m, std::mt19937& g, double eq, bool vary) {
+  if (eq < 0) throw std::runtime_error("reptation needs a bond length (rigid bond or spring)");
+  double bl = eq;
+  if (vary) bl += (eq / 5.0) * (u01(g) - 0.5);
+  const int len = m.Size();
+  int dir = 1, begin = 0, end = len - 1;
+  if (g() % 2 == 0) {
+    dir = -1;
+    begin = len - 1;
+    end = 0;
+  }
+  for (int i = begin; i != end; i += dir)
+    for (int k = 0; k < 3; k++) m.bds[i].SetCrd(1, k, m.bds[i + dir].GetCrd(0, k));
+  double v[3];
+  rand_sphere(v, g);
+  const double vl = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int k = 0; k < 3; k++) m.bds[end].SetCrd(1, k, m.bds[end].GetCrd(0, k) + bl * v[k] / vl);
+  for (Bead& b : m.bds) b.SetMoved();
+}
+
+Simulation::Simulation(std::istream& in, unsigned seed, const std::string& base_dir) : rng(seed) {
+  std::string err;
+  if (!read_sim_block(in, r, err)) throw std::runtime_error(err);
+  if (r.npbc < 2 || r.npbc > 3) throw std::runtime_error("invalid number of periodic dimensions");
+  if (r.beta <= 0) throw std::runtime_error("beta must be > 0");
+  double tot = 0;
+  for (double p : r.move_prob) tot += p;
+  if (std::fabs(tot - 1.0) > 1e-4) throw std::runtime_error("MC move probabilities do not sum up to 1.0");
+  int n_bead = 0, n_mol = 0, id_counter = 0;
+  if (!read_top(join(base_dir, r.top_file), n_bead, n_mol, box, mols, err)) throw std::runtime_error(err);
+  if (!read_crd(join(base_dir, r.crd_file), mols, id_counter, err)) throw std::runtime_error(err);
+  RunIn ffr = r;
+  if (!read_ff_block(in, ffr, err)) throw std::runtime_error(err);
+  r = ffr;
+  if (r.move_prob[4] > 0 && !r.use_rigid_bond)
+    throw std::runtime_error("reptation requires the rigid-bond setting (simulation.cc:182-186)");
+  ff.Initialize(r, box, mols);
+  update_counts();
+}
+
+// Simulation::UpdateMolCounts (simulation.cc:1066-1094)
+void Simulation::update_counts() {
+  n_chain = n_cion = n_aion = n_nion = 0;
+  for (int i = r.phantom; i < (int)mols.size(); i++) {
+    if (mols[i].Size() > 1) n_chain++;
+    else if (mols[i].bds[0].Charge() > 0) n_cion++;
+    else if (mols[i].bds[0].Charge() < 0) n_aion++;
+    else n_nion++;
+  }
+  n_chain -= r.grafted;
+}
+
+// Simulation::Run (simulation.cc:216-239) without GC and without output.
+void Simulation::Run(long long steps) {
+  for (long long s = 0; s < steps; s++) {
+    (void)rng();  // the per-step draw of the GC coin (simulation.cc:221)
+    TranslationalMove();
+    steps_done++;
+  }
+}
+
+// Simulation::TranslationalMove (simulation.cc:241-355)
+void Simulation::TranslationalMove() {
+  const int n_mol = (int)mols.size();
+  if (n_mol - r.phantom <= 0) return;
+  const int n_ions = n_cion + n_aion + n_nion;
+  int which_chain = (int)std::floor(u01(rng) * (r.grafted + n_chain));
+  int which_ion = (int)std::floor(u01(rng) * n_ions);
+  if (which_chain == r.grafted + n_chain) which_chain--;
+  if (which_ion == n_ions) which_ion--;
+  int chain_id = -1, ion_id = -1, cc = -1, ic = -1;
+  for (int i = r.phantom; i < n_mol; i++) {
+    if (mols[i].Size() > 1) cc++;
+    else ic++;
+    if (cc == which_chain) { chain_id = i; which_chain = -2; }
+    if (ic == which_ion) { ion_id = i; which_ion = -2; }
+  }
+  int mt = 0;
+  const double rn = u01(rng);
+  double cur = r.move_prob[0];
+  while (cur < rn) {
+    mt++;
+    cur += r.move_prob[mt];
+  }
+  if (mt >= 5) mt = 4;
+  int mol_id = -1;
+  if (mt == 0 && n_ions > 0) {
+    mol_id = ion_id;
+    BeadTranslate(mols[mol_id], r.move_size, rng);
+  } else if (r.grafted + n_chain > 0) {
+    mol_id = chain_id;
+    double bond_len = 0;
+    bool vary = false;
+    if (ff.UseBondRigid()) bond_len = ff.RigidBondLen();
+    if (ff.UseBondPot()) {
+      bond_len = ff.EqBondLen();
+      vary = true;
+    }
+    switch (mt) {
+      case 1: COMTranslate(mols[mol_id], r.move_size, rng); break;
+      case 2: Pivot(mols[mol_id], r.move_size, rng, bond_len, vary); break;
+      case 3: Crankshaft(mols[mol_id], r.move_size, rng); break;
+      case 4: RandomReptation(mols[mol_id], rng, bond_len, vary); break;
+    }
+  }
+  if ((mt == 0 && n_ions > 0) || (mt > 0 && r.grafted + n_chain > 0)) {
+    attempted[mt]++;
+    const double dE = ff.EnergyDifference(mols, mol_id);
+    bool accept = false;
+    if (dE < 1e8) accept = u01(rng) < std::exp(-r.beta * dE);
+    ff.FinalizeEnergies(mols, accept, mol_id);
+    for (Bead& b : mols[mol_id].bds) {
+      if (accept) b.UpdateCurrentPos();
+      else b.UpdateTrialPos();
+      b.UnsetMoved();
+    }
+    if (accept) accepted[mt]++;
+  }
+}
+
+// Simulation::RadiusOfGyration (simulation.cc:406-434)
+double Simulation::RadiusOfGyration() const {
+  double rg = 0;
+  for (const Molecule& m : mols) {
+    if (m.Size() <= 1) continue;
+    double rm = 0;
+    for (int j = 0; j < m.Size() - 1; j++)
+      for (int k = j + 1; k < m.Size(); k++) {
+        double s = 0;
+        for (int e = 0; e < 3; e++) {
+          const double d = m.bds[j].GetCrd(1, e) - m.bds[k].GetCrd(1, e);
+          s += d * d;
+        }
+        rm += s;
+      }
+    rg += rm / (m.Size() * m.Size());
+  }
+  return (r.grafted + n_chain) > 0 ? rg / (r.grafted + n_chain) : 0;
+}
+
+// Simulation::EndToEndDistance (simulation.cc:483-501), mean squared
+double Simulation::EndToEndDistance() const {
+  double e = 0;
+  for (const Molecule& m : mols) {
+    if (m.Size() <= 1) continue;
+    double s = 0;
+    for (int k = 0; k < 3; k++) {
+      const double d = m.bds[0].GetCrd(1, k) - m.bds[m.Size() - 1].GetCrd(1, k);
+      s += d * d;
+    }
+    e += s;
+  }
+  return (r.grafted + n_chain) > 0 ? e / (r.grafted + n_chain) : 0;
+}
+
+}  // namespace plum

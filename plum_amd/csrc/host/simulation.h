@@ -1,0 +1,49 @@
+// Host Monte Carlo driver with the reference's move semantics
+// (Simulation::Run / TranslationalMove, simulation.cc:216-355, and the move
+// generators of molecule.cc:103-312) on std::mt19937, calling the GPU
+// engine through the drop-in ForceField.  Unlike the reference, the seed is
+// a parameter (the reference seeds from the clock, simulation.cc:133-136).
+#pragma once
+#include <istream>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "force_field.h"
+#include "model.h"
+#include "runin.h"
+
+namespace plum {
+
+// Move generators: write trial coordinates and moved flags of one molecule.
+void BeadTranslate(Molecule& m, double move_size, std::mt19937& rng);
+void COMTranslate(Molecule& m, double move_size, std::mt19937& rng);
+void Pivot(Molecule& m, double move_size, std::mt19937& rng, double eq_bond_len, bool vary_bond);
+void Crankshaft(Molecule& m, double move_size, std::mt19937& rng);
+void RandomReptation(Molecule& m, std::mt19937& rng, double eq_bond_len, bool vary_bond);
+
+class Simulation {
+ public:
+  // Reads the whole run.in from `in`; crd/top paths are taken relative to
+  // base_dir when they are not absolute.  Throws on malformed input.
+  Simulation(std::istream& in, unsigned seed, const std::string& base_dir = "");
+  void Run(long long steps);
+  void TranslationalMove();
+  double RadiusOfGyration() const;
+  double EndToEndDistance() const;
+
+  RunIn r;
+  double box[3] = {0, 0, 0};
+  std::vector<Molecule> mols;
+  ForceField ff;
+  std::mt19937 rng;
+  long long attempted[5] = {0, 0, 0, 0, 0};
+  long long accepted[5] = {0, 0, 0, 0, 0};
+  long long steps_done = 0;
+
+ private:
+  void update_counts();
+  int n_chain = 0, n_cion = 0, n_aion = 0, n_nion = 0;
+};
+
+}  // namespace plum

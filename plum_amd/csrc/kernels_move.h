@@ -6,7 +6,7 @@
 
 namespace plum {
 
-constexpr int kKGroups = 4;             // bead groups per k-space block
+constexpr int kKGroups = 16;            // bead groups per k-space block
 constexpr int kKPerBlock = kBlock / kKGroups;
 constexpr int kMaxChainDev = 2048;      // longest molecule the device loop moves
 
@@ -50,20 +50,26 @@ __global__ void k_tables(Geo g, Dev d) {
 
 // ------------------------------------------------------------ k space
 // dS(k) = sum_m q_m (e^{ik.r'_m} - e^{ik.r_m}); dE_k = c_k (|S+dS|^2 - |S|^2).
-// Block = 64 k vectors x 4 groups of moved beads; the groups are reduced in
+// Block = 16 k vectors x 16 groups of moved beads; the groups are reduced in
 // LDS.  A still-pending S(k) += dS(k) of the previous accepted move is applied
 // first by the thread that owns k (device loop).  FP64 in both precision
 // modes: it is a small part of the per-move work and keeps S(k) exact.
-__global__ __launch_bounds__(kBlock) void k_kspace_delta(Geo g, Dev d) {
-  __shared__ double2 s_ds[kKGroups][kKPerBlock];
-  __shared__ double sh[kBlock / kWave];
+struct KSmem {
+  double2 s_ds[kKGroups][kKPerBlock];
+  double sh[kBlock / kWave];
+};
+
+__device__ void kspace_block(const Geo& g, const Dev& d, int blk, KSmem& sm) {
+  double2 (&s_ds)[kKGroups][kKPerBlock] = sm.s_ds;
+  double* sh = sm.sh;
   const int kk = threadIdx.x % kKPerBlock, grp = threadIdx.x / kKPerBlock;
-  const int k = blockIdx.x * kKPerBlock + kk;
+  const int k = blk * kKPerBlock + kk;
   const int nm = d.rec->n_moved;
   double2 dS = make_double2(0, 0);
   if (k < g.n_kh) {
     const int4 l = d.kl[k];
     const int ox = l.x, oy = g.nkx + 1 + l.y + g.nky, oz = g.nkx + 1 + 2 * g.nky + 1 + l.z + g.nkz;
+#pragma unroll 2
     for (int m = grp; m < nm; m += kKGroups) {
       const double qm = d.mv_old[m].w;
       const double2* to = d.tab + (size_t)(2 * m) * g.tab_w;
@@ -92,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_kspace_delta(Geo g, Dev d) {
     d.dS[k] = t;
   }
   const double r = block_sum<kBlock>(e, sh);
-  if (threadIdx.x == 0) d.part_k[blockIdx.x] = r;
+  if (threadIdx.x == 0) d.part_k[blk] = r;
 }
 
 // Applies a pending S(k) += dS(k) (end of a device run).
@@ -177,28 +183,40 @@ __device__ void mm_block(const Geo& g, const Dev& d, int blk, int nblk, double* 
 
 // Real-space (LJ + Ewald erfc) energy of one moved bead at its old (sign -1)
 // or new (sign +1) position against every unmoved bead image within the
-// search radius.  Blocks [0, nreal) = (moved bead, old/new); blocks
-// [nreal, nreal + nmm) = moved-moved pairs.  The visited cells are listed in
-// LDS with a prefix over their counts, so the candidate slots are spread
-// evenly over the block.
+// search radius.  Blocks [0, nreal) = (moved bead, old/new, part): each query
+// is split over `nsplit` blocks that take interleaved candidate slots;
+// blocks [nreal, nreal + nmm) = moved-moved pairs.  The visited cells are
+// listed in LDS with a prefix over their counts, so the candidate slots are
+// spread evenly; the moved beads are skipped by index range when they are
+// contiguous (every reference move generator moves a contiguous range).
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_real_delta(Geo g, Dev d, int nreal, int nmm) {
-  __shared__ int s_cell[kMaxQueryCells];
-  __shared__ int s_img[kMaxQueryCells];
-  __shared__ int s_end[kMaxQueryCells];
-  __shared__ float s_rel[P ? kMaxQueryCells : 1][3];
-  __shared__ double sh[kBlock / kWave];
-  const int b = blockIdx.x;
+struct RSmem {
+  int s_cell[kMaxQueryCells];
+  int s_img[kMaxQueryCells];
+  int s_end[kMaxQueryCells];
+  float s_rel[P ? kMaxQueryCells : 1][3];
+  double sh[kBlock / kWave];
+};
+
+template <int P>
+__device__ void real_block(const Geo& g, const Dev& d, int b, int nreal, int nmm, int nsplit, RSmem<P>& sm) {
+  int* s_cell = sm.s_cell;
+  int* s_img = sm.s_img;
+  int* s_end = sm.s_end;
+  float (&s_rel)[P ? kMaxQueryCells : 1][3] = sm.s_rel;
+  double* sh = sm.sh;
   if (b >= nreal) {
     mm_block(g, d, b - nreal, nmm, sh);
     return;
   }
-  const int m = b >> 1, on = b & 1;
+  const int q2 = b / nsplit, part = b - q2 * nsplit;
+  const int m = q2 >> 1, on = q2 & 1;
   const int nm = d.rec->n_moved;
   if (m >= nm) {
     if (threadIdx.x == 0) d.part_r[b] = make_double2(0, 0);
     return;
   }
+  const int contig = d.rec->contig, mlo = d.rec->lo, mhi = d.rec->hi;
   const double4 p = on ? d.mv_new[m] : d.mv_old[m];
   const int tm = d.mv_type[m];
   const double pw[3] = {wrap_coord(p.x, g.L[0], g.inv_L[0]), wrap_coord(p.y, g.L[1], g.inv_L[1]),
@@ -240,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_real_delta(Geo g, Dev d, int nreal, 
     __syncthreads();
     lds_scan(s_end, nloc);
     const int total = s_end[nloc - 1];
-    for (int it = threadIdx.x; it < total; it += kBlock) {
+    for (int it = part * kBlock + threadIdx.x; it < total; it += nsplit * kBlock) {
       int lo_i = 0, hi_i = nloc - 1;
       while (lo_i < hi_i) {
         const int mid = (lo_i + hi_i) >> 1;
@@ -250,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_real_delta(Geo g, Dev d, int nreal, 
       const int slot = it - (t ? s_end[t - 1] : 0);
       const size_t si = (size_t)s_cell[t] * g.cap + slot;
       const int j = d.cbead[si];
-      if (j < 0 || d.moved[j]) continue;
+      if (j < 0 || (contig ? (j >= mlo && j <= mhi) : d.moved[j])) continue;
       const int iz = ((s_img[t] >> 16) & 0xff) - 128;
       const bool lj_ok = g.use_pair && (g.npbc == 3 || iz == 0);
       if constexpr (P == 0) {
@@ -319,16 +337,29 @@ __global__ __launch_bounds__(kBlock) void k_real_delta(Geo g, Dev d, int nreal, 
   if (threadIdx.x == 0) d.part_r[b] = make_double2(sgn * a, sgn * c);
 }
 
+// All dE work of one trial move in one launch: blocks [0, nkb) k space,
+// then nreal real-space query blocks, then nmm moved-moved pair blocks.
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_compute(Geo g, Dev d, int nkb, int nreal, int nmm, int nsplit) {
+  __shared__ union U {
+    KSmem k;
+    RSmem<P> r;
+  } sm;
+  const int b = blockIdx.x;
+  if (b < nkb) kspace_block(g, d, b, sm.k);
+  else real_block<P>(g, d, b - nkb, nreal, nmm, nsplit, sm.r);
+}
+
 // ------------------------------------------------------------ epilogue
-// Spring energy of molecule `mol` in the reference's summation order
-// (potential_spring.cc:22-47: consecutive beads, no PBC).  Per-bond terms are
-// evaluated in parallel into LDS and summed in bond order by thread 0, so the
-// same bits come out whichever kernel evaluates the molecule.
+// Spring energy of molecule `mol` (potential_spring.cc:22-47: consecutive
+// beads, no PBC).  Per-bond terms are summed by a fixed-order block tree, so
+// the same bits come out whichever kernel evaluates the molecule.
 __device__ double molecule_spring(const Geo& g, const Dev& d, int mol, bool trial, double* sbuf) {
   const int a = d.mol_start[mol], b = d.mol_start[mol + 1];
   double acc = 0;
   for (int base = a; base < b - 1; base += kBlock) {
     const int i = base + threadIdx.x;
+    double e = 0;
     if (i < b - 1) {
       double p0[3] = {d.x[i], d.y[i], d.z[i]};
       double p1[3] = {d.x[i + 1], d.y[i + 1], d.z[i + 1]};
@@ -339,14 +370,9 @@ __device__ double molecule_spring(const Geo& g, const Dev& d, int mol, bool tria
       }
       const double dx = p0[0] - p1[0], dy = p0[1] - p1[1], dz = p0[2] - p1[2];
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      sbuf[threadIdx.x] = 0.5 * g.k_bond * (r - g.r0) * (r - g.r0);
+      e = 0.5 * g.k_bond * (r - g.r0) * (r - g.r0);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int cnt = min(kBlock, b - 1 - base);
-      for (int k = 0; k < cnt; k++) acc += sbuf[k];
-    }
-    __syncthreads();
+    acc += block_sum<kBlock>(e, sbuf);
   }
   return acc;  // valid in thread 0
 }
@@ -424,12 +450,7 @@ __device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int
   // dipole moment change, in moved-list order
   if (threadIdx.x == 0) {
     if (acc) {
-      double dmz = 0;
-      for (int k = 0; k < nm; k++) {
-        const double4 pn = d.mv_new[k], po = d.mv_old[k];
-        dmz += pn.w * (pn.z - po.z);
-      }
-      C->Mz += dmz;
+      C->Mz += C->dMz;
       if (g.use_bond && d.mol_start[mol + 1] - d.mol_start[mol] > 1) d.bond_cur[mol] = C->bond_trial;
       if (g.dipole) C->cur_dipl = C->trial_dipl;
       for (int k = 0; k < 8; k++) C->running[k] += C->terms[k];
@@ -460,12 +481,21 @@ __device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int
 // (force_field.cc:407-434).  mode 0: store dE only (host path); mode 1: also
 // take the Metropolis decision (simulation.cc:327-332) from the Philox stream
 // and commit the move (device loop).
-__global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int nrb, int nmmb,
-                                                     int mode, McDev mc) {
-  __shared__ double sh[kBlock / kWave];
-  __shared__ double sbuf[kBlock];
-  __shared__ int s_forbid, s_acc;
-  __shared__ int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
+struct EpSmem {
+  double sh[kBlock / kWave];
+  double sbuf[kBlock];
+  int s_forbid, s_acc;
+  int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
+};
+
+__device__ void epilogue_block(const Geo& g, const Dev& d, int nkb, int nrb, int nmmb, int mode,
+                               const McDev& mc, EpSmem& sm) {
+  double* sh = sm.sh;
+  double* sbuf = sm.sbuf;
+  int& s_forbid = sm.s_forbid;
+  int& s_acc = sm.s_acc;
+  int* s_nc = sm.s_nc;
+  int* s_flag = sm.s_flag;
   Ctl* C = d.ctl;
   const int nm = d.rec->n_moved;
   const int mol = d.rec->mol;
@@ -533,6 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int 
       dE += t[T_BOND];
     }
     C->bond_trial = ebt;
+    C->dMz = dmzt;
     for (int k = 0; k < 8; k++) C->terms[k] = t[k];
     C->total = dE;
     C->pending = 1;
@@ -553,6 +584,12 @@ __global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int 
     commit_block(g, d, acc, s_nc, s_flag);
     if (threadIdx.x == 0) C->sk_pending = acc && g.use_ewald && g.n_kh > 0;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int nrb, int nmmb, int mode,
+                                                     McDev mc) {
+  __shared__ EpSmem sm;
+  epilogue_block(g, d, nkb, nrb, nmmb, mode, mc, sm);
 }
 
 // Host path: commit / roll back after the caller's decision.
@@ -616,11 +653,22 @@ __device__ void sphere(const McDev& mc, unsigned long long ctr, uint32_t stream,
 // One proposal of Simulation::TranslationalMove (simulation.cc:241-312) with
 // the move generators of molecule.cc:103-312, drawing from Philox4x32-10
 // (counter = move number, streams: 0 selection, 1 move parameters, 2+i bead i).
-__global__ __launch_bounds__(kBlock) void k_propose(Geo g, Dev d, McDev mc) {
-  __shared__ double s_b[kMaxChainDev][3];
-  __shared__ int s_mol, s_type, s_valid, s_a, s_len;
-  __shared__ double s_par[4];
-  __shared__ int s_ipar[3];
+struct PrSmem {
+  double s_b[kMaxChainDev][3];
+  int s_mol, s_type, s_valid, s_a, s_len;
+  double s_par[4];
+  int s_ipar[3];
+};
+
+__device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, PrSmem& sm) {
+  double (&s_b)[kMaxChainDev][3] = sm.s_b;
+  int& s_mol = sm.s_mol;
+  int& s_type = sm.s_type;
+  int& s_valid = sm.s_valid;
+  int& s_a = sm.s_a;
+  int& s_len = sm.s_len;
+  double* s_par = sm.s_par;
+  int* s_ipar = sm.s_ipar;
   Ctl* C = d.ctl;
   const unsigned long long ctr = C->move_counter;
   if (threadIdx.x == 0) {
@@ -664,6 +712,9 @@ __global__ __launch_bounds__(kBlock) void k_propose(Geo g, Dev d, McDev mc) {
       d.rec->mol = mol;
       d.rec->move_type = mt;
       d.rec->valid = 0;
+      d.rec->contig = 1;
+      d.rec->lo = 0;
+      d.rec->hi = -1;
     }
     return;
   }
@@ -794,6 +845,62 @@ __global__ __launch_bounds__(kBlock) void k_propose(Geo g, Dev d, McDev mc) {
     d.rec->mol = mol;
     d.rec->move_type = mt;
     d.rec->valid = 1;
+    // every generator moves a contiguous index range
+    const int lo = (mt == 3) ? a + (int)floor((len - 2) * s_par[0]) + 1 : a;
+    d.rec->contig = 1;
+    d.rec->lo = lo;
+    d.rec->hi = lo + nmv - 1;
+  }
+}
+
+}  // namespace plum
+
+namespace plum {
+
+// Phase tables of the pending move by recurrence, inside one block:
+// e^{i l theta} = e^{i (l-1) theta} e^{i theta}, l = 0..n_d, negative l by
+// conjugation; theta_d = (2 kPi / L_d) x_d on the wrapped coordinate.
+__device__ void tables_block(const Geo& g, const Dev& d) {
+  const int nm = d.rec->n_moved;
+  const int nseq = nm * 2 * 3;
+  for (int sidx = threadIdx.x; sidx < nseq; sidx += kBlock) {
+    const int row = sidx / 3, dim = sidx - row * 3;
+    const int m = row >> 1, on = row & 1;
+    const double4 p = on ? d.mv_new[m] : d.mv_old[m];
+    const double xv = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
+    const double xw = wrap_coord(xv, g.L[dim], g.inv_L[dim]);
+    const double k1 = (double)2 * kPi / g.L[dim];
+    double sn, cs;
+    sincos(k1 * xw, &sn, &cs);
+    const double2 base = make_double2(cs, sn);
+    double2* t = d.tab + (size_t)row * g.tab_w;
+    const int n = dim == 0 ? g.nkx : (dim == 1 ? g.nky : g.nkz);
+    const int off = dim == 0 ? 0 : (dim == 1 ? g.nkx + 1 + g.nky : g.nkx + 1 + 2 * g.nky + 1 + g.nkz);
+    double2 e = make_double2(1.0, 0.0);
+    t[off] = e;
+    for (int l = 1; l <= n; l++) {
+      e = cmul(e, base);
+      t[off + l] = e;
+      if (dim != 0) t[off - l] = conj2(e);
+    }
+  }
+}
+
+// The serial part of the device loop, one block: dE epilogue, Metropolis
+// decision and commit of the pending move (do_epi), then the proposal of
+// the next move and its phase tables (do_prop).
+__global__ __launch_bounds__(kBlock) void k_serial(Geo g, Dev d, McDev mc, int nkb, int nrb, int nmmb,
+                                                   int do_epi, int do_prop) {
+  __shared__ union U {
+    EpSmem e;
+    PrSmem p;
+  } sm;
+  if (do_epi) epilogue_block(g, d, nkb, nrb, nmmb, 1, mc, sm.e);
+  __syncthreads();
+  if (do_prop) {
+    propose_block(g, d, mc, sm.p);
+    __syncthreads();
+    if (g.use_ewald && g.n_kh > 0) tables_block(g, d);
   }
 }
 

@@ -1,0 +1,104 @@
+"""ctypes binding of include/plum_host.h (libplum_host.so): run.in reader and
+the host Monte Carlo driver over the GPU engine."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import engine as E
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libplum_host.so")
+HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim_energy",
+                "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
+                "plum_host_last_error")
+
+
+class RunInSummary(C.Structure):
+    _fields_ = [
+        ("n_beads", C.c_int), ("n_mols", C.c_int), ("n_types", C.c_int),
+        ("npbc", C.c_int), ("use_pair", C.c_int), ("use_ewald", C.c_int), ("use_bond", C.c_int),
+        ("use_ext", C.c_int), ("use_gc", C.c_int), ("dipole_correction", C.c_int),
+        ("box", C.c_double * 3), ("beta", C.c_double), ("move_size", C.c_double),
+        ("move_prob", C.c_double * 5), ("steps", C.c_longlong),
+        ("lj_cutoff", C.c_double), ("lB", C.c_double), ("alpha", C.c_double),
+        ("k_bond", C.c_double), ("r0", C.c_double), ("rigid_bond", C.c_double),
+        ("wall_cut", C.c_double), ("phantom", C.c_int), ("grafted", C.c_int),
+    ]
+
+
+class HostError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HostError("libplum_host.so is not built; run __graft_entry__.build()")
+        E.lib()  # the engine library first (libplum_host links against it)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.plum_runin_parse.argtypes = [C.c_char_p, C.POINTER(RunInSummary)]
+        L.plum_sim_create.argtypes = [C.c_char_p, C.c_uint, C.POINTER(vp)]
+        L.plum_sim_run.argtypes = [vp, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.plum_sim_energy.argtypes = [vp, C.POINTER(E.Terms), C.POINTER(E.Terms)]
+        L.plum_sim_positions.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
+        L.plum_sim_observables.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.plum_sim_destroy.argtypes = [vp]
+        L.plum_host_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HostError("plum host error %d: %s" % (rc, lib().plum_host_last_error().decode()))
+
+
+def parse_runin(path):
+    s = RunInSummary()
+    _check(lib().plum_runin_parse(path.encode(), C.byref(s)))
+    return s
+
+
+class HostSimulation:
+    def __init__(self, run_in_path, seed):
+        h = C.c_void_p()
+        _check(lib().plum_sim_create(run_in_path.encode(), seed, C.byref(h)))
+        self.h = h
+
+    def run(self, steps):
+        att = (C.c_longlong * 5)()
+        acc = (C.c_longlong * 5)()
+        _check(lib().plum_sim_run(self.h, steps, att, acc))
+        return list(att), list(acc)
+
+    def energies(self):
+        r, f = E.Terms(), E.Terms()
+        _check(lib().plum_sim_energy(self.h, C.byref(r), C.byref(f)))
+        return r.as_dict(), f.as_dict()
+
+    def positions(self, n):
+        out = np.zeros(3 * n)
+        _check(lib().plum_sim_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return out.reshape(-1, 3)
+
+    def observables(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().plum_sim_observables(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().plum_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -180,10 +180,10 @@ def test_large_system_properties():
     eng.finalize(True)
     e1 = eng.energy_full()
     for k in ("lj", "real", "kspace", "bond"):
-        assert abs(t[k] - (e1[k] - e0[k])) <= 1e-10 * (abs(e0[k]) + abs(e1[k])), k
+        assert abs(t[k] - (e1[k] - e0[k])) <= 1e-10 * max(abs(e0[k]) + abs(e1[k]), FLOOR), k
     tot_b, tb = eng.delta(mol, idx, sd["xyz"][a:b])
     for k in ("lj", "real", "kspace", "bond"):
-        assert abs(tb[k] + t[k]) <= 1e-10 * max(abs(t[k]), 1e-3 * abs(e0[k])), k
+        assert abs(tb[k] + t[k]) <= 1e-10 * _den(t[k], e0[k]), k
     eng.finalize(True)
     drift = eng.resync()
     for k in ("lj", "real", "kspace", "bond"):

@@ -1,0 +1,104 @@
+"""Host layer: the run.in / crd / top readers on the reference's own example
+decks (CPU), the exported ABI, and the host Monte Carlo driver replaying the
+reference move sequence against the oracle (GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from plum_amd import host as H
+from plum_amd import systems
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EX = os.path.join(HERE, "golden", "examples")
+ROOT = os.path.dirname(HERE)
+
+
+def test_host_library_exports_every_declared_symbol():
+    text = open(os.path.join(ROOT, "include", "plum_host.h")).read()
+    decl = sorted(set(re.findall(r"\b(plum_[a-z_]+)\s*\(", text)))
+    assert decl == sorted(H.HOST_SYMBOLS)
+    lib = H.lib()
+    for s in decl:
+        assert hasattr(lib, s), s
+
+
+def test_parse_reference_example_decks():
+    s = H.parse_runin(os.path.join(EX, "bulk_nvt", "run.in"))
+    assert (s.n_beads, s.n_mols, s.npbc) == (512, 272, 3)
+    assert s.box[0] == pytest.approx(58.4803547642572994)
+    assert (s.use_pair, s.use_ewald, s.use_bond, s.use_ext, s.use_gc) == (1, 1, 0, 0, 0)
+    assert s.lj_cutoff == -1 and s.lB == 2.5 and s.alpha == 0.001 and s.rigid_bond == 2.5
+    assert list(s.move_prob) == [0.95, 0.0, 0.05, 0.0, 0.0]
+    assert s.n_types == 5  # P L R C Z
+    c = H.parse_runin(os.path.join(EX, "confined_nvt", "run.in"))
+    assert c.npbc == 2 and c.use_ext == 1 and c.dipole_correction == 1
+    assert c.box[2] == 60.0
+    g = H.parse_runin(os.path.join(EX, "bulk_muvt", "run.in"))
+    assert g.use_gc == 1 and g.n_beads == 0
+
+
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_parse_errors_are_status_codes(tmp_path):
+    # truncated deck: the reference would read garbage / exit(1)
+    src = open(os.path.join(EX, "bulk_nvt", "run.in")).read().splitlines()
+    for f in ("input_top.dat", "input_crd.dat"):
+        (tmp_path / f).write_text(open(os.path.join(EX, "bulk_nvt", f)).read())
+    p = _write(tmp_path, "run.in", "\n".join(src[:30]) + "\n")
+    with pytest.raises(H.HostError) as e:
+        H.parse_runin(p)
+    assert "run.in" in str(e.value)
+    # molecule id out of range in the coordinate file (simulation.cc:594-598)
+    crd = open(os.path.join(EX, "bulk_nvt", "input_crd.dat")).read().splitlines()
+    crd[2] = "272 " + crd[2].split(" ", 1)[1]
+    (tmp_path / "input_crd.dat").write_text("\n".join(crd) + "\n")
+    p = _write(tmp_path, "run.in", "\n".join(src) + "\n")
+    with pytest.raises(H.HostError) as e:
+        H.parse_runin(p)
+    assert "out of bounds" in str(e.value)
+
+
+def test_generated_decks_round_trip(tmp_path):
+    sd = systems.config("slab_small")
+    path = systems.write_reference_inputs(sd, str(tmp_path))
+    s = H.parse_runin(path)
+    assert s.n_beads == len(sd["q"]) and s.n_mols == len(sd["mol_start"]) - 1
+    assert s.npbc == 2 and s.use_ext == 1 and s.dipole_correction == 1
+    assert s.alpha == pytest.approx(sd["alpha"], rel=1e-15)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,steps", [("small", 120), ("slab_small", 120)])
+def test_host_driver_replays_reference_sequence(tmp_path, name, steps):
+    """Same run.in, same seed: the host driver (GPU energies) and the oracle
+    (CPU reference algorithm) take the same accept/reject decisions and end
+    in the same configuration."""
+    from oracle import oracle as O
+    sd = systems.config(name)
+    path = systems.write_reference_inputs(sd, str(tmp_path))
+    sim = H.HostSimulation(path, 77)
+    att, acc = sim.run(steps)
+    orc = O.Oracle(sd)
+    rng = O.MT19937(77)
+    oatt, oacc = [0] * 5, [0] * 5
+    for _ in range(steps):
+        inf = O.mc_step(orc, rng, sd["move_prob"], sd["move_size"], sd.get("phantom", 0),
+                        sd.get("grafted", 0), sd["r0"], True)
+        if inf.attempted:
+            oatt[inf.move_type] += 1
+            oacc[inf.move_type] += inf.accepted
+    print(name, "host", att, acc, "oracle", oatt, oacc, flush=True)
+    assert att == oatt and acc == oacc
+    pos = sim.positions(len(sd["q"]))
+    assert np.max(np.abs(pos - orc.positions())) <= 1e-9
+    run, full = sim.energies()
+    ref = orc.energy_full()
+    for k in ref:
+        assert abs(full[k] - ref[k]) <= 1e-10 * max(abs(ref[k]), 1e-2), k
