@@ -174,6 +174,45 @@ int plum_set_profiling(plum_ctx* ctx, int on);
 int plum_kernel_times(plum_ctx* ctx, int cap, const char** names,
                       double* avg_us, long long* count);
 
+/* ---- batched replicas -------------------------------------------------- */
+
+/* R independent systems (contexts created with plum_create, same precision)
+   advanced together: every k_compute_batch launch carries one trial move of
+   each replica, and `sweeps_per_graph` sweeps are captured in a HIP graph and
+   replayed.  No reference counterpart: the reference runs one system per
+   process (simulation.cc:216-355 per replica).  The contexts must outlive the
+   batch and must not be used on their own while a batch call runs. */
+typedef struct plum_batch plum_batch;
+
+typedef struct {
+  int blocks_per_cu;     /* k_compute_batch work-groups per compute unit */
+  int split_div;         /* candidate slots per real-space query block */
+  int sweeps_per_graph;  /* sweeps captured in one graph (also the cell
+                            compaction period) */
+  double sweep_us;       /* measured time of one sweep (plum_batch_tune) */
+} plum_batch_tuning;
+
+typedef struct {
+  long long moves;       /* trial moves done, all replicas */
+  long long sweeps;      /* moves per replica requested */
+  long long attempted[5], accepted[5];
+  long long overflow_replicas;  /* replicas finished alone after a cell overflow */
+  double seconds;        /* wall time of the batched part (host clock, synchronized) */
+} plum_batch_stats;
+
+int plum_batch_create(plum_ctx* const* replicas, int n_replicas, plum_batch** out);
+int plum_batch_destroy(plum_batch* batch);
+/* mc[r] are replica r's move-mix parameters and seed; per (may be NULL)
+   receives per-replica stats.  use_graph = 0 launches the same kernels
+   directly (no graph). */
+int plum_batch_mc_run(plum_batch* batch, const plum_mc_params* mc, long long moves_per_replica, int use_graph,
+                      plum_batch_stats* total, plum_mc_stats* per);
+/* Autotuner: times `moves_per_trial` sweeps for each candidate launch shape
+   and keeps the fastest (the trial sweeps are ordinary moves). */
+int plum_batch_tune(plum_batch* batch, const plum_mc_params* mc, int moves_per_trial, plum_batch_tuning* best);
+int plum_batch_get_tuning(const plum_batch* batch, plum_batch_tuning* out);
+int plum_batch_set_tuning(plum_batch* batch, const plum_batch_tuning* tuning);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,25 @@ int plum_sim_observables(plum_sim* sim, double* rg2, double* ree2);
 int plum_sim_destroy(plum_sim* sim);
 const char* plum_host_last_error(void);
 
+/* Seeded synthetic system (chains as random walks of step r0 along Marsaglia
+   directions, then counterions and salt pairs uniform; a bead landing within
+   min_sep of another is redrawn).  No reference counterpart: the reference
+   reads configurations from crd files (simulation.cc:582-646).  Types: 0 chain
+   bead (2 for the grafted first bead when graft_z >= 0), 1 ion. */
+typedef struct {
+  int n_chains, chain_len, n_counterions, n_salt_pairs;
+  double bead_q, counterion_q;
+  double box[3];
+  int npbc;
+  double min_sep, r0;
+  double graft_z;        /* < 0: free chains */
+  double z_lo, z_hi;     /* growth / ion z range; z_hi <= z_lo: whole box */
+  unsigned long long seed;
+} plum_synth_spec;
+
+int plum_synth_count(const plum_synth_spec* spec, int* n_beads, int* n_mols);
+int plum_synth_system(const plum_synth_spec* spec, double* xyz, double* q, int* type, int* mol_start);
+
 #ifdef __cplusplus
 }
 #endif

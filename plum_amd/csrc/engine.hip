@@ -13,6 +13,7 @@
 //                         per-bead phase tables e^{i l theta_d}, |l| <= n_d
 //   control block:        dipole moment, dipole lag state, running totals,
 //                         the last dE per term, status flags, move counters
+#include "kernels_batch.h"
 #include "kernels_full.h"
 #include "kernels_move.h"
 #include "plum_gpu.h"
@@ -40,6 +41,7 @@ struct plum_ctx {
   int chunk = 512;
   int nrealb = 1;        // blocks of the full real-space kernel
   int nsplit = 1;        // blocks per real-space query
+  double cand = 0;       // expected candidate pair-images per query
   double* epart_k = nullptr;
   std::vector<int> mol_start;
   std::vector<int> type_h;
@@ -250,9 +252,11 @@ int validate(const plum_params* p, int n, const double* xyz, const double* q, co
   return PLUM_OK;
 }
 
-int nmm_for(const plum_ctx* c, int nm) {
-  const int npair = nm * (nm - 1) / 2;
-  return std::max(1, std::min(c->nmmb, (npair + kBlock - 1) / kBlock));
+int nmm_for(const plum_ctx*, int nm) { return nmm_blocks(nm); }
+
+// Blocks per real-space query: about 1.5 candidate slots per thread.
+int split_for(double cand, int split_div) {
+  return std::max(1, std::min(kMaxSplit, (int)std::lround(cand / split_div)));
 }
 
 // k-space and real-space work of one trial move in one launch, once the
@@ -419,7 +423,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   info.n_beads = n;
   info.n_mols = n_mols;
   info.max_mol_len = maxm;
-  c->nmmb = std::max(1, std::min(64, (maxm * (maxm - 1) / 2 + kBlock - 1) / kBlock));
+  c->nmmb = nmm_blocks(maxm);
   c->nchunk = (n + c->chunk - 1) / c->chunk;
 
   if (hipStreamCreate(&c->st) != hipSuccess) {
@@ -450,9 +454,11 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
     const double rho = n / info.vol_e;
     double cand = rho;
     for (int k = 0; k < 3; k++) cand *= 2 * g.rs + g.h[k];
-    c->nsplit = std::max(1, std::min(8, (int)std::lround(cand / 384.0)));
+    c->cand = cand;
+    c->nsplit = split_for(cand, kDefaultSplitDiv);
   }
-  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, 2 * maxm * c->nsplit); ALLOC(d.part_mm, c->nmmb);
+  // room for any split (the batch autotuner may change it)
+  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, 2 * maxm * kMaxSplit); ALLOC(d.part_mm, c->nmmb);
   ALLOC(d.sk_part, (size_t)c->nchunk * std::max(1, g.n_kh));
   ALLOC(c->epart_k, c->nkb_full);
   ALLOC(d.chain_ids, n_mols); ALLOC(d.ion_ids, n_mols);
@@ -803,3 +809,5 @@ extern "C" int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_m
   }
   return PLUM_OK;
 }
+
+#include "batch_api.h"

@@ -377,6 +377,28 @@ __device__ double molecule_spring(const Geo& g, const Dev& d, int mol, bool tria
   return acc;  // valid in thread 0
 }
 
+// Removes the tombstones of one cell, keeping slot order (one thread).
+__device__ void compact_cell(const Geo& g, const Dev& d, int cell) {
+  const int cnt = d.ccount[cell];
+  int w = 0;
+  for (int s = 0; s < cnt; s++) {
+    const size_t si = (size_t)cell * g.cap + s;
+    const int j = d.cbead[si];
+    if (j < 0) continue;
+    if (w != s) {
+      const size_t sw = (size_t)cell * g.cap + w;
+      d.cbead[sw] = j;
+      d.cpos[sw] = d.cpos[si];
+      d.cpos32[sw] = d.cpos32[si];
+      d.ctype[sw] = d.ctype[si];
+      d.bslot[j] = w;
+    }
+    w++;
+  }
+  for (int s = w; s < cnt; s++) d.cbead[(size_t)cell * g.cap + s] = -1;
+  d.ccount[cell] = w;
+}
+
 // Commit (accept) or roll back (reject) the pending move: positions, the
 // cell list (parallel, deterministic: a bead leaving its cell leaves a
 // tombstone; arrivals in one cell take consecutive slots in moved-list
@@ -399,6 +421,21 @@ __device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int
       const int i = d.mv_idx[k];
       s_nc[k] = nc;
       s_flag[k] = (d.bcell[i] != nc) ? 1 : 0;
+    }
+    __syncthreads();
+    // a destination cell whose high-water mark would pass the capacity is
+    // compacted first (by the thread of its first arrival)
+    for (int k = threadIdx.x; k < nm; k += kBlock) {
+      if (!s_flag[k]) continue;
+      const int nc = s_nc[k];
+      int arrivals = 0;
+      bool first = true;
+      for (int k2 = 0; k2 < nm; k2++) {
+        if (!s_flag[k2] || s_nc[k2] != nc) continue;
+        if (k2 < k) first = false;
+        arrivals++;
+      }
+      if (first && d.ccount[nc] + arrivals > g.cap) compact_cell(g, d, nc);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nm; k += kBlock) {
@@ -602,24 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_commit(Geo g, Dev d, int accept) {
 __global__ void k_compact(Geo g, Dev d) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= g.ncell) return;
-  const int cnt = d.ccount[cell];
-  int w = 0;
-  for (int s = 0; s < cnt; s++) {
-    const size_t si = (size_t)cell * g.cap + s;
-    const int j = d.cbead[si];
-    if (j < 0) continue;
-    if (w != s) {
-      const size_t sw = (size_t)cell * g.cap + w;
-      d.cbead[sw] = j;
-      d.cpos[sw] = d.cpos[si];
-      d.cpos32[sw] = d.cpos32[si];
-      d.ctype[sw] = d.ctype[si];
-      d.bslot[j] = w;
-    }
-    w++;
-  }
-  for (int s = w; s < cnt; s++) d.cbead[(size_t)cell * g.cap + s] = -1;
-  d.ccount[cell] = w;
+  compact_cell(g, d, cell);
 }
 
 // ------------------------------------------------------------ proposals

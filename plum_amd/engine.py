@@ -21,7 +21,8 @@ ABI_SYMBOLS = (
     "plum_create", "plum_destroy", "plum_get_info", "plum_energy_full",
     "plum_energy_difference", "plum_finalize", "plum_totals", "plum_resync",
     "plum_get_positions", "plum_last_error", "plum_mc_run", "plum_set_profiling",
-    "plum_kernel_times",
+    "plum_kernel_times", "plum_batch_create", "plum_batch_destroy", "plum_batch_mc_run",
+    "plum_batch_tune", "plum_batch_get_tuning", "plum_batch_set_tuning",
 )
 
 
@@ -78,6 +79,19 @@ class MCStats(C.Structure):
     ]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int), ("split_div", C.c_int),
+                ("sweeps_per_graph", C.c_int), ("sweep_us", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("moves", C.c_longlong), ("sweeps", C.c_longlong), ("attempted", C.c_longlong * 5),
+        ("accepted", C.c_longlong * 5), ("overflow_replicas", C.c_longlong),
+        ("seconds", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -104,6 +118,13 @@ def lib():
         L.plum_set_profiling.argtypes = [vp, C.c_int]
         L.plum_kernel_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), dp,
                                         C.POINTER(C.c_longlong)]
+        L.plum_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+        L.plum_batch_destroy.argtypes = [vp]
+        L.plum_batch_mc_run.argtypes = [vp, C.POINTER(MCParams), C.c_longlong, C.c_int,
+                                        C.POINTER(BatchStats), C.POINTER(MCStats)]
+        L.plum_batch_tune.argtypes = [vp, C.POINTER(MCParams), C.c_int, C.POINTER(BatchTuning)]
+        L.plum_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+        L.plum_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
         for f in ABI_SYMBOLS:
             if f != "plum_last_error":
                 getattr(L, f).restype = C.c_int
@@ -156,6 +177,21 @@ def make_params(sysdef, precision=PLUM_FP64, dipole_exact=0, cell_div=0):
     p.dipole_exact = dipole_exact
     p.cell_div = cell_div
     return p
+
+
+def mc_params(move_prob, move_size=1.0, seed=1, phantom=0, grafted=0, bond_len=1.0,
+              vary_bond=True, resync_every=0, out=None):
+    mp = MCParams() if out is None else out
+    for i in range(5):
+        mp.move_prob[i] = move_prob[i]
+    mp.move_size = move_size
+    mp.phantom = phantom
+    mp.grafted = grafted
+    mp.bond_len = bond_len
+    mp.vary_bond = int(vary_bond)
+    mp.seed = seed
+    mp.resync_every = resync_every
+    return mp
 
 
 class Engine:
@@ -225,16 +261,8 @@ class Engine:
 
     def mc_run(self, n_moves, move_prob, move_size=1.0, seed=1, phantom=0, grafted=0,
                bond_len=1.0, vary_bond=True, resync_every=0):
-        mp = MCParams()
-        for i in range(5):
-            mp.move_prob[i] = move_prob[i]
-        mp.move_size = move_size
-        mp.phantom = phantom
-        mp.grafted = grafted
-        mp.bond_len = bond_len
-        mp.vary_bond = int(vary_bond)
-        mp.seed = seed
-        mp.resync_every = resync_every
+        mp = mc_params(move_prob, move_size, seed, phantom, grafted, bond_len, vary_bond,
+                       resync_every)
         st = MCStats()
         _check(lib().plum_mc_run(self.h, C.byref(mp), n_moves, C.byref(st)))
         return st
@@ -251,3 +279,62 @@ class Engine:
         if n < 0:
             _check(n)
         return {names[i].decode(): (avg[i], int(cnt[i])) for i in range(n)}
+
+
+class Batch:
+    """Replicas (Engine objects, same precision) advanced together by the
+    batched kernels.  The engines must stay alive while the batch exists."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        _check(lib().plum_batch_create(arr, len(self.engines), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().plum_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _mc_array(self, mc):
+        """mc: one dict of mc_params keywords per replica."""
+        arr = (MCParams * len(self.engines))()
+        for r, kw in enumerate(mc):
+            mc_params(out=arr[r], **kw)
+        return arr
+
+    def mc_run(self, moves_per_replica, mc, use_graph=True, per_replica=False):
+        arr = self._mc_array(mc)
+        st = BatchStats()
+        per = (MCStats * len(self.engines))() if per_replica else None
+        _check(lib().plum_batch_mc_run(self.h, arr, moves_per_replica, int(use_graph),
+                                       C.byref(st), per))
+        return (st, per) if per_replica else st
+
+    def tune(self, mc, moves_per_trial=16):
+        arr = self._mc_array(mc)
+        t = BatchTuning()
+        _check(lib().plum_batch_tune(self.h, arr, moves_per_trial, C.byref(t)))
+        return t
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(lib().plum_batch_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, blocks_per_cu=None, split_div=None, sweeps_per_graph=None):
+        t = self.tuning()
+        if blocks_per_cu is not None:
+            t.blocks_per_cu = blocks_per_cu
+        if split_div is not None:
+            t.split_div = split_div
+        if sweeps_per_graph is not None:
+            t.sweeps_per_graph = sweeps_per_graph
+        _check(lib().plum_batch_set_tuning(self.h, C.byref(t)))

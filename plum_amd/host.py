@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libplum_host.so")
 HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim_energy",
                 "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
-                "plum_host_last_error")
+                "plum_host_last_error", "plum_synth_count", "plum_synth_system")
 
 
 class RunInSummary(C.Structure):
@@ -24,6 +24,16 @@ class RunInSummary(C.Structure):
         ("lj_cutoff", C.c_double), ("lB", C.c_double), ("alpha", C.c_double),
         ("k_bond", C.c_double), ("r0", C.c_double), ("rigid_bond", C.c_double),
         ("wall_cut", C.c_double), ("phantom", C.c_int), ("grafted", C.c_int),
+    ]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [
+        ("n_chains", C.c_int), ("chain_len", C.c_int), ("n_counterions", C.c_int),
+        ("n_salt_pairs", C.c_int), ("bead_q", C.c_double), ("counterion_q", C.c_double),
+        ("box", C.c_double * 3), ("npbc", C.c_int), ("min_sep", C.c_double), ("r0", C.c_double),
+        ("graft_z", C.c_double), ("z_lo", C.c_double), ("z_hi", C.c_double),
+        ("seed", C.c_ulonglong),
     ]
 
 
@@ -50,6 +60,10 @@ def lib():
         L.plum_sim_observables.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.plum_sim_destroy.argtypes = [vp]
         L.plum_host_last_error.restype = C.c_char_p
+        L.plum_synth_count.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.plum_synth_system.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -102,3 +116,32 @@ class HostSimulation:
             self.close()
         except Exception:
             pass
+
+
+def synth_system(n_chains, chain_len, n_counterions, n_salt_pairs, box, seed, bead_q=-1.0,
+                 counterion_q=1.0, npbc=3, min_sep=0.8, r0=1.0, graft_z=-1.0, z_range=None):
+    """Seeded chains + ions built by the native generator.  Returns
+    (xyz (N,3), q, type, mol_start)."""
+    L = lib()
+    sp = SynthSpec(n_chains=n_chains, chain_len=chain_len, n_counterions=n_counterions,
+                   n_salt_pairs=n_salt_pairs, bead_q=bead_q, counterion_q=counterion_q,
+                   npbc=npbc, min_sep=min_sep, r0=r0, graft_z=graft_z,
+                   z_lo=z_range[0] if z_range else 0.0, z_hi=z_range[1] if z_range else 0.0,
+                   seed=seed)
+    for d in range(3):
+        sp.box[d] = box[d]
+    nb, nm = C.c_int(), C.c_int()
+    rc = L.plum_synth_count(C.byref(sp), C.byref(nb), C.byref(nm))
+    if rc != 0:
+        raise HostError("plum_synth_count failed (%d)" % rc)
+    xyz = np.zeros(3 * nb.value)
+    q = np.zeros(nb.value)
+    ty = np.zeros(nb.value, dtype=np.int32)
+    ms = np.zeros(nm.value + 1, dtype=np.int32)
+    rc = L.plum_synth_system(C.byref(sp), xyz.ctypes.data_as(C.POINTER(C.c_double)),
+                             q.ctypes.data_as(C.POINTER(C.c_double)),
+                             ty.ctypes.data_as(C.POINTER(C.c_int)),
+                             ms.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        raise HostError("plum_synth_system failed (%d): too dense for the overlap rule?" % rc)
+    return xyz.reshape(-1, 3), q, ty, ms

@@ -211,6 +211,81 @@ def config(name, seed=None, **over):
     return base
 
 
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    """SplitMix64 output for state x (Steele, Lea, Flood 2014)."""
+    z = (x + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def alpha_for_cutoff(rc, lB=1.0):
+    """Ewald alpha (kappa^2) for which the reference's cutoff rule
+    (potential_ewald_coul.cc:56-61: smallest integer r with
+    lB/2 erfc(sqrt(alpha) r)/r <= 1e-7) lands exactly on the integer rc."""
+    target = 2e-7 * rc / lB
+    lo, hi = 0.0, 20.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if math.erfc(mid) > target:
+            lo = mid
+        else:
+            hi = mid
+    return (hi / rc) ** 2 * (1 + 1e-9)
+
+
+def replica_spec(r, seed=1002, n_max=None):
+    """Draw replica r of the batched-replica workload: chain length U{10..200},
+    chain count U{5..100}, added 1:1 salt U{0..2000} ion pairs, bead charge -1
+    or 0 with p = 0.5 (charged chains carry one +1 counterion per bead),
+    density U[0.02, 0.2].  The real-space cutoff is fixed by the builder at
+    min(12, floor(L/2)) (at least 3) and alpha follows from it."""
+    rng = np.random.default_rng(splitmix64((seed << 20) + r))
+    clen = int(rng.integers(10, 201))
+    nch = int(rng.integers(5, 101))
+    salt = int(rng.integers(0, 2001))
+    charged = bool(rng.random() < 0.5)
+    rho = float(rng.uniform(0.02, 0.2))
+    if n_max is not None:
+        # reduced copies for tests: shrink chains and salt in proportion
+        while nch * clen * (2 if charged else 1) + 2 * salt > n_max:
+            nch = max(1, nch // 2)
+            salt //= 2
+            if nch == 1 and salt == 0 and clen > 10:
+                clen = max(10, clen // 2)
+            if nch == 1 and salt == 0 and clen == 10:
+                break
+    n = nch * clen * (2 if charged else 1) + 2 * salt
+    L = (n / rho) ** (1.0 / 3.0)
+    rc = int(max(3, min(12, math.floor(L / 2))))
+    return dict(chain_len=clen, n_chains=nch, salt_pairs=salt, charged=charged, rho=rho,
+                n=n, box=L, rc=rc, seed=splitmix64((seed << 20) + r + 0x5EED))
+
+
+def replica_system(spec):
+    """Sysdef of one replica, built by the native generator."""
+    from . import host
+    L = spec["box"]
+    nci = spec["n_chains"] * spec["chain_len"] if spec["charged"] else 0
+    xyz, q, ty, ms = host.synth_system(spec["n_chains"], spec["chain_len"], nci,
+                                       spec["salt_pairs"], [L, L, L], spec["seed"] & _M64,
+                                       bead_q=-1.0 if spec["charged"] else 0.0)
+    return dict(npbc=3, beta=1.0, use_pair=1, use_ewald=1, use_bond=1, use_ext=0,
+                lj_cutoff=2.5, lB=1.0, dielectric=80.0, dipole=0, k_bond=400.0, r0=1.0,
+                wall_cut=-1.0, wall_sigma_w=1.0, wall_eps_w=1.0, phantom=0, grafted=0,
+                move_prob=[0.0, 0.1, 0.4, 0.3, 0.2], move_size=1.0, types=_std_types(),
+                box=[L, L, L], alpha=alpha_for_cutoff(spec["rc"]), xyz=xyz, q=q, type=ty,
+                mol_start=ms, name="replica")
+
+
+def replica_set(n=256, seed=1002, n_max=None):
+    """The batched-replica workload: n independent bulk systems."""
+    return [replica_system(replica_spec(r, seed, n_max)) for r in range(n)]
+
+
 def write_reference_inputs(sysdef, outdir, steps=1000, prefix="output"):
     """Write run.in / input_crd.dat / input_top.dat in the reference format
     (Simulation ctor simulation.cc:30-50, ReadTop :610-646, ReadCrd :582-608,
