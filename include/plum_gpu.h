@@ -163,7 +163,12 @@ typedef struct {
 } plum_mc_stats;
 
 /* Runs n_moves trial moves entirely on the GPU (proposal, dE, Metropolis,
-   commit) on the context's stream. */
+   commit) on the context's stream, in speculative batches: up to `pmax`
+   proposals on distinct molecules are evaluated against the same state in
+   one launch and then accepted or rejected in order, each dE corrected by
+   the exact cross terms of the proposals accepted before it (k space:
+   2 sum_k c_k Re(dS_p* dS_q); LJ and real space: the pairs between the two
+   moved sets), so the chain is the sequential one. */
 int plum_mc_run(plum_ctx* ctx, const plum_mc_params* mc, long long n_moves,
                 plum_mc_stats* stats);
 
@@ -177,19 +182,20 @@ int plum_kernel_times(plum_ctx* ctx, int cap, const char** names,
 /* ---- batched replicas -------------------------------------------------- */
 
 /* R independent systems (contexts created with plum_create, same precision)
-   advanced together: every k_compute_batch launch carries one trial move of
-   each replica, and `sweeps_per_graph` sweeps are captured in a HIP graph and
+   advanced together: every launch carries a speculative batch of every
+   replica, and `batches_per_graph` batches are captured in a HIP graph and
    replayed.  No reference counterpart: the reference runs one system per
    process (simulation.cc:216-355 per replica).  The contexts must outlive the
    batch and must not be used on their own while a batch call runs. */
 typedef struct plum_batch plum_batch;
 
 typedef struct {
-  int blocks_per_cu;     /* k_compute_batch work-groups per compute unit */
+  int blocks_per_cu;     /* k_compute_spec work-groups per compute unit */
   int split_div;         /* candidate slots per real-space query block */
-  int sweeps_per_graph;  /* sweeps captured in one graph (also the cell
-                            compaction period) */
-  double sweep_us;       /* measured time of one sweep (plum_batch_tune) */
+  int batches_per_graph; /* speculative batches captured in one graph (also
+                            the cell compaction period) */
+  int pmax;              /* proposals per speculative batch, 1..8 */
+  double sweep_us;       /* measured time per move of every system (plum_batch_tune) */
 } plum_batch_tuning;
 
 typedef struct {
@@ -212,6 +218,9 @@ int plum_batch_mc_run(plum_batch* batch, const plum_mc_params* mc, long long mov
 int plum_batch_tune(plum_batch* batch, const plum_mc_params* mc, int moves_per_trial, plum_batch_tuning* best);
 int plum_batch_get_tuning(const plum_batch* batch, plum_batch_tuning* out);
 int plum_batch_set_tuning(plum_batch* batch, const plum_batch_tuning* tuning);
+/* Launch shape of plum_mc_run (one system). */
+int plum_mc_get_tuning(plum_ctx* ctx, plum_batch_tuning* out);
+int plum_mc_set_tuning(plum_ctx* ctx, const plum_batch_tuning* tuning);
 
 #ifdef __cplusplus
 }

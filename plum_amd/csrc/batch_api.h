@@ -1,13 +1,14 @@
-// Host side of the batched-replica entry points (plum_batch_* in
-// include/plum_gpu.h).  Included once, at the end of engine.hip: it shares
-// that translation unit's kernels and helpers (build_cells, flush_sk, the
-// control-block copies).
+// Host side of the device-resident Monte Carlo loop: plum_mc_run (one
+// system, run as a batch of one) and plum_batch_* (R systems together),
+// include/plum_gpu.h.  Included once, at the end of engine.hip: it shares
+// that translation unit's kernels and helpers (build_cells, run_full).
 //
-// A sweep = one trial move of every replica: k_compute_batch (all replicas'
-// k-space / real-space / moved-moved blocks over one fixed grid) followed by
-// k_serial_batch (one block per replica).  `sweeps_per_graph` sweeps, the
-// cell compaction and the overflow-flag gather are captured once in a HIP
-// graph and replayed; the host looks at the flags between replays.
+// One speculative batch of every system = k_propose_spec, k_plan_spec,
+// k_compute_spec, k_reduce_spec, k_resolve_spec (kernels_batch.h).  `batches_per_graph` batches, the cell
+// compaction and the status gather are captured once in a HIP graph and
+// replayed until every system has reached its move count; the host looks at
+// the status between replays (cell overflow: rebuild that system's cell list
+// with more room and give it back the skipped moves).
 
 struct plum_batch {
   std::vector<plum_ctx*> reps;
@@ -20,19 +21,25 @@ struct plum_batch {
   RepCfg* dRC = nullptr;
   int* d_vb = nullptr;
   int* d_cell_off = nullptr;
-  int* d_flags = nullptr;
-  int* h_flags = nullptr;     // pinned
+  int2* d_status = nullptr;
+  int2* h_status = nullptr;   // pinned
   int total_cells = 0;
+  int max_nkb_full = 1;
   int n_cu = 0;
   plum_batch_tuning tune{};
-  std::vector<RepCfg> rc;
+  std::vector<McDev> mc;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int g_sweeps = 0, g_grid = 0;  // shape the graph was captured with
+  int g_batches = 0, g_grid = 0, g_pmax = 0;  // shape the graph was captured with
+  bool profiling = false;
   std::vector<void*> allocs;
 };
 
 namespace {
+
+constexpr int kSpecKernels = 5;
+const char* kSpecKernelNames[kSpecKernels] = {"k_propose_spec", "k_plan_spec", "k_compute_spec", "k_reduce_spec",
+                                              "k_resolve_spec"};
 
 template <typename T>
 int balloc(plum_batch* b, T** ptr, size_t count) {
@@ -46,46 +53,67 @@ int balloc(plum_batch* b, T** ptr, size_t count) {
 
 int batch_grid(const plum_batch* b) { return std::max(1, b->n_cu * b->tune.blocks_per_cu); }
 
-// Uploads every replica's Geo / Dev (a cell rebuild can change both) and the
-// launch shapes.
+// Uploads every system's Geo / Dev (a cell rebuild changes both), launch
+// shape and Monte Carlo parameters.
 int batch_upload_state(plum_batch* b) {
   std::vector<Geo> G(b->R);
   std::vector<Dev> D(b->R);
-  b->rc.resize(b->R);
+  std::vector<RepCfg> RCv(b->R);
+  b->max_nkb_full = 1;
   for (int r = 0; r < b->R; r++) {
     plum_ctx* c = b->reps[r];
     c->nsplit = split_for(c->cand, b->tune.split_div);
     G[r] = c->g;
     D[r] = c->d;
-    RepCfg& q = b->rc[r];
-    q.nkb = (c->g.use_ewald && c->g.n_kh > 0) ? c->nkb : 0;
+    RepCfg& q = RCv[r];
+    q.nkb_s = c->nkb_s;
     q.nsplit = c->nsplit;
     q.ncell = c->g.ncell;
-    q.pad = 0;
+    q.pmax = b->tune.pmax;
+    q.maxm = c->maxm;
+    q.nmmb = c->nmmb;
+    q.rstride = 2 * c->maxm * kMaxSplit;
+    q.xcap = c->xcap;
+    b->max_nkb_full = std::max(b->max_nkb_full, c->nkb_full);
   }
   HIPCHK(hipMemcpyAsync(b->dG, G.data(), sizeof(Geo) * b->R, hipMemcpyHostToDevice, b->st));
   HIPCHK(hipMemcpyAsync(b->dD, D.data(), sizeof(Dev) * b->R, hipMemcpyHostToDevice, b->st));
-  HIPCHK(hipMemcpyAsync(b->dRC, b->rc.data(), sizeof(RepCfg) * b->R, hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->dRC, RCv.data(), sizeof(RepCfg) * b->R, hipMemcpyHostToDevice, b->st));
+  if (!b->mc.empty())
+    HIPCHK(hipMemcpyAsync(b->dMC, b->mc.data(), sizeof(McDev) * b->R, hipMemcpyHostToDevice, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   return PLUM_OK;
 }
 
-// `sweeps` trial moves of every replica, then compaction and the flag gather.
-void batch_enqueue(plum_batch* b, int sweeps) {
-  const int R = b->R, grid = batch_grid(b);
-  hipLaunchKernelGGL(k_serial_batch, dim3(R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, b->d_vb, R,
-                     0, 1);
-  for (int i = 0; i < sweeps; i++) {
-    if (b->precision == PLUM_FP32_PAIR)
-      hipLaunchKernelGGL(k_compute_batch<1>, dim3(grid), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC, b->d_vb, R);
-    else
-      hipLaunchKernelGGL(k_compute_batch<0>, dim3(grid), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC, b->d_vb, R);
-    hipLaunchKernelGGL(k_serial_batch, dim3(R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, b->d_vb, R,
-                       1, i < sweeps - 1 ? 1 : 0);
-  }
+// One speculative batch of every system; with `ev` (profiling) each kernel is
+// bracketed by events ev[k], ev[k + 1].
+void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
+  const int R = b->R;
+  if (ev) (void)hipEventRecord(ev[0], b->st);
+  hipLaunchKernelGGL(k_propose_spec, dim3(b->tune.pmax, R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC,
+                     b->d_vb, R);
+  if (ev) (void)hipEventRecord(ev[1], b->st);
+  hipLaunchKernelGGL(k_plan_spec, dim3(R), dim3(kWave), 0, b->st, b->dG, b->dD, b->dRC, b->d_vb, R);
+  if (ev) (void)hipEventRecord(ev[2], b->st);
+  if (b->precision == PLUM_FP32_PAIR)
+    hipLaunchKernelGGL(k_compute_spec<1>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
+                       b->d_vb, R);
+  else
+    hipLaunchKernelGGL(k_compute_spec<0>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
+                       b->d_vb, R);
+  if (ev) (void)hipEventRecord(ev[3], b->st);
+  hipLaunchKernelGGL(k_reduce_spec, dim3(kKVals + kMaxProp + kMaxPairs, R), dim3(kBlock), 0, b->st, b->dG, b->dD,
+                     b->dRC, R);
+  if (ev) (void)hipEventRecord(ev[4], b->st);
+  hipLaunchKernelGGL(k_resolve_spec, dim3(R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, R);
+  if (ev) (void)hipEventRecord(ev[5], b->st);
+}
+
+void launch_tail(plum_batch* b) {
   hipLaunchKernelGGL(k_compact_batch, dim3((b->total_cells + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dG,
-                     b->dD, b->d_cell_off, R);
-  hipLaunchKernelGGL(k_batch_flags, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dD, R, b->d_flags);
+                     b->dD, b->d_cell_off, b->R);
+  hipLaunchKernelGGL(k_batch_status, dim3((b->R + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dD, b->dMC, b->R,
+                     b->d_status);
 }
 
 void batch_drop_graph(plum_batch* b) {
@@ -93,37 +121,59 @@ void batch_drop_graph(plum_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
   b->graph = nullptr;
-  b->g_sweeps = 0;
-  b->g_grid = 0;
+  b->g_batches = b->g_grid = b->g_pmax = 0;
 }
 
-int batch_launch(plum_batch* b, int sweeps, bool use_graph) {
-  if (!use_graph) {
-    batch_enqueue(b, sweeps);
+// `nb` batches of every system, then compaction and the status gather.  With
+// profiling, direct launches bracketed by events (per-kernel device times).
+int batch_replay(plum_batch* b, int nb, bool use_graph, plum_ctx* prof) {
+  if (prof) {
+    std::vector<hipEvent_t> ev(kSpecKernels + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (int i = 0; i < nb; i++) {
+      launch_batch(b, ev.data());
+      HIPCHK(hipEventSynchronize(ev[kSpecKernels]));
+      for (int k = 0; k < kSpecKernels; k++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) {
+          prof->ksum_us[k] += 1000.0 * ms;
+          prof->kcount[k] += 1;
+        }
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    launch_tail(b);
     HIPCHK(hipGetLastError());
     return PLUM_OK;
   }
-  if (!b->gexec || b->g_sweeps != sweeps || b->g_grid != batch_grid(b)) {
+  if (!use_graph) {
+    for (int i = 0; i < nb; i++) launch_batch(b);
+    launch_tail(b);
+    HIPCHK(hipGetLastError());
+    return PLUM_OK;
+  }
+  if (!b->gexec || b->g_batches != nb || b->g_grid != batch_grid(b) || b->g_pmax != b->tune.pmax) {
     batch_drop_graph(b);
     HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    batch_enqueue(b, sweeps);
+    for (int i = 0; i < nb; i++) launch_batch(b);
+    launch_tail(b);
     hipGraph_t gr = nullptr;
     HIPCHK(hipStreamEndCapture(b->st, &gr));
     b->graph = gr;
     HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-    b->g_sweeps = sweeps;
+    b->g_batches = nb;
     b->g_grid = batch_grid(b);
+    b->g_pmax = b->tune.pmax;
   }
   HIPCHK(hipGraphLaunch(b->gexec, b->st));
   return PLUM_OK;
 }
 
+// Pending S(k) of every system folded in (end of a run, before a rebuild).
 int batch_flush(plum_batch* b) {
-  int maxk = 1;
-  for (plum_ctx* c : b->reps) maxk = std::max(maxk, c->nkb_full);
-  hipLaunchKernelGGL(k_flush_sk_batch, dim3(std::min(maxk, 64), b->R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->R);
-  hipLaunchKernelGGL(k_clear_sk_pending_batch, dim3((b->R + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dD,
+  hipLaunchKernelGGL(k_flush_spec, dim3(std::min(b->max_nkb_full, 64), b->R), dim3(kBlock), 0, b->st, b->dG, b->dD,
                      b->R);
+  hipLaunchKernelGGL(k_clear_mask, dim3((b->R + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dD, b->R);
   HIPCHK(hipGetLastError());
   return PLUM_OK;
 }
@@ -133,7 +183,8 @@ int read_ctl(plum_ctx* c, hipStream_t st) {
   return PLUM_OK;
 }
 
-// Chain / ion lists and the McDev of one replica (as plum_mc_run sets them).
+// Chain / ion lists (simulation.cc:247-266: from the phantom index on) and
+// the McDev of one system.
 int mc_setup(plum_ctx* c, const plum_mc_params* mcp, hipStream_t st, McDev* out) {
   double tot = 0;
   for (int k = 0; k < 5; k++) {
@@ -160,33 +211,20 @@ int mc_setup(plum_ctx* c, const plum_mc_params* mcp, hipStream_t st, McDev* out)
   mc.n_chains = (int)chains.size();
   mc.n_ions = (int)ions.size();
   mc.seed = mcp->seed;
+  mc.ctr_limit = 0;
   *out = mc;
   return PLUM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int plum_batch_create(plum_ctx* const* replicas, int n, plum_batch** out) {
-  if (!out || !replicas) return fail(PLUM_E_ARG, "null argument");
-  *out = nullptr;
-  if (n <= 0 || n > kMaxReplicas) return fail(PLUM_E_ARG, "replica count must be 1..1024");
-  for (int r = 0; r < n; r++) {
-    if (!replicas[r]) return fail(PLUM_E_ARG, "null replica");
-    if (replicas[r]->pending) return fail(PLUM_E_STATE, "a replica has a pending trial move");
-    if (replicas[r]->p.precision != replicas[0]->p.precision)
-      return fail(PLUM_E_ARG, "all replicas must use the same precision");
-    for (int q = 0; q < r; q++)
-      if (replicas[q] == replicas[r]) return fail(PLUM_E_ARG, "replica listed twice");
-  }
+int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   plum_batch* b = new plum_batch();
   b->reps.assign(replicas, replicas + n);
   b->R = n;
   b->precision = replicas[0]->p.precision;
   b->tune.blocks_per_cu = 8;
   b->tune.split_div = kDefaultSplitDiv;
-  b->tune.sweeps_per_graph = 256;
+  b->tune.batches_per_graph = 32;
+  b->tune.pmax = kMaxProp;
   b->tune.sweep_us = 0;
   int dev = 0;
   int rc = PLUM_OK;
@@ -200,9 +238,9 @@ int plum_batch_create(plum_ctx* const* replicas, int n, plum_batch** out) {
   if (hipStreamCreate(&b->st) != hipSuccess) return bail(fail(PLUM_E_HIP, "hipStreamCreate failed"));
   if ((rc = balloc(b, &b->dG, n)) || (rc = balloc(b, &b->dD, n)) || (rc = balloc(b, &b->dMC, n)) ||
       (rc = balloc(b, &b->dRC, n)) || (rc = balloc(b, &b->d_vb, n)) || (rc = balloc(b, &b->d_cell_off, n + 1)) ||
-      (rc = balloc(b, &b->d_flags, n)))
+      (rc = balloc(b, &b->d_status, n)))
     return bail(rc);
-  if (hipHostMalloc((void**)&b->h_flags, sizeof(int) * n, hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void**)&b->h_status, sizeof(int2) * n, hipHostMallocDefault) != hipSuccess)
     return bail(fail(PLUM_E_NOMEM, "hipHostMalloc failed"));
   std::vector<int> off(n + 1, 0);
   for (int r = 0; r < n; r++) off[r + 1] = off[r] + replicas[r]->g.ncell;
@@ -216,12 +254,189 @@ int plum_batch_create(plum_ctx* const* replicas, int n, plum_batch** out) {
   return PLUM_OK;
 }
 
+// The device loop proper: n_moves trial moves of every system.
+int batch_run(plum_batch* b, const plum_mc_params* mcp, long long n_moves, bool use_graph, plum_ctx* prof,
+              plum_batch_stats* total, plum_mc_stats* per) {
+  const int R = b->R;
+  for (int r = 0; r < R; r++)
+    if (b->reps[r]->pending) return fail(PLUM_E_STATE, "a system has a pending trial move");
+  b->mc.assign(R, McDev{});
+  for (int r = 0; r < R; r++) CHK(mc_setup(b->reps[r], &mcp[r], b->st, &b->mc[r]));
+  std::vector<unsigned long long> m0(R);
+  std::vector<long long> sk0(R), att0(5 * (size_t)R), acc0(5 * (size_t)R);
+  for (int r = 0; r < R; r++) CHK(read_ctl(b->reps[r], b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (int r = 0; r < R; r++) {
+    const Ctl* h = b->reps[r]->h_ctl;
+    m0[r] = h->move_counter;
+    sk0[r] = h->skipped;
+    for (int k = 0; k < 5; k++) {
+      att0[5 * (size_t)r + k] = h->attempted[k];
+      acc0[5 * (size_t)r + k] = h->accepted[k];
+    }
+    b->mc[r].ctr_limit = m0[r] + (unsigned long long)n_moves;
+  }
+  CHK(batch_upload_state(b));
+  long long resync_every = 0;
+  for (int r = 0; r < R; r++)
+    if (mcp[r].resync_every > 0)
+      resync_every = resync_every ? std::min<long long>(resync_every, mcp[r].resync_every) : mcp[r].resync_every;
+  std::vector<long long> credited(R, 0);   // skipped moves already given back
+  const auto t0 = std::chrono::steady_clock::now();
+  long long since_resync = 0, rebuilds = 0;
+  for (;;) {
+    CHK(batch_replay(b, b->tune.batches_per_graph, use_graph, prof));
+    HIPCHK(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int2) * R, hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    since_resync += (long long)b->tune.batches_per_graph * b->tune.pmax;
+    bool busy = false, fix = false;
+    for (int r = 0; r < R; r++) {
+      busy |= b->h_status[r].y > 0;
+      fix |= b->h_status[r].x != 0;
+    }
+    const bool resync = resync_every > 0 && since_resync >= resync_every && busy;
+    if (fix || resync) {
+      CHK(batch_flush(b));
+      HIPCHK(hipStreamSynchronize(b->st));
+      for (int r = 0; r < R; r++) {
+        plum_ctx* c = b->reps[r];
+        if (b->h_status[r].x) {
+          // cell overflow: more room, and the moves skipped since get redone
+          if (++rebuilds > 16L * R) return fail(PLUM_E_CAPACITY, "cell list keeps overflowing");
+          int zero = 0;
+          HIPCHK(hipMemcpyAsync(&c->d.ctl->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, b->st));
+          CHK(read_ctl(c, b->st));
+          HIPCHK(hipStreamSynchronize(b->st));
+          CHK(build_cells(c, 2 * c->g.cap));
+          const long long sk = c->h_ctl->skipped - sk0[r];
+          b->mc[r].ctr_limit += (unsigned long long)(sk - credited[r]);
+          credited[r] = sk;
+          busy = true;
+        } else if (resync) {
+          CHK(build_cells(c, c->g.cap));
+          double f[8];
+          CHK(run_full(c, 1, f));
+        }
+      }
+      if (resync) since_resync = 0;
+      CHK(batch_upload_state(b));
+    }
+    if (!busy) break;
+  }
+  CHK(batch_flush(b));
+  hipLaunchKernelGGL(k_compact_batch, dim3((b->total_cells + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dG,
+                     b->dD, b->d_cell_off, b->R);
+  HIPCHK(hipGetLastError());
+  for (int r = 0; r < R; r++) CHK(read_ctl(b->reps[r], b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  const auto t1 = std::chrono::steady_clock::now();
+  const double secs = std::chrono::duration<double>(t1 - t0).count();
+  plum_batch_stats st{};
+  for (int r = 0; r < R; r++) {
+    const Ctl* h = b->reps[r]->h_ctl;
+    const long long skipped = h->skipped - sk0[r];
+    const long long moves = ((long long)h->move_counter - (long long)m0[r]) - skipped;
+    st.moves += moves;
+    for (int k = 0; k < 5; k++) {
+      st.attempted[k] += h->attempted[k] - att0[5 * (size_t)r + k];
+      st.accepted[k] += h->accepted[k] - acc0[5 * (size_t)r + k];
+    }
+    if (skipped > 0) st.overflow_replicas += 1;
+    if (per) {
+      plum_mc_stats& p = per[r];
+      std::memset(&p, 0, sizeof(p));
+      for (int k = 0; k < 5; k++) {
+        p.attempted[k] = h->attempted[k] - att0[5 * (size_t)r + k];
+        p.accepted[k] = h->accepted[k] - acc0[5 * (size_t)r + k];
+      }
+      p.moves = moves;
+      p.seconds = secs;
+      to_terms(h->running, &p.totals);
+    }
+  }
+  st.sweeps = n_moves;
+  st.seconds = secs;
+  if (total) *total = st;
+  return PLUM_OK;
+}
+
+int check_tuning(const plum_batch_tuning* t) {
+  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(PLUM_E_ARG, "blocks_per_cu must be 1..64");
+  if (t->split_div < 16) return fail(PLUM_E_ARG, "split_div must be >= 16");
+  if (t->batches_per_graph < 1 || t->batches_per_graph > 1024)
+    return fail(PLUM_E_ARG, "batches_per_graph must be 1..1024");
+  if (t->pmax < 1 || t->pmax > kMaxProp) return fail(PLUM_E_ARG, "pmax must be 1..8");
+  return PLUM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_moves, plum_mc_stats* stats) {
+  if (!c || !mcp) return fail(PLUM_E_ARG, "null argument");
+  if (c->pending) return fail(PLUM_E_STATE, "a trial move is pending finalize");
+  if (n_moves < 0) return fail(PLUM_E_ARG, "negative move count");
+  if (!c->self) {
+    plum_ctx* one[1] = {c};
+    CHK(batch_new(one, 1, &c->self));
+  }
+  if (c->profiling) {
+    c->kname.assign(kSpecKernelNames, kSpecKernelNames + kSpecKernels);
+    c->ksum_us.assign(kSpecKernels, 0.0);
+    c->kcount.assign(kSpecKernels, 0);
+  }
+  plum_mc_stats st{};
+  CHK(batch_run(c->self, mcp, n_moves, true, c->profiling ? c : nullptr, nullptr, &st));
+  if (stats) *stats = st;
+  return PLUM_OK;
+}
+
+int plum_mc_get_tuning(plum_ctx* c, plum_batch_tuning* out) {
+  if (!c || !out) return fail(PLUM_E_ARG, "null argument");
+  if (!c->self) {
+    plum_ctx* one[1] = {c};
+    CHK(batch_new(one, 1, &c->self));
+  }
+  *out = c->self->tune;
+  return PLUM_OK;
+}
+
+int plum_mc_set_tuning(plum_ctx* c, const plum_batch_tuning* t) {
+  if (!c || !t) return fail(PLUM_E_ARG, "null argument");
+  CHK(check_tuning(t));
+  if (!c->self) {
+    plum_ctx* one[1] = {c};
+    CHK(batch_new(one, 1, &c->self));
+  }
+  const double us = c->self->tune.sweep_us;
+  c->self->tune = *t;
+  c->self->tune.sweep_us = us;
+  CHK(batch_upload_state(c->self));
+  return PLUM_OK;
+}
+
+int plum_batch_create(plum_ctx* const* replicas, int n, plum_batch** out) {
+  if (!out || !replicas) return fail(PLUM_E_ARG, "null argument");
+  *out = nullptr;
+  if (n <= 0 || n > kMaxReplicas) return fail(PLUM_E_ARG, "replica count must be 1..1024");
+  for (int r = 0; r < n; r++) {
+    if (!replicas[r]) return fail(PLUM_E_ARG, "null replica");
+    if (replicas[r]->pending) return fail(PLUM_E_STATE, "a replica has a pending trial move");
+    if (replicas[r]->p.precision != replicas[0]->p.precision)
+      return fail(PLUM_E_ARG, "all replicas must use the same precision");
+    for (int q = 0; q < r; q++)
+      if (replicas[q] == replicas[r]) return fail(PLUM_E_ARG, "replica listed twice");
+  }
+  return batch_new(replicas, n, out);
+}
+
 int plum_batch_destroy(plum_batch* b) {
   if (!b) return PLUM_OK;
   if (b->st) (void)hipStreamSynchronize(b->st);
   batch_drop_graph(b);
   for (void* p : b->allocs) (void)hipFree(p);
-  if (b->h_flags) (void)hipHostFree(b->h_flags);
+  if (b->h_status) (void)hipHostFree(b->h_status);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
   return PLUM_OK;
@@ -235,13 +450,10 @@ int plum_batch_get_tuning(const plum_batch* b, plum_batch_tuning* out) {
 
 int plum_batch_set_tuning(plum_batch* b, const plum_batch_tuning* t) {
   if (!b || !t) return fail(PLUM_E_ARG, "null argument");
-  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(PLUM_E_ARG, "blocks_per_cu must be 1..64");
-  if (t->split_div < 16) return fail(PLUM_E_ARG, "split_div must be >= 16");
-  if (t->sweeps_per_graph < 1 || t->sweeps_per_graph > 4096)
-    return fail(PLUM_E_ARG, "sweeps_per_graph must be 1..4096");
-  b->tune.blocks_per_cu = t->blocks_per_cu;
-  b->tune.split_div = t->split_div;
-  b->tune.sweeps_per_graph = t->sweeps_per_graph;
+  CHK(check_tuning(t));
+  const double us = b->tune.sweep_us;
+  b->tune = *t;
+  b->tune.sweep_us = us;
   CHK(batch_upload_state(b));
   return PLUM_OK;
 }
@@ -250,129 +462,40 @@ int plum_batch_mc_run(plum_batch* b, const plum_mc_params* mcp, long long n_move
                       plum_batch_stats* total, plum_mc_stats* per) {
   if (!b || !mcp) return fail(PLUM_E_ARG, "null argument");
   if (n_moves < 0) return fail(PLUM_E_ARG, "negative move count");
-  const int R = b->R;
-  for (int r = 0; r < R; r++)
-    if (b->reps[r]->pending) return fail(PLUM_E_STATE, "a replica has a pending trial move");
-  std::vector<McDev> mc(R);
-  for (int r = 0; r < R; r++) CHK(mc_setup(b->reps[r], &mcp[r], b->st, &mc[r]));
-  HIPCHK(hipMemcpyAsync(b->dMC, mc.data(), sizeof(McDev) * R, hipMemcpyHostToDevice, b->st));
-  CHK(batch_upload_state(b));
-  std::vector<unsigned long long> m0(R);
-  std::vector<long long> sk0(R), att0(5 * (size_t)R), acc0(5 * (size_t)R);
-  for (int r = 0; r < R; r++) CHK(read_ctl(b->reps[r], b->st));
-  HIPCHK(hipStreamSynchronize(b->st));
-  for (int r = 0; r < R; r++) {
-    const Ctl* h = b->reps[r]->h_ctl;
-    m0[r] = h->move_counter;
-    sk0[r] = h->skipped;
-    for (int k = 0; k < 5; k++) {
-      att0[5 * (size_t)r + k] = h->attempted[k];
-      acc0[5 * (size_t)r + k] = h->accepted[k];
-    }
-  }
-  long long resync_every = 0;
-  for (int r = 0; r < R; r++)
-    if (mcp[r].resync_every > 0)
-      resync_every = resync_every ? std::min<long long>(resync_every, mcp[r].resync_every) : mcp[r].resync_every;
-  const auto t0 = std::chrono::steady_clock::now();
-  long long done = 0, since_resync = 0;
-  const int M = b->tune.sweeps_per_graph;
-  while (done < n_moves) {
-    const int m = (int)std::min<long long>(M, n_moves - done);
-    CHK(batch_launch(b, m, use_graph != 0 && m == M));
-    HIPCHK(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int) * R, hipMemcpyDeviceToHost, b->st));
-    HIPCHK(hipStreamSynchronize(b->st));
-    done += m;
-    since_resync += m;
-    // a replica whose cell list overflowed skips its remaining sweeps; it is
-    // finished alone once the batch is through
-    if (resync_every > 0 && since_resync >= resync_every && done < n_moves) {
-      CHK(batch_flush(b));
-      HIPCHK(hipStreamSynchronize(b->st));
-      for (plum_ctx* c : b->reps) {
-        CHK(build_cells(c, c->g.cap));
-        double f[8];
-        CHK(run_full(c, 1, f));
-      }
-      CHK(batch_upload_state(b));
-      since_resync = 0;
-    }
-  }
-  CHK(batch_flush(b));
-  HIPCHK(hipStreamSynchronize(b->st));
-  const auto t1 = std::chrono::steady_clock::now();
-  // replicas that overflowed: rebuild with more room, redo the skipped moves
-  for (int r = 0; r < R; r++) {
-    plum_ctx* c = b->reps[r];
-    CHK(read_ctl(c, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    const long long skipped = c->h_ctl->skipped - sk0[r];
-    if (skipped <= 0 && !c->h_ctl->overflow) continue;
-    int zero = 0;
-    HIPCHK(hipMemcpyAsync(&c->d.ctl->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, c->st));
-    CHK(build_cells(c, 2 * c->g.cap));
-    if (skipped > 0) CHK(plum_mc_run(c, &mcp[r], skipped, nullptr));
-  }
-  CHK(batch_upload_state(b));
-  plum_batch_stats st{};
-  for (int r = 0; r < R; r++) {
-    plum_ctx* c = b->reps[r];
-    CHK(read_ctl(c, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    const Ctl* h = c->h_ctl;
-    const long long moves = ((long long)h->move_counter - (long long)m0[r]) - (h->skipped - sk0[r]);
-    st.moves += moves;
-    for (int k = 0; k < 5; k++) {
-      st.attempted[k] += h->attempted[k] - att0[5 * (size_t)r + k];
-      st.accepted[k] += h->accepted[k] - acc0[5 * (size_t)r + k];
-    }
-    if (h->skipped - sk0[r] > 0) st.overflow_replicas += 1;
-    if (per) {
-      plum_mc_stats& p = per[r];
-      std::memset(&p, 0, sizeof(p));
-      for (int k = 0; k < 5; k++) {
-        p.attempted[k] = h->attempted[k] - att0[5 * (size_t)r + k];
-        p.accepted[k] = h->accepted[k] - acc0[5 * (size_t)r + k];
-      }
-      p.moves = moves;
-      to_terms(h->running, &p.totals);
-    }
-  }
-  st.sweeps = n_moves;
-  st.seconds = std::chrono::duration<double>(t1 - t0).count();
-  if (per)
-    for (int r = 0; r < R; r++) per[r].seconds = st.seconds;
-  if (total) *total = st;
-  return PLUM_OK;
+  return batch_run(b, mcp, n_moves, use_graph != 0, nullptr, total, per);
 }
 
-// Times `moves_per_trial` sweeps for each candidate launch shape (work-groups
-// per CU of k_compute_batch x candidate slots per real-space query block) and
-// keeps the fastest.  The trial sweeps are ordinary Monte Carlo moves of the
-// replicas.
+// Times `moves_per_trial` moves of every system for each candidate launch
+// shape (work-groups per CU of k_compute_spec, candidate slots per
+// real-space query block, proposals per speculative batch) and keeps the
+// fastest.  The trial moves are ordinary Monte Carlo moves.
 int plum_batch_tune(plum_batch* b, const plum_mc_params* mcp, int moves_per_trial, plum_batch_tuning* best) {
   if (!b || !mcp) return fail(PLUM_E_ARG, "null argument");
   if (moves_per_trial < 1) return fail(PLUM_E_ARG, "moves_per_trial must be >= 1");
-  const int bpc[] = {2, 4, 8, 16};
-  const int div[] = {192, 384, 768, 1536};
-  plum_batch_tuning keep = b->tune, win = b->tune;
+  const int bpc[] = {4, 8, 16};
+  const int div[] = {192, 384, 768};
+  const int pm[] = {1, 2, 4, 8};
+  const plum_batch_tuning keep = b->tune;
+  plum_batch_tuning win = keep;
   double best_s = 1e300;
   for (int i : bpc)
-    for (int j : div) {
-      plum_batch_tuning t = keep;
-      t.blocks_per_cu = i;
-      t.split_div = j;
-      t.sweeps_per_graph = moves_per_trial;
-      CHK(plum_batch_set_tuning(b, &t));
-      plum_batch_stats st{};
-      CHK(plum_batch_mc_run(b, mcp, moves_per_trial, 1, &st, nullptr));
-      if (st.seconds < best_s) {
-        best_s = st.seconds;
-        win = t;
-        win.sweep_us = 1e6 * st.seconds / moves_per_trial;
+    for (int j : div)
+      for (int k : pm) {
+        plum_batch_tuning t = keep;
+        t.blocks_per_cu = i;
+        t.split_div = j;
+        t.pmax = k;
+        t.batches_per_graph = std::max(1, std::min(64, moves_per_trial / k));
+        CHK(plum_batch_set_tuning(b, &t));
+        plum_batch_stats st{};
+        CHK(batch_run(b, mcp, moves_per_trial, true, nullptr, &st, nullptr));
+        if (st.seconds < best_s) {
+          best_s = st.seconds;
+          win = t;
+          win.sweep_us = 1e6 * st.seconds / moves_per_trial;
+        }
       }
-    }
-  win.sweeps_per_graph = keep.sweeps_per_graph;
+  win.batches_per_graph = keep.batches_per_graph;
   CHK(plum_batch_set_tuning(b, &win));
   b->tune.sweep_us = win.sweep_us;
   if (best) *best = b->tune;

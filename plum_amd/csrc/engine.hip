@@ -27,6 +27,9 @@
 
 using namespace plum;
 
+struct plum_batch;
+extern "C" int plum_batch_destroy(plum_batch* batch);
+
 struct plum_ctx {
   plum_params p{};
   Geo g{};
@@ -42,6 +45,9 @@ struct plum_ctx {
   int nrealb = 1;        // blocks of the full real-space kernel
   int nsplit = 1;        // blocks per real-space query
   double cand = 0;       // expected candidate pair-images per query
+  int nkb_s = 0;         // k-space blocks of the speculative loop
+  int xcap = 0;          // cross-term partial capacity
+  plum_batch* self = nullptr;  // this system as a batch of one (device loop)
   double* epart_k = nullptr;
   std::vector<int> mol_start;
   std::vector<int> type_h;
@@ -443,11 +449,20 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   ALLOC(d.type, n); ALLOC(d.mol_of, n); ALLOC(d.mol_start, n_mols + 1);
   ALLOC(d.bcell, n); ALLOC(d.bslot, n); ALLOC(d.moved, n); ALLOC(d.mv_slot, n);
   ALLOC(d.ccount, g.ncell);
-  ALLOC(d.kl, g.n_kh); ALLOC(d.ck, g.n_kh); ALLOC(d.Sk, g.n_kh); ALLOC(d.dS, g.n_kh);
-  ALLOC(d.tab, (size_t)maxm * 2 * std::max(1, g.tab_w));
+  // per-proposal slots (kMaxProp) for the speculative device loop; the host
+  // path uses slot 0
+  ALLOC(d.kl, g.n_kh); ALLOC(d.ck, g.n_kh); ALLOC(d.Sk, g.n_kh); ALLOC(d.dS, (size_t)kMaxProp * g.n_kh);
+  ALLOC(d.tab, (size_t)kMaxProp * maxm * 2 * std::max(1, g.tab_w));
   ALLOC(d.lj, lj.size()); ALLOC(d.lj32, lj32.size()); ALLOC(d.wall, wall.size());
   ALLOC(d.bond_cur, n_mols);
-  ALLOC(d.mv_old, maxm); ALLOC(d.mv_new, maxm); ALLOC(d.mv_type, maxm);
+  ALLOC(d.mv_old, (size_t)kMaxProp * maxm); ALLOC(d.mv_new, (size_t)kMaxProp * maxm);
+  ALLOC(d.mv_type, (size_t)kMaxProp * maxm);
+  ALLOC(d.recs, kMaxProp); ALLOC(d.smv_idx, (size_t)kMaxProp * maxm);
+  ALLOC(d.prop, kMaxProp); ALLOC(d.plan, 1);
+  c->nkb_s = (g.use_ewald && g.n_kh > 0) ? (g.n_kh + kKSpec - 1) / kKSpec : 0;
+  ALLOC(d.part_ks, (size_t)kKVals * std::max(1, c->nkb_s));
+  c->xcap = kMaxPairs * cross_blocks(maxm, maxm);
+  ALLOC(d.part_x, c->xcap);
   ALLOC(d.ctl, 1);
   {
     // split each real-space query so a thread sees ~1.5 candidate slots
@@ -458,7 +473,8 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
     c->nsplit = split_for(cand, kDefaultSplitDiv);
   }
   // room for any split (the batch autotuner may change it)
-  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, 2 * maxm * kMaxSplit); ALLOC(d.part_mm, c->nmmb);
+  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, (size_t)kMaxProp * 2 * maxm * kMaxSplit);
+  ALLOC(d.part_mm, (size_t)kMaxProp * c->nmmb);
   ALLOC(d.sk_part, (size_t)c->nchunk * std::max(1, g.n_kh));
   ALLOC(c->epart_k, c->nkb_full);
   ALLOC(d.chain_ids, n_mols); ALLOC(d.ion_ids, n_mols);
@@ -495,6 +511,8 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   if (hipMemsetAsync(d.moved, 0, n, c->st) != hipSuccess ||
       hipMemsetAsync(d.mv_slot, 0xff, sizeof(int) * n, c->st) != hipSuccess ||
       hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->st) != hipSuccess ||
+      hipMemsetAsync(d.plan, 0, sizeof(SpecPlan), c->st) != hipSuccess ||
+      hipMemsetAsync(d.recs, 0, sizeof(MoveRec) * kMaxProp, c->st) != hipSuccess ||
       hipMemsetAsync(c->d_in, 0, c->in_bytes, c->st) != hipSuccess)
     return bail(fail(PLUM_E_HIP, "hipMemsetAsync failed"));
   if ((rc = build_cells(c, 0)) != PLUM_OK) return bail(rc);
@@ -506,6 +524,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
 
 int plum_destroy(plum_ctx* c) {
   if (!c) return PLUM_OK;
+  if (c->self) plum_batch_destroy(c->self);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void* p : c->dev_allocs) (void)hipFree(p);
   if (c->h_in) (void)hipHostFree(c->h_in);
@@ -574,10 +593,8 @@ int plum_energy_difference(plum_ctx* c, int mol, int n_moved, const int* idx, co
                        c->st, g, c->d);
   }
   launch_compute(c, n_moved);
-  McDev none{};
   hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(kBlock), 0, c->st, g, c->d,
-                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * n_moved * c->nsplit, nmm_for(c, n_moved), 0,
-                     none);
+                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * n_moved * c->nsplit, nmm_for(c, n_moved));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -669,145 +686,5 @@ int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, 
 }
 
 }  // extern "C"
-
-// ---------------------------------------------------------------- device loop
-namespace {
-
-constexpr int kDevKernels = 2;
-const char* kDevKernelNames[kDevKernels] = {"k_compute", "k_serial"};
-
-// One k_serial launch (epilogue of the pending move and/or proposal of the
-// next); with `ev` the launch is bracketed by events.
-void launch_serial(plum_ctx* c, const McDev& mc, int do_epi, int do_prop) {
-  const Geo& g = c->g;
-  hipLaunchKernelGGL(k_serial, dim3(1), dim3(kBlock), 0, c->st, g, c->d, mc,
-                     (g.use_ewald && g.n_kh > 0) ? c->nkb : 0, 2 * c->maxm * c->nsplit, nmm_for(c, c->maxm),
-                     do_epi, do_prop);
-}
-
-void flush_sk(plum_ctx* c) {
-  hipLaunchKernelGGL(k_flush_sk, dim3(c->nkb_full), dim3(kBlock), 0, c->st, c->g, c->d);
-  hipLaunchKernelGGL(k_clear_sk_pending, dim3(1), dim3(1), 0, c->st, c->d);
-}
-
-}  // namespace
-
-extern "C" int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_moves, plum_mc_stats* stats) {
-  if (!c || !mcp) return fail(PLUM_E_ARG, "null argument");
-  if (c->pending) return fail(PLUM_E_STATE, "a trial move is pending finalize");
-  if (n_moves < 0) return fail(PLUM_E_ARG, "negative move count");
-  double tot = 0;
-  for (int k = 0; k < 5; k++) {
-    if (!(mcp->move_prob[k] >= 0)) return fail(PLUM_E_ARG, "negative move probability");
-    tot += mcp->move_prob[k];
-  }
-  if (std::fabs(tot - 1.0) > 1e-4)
-    return fail(PLUM_E_ARG, "move probabilities must sum to 1 (simulation.cc:158-166)");
-  if (mcp->phantom < 0 || mcp->phantom > c->g.n_mols) return fail(PLUM_E_ARG, "phantom count out of range");
-  // chain and ion lists from the phantom index on (simulation.cc:247-266)
-  std::vector<int> chains, ions;
-  for (int m = mcp->phantom; m < c->g.n_mols; m++) {
-    if (c->mol_start[m + 1] - c->mol_start[m] > 1) chains.push_back(m);
-    else ions.push_back(m);
-  }
-  if (!chains.empty())
-    HIPCHK(hipMemcpyAsync(c->d.chain_ids, chains.data(), sizeof(int) * chains.size(), hipMemcpyHostToDevice, c->st));
-  if (!ions.empty())
-    HIPCHK(hipMemcpyAsync(c->d.ion_ids, ions.data(), sizeof(int) * ions.size(), hipMemcpyHostToDevice, c->st));
-  McDev mc{};
-  for (int k = 0; k < 5; k++) mc.cum_prob[k] = mcp->move_prob[k];
-  mc.move_size = mcp->move_size;
-  mc.bond_len = mcp->bond_len;
-  mc.vary_bond = mcp->vary_bond;
-  mc.n_chains = (int)chains.size();
-  mc.n_ions = (int)ions.size();
-  mc.seed = mcp->seed;
-  HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  const long long m0 = (long long)c->h_ctl->move_counter;
-  long long att0[5], acc0[5];
-  for (int k = 0; k < 5; k++) {
-    att0[k] = c->h_ctl->attempted[k];
-    acc0[k] = c->h_ctl->accepted[k];
-  }
-  const int compact_every = 64;
-  const int batch = 256;
-  std::vector<hipEvent_t> ev;
-  if (c->profiling) {
-    ev.resize((size_t)3 * batch);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    c->kname.assign(kDevKernelNames, kDevKernelNames + kDevKernels);
-    c->ksum_us.assign(kDevKernels, 0.0);
-    c->kcount.assign(kDevKernels, 0);
-  }
-  const auto t0 = std::chrono::steady_clock::now();
-  const long long sk0 = c->h_ctl->skipped;
-  long long issued = 0;     // moves completed, not counting skipped ones
-  long long since_resync = 0;
-  int guard = 0;
-  while (issued < n_moves) {
-    const long long todo = std::min<long long>(batch, n_moves - issued);
-    // pipeline: propose; (compute, epilogue+propose) x (todo-1); compute, epilogue
-    launch_serial(c, mc, 0, 1);
-    for (long long i = 0; i < todo; i++) {
-      const bool last = (i == todo - 1);
-      if (c->profiling) (void)hipEventRecord(ev[3 * i], c->st);
-      launch_compute(c, c->maxm);
-      if (c->profiling) (void)hipEventRecord(ev[3 * i + 1], c->st);
-      launch_serial(c, mc, 1, last ? 0 : 1);
-      if (c->profiling) (void)hipEventRecord(ev[3 * i + 2], c->st);
-    }
-    hipLaunchKernelGGL(k_compact, dim3((c->g.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, c->st, c->g, c->d);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (c->profiling) {
-      for (long long i = 0; i < todo; i++)
-        for (int k = 0; k < kDevKernels; k++) {
-          float ms = 0;
-          if (hipEventElapsedTime(&ms, ev[3 * i + k], ev[3 * i + k + 1]) == hipSuccess) {
-            c->ksum_us[k] += 1000.0 * ms;
-            c->kcount[k] += 1;
-          }
-        }
-    }
-    // moves proposed after a cell overflow were skipped on the device; they
-    // are redone after the cell list is rebuilt with more room
-    issued = ((long long)c->h_ctl->move_counter - m0) - (c->h_ctl->skipped - sk0);
-    if (c->h_ctl->overflow) {
-      flush_sk(c);
-      int zero = 0;
-      HIPCHK(hipMemcpyAsync(&c->d.ctl->overflow, &zero, sizeof(int), hipMemcpyHostToDevice, c->st));
-      CHK(build_cells(c, 2 * c->g.cap));
-      if (++guard > 16) return fail(PLUM_E_CAPACITY, "cell list keeps overflowing");
-    }
-    since_resync += todo;
-    if (mcp->resync_every > 0 && since_resync >= mcp->resync_every && issued < n_moves) {
-      flush_sk(c);
-      CHK(build_cells(c, c->g.cap));
-      double f[8];
-      CHK(run_full(c, 1, f));
-      since_resync = 0;
-    }
-  }
-  flush_sk(c);
-  hipLaunchKernelGGL(k_compact, dim3((c->g.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, c->st, c->g, c->d);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  const auto t1 = std::chrono::steady_clock::now();
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
-    for (int k = 0; k < 5; k++) {
-      stats->attempted[k] = c->h_ctl->attempted[k] - att0[k];
-      stats->accepted[k] = c->h_ctl->accepted[k] - acc0[k];
-    }
-    stats->moves = ((long long)c->h_ctl->move_counter - m0) - (c->h_ctl->skipped - sk0);
-    stats->seconds = std::chrono::duration<double>(t1 - t0).count();
-    to_terms(c->h_ctl->running, &stats->totals);
-  }
-  return PLUM_OK;
-}
 
 #include "batch_api.h"

@@ -17,6 +17,38 @@ struct MoveRec {
   int pad;
 };
 
+constexpr int kMaxProp = 8;                                   // proposals per speculative batch
+constexpr int kMaxPairs = kMaxProp * (kMaxProp - 1) / 2;       // proposal pairs
+constexpr int kKVals = kMaxProp + kMaxPairs;                   // k-space sums per batch
+
+// Pair slot of proposals p < q.
+__host__ __device__ __forceinline__ int pair_index(int p, int q) { return q * (q - 1) / 2 + p; }
+
+// One proposal of a speculative batch (written by the proposal kernel).
+struct PropInfo {
+  double bond_trial;   // spring energy of the molecule at its trial coordinates
+  double ew;           // wall dE of the moved beads
+  double dmz;          // sum q (z' - z) of the moved beads
+  double c[3], rad;    // bounding sphere of the moved beads, old and new
+  int forbid;          // a moved bead entered a wall (1e8)
+  int n, mol, type;    // moved beads, molecule, move type
+  int valid, skipped;  // attempted move / dropped after a cell overflow
+};
+
+// Plan and reduced sums of a speculative batch.
+struct SpecPlan {
+  int n_prop;                    // proposals in the batch
+  int n_near;                    // proposal pairs within interaction range
+  int pair_p[kMaxPairs], pair_q[kMaxPairs];
+  int pair_off[kMaxPairs + 1];   // cross-term blocks per near pair (prefix)
+  int prop_off[kMaxProp + 1];    // real-space + moved-moved blocks per proposal (prefix)
+  unsigned acc_mask;             // accepted by the last resolve: S(k) += dS_p pending
+  int pad;
+  double ksum[kKVals];           // k space: dE per proposal, then 2 sum c_k Re(dS_p* dS_q)
+  double2 rsum[kMaxProp];        // (LJ, real) dE per proposal against the unmoved state
+  double2 xsum[kMaxPairs];       // (LJ, real) cross term per near pair
+};
+
 // Control block: scalars of the energy state and the run.
 struct Ctl {
   double Mz;            // sum q z over current coordinates
@@ -77,6 +109,15 @@ struct Dev {
   double* fpart;
   // device Monte Carlo
   int *chain_ids, *ion_ids;
+  // speculative batches: kMaxProp proposals, proposal p's moved beads at
+  // p * maxm in mv_old / mv_new / mv_type / smv_idx, its phase tables at
+  // p * maxm * 2 * tab_w, its dS(k) at p * n_kh
+  MoveRec* recs;
+  int* smv_idx;
+  PropInfo* prop;
+  SpecPlan* plan;
+  double* part_ks;        // [kKVals][k blocks]
+  double2* part_x;        // cross-term block partials
 };
 
 // Parameters of the device-resident loop, by value.
@@ -85,6 +126,7 @@ struct McDev {
   double move_size, bond_len;
   int vary_bond, n_chains, n_ions;
   unsigned long long seed;
+  unsigned long long ctr_limit;   // proposals stop at this move number
 };
 
 }  // namespace plum

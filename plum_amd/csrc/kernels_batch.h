@@ -1,9 +1,28 @@
-// Batched replicas: R independent systems advance together, every launch
-// carrying one trial move of each.  A k_compute_batch launch spreads the
-// virtual blocks of all replicas (k space, real-space queries, moved-moved
-// pairs; the same block functions as the single-system kernel) over a fixed
-// grid; k_serial_batch runs one block per replica (epilogue, commit, next
-// proposal).  Kernel boundaries are the only synchronisation.
+// Device-resident Monte Carlo over R independent systems (R = 1 for a single
+// system), in speculative batches of up to kMaxProp trial moves per system.
+//
+// One batch = five launches (kernel boundaries order them; no grid-wide
+// fences, which on a multi-XCD part would write back and invalidate L2):
+//   k_propose_spec   grid (Pmax, R): proposal p of system r (move number
+//                    counter + p), its phase tables, spring / wall / dipole
+//                    parts and bounding sphere (proposals stop at the first
+//                    molecule already proposed).
+//   k_plan_spec      grid R: block counts; pairs of proposals within
+//                    interaction range get cross-term blocks.
+//   k_compute_spec   all systems' k-space blocks (dS_p(k) of every proposal,
+//                    dE_k per proposal and the cross sums 2 c_k Re(dS_p* dS_q)),
+//                    real-space query blocks and moved-moved blocks per
+//                    proposal, and cross blocks (LJ + real pairs between two
+//                    proposals' moved beads, old/new combinations), spread over
+//                    one grid.
+//   k_reduce_spec    one block per sum and system, fixed order.
+//   k_resolve_spec   grid R: the reference's sequential Metropolis chain over
+//                    the batch (simulation.cc:241-355), each dE corrected by
+//                    the cross terms of the proposals accepted before it, then
+//                    the commits.
+// Every proposal is evaluated against the state before the batch; accepting
+// p changes q's dE by exactly the cross terms, so the chain is the sequential
+// one (the random numbers depend only on the move number).
 #pragma once
 #include "kernels_move.h"
 
@@ -12,41 +31,29 @@ namespace plum {
 constexpr int kMaxReplicas = 1024;
 constexpr int kMaxSplit = 8;           // most blocks per real-space query
 constexpr int kDefaultSplitDiv = 384;  // candidate slots per query block
+constexpr int kKSpec = 16;             // k vectors per speculative k-space block
+constexpr int kKSpecGroups = kBlock / kKSpec;
 
-// Per-replica launch shape.
+// Per-system launch shape.
 struct RepCfg {
-  int nkb;      // k-space blocks (0 without Ewald)
+  int nkb_s;    // k-space blocks of kKSpec vectors (0 without Ewald)
   int nsplit;   // blocks per real-space query
   int ncell;    // cells
-  int pad;
+  int pmax;     // proposals per batch (1..kMaxProp)
+  int maxm;     // moved-list capacity per proposal (longest molecule)
+  int nmmb;     // moved-moved block capacity per proposal
+  int rstride;  // real-space partials per proposal
+  int xcap;     // cross-term partial capacity
 };
 
-// Moved-moved pair blocks: 256 pairs each, independent of the launch size, so
-// the partial sums come out the same in every launch shape.
+// Moved-moved pair blocks: 256 pairs each, independent of the launch size.
 __host__ __device__ __forceinline__ int nmm_blocks(int nm) {
   const int npair = nm * (nm - 1) / 2;
   return npair > 0 ? (npair + kBlock - 1) / kBlock : 1;
 }
 
-__device__ __forceinline__ int virtual_blocks(const RepCfg& rc, int nm) {
-  return rc.nkb + 2 * nm * rc.nsplit + nmm_blocks(nm);
-}
+__host__ __device__ __forceinline__ int cross_blocks(int np, int nq) { return (np * nq + kBlock - 1) / kBlock; }
 
-// Replica r owning virtual block v: first r with pre[r] > v (inclusive prefix).
-__device__ __forceinline__ int find_replica(const int* pre, int R, int v) {
-  int lo = 0, hi = R - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (pre[mid] > v) hi = mid;
-    else lo = mid + 1;
-  }
-  return lo;
-}
-
-// The per-replica records are read through the constant address space and
-// copied by value for each virtual block: the loads then stay scalar and
-// rematerialisable across the block functions' barriers (through a generic
-// pointer they turn into per-lane loads and double the register count).
 template <typename T>
 __device__ __forceinline__ T load_const(const T* p, int i) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -57,65 +64,658 @@ __device__ __forceinline__ T load_const(const T* p, int i) {
 #endif
 }
 
-template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_compute_batch(
-    const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
-    const int* __restrict__ vb_count, int R) {
-  __shared__ int s_pre[kMaxReplicas];
-  __shared__ union U {
-    KSmem k;
-    RSmem<P> r;
-  } sm;
-  for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];
-  __syncthreads();
-  lds_scan(s_pre, R);
-  const int total = s_pre[R - 1];
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
-    const int r = __builtin_amdgcn_readfirstlane(find_replica(s_pre, R, v));
-    const int b = __builtin_amdgcn_readfirstlane(v - (r ? s_pre[r - 1] : 0));
-    const Geo g = load_const(G, r);
-    const Dev d = load_const(D, r);
-    const RepCfg rc = load_const(RC, r);
-    if (b < rc.nkb) {
-      kspace_block(g, d, b, sm.k);
-    } else {
-      const int nm = d.rec->n_moved;
-      real_block<P>(g, d, b - rc.nkb, 2 * nm * rc.nsplit, nmm_blocks(nm), rc.nsplit, sm.r);
-    }
-    __syncthreads();  // the LDS union is reused by the next virtual block
+__device__ __forceinline__ int uload(const int* p) { return __builtin_amdgcn_readfirstlane(*p); }
+
+// System r owning virtual block v: first r with pre[r] > v (inclusive prefix).
+__device__ __forceinline__ int find_replica(const int* pre, int R, int v) {
+  int lo = 0, hi = R - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] > v) hi = mid;
+    else lo = mid + 1;
   }
+  return lo;
 }
 
-// One block per replica: epilogue + Metropolis + commit of the pending move
-// (do_epi), then the next proposal, its phase tables and its virtual-block
-// count (do_prop).
-__global__ __launch_bounds__(kBlock) void k_serial_batch(const Geo* __restrict__ G, const Dev* __restrict__ D,
+// View of proposal slot p: its move record, moved list, phase tables, dS(k)
+// and partial sums.
+__device__ __forceinline__ Dev prop_view(const Geo& g, const Dev& d, const RepCfg& rc, int p) {
+  Dev v = d;
+  v.rec = d.recs + p;
+  v.mv_idx = d.smv_idx + p * rc.maxm;
+  v.mv_old = d.mv_old + p * rc.maxm;
+  v.mv_new = d.mv_new + p * rc.maxm;
+  v.mv_type = d.mv_type + p * rc.maxm;
+  v.tab = d.tab + (size_t)p * rc.maxm * 2 * g.tab_w;
+  v.dS = d.dS + (size_t)p * g.n_kh;
+  v.part_r = d.part_r + (size_t)p * rc.rstride;
+  v.part_mm = d.part_mm + (size_t)p * rc.nmmb;
+  return v;
+}
+
+// Minimum-image distance of two points in the Ewald box (x, y periodic; z
+// periodic in the padded box as the real-space image sum is).
+__device__ __forceinline__ double center_dist(const double* a, const double* b, const Geo& g) {
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double dk = a[k] - b[k];
+    dk -= g.L[k] * rint(dk * g.inv_L[k]);
+    s += dk * dk;
+  }
+  return sqrt(s);
+}
+
+// ------------------------------------------------------------ proposals
+struct PropSmem {
+  PrSmem pr;
+  double sbuf[kBlock];
+  double sh[kBlock / kWave];
+  int s_forbid, s_cut;
+  MoveSel s_sel;
+  MoveSel s_sels[kMaxProp];
+};
+
+__global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                          const McDev* __restrict__ MC,
-                                                         const RepCfg* __restrict__ RC, int* vb_count, int R,
-                                                         int do_epi, int do_prop) {
-  __shared__ union U {
-    EpSmem e;
-    PrSmem p;
-  } sm;
+                                                         const RepCfg* __restrict__ RC, int* vb_count, int R) {
+  __shared__ PropSmem sm;
+  const int p = blockIdx.x, r = blockIdx.y;
+  if (r >= R) return;
+  const Geo& g = G[r];
+  const Dev& d = D[r];
+  const McDev& mc = MC[r];
+  const RepCfg& rc = RC[r];
+  Ctl* C = d.ctl;
+  SpecPlan* plan = d.plan;
+  const unsigned long long ctr0 = C->move_counter;
+  if (threadIdx.x < rc.pmax) sm.s_sels[threadIdx.x] = select_move(d, mc, ctr0 + threadIdx.x);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // proposals stop at the first molecule already in the batch (its
+    // coordinates would depend on an earlier decision) and at the run's end
+    int cut = rc.pmax;
+    for (int q = 1; q < rc.pmax && cut == rc.pmax; q++)
+      for (int q2 = 0; q2 < q; q2++)
+        if (sm.s_sels[q2].mol == sm.s_sels[q].mol) cut = q;
+    sm.s_sel = sm.s_sels[p];
+    if (mc.ctr_limit <= ctr0) cut = 0;
+    else if ((long long)(mc.ctr_limit - ctr0) < cut) cut = (int)(mc.ctr_limit - ctr0);
+    sm.s_cut = cut;
+  }
+  __syncthreads();
+  const int cut = sm.s_cut;
+  if (p < cut) {
+    const int overflow = C->overflow;
+    MoveSel sel = sm.s_sel;
+    const int was_valid = sel.valid;
+    if (overflow) sel.valid = 0;   // the cell list is incomplete: skip, redo later
+    const Dev v = prop_view(g, d, rc, p);
+    propose_block(g, v, mc, ctr0 + p, sel, p * rc.maxm, sm.pr);
+    __syncthreads();
+    if (g.use_ewald && g.n_kh > 0) tables_block(g, v);
+    // spring energy of the trial molecule, wall and dipole parts, bounding
+    // sphere: everything of the dE that involves this proposal alone
+    const int nm = v.rec->n_moved;
+    const int mol = v.rec->mol;
+    const int a = d.mol_start[mol], bend = d.mol_start[mol + 1];
+    double ebt = 0;
+    // (a valid move that moves no bead keeps the molecule's current energy)
+    if (v.rec->valid && g.use_bond && bend - a > 1) ebt = molecule_spring(g, d, mol, true, sm.sbuf);
+    if (threadIdx.x == 0) sm.s_forbid = 0;
+    __syncthreads();
+    double ew = 0, dmz = 0, cx = 0, cy = 0, cz = 0;
+    for (int k = threadIdx.x; k < nm; k += kBlock) {
+      const double4 pn = v.mv_new[k], po = v.mv_old[k];
+      if (g.use_ext) {
+        const WallType w = d.wall[v.mv_type[k]];
+        const double en = wall_energy(pn.z, w, g);
+        if (en >= kVeryLargeEnergy) atomicOr(&sm.s_forbid, 1);
+        ew += en - wall_energy(po.z, w, g);
+      }
+      dmz += pn.w * (pn.z - po.z);
+      cx += pn.x + po.x;
+      cy += pn.y + po.y;
+      cz += pn.z + po.z;
+    }
+    const double ewt = block_sum<kBlock>(ew, sm.sh);
+    const double dmzt = block_sum<kBlock>(dmz, sm.sh);
+    const double cxt = block_sum<kBlock>(cx, sm.sh);
+    const double cyt = block_sum<kBlock>(cy, sm.sh);
+    const double czt = block_sum<kBlock>(cz, sm.sh);
+    if (threadIdx.x == 0) {
+      const double inv = nm > 0 ? 1.0 / (2.0 * nm) : 0.0;
+      sm.sbuf[0] = cxt * inv;
+      sm.sbuf[1] = cyt * inv;
+      sm.sbuf[2] = czt * inv;
+    }
+    __syncthreads();
+    const double c0 = sm.sbuf[0], c1 = sm.sbuf[1], c2 = sm.sbuf[2];
+    __syncthreads();
+    double rad = 0;
+    for (int k = threadIdx.x; k < nm; k += kBlock) {
+      const double4 pn = v.mv_new[k], po = v.mv_old[k];
+      const double dn = (pn.x - c0) * (pn.x - c0) + (pn.y - c1) * (pn.y - c1) + (pn.z - c2) * (pn.z - c2);
+      const double dd = (po.x - c0) * (po.x - c0) + (po.y - c1) * (po.y - c1) + (po.z - c2) * (po.z - c2);
+      rad = fmax(rad, fmax(dn, dd));
+    }
+    sm.sbuf[threadIdx.x] = rad;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) sm.sbuf[threadIdx.x] = fmax(sm.sbuf[threadIdx.x], sm.sbuf[threadIdx.x + off]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      PropInfo& pi = d.prop[p];
+      pi.bond_trial = ebt;
+      pi.ew = ewt;
+      pi.dmz = dmzt;
+      pi.c[0] = c0;
+      pi.c[1] = c1;
+      pi.c[2] = c2;
+      pi.rad = sqrt(sm.sbuf[0]);
+      pi.forbid = sm.s_forbid;
+      pi.n = nm;
+      pi.mol = mol;
+      pi.type = v.rec->move_type;
+      pi.valid = v.rec->valid;
+      pi.skipped = (overflow && was_valid) ? 1 : 0;
+    }
+  }
+  if (p == 0 && threadIdx.x == 0) plan->n_prop = cut;
+}
+
+// Plan of a batch, one block per system: real-space + moved-moved blocks of
+// every proposal, the proposal pairs within interaction range (bounding
+// spheres closer than the search radius) and their cross-term blocks, and
+// the system's virtual-block count.
+struct PlanSmem {
+  PropInfo s_prop[kMaxProp];
+};
+
+__global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
+                            int* vb_count, int R) {
+  __shared__ PlanSmem sm;
   const int r = blockIdx.x;
   if (r >= R) return;
   const Geo& g = G[r];
   const Dev& d = D[r];
   const RepCfg& rc = RC[r];
-  if (do_epi) {
-    const int nm = d.rec->n_moved;
-    epilogue_block(g, d, rc.nkb, 2 * nm * rc.nsplit, nmm_blocks(nm), 1, MC[r], sm.e);
-  }
+  SpecPlan* plan = d.plan;
+  const int cut = plan->n_prop;
+  if (threadIdx.x < cut) sm.s_prop[threadIdx.x] = d.prop[threadIdx.x];
   __syncthreads();
-  if (do_prop) {
-    propose_block(g, d, MC[r], sm.p);
-    __syncthreads();
-    if (g.use_ewald && g.n_kh > 0) tables_block(g, d);
-    if (threadIdx.x == 0) vb_count[r] = virtual_blocks(rc, d.rec->n_moved);
+  if (threadIdx.x == 0) {
+    int off = 0;
+    plan->prop_off[0] = 0;
+    for (int q = 0; q < cut; q++) {
+      const int n = sm.s_prop[q].n;
+      off += n > 0 ? 2 * n * rc.nsplit + nmm_blocks(n) : 0;
+      plan->prop_off[q + 1] = off;
+    }
+    int nn = 0, xo = 0;
+    plan->pair_off[0] = 0;
+    for (int q = 1; q < cut; q++) {
+      const PropInfo& pq = sm.s_prop[q];
+      if (pq.n == 0) continue;
+      for (int q0 = 0; q0 < q; q0++) {
+        const PropInfo& pp = sm.s_prop[q0];
+        if (pp.n == 0) continue;
+        if (center_dist(pp.c, pq.c, g) > pp.rad + pq.rad + g.rs + 1e-9) continue;
+        plan->pair_p[nn] = q0;
+        plan->pair_q[nn] = q;
+        xo += cross_blocks(pp.n, pq.n);
+        nn++;
+        plan->pair_off[nn] = xo;
+      }
+    }
+    plan->n_near = nn;
+    vb_count[r] = cut > 0 ? rc.nkb_s + off + xo : 0;
   }
 }
 
-// Tombstone compaction of every replica's cells; cell_off is the exclusive
+// ------------------------------------------------------------ k space
+// 16 k vectors x 16 bead groups per block; every proposal's dS_p(k)
+// accumulates in registers, the groups meet in LDS, and the 16 threads of
+// group 0 form dE_p = c_k (2 Re(S* dS_p) + |dS_p|^2) and the cross sums.  The S(k) of the
+// proposals accepted by the previous batch is folded in first.
+struct KSpecSmem {
+  double2 s_ds[kMaxProp][kKSpecGroups][kKSpec];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;  // valid in lane 0
+}
+
+__device__ __forceinline__ double sum16(double v) {
+#pragma unroll
+  for (int off = kKSpec / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kKSpec);
+  return v;  // valid in lane 0 of each 16-lane group
+}
+
+__device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, int np, unsigned acc_mask, int blk,
+                                  KSpecSmem& sm) {
+  const int kk = threadIdx.x % kKSpec, grp = threadIdx.x / kKSpec;
+  const int k = blk * kKSpec + kk;
+  int ox = 0, oy = 0, oz = 0;
+  if (k < g.n_kh) {
+    const int4 l = d.kl[k];
+    ox = l.x;
+    oy = g.nkx + 1 + l.y + g.nky;
+    oz = g.nkx + 1 + 2 * g.nky + 1 + l.z + g.nkz;
+  }
+#pragma unroll
+  for (int p = 0; p < kMaxProp; p++) {
+    if (p < np) {
+      double2 acc = make_double2(0, 0);
+      const int n = __builtin_amdgcn_readfirstlane(d.recs[p].n_moved);
+      if (k < g.n_kh) {
+        const double4* mo = d.mv_old + p * rc.maxm;
+        const double2* base = d.tab + (size_t)p * rc.maxm * 2 * g.tab_w;
+        for (int m = grp; m < n; m += kKSpecGroups) {
+          const double qm = mo[m].w;
+          const double2* to = base + (size_t)(2 * m) * g.tab_w;
+          const double2* tn = to + g.tab_w;
+          const double2 eo = cmul(cmul(to[ox], to[oy]), to[oz]);
+          const double2 en = cmul(cmul(tn[ox], tn[oy]), tn[oz]);
+          acc.x += qm * (en.x - eo.x);
+          acc.y += qm * (en.y - eo.y);
+        }
+      }
+      sm.s_ds[p][grp][kk] = acc;
+    }
+  }
+  __syncthreads();
+  if (grp != 0) return;
+  double2 t[kMaxProp];
+#pragma unroll
+  for (int p = 0; p < kMaxProp; p++) t[p] = make_double2(0, 0);
+  double2 S = make_double2(0, 0);
+  double ck = 0;
+  if (k < g.n_kh) {
+    S = d.Sk[k];
+    if (acc_mask) {
+#pragma unroll
+      for (int p = 0; p < kMaxProp; p++)
+        if (acc_mask & (1u << p)) {
+          const double2 prev = d.dS[(size_t)p * g.n_kh + k];
+          S.x += prev.x;
+          S.y += prev.y;
+        }
+      d.Sk[k] = S;
+    }
+    ck = d.ck[k];
+#pragma unroll
+    for (int p = 0; p < kMaxProp; p++) {
+      if (p < np) {
+#pragma unroll
+        for (int r = 0; r < kKSpecGroups; r++) {
+          t[p].x += sm.s_ds[p][r][kk].x;
+          t[p].y += sm.s_ds[p][r][kk].y;
+        }
+        d.dS[(size_t)p * g.n_kh + k] = t[p];
+      }
+    }
+  }
+  // sums over the block's 16 k vectors (t = 0 past n_kh)
+  const int nkb = rc.nkb_s;
+#pragma unroll
+  for (int p = 0; p < kMaxProp; p++) {
+    if (p >= np) break;
+    const double e = ck * (2.0 * (S.x * t[p].x + S.y * t[p].y) + t[p].x * t[p].x + t[p].y * t[p].y);
+    const double sv = sum16(e);
+    if (threadIdx.x == 0) d.part_ks[(size_t)p * nkb + blk] = sv;
+  }
+#pragma unroll
+  for (int q = 1; q < kMaxProp; q++) {
+    if (q >= np) break;
+#pragma unroll
+    for (int p = 0; p < q; p++) {
+      const double x = 2.0 * ck * (t[p].x * t[q].x + t[p].y * t[q].y);
+      const double sv = sum16(x);
+      if (threadIdx.x == 0) d.part_ks[(size_t)(kMaxProp + pair_index(p, q)) * nkb + blk] = sv;
+    }
+  }
+}
+
+// Cross block: LJ + real pairs between proposal q's and proposal p's moved
+// beads, u(i',j') - u(i,j') - u(i',j) + u(i,j): the change of q's dE when p
+// has been accepted (same pair functions as the moved-moved pairs).
+__device__ void cross_block(const Geo& g, const Dev& d, const RepCfg& rc, int p, int q, int np_, int nq, int local,
+                            double* sh, double2* out) {
+  const int t = local * kBlock + threadIdx.x;
+  double elj = 0, ere = 0;
+  if (t < np_ * nq) {
+    const int i = t / np_, j = t - (t / np_) * np_;
+    const int si = q * rc.maxm + i, sj = p * rc.maxm + j;
+    const double4 a0 = d.mv_old[si], a1 = d.mv_new[si], b0 = d.mv_old[sj], b1 = d.mv_new[sj];
+    const double ao[3] = {a0.x, a0.y, a0.z}, an[3] = {a1.x, a1.y, a1.z};
+    const double bo[3] = {b0.x, b0.y, b0.z}, bn[3] = {b1.x, b1.y, b1.z};
+    if (g.use_pair) {
+      const LJPair lp = d.lj[d.mv_type[si] * g.n_types + d.mv_type[sj]];
+      elj = lj_energy(bb_dist(an, bn, g.Lsim, g.npbc), lp) - lj_energy(bb_dist(ao, bn, g.Lsim, g.npbc), lp) -
+            lj_energy(bb_dist(an, bo, g.Lsim, g.npbc), lp) + lj_energy(bb_dist(ao, bo, g.Lsim, g.npbc), lp);
+    }
+    if (g.use_ewald && a0.w * b0.w != 0)
+      ere = real_pair_ref(an, a1.w, bn, b1.w, g) - real_pair_ref(ao, a0.w, bn, b1.w, g) -
+            real_pair_ref(an, a1.w, bo, b0.w, g) + real_pair_ref(ao, a0.w, bo, b0.w, g);
+  }
+  const double a = block_sum<kBlock>(elj, sh);
+  const double c = block_sum<kBlock>(ere, sh);
+  if (threadIdx.x == 0) *out = make_double2(a, c);
+}
+
+struct CSpecSmem {
+  double sh[kBlock / kWave];
+};
+
+template <int P>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_compute_spec(
+    const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
+    const int* __restrict__ vb_count, int R) {
+  __shared__ int s_pre[kMaxReplicas];
+  __shared__ union U {
+    KSpecSmem k;
+    RSmem<P> r;
+  } sm;
+  __shared__ CSpecSmem cs;
+  for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];
+  __syncthreads();
+  lds_scan(s_pre, R);
+  const int total = s_pre[R - 1];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int r = __builtin_amdgcn_readfirstlane(find_replica(s_pre, R, vb));
+    int b = __builtin_amdgcn_readfirstlane(vb - (r ? s_pre[r - 1] : 0));
+    const Geo g = load_const(G, r);
+    const Dev d = load_const(D, r);
+    const RepCfg rc = load_const(RC, r);
+    SpecPlan* plan = d.plan;
+    const int np = uload(&plan->n_prop);
+    if (b < rc.nkb_s) {
+      kspace_spec_block(g, d, rc, np, (unsigned)uload((const int*)&plan->acc_mask), b, sm.k);
+    } else {
+      b -= rc.nkb_s;
+      const int nprop_blocks = uload(&plan->prop_off[np]);
+      if (b < nprop_blocks) {
+        int p = 0;
+        while (p + 1 < np && uload(&plan->prop_off[p + 1]) <= b) p++;
+        const int local = b - uload(&plan->prop_off[p]);
+        const Dev v = prop_view(g, d, rc, p);
+        const int nm = uload(&v.rec->n_moved);
+        real_block<P>(g, v, local, 2 * nm * rc.nsplit, nmm_blocks(nm), rc.nsplit, sm.r);
+      } else {
+        b -= nprop_blocks;
+        const int nn = uload(&plan->n_near);
+        int j = 0;
+        while (j + 1 < nn && uload(&plan->pair_off[j + 1]) <= b) j++;
+        const int off = uload(&plan->pair_off[j]);
+        const int p = uload(&plan->pair_p[j]), q = uload(&plan->pair_q[j]);
+        const int np_ = uload(&d.recs[p].n_moved), nq = uload(&d.recs[q].n_moved);
+        cross_block(g, d, rc, p, q, np_, nq, b - off, cs.sh, d.part_x + b);
+      }
+    }
+    __syncthreads();  // the LDS union is reused by the next virtual block
+  }
+}
+
+// Sums of a batch, one block per (sum, system), fixed order: blockIdx.x in
+// [0, kKVals) k-space sums, then kMaxProp (LJ, real) sums per proposal
+// (real-space query blocks + moved-moved blocks), then kMaxPairs cross sums.
+__global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
+                                                        const RepCfg* __restrict__ RC, int R) {
+  __shared__ double sh[kBlock / kWave];
+  const int j = blockIdx.x, r = blockIdx.y;
+  if (r >= R) return;
+  const Dev& d = D[r];
+  const RepCfg& rc = RC[r];
+  SpecPlan* plan = d.plan;
+  const int np = plan->n_prop;
+  if (np == 0) return;
+  if (j < kKVals) {
+    if (j >= np && (j < kMaxProp || j >= kMaxProp + np * (np - 1) / 2)) return;
+    if (rc.nkb_s == 0) {
+      if (threadIdx.x == 0) plan->ksum[j] = 0;
+      return;
+    }
+    double s = 0;
+    for (int i = threadIdx.x; i < rc.nkb_s; i += kBlock) s += d.part_ks[(size_t)j * rc.nkb_s + i];
+    const double t = block_sum<kBlock>(s, sh);
+    if (threadIdx.x == 0) plan->ksum[j] = t;
+  } else if (j < kKVals + kMaxProp) {
+    const int p = j - kKVals;
+    if (p >= np) return;
+    const int nm = d.recs[p].n_moved;
+    if (nm == 0) return;
+    const int nreal = 2 * nm * rc.nsplit, nmm = nmm_blocks(nm);
+    const double2* pr = d.part_r + (size_t)p * rc.rstride;
+    const double2* pm = d.part_mm + (size_t)p * rc.nmmb;
+    double lj = 0, re = 0;
+    for (int i = threadIdx.x; i < nreal; i += kBlock) {
+      lj += pr[i].x;
+      re += pr[i].y;
+    }
+    for (int i = threadIdx.x; i < nmm; i += kBlock) {
+      lj += pm[i].x;
+      re += pm[i].y;
+    }
+    const double a = block_sum<kBlock>(lj, sh);
+    const double c = block_sum<kBlock>(re, sh);
+    if (threadIdx.x == 0) plan->rsum[p] = make_double2(a, c);
+  } else {
+    const int jj = j - kKVals - kMaxProp;
+    if (jj >= plan->n_near) return;
+    const int off = plan->pair_off[jj], nblk = plan->pair_off[jj + 1] - off;
+    double lj = 0, re = 0;
+    for (int i = threadIdx.x; i < nblk; i += kBlock) {
+      lj += d.part_x[off + i].x;
+      re += d.part_x[off + i].y;
+    }
+    const double a = block_sum<kBlock>(lj, sh);
+    const double c = block_sum<kBlock>(re, sh);
+    if (threadIdx.x == 0) plan->xsum[jj] = make_double2(a, c);
+  }
+}
+
+// ------------------------------------------------------------ resolve
+struct ResSmem {
+  int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
+  PropInfo s_prop[kMaxProp];
+  double2 s_rsum[kMaxProp];
+  double s_ksum[kKVals];
+  double2 s_xsum[kMaxPairs];
+  int s_near[kMaxPairs];
+  double s_bcur[kMaxProp];
+  int s_len[kMaxProp];
+  double s_u[kMaxProp];
+  int s_mask, s_np;
+};
+
+__global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
+                                                         const McDev* __restrict__ MC,
+                                                         const RepCfg* __restrict__ RC, int R) {
+  __shared__ ResSmem sm;
+  const int r = blockIdx.x;
+  if (r >= R) return;
+  const Geo& g = G[r];
+  const Dev& d = D[r];
+  const McDev& mc = MC[r];
+  const RepCfg& rc = RC[r];
+  Ctl* C = d.ctl;
+  SpecPlan* plan = d.plan;
+  const int np = plan->n_prop;
+  if (np == 0) return;
+  const unsigned long long ctr0 = C->move_counter;
+  // everything the sequential decisions read, fetched in parallel
+  const int t = threadIdx.x;
+  if (t < np) {
+    const PropInfo P = d.prop[t];
+    sm.s_prop[t] = P;
+    sm.s_rsum[t] = plan->rsum[t];
+    sm.s_bcur[t] = d.bond_cur[P.mol];
+    sm.s_len[t] = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
+    const unsigned long long ctr = ctr0 + t;
+    const uint4 u = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xACCu, 0u),
+                                make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
+    sm.s_u[t] = u01_32(u.x);
+  }
+  if (t >= 64 && t < 64 + kKVals) sm.s_ksum[t - 64] = plan->ksum[t - 64];
+  if (t >= 128 && t < 128 + kMaxPairs) {
+    const int j = t - 128;
+    sm.s_near[j] = -1;
+    sm.s_xsum[j] = plan->xsum[j];
+  }
+  __syncthreads();
+  if (t == 0) {
+    const int nn = plan->n_near;
+    for (int j = 0; j < nn; j++) sm.s_near[pair_index(plan->pair_p[j], plan->pair_q[j])] = j;
+  }
+  __syncthreads();
+  if (t == 0) {
+    unsigned mask = 0;
+    double Mz = C->Mz, cur_dipl = C->cur_dipl;
+    double run[8], last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; k++) run[k] = C->running[k];
+    long long att[5] = {0, 0, 0, 0, 0}, acc5[5] = {0, 0, 0, 0, 0};
+    long long skipped = 0, valid = 0;
+    double last_total = C->total;
+    for (int q = 0; q < np; q++) {
+      const PropInfo& P = sm.s_prop[q];
+      if (!P.valid) {
+        if (P.skipped) skipped++;
+        continue;
+      }
+      // a proposal that moves no bead (crankshaft between neighbours) has no
+      // real-space blocks: its pair terms are zero
+      double lj = P.n > 0 ? sm.s_rsum[q].x : 0.0, re = P.n > 0 ? sm.s_rsum[q].y : 0.0;
+      double kk = sm.s_ksum[q];
+      for (int p = 0; p < q; p++) {
+        if (!(mask & (1u << p))) continue;
+        const int pi = pair_index(p, q);
+        kk += sm.s_ksum[kMaxProp + pi];
+        const int j = sm.s_near[pi];
+        if (j >= 0) {
+          lj += sm.s_xsum[j].x;
+          re += sm.s_xsum[j].y;
+        }
+      }
+      // ForceField::EnergyDifference with its early exits (force_field.cc:407-434)
+      double tt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      double dE = 0;
+      bool early = false;
+      if (g.use_pair) {
+        tt[T_LJ] = lj;
+        dE += lj;
+        if (dE >= kVeryLargeEnergy) early = true;
+      }
+      if (!early && g.use_ext) {
+        tt[T_WALL] = P.forbid ? kVeryLargeEnergy : P.ew;
+        dE += tt[T_WALL];
+        if (dE >= kVeryLargeEnergy) early = true;
+      }
+      double trial_dipl = cur_dipl;
+      if (!early && g.use_ewald) {
+        tt[T_REAL] = re;
+        tt[T_KSPACE] = kk;
+        double dd = 0;
+        if (g.dipole) {
+          if (g.dipole_exact) {
+            const double mz1 = Mz + P.dmz;
+            trial_dipl = g.dip_coef * mz1 * mz1;
+            dd = trial_dipl - g.dip_coef * Mz * Mz;
+          } else {
+            // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
+            trial_dipl = g.dip_coef * Mz * Mz;
+            dd = trial_dipl - cur_dipl;
+          }
+        }
+        tt[T_DIPOLE] = dd;
+        dE += re + kk + dd;
+      }
+      const int len = sm.s_len[q];
+      if (!early && g.use_bond && len > 1) {
+        tt[T_BOND] = P.bond_trial - sm.s_bcur[q];
+        dE += tt[T_BOND];
+      }
+      // Metropolis (simulation.cc:327-332): no draw for a forbidden move
+      const int acc = (dE < kVeryLargeEnergy) && (sm.s_u[q] < exp(-g.beta * dE));
+      for (int k = 0; k < 8; k++) last[k] = tt[k];
+      last_total = dE;
+      if (P.type >= 0 && P.type < 5) {
+        att[P.type] += 1;
+        if (acc) acc5[P.type] += 1;
+      }
+      valid++;
+      if (acc) {
+        mask |= 1u << q;
+        Mz += P.dmz;
+        if (g.dipole) cur_dipl = trial_dipl;
+        for (int k = 0; k < 8; k++) run[k] += tt[k];
+        if (g.use_bond && len > 1) d.bond_cur[P.mol] = P.bond_trial;
+      }
+    }
+    C->Mz = Mz;
+    C->cur_dipl = cur_dipl;
+    C->trial_dipl = cur_dipl;
+    for (int k = 0; k < 8; k++) {
+      C->running[k] = run[k];
+      C->terms[k] = last[k];
+    }
+    C->total = last_total;
+    for (int k = 0; k < 5; k++) {
+      C->attempted[k] += att[k];
+      C->accepted[k] += acc5[k];
+    }
+    C->skipped += skipped;
+    C->valid_moves += valid;
+    C->move_counter = ctr0 + np;
+    plan->acc_mask = mask;
+    sm.s_mask = mask;
+  }
+  __syncthreads();
+  const unsigned mask = sm.s_mask;
+  for (int q = 0; q < np; q++) {
+    if (!(mask & (1u << q))) continue;
+    commit_cells(g, prop_view(g, d, rc, q), sm.s_nc, sm.s_flag);
+  }
+  for (int q = 0; q < np; q++) {
+    const int n = sm.s_prop[q].n;
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+      const int i = d.smv_idx[q * rc.maxm + k];
+      d.moved[i] = 0;
+      d.mv_slot[i] = -1;
+    }
+  }
+}
+
+// Folds the pending accepted dS_p(k) into S(k) (end of a run); one block row
+// per system, grid-stride over its k vectors.
+__global__ void k_flush_spec(const Geo* __restrict__ G, const Dev* __restrict__ D, int R) {
+  const int r = blockIdx.y;
+  if (r >= R) return;
+  const Geo& g = G[r];
+  const Dev& d = D[r];
+  const unsigned mask = d.plan->acc_mask;
+  if (!mask) return;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g.n_kh; k += gridDim.x * blockDim.x) {
+    double2 s = d.Sk[k];
+    for (int p = 0; p < kMaxProp; p++)
+      if (mask & (1u << p)) {
+        const double2 ds = d.dS[(size_t)p * g.n_kh + k];
+        s.x += ds.x;
+        s.y += ds.y;
+      }
+    d.Sk[k] = s;
+  }
+}
+
+__global__ void k_clear_mask(const Dev* __restrict__ D, int R) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) D[r].plan->acc_mask = 0;
+}
+
+// Tombstone compaction of every system's cells; cell_off is the exclusive
 // prefix of the cell counts (R + 1 entries).
 __global__ void k_compact_batch(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                 const int* __restrict__ cell_off, int R) {
@@ -127,39 +727,16 @@ __global__ void k_compact_batch(const Geo* __restrict__ G, const Dev* __restrict
     if (cell_off[mid] <= gid) lo = mid;
     else hi = mid - 1;
   }
-  const int r = lo;
-  const Geo& g = G[r];
-  const Dev& d = D[r];
-  compact_cell(g, d, gid - cell_off[r]);
+  compact_cell(G[lo], D[lo], gid - cell_off[lo]);
 }
 
-// Overflow flags of all replicas into one array (read by the host between
-// graph replays).
-__global__ void k_batch_flags(const Dev* __restrict__ D, int R, int* out) {
+// Per system: overflow flag and moves still to propose (read by the host
+// between graph replays).
+__global__ void k_batch_status(const Dev* __restrict__ D, const McDev* __restrict__ MC, int R, int2* out) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < R) out[r] = D[r].ctl->overflow;
-}
-
-// Pending S(k) += dS(k) of every replica (end of a run); one block row per
-// replica, grid-stride over its k vectors.
-__global__ void k_flush_sk_batch(const Geo* __restrict__ G, const Dev* __restrict__ D, int R) {
-  const int r = blockIdx.y;
   if (r >= R) return;
-  const Geo& g = G[r];
-  const Dev& d = D[r];
-  if (!d.ctl->sk_pending) return;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g.n_kh; k += gridDim.x * blockDim.x) {
-    double2 s = d.Sk[k];
-    const double2 ds = d.dS[k];
-    s.x += ds.x;
-    s.y += ds.y;
-    d.Sk[k] = s;
-  }
-}
-
-__global__ void k_clear_sk_pending_batch(const Dev* __restrict__ D, int R) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < R) D[r].ctl->sk_pending = 0;
+  const unsigned long long c = D[r].ctl->move_counter, lim = MC[r].ctr_limit;
+  out[r] = make_int2(D[r].ctl->overflow, c < lim ? (int)min(lim - c, 1ull << 30) : 0);
 }
 
 }  // namespace plum

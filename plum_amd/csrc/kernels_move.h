@@ -405,11 +405,10 @@ __device__ void compact_cell(const Geo& g, const Dev& d, int cell) {
 // order), dipole moment, spring cache, dipole lag state, running totals.
 // Runs inside one block.  S(k) is left to k_commit_sk / the next k-space
 // pass (sk_pending).
-__device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int* s_flag) {
+__device__ void commit_cells(const Geo& g, const Dev& d, int* s_nc, int* s_flag) {
   Ctl* C = d.ctl;
   const int nm = d.rec->n_moved;
-  const int mol = d.rec->mol;
-  if (acc) {
+  {
     // new cell of every moved bead
     for (int k = threadIdx.x; k < nm; k += kBlock) {
       const double4 pn = d.mv_new[k];
@@ -484,7 +483,14 @@ __device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int
     for (int k = threadIdx.x; k < nm; k += kBlock)
       if (s_flag[k] < -1) d.ccount[s_nc[k]] = -s_flag[k] - 1;
   }
-  // dipole moment change, in moved-list order
+  __syncthreads();
+}
+
+__device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int* s_flag) {
+  Ctl* C = d.ctl;
+  const int nm = d.rec->n_moved;
+  const int mol = d.rec->mol;
+  if (acc) commit_cells(g, d, s_nc, s_flag);
   if (threadIdx.x == 0) {
     if (acc) {
       C->Mz += C->dMz;
@@ -515,24 +521,18 @@ __device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int
 
 // Reduction of all partials, bonded / wall / dipole terms and the
 // ForceField::EnergyDifference combination with its early exits
-// (force_field.cc:407-434).  mode 0: store dE only (host path); mode 1: also
-// take the Metropolis decision (simulation.cc:327-332) from the Philox stream
-// and commit the move (device loop).
+// (force_field.cc:407-434) for the caller's move (host path); the caller
+// decides and calls k_commit.
 struct EpSmem {
   double sh[kBlock / kWave];
   double sbuf[kBlock];
-  int s_forbid, s_acc;
-  int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
+  int s_forbid;
 };
 
-__device__ void epilogue_block(const Geo& g, const Dev& d, int nkb, int nrb, int nmmb, int mode,
-                               const McDev& mc, EpSmem& sm) {
+__device__ void epilogue_block(const Geo& g, const Dev& d, int nkb, int nrb, int nmmb, EpSmem& sm) {
   double* sh = sm.sh;
   double* sbuf = sm.sbuf;
   int& s_forbid = sm.s_forbid;
-  int& s_acc = sm.s_acc;
-  int* s_nc = sm.s_nc;
-  int* s_flag = sm.s_flag;
   Ctl* C = d.ctl;
   const int nm = d.rec->n_moved;
   const int mol = d.rec->mol;
@@ -605,28 +605,14 @@ __device__ void epilogue_block(const Geo& g, const Dev& d, int nkb, int nrb, int
     C->total = dE;
     C->pending = 1;
     C->sk_pending = 0;
-    int acc = 0;
-    if (mode == 1 && valid && dE < kVeryLargeEnergy) {
-      const unsigned long long ctr = C->move_counter;
-      const uint4 r = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xACCu, 0u),
-                                  make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
-      acc = u01_32(r.x) < exp(-g.beta * dE);
-    }
-    C->accept = acc;
-    s_acc = acc;
+    C->accept = 0;
   }
   __syncthreads();
-  if (mode == 1) {
-    const int acc = s_acc;
-    commit_block(g, d, acc, s_nc, s_flag);
-    if (threadIdx.x == 0) C->sk_pending = acc && g.use_ewald && g.n_kh > 0;
-  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int nrb, int nmmb, int mode,
-                                                     McDev mc) {
+__global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int nrb, int nmmb) {
   __shared__ EpSmem sm;
-  epilogue_block(g, d, nkb, nrb, nmmb, mode, mc, sm);
+  epilogue_block(g, d, nkb, nrb, nmmb, sm);
 }
 
 // Host path: commit / roll back after the caller's decision.
@@ -670,9 +656,39 @@ __device__ void sphere(const McDev& mc, unsigned long long ctr, uint32_t stream,
   v[2] = 1 - 2 * s;
 }
 
+// Simulation::TranslationalMove's choice of move type and molecule
+// (simulation.cc:241-266) from Philox stream 0 of move `ctr`.
+struct MoveSel {
+  int mol, mt, valid;
+};
+
+__device__ MoveSel select_move(const Dev& d, const McDev& mc, unsigned long long ctr) {
+  MoveSel s;
+  const uint4 r0 = rnd(mc, ctr, 0, 0);
+  int which_chain = (int)floor(u01_32(r0.x) * mc.n_chains);
+  int which_ion = (int)floor(u01_32(r0.y) * mc.n_ions);
+  if (which_chain == mc.n_chains) which_chain--;
+  if (which_ion == mc.n_ions) which_ion--;
+  int mt = 0;
+  const double rn = u01_32(r0.z);
+  double cum = mc.cum_prob[0];
+  while (cum < rn && mt < 4) { mt++; cum += mc.cum_prob[mt]; }
+  int mol = -1, valid = 1;
+  if (mt == 0 && mc.n_ions > 0) mol = d.ion_ids[which_ion];
+  else if (mt > 0 && mc.n_chains > 0) mol = d.chain_ids[which_chain];
+  else valid = 0;
+  if (mol < 0) { mol = 0; valid = 0; }
+  s.mol = mol;
+  s.mt = mt;
+  s.valid = valid;
+  return s;
+}
+
 // One proposal of Simulation::TranslationalMove (simulation.cc:241-312) with
 // the move generators of molecule.cc:103-312, drawing from Philox4x32-10
-// (counter = move number, streams: 0 selection, 1 move parameters, 2+i bead i).
+// (counter = move number, streams: 1 move parameters, 2+i bead i).  `d` is
+// the view of one proposal slot (its rec / mv_* arrays); moved-list indices
+// are stored in mv_slot as slot_base + k.
 struct PrSmem {
   double s_b[kMaxChainDev][3];
   int s_mol, s_type, s_valid, s_a, s_len;
@@ -680,7 +696,8 @@ struct PrSmem {
   int s_ipar[3];
 };
 
-__device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, PrSmem& sm) {
+__device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, unsigned long long ctr, MoveSel sel,
+                              int slot_base, PrSmem& sm) {
   double (&s_b)[kMaxChainDev][3] = sm.s_b;
   int& s_mol = sm.s_mol;
   int& s_type = sm.s_type;
@@ -689,34 +706,13 @@ __device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, PrSme
   int& s_len = sm.s_len;
   double* s_par = sm.s_par;
   int* s_ipar = sm.s_ipar;
-  Ctl* C = d.ctl;
-  const unsigned long long ctr = C->move_counter;
   if (threadIdx.x == 0) {
-    int valid = 1;
-    const uint4 r0 = rnd(mc, ctr, 0, 0);
     const uint4 r1 = rnd(mc, ctr, 1, 0);
-    int which_chain = (int)floor(u01_32(r0.x) * mc.n_chains);
-    int which_ion = (int)floor(u01_32(r0.y) * mc.n_ions);
-    if (which_chain == mc.n_chains) which_chain--;
-    if (which_ion == mc.n_ions) which_ion--;
-    int mt = 0;
-    const double rn = u01_32(r0.z);
-    double cum = mc.cum_prob[0];
-    while (cum < rn && mt < 4) { mt++; cum += mc.cum_prob[mt]; }
-    int mol = -1;
-    if (mt == 0 && mc.n_ions > 0) mol = d.ion_ids[which_ion];
-    else if (mt > 0 && mc.n_chains > 0) mol = d.chain_ids[which_chain];
-    else valid = 0;
-    if (mol < 0) { mol = 0; valid = 0; }
-    if (C->overflow) {
-      valid = 0;
-      C->skipped += 1;
-    }
-    s_mol = mol;
-    s_type = mt;
-    s_valid = valid;
-    s_a = d.mol_start[mol];
-    s_len = d.mol_start[mol + 1] - s_a;
+    s_mol = sel.mol;
+    s_type = sel.mt;
+    s_valid = sel.valid;
+    s_a = d.mol_start[sel.mol];
+    s_len = d.mol_start[sel.mol + 1] - s_a;
     s_par[0] = u01_32(r1.x);
     s_par[1] = u01_32(r1.y);
     s_par[2] = u01_32(r1.z);
@@ -856,7 +852,7 @@ __device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, PrSme
     const int i = (mt == 3) ? d.mv_idx[k] : a + k;
     if (mt != 3) d.mv_idx[k] = i;
     d.moved[i] = 1;
-    d.mv_slot[i] = k;
+    d.mv_slot[i] = slot_base + k;
     d.mv_old[k] = make_double4(d.x[i], d.y[i], d.z[i], d.q[i]);
     d.mv_type[k] = d.type[i];
   }
@@ -903,24 +899,6 @@ __device__ void tables_block(const Geo& g, const Dev& d) {
       t[off + l] = e;
       if (dim != 0) t[off - l] = conj2(e);
     }
-  }
-}
-
-// The serial part of the device loop, one block: dE epilogue, Metropolis
-// decision and commit of the pending move (do_epi), then the proposal of
-// the next move and its phase tables (do_prop).
-__global__ __launch_bounds__(kBlock) void k_serial(Geo g, Dev d, McDev mc, int nkb, int nrb, int nmmb,
-                                                   int do_epi, int do_prop) {
-  __shared__ union U {
-    EpSmem e;
-    PrSmem p;
-  } sm;
-  if (do_epi) epilogue_block(g, d, nkb, nrb, nmmb, 1, mc, sm.e);
-  __syncthreads();
-  if (do_prop) {
-    propose_block(g, d, mc, sm.p);
-    __syncthreads();
-    if (g.use_ewald && g.n_kh > 0) tables_block(g, d);
   }
 }
 

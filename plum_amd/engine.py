@@ -22,7 +22,8 @@ ABI_SYMBOLS = (
     "plum_energy_difference", "plum_finalize", "plum_totals", "plum_resync",
     "plum_get_positions", "plum_last_error", "plum_mc_run", "plum_set_profiling",
     "plum_kernel_times", "plum_batch_create", "plum_batch_destroy", "plum_batch_mc_run",
-    "plum_batch_tune", "plum_batch_get_tuning", "plum_batch_set_tuning",
+    "plum_batch_tune", "plum_batch_get_tuning", "plum_batch_set_tuning", "plum_mc_get_tuning",
+    "plum_mc_set_tuning",
 )
 
 
@@ -81,7 +82,7 @@ class MCStats(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int), ("split_div", C.c_int),
-                ("sweeps_per_graph", C.c_int), ("sweep_us", C.c_double)]
+                ("batches_per_graph", C.c_int), ("pmax", C.c_int), ("sweep_us", C.c_double)]
 
 
 class BatchStats(C.Structure):
@@ -125,6 +126,8 @@ def lib():
         L.plum_batch_tune.argtypes = [vp, C.POINTER(MCParams), C.c_int, C.POINTER(BatchTuning)]
         L.plum_batch_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
         L.plum_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+        L.plum_mc_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+        L.plum_mc_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
         for f in ABI_SYMBOLS:
             if f != "plum_last_error":
                 getattr(L, f).restype = C.c_int
@@ -270,6 +273,19 @@ class Engine:
     def set_profiling(self, on):
         _check(lib().plum_set_profiling(self.h, int(on)))
 
+    def tuning(self):
+        t = BatchTuning()
+        _check(lib().plum_mc_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        """Launch shape of mc_run: blocks_per_cu, split_div, batches_per_graph,
+        pmax (proposals per speculative batch)."""
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().plum_mc_set_tuning(self.h, C.byref(t)))
+
     def kernel_times(self, cap=32):
         names = (C.c_char_p * cap)()
         avg = np.zeros(cap)
@@ -329,12 +345,8 @@ class Batch:
         _check(lib().plum_batch_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, blocks_per_cu=None, split_div=None, sweeps_per_graph=None):
+    def set_tuning(self, **kw):
         t = self.tuning()
-        if blocks_per_cu is not None:
-            t.blocks_per_cu = blocks_per_cu
-        if split_div is not None:
-            t.split_div = split_div
-        if sweeps_per_graph is not None:
-            t.sweeps_per_graph = sweeps_per_graph
+        for k, v in kw.items():
+            setattr(t, k, v)
         _check(lib().plum_batch_set_tuning(self.h, C.byref(t)))
