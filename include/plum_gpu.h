@@ -222,6 +222,63 @@ int plum_batch_set_tuning(plum_batch* batch, const plum_batch_tuning* tuning);
 int plum_mc_get_tuning(plum_ctx* ctx, plum_batch_tuning* out);
 int plum_mc_set_tuning(plum_ctx* ctx, const plum_batch_tuning* tuning);
 
+/* ---- delayed acceptance with a learned surrogate ---------------------- */
+
+/* No reference counterpart (the reference evaluates every trial move
+   exactly).  With the surrogate on, plum_mc_run screens up to `screen`
+   proposals per batch with a network estimate dE_s (exact spring and wall
+   parts + a 64-128-128-128-1 tanh network over local radial-basis
+   descriptors, BF16 matrix-core inference); the first stage accepts with
+   min(1, exp(-beta dE_s)), and only its passes are evaluated exactly and
+   accepted with min(1, exp(-beta (dE - dE_s))).  The sampled ensemble is the
+   exact one; the network (FP32 master weights, Adam) is trained online on
+   the exact dE of evaluated proposals. */
+typedef struct {
+  int screen;           /* proposals screened per batch, 1..32 */
+  int fill;             /* exact slots (<= 8) also given to first-stage rejections, as training data */
+  int train_every;      /* batches between online training steps; 0 = frozen network */
+  int train_samples;    /* samples per training step (newest first) */
+  int ring;             /* capacity of the training-sample ring */
+  double lr;            /* Adam step size */
+  double r_d;           /* descriptor radius (sigma) */
+  double target_clip;   /* |target| clip of the training loss (kT) */
+  unsigned long long seed;  /* weight initialisation */
+} plum_surrogate_params;
+
+typedef struct {
+  long long s1_rejected;   /* first-stage rejections */
+  long long s1_passed;     /* first-stage passes (exact evaluations decided) */
+  long long train_evals;   /* first-stage rejections evaluated for training */
+  long long stale_cuts;    /* batches ended early by a stale surrogate value */
+  long long samples;       /* training samples collected */
+  long long train_steps;
+  double last_loss;        /* mean squared error of the last training step (kT^2) */
+} plum_surrogate_stats;
+
+#define PLUM_SURROGATE_PARAMS 41473   /* 64*128 + 128 + 2*(128*128 + 128) + 128 + 1 */
+
+int plum_surrogate_enable(plum_ctx* ctx, const plum_surrogate_params* params);
+int plum_surrogate_disable(plum_ctx* ctx);
+int plum_surrogate_stats_get(plum_ctx* ctx, plum_surrogate_stats* out);
+/* FP32 master weights: W1[64][128], b1, W2[128][128], b2, W3[128][128], b3,
+   w4[128], b4 (PLUM_SURROGATE_PARAMS floats). */
+int plum_surrogate_get_weights(plum_ctx* ctx, float* w, int n);
+int plum_surrogate_set_weights(plum_ctx* ctx, const float* w, int n);
+/* `steps` training steps on the newest samples; loss of the last one. */
+int plum_surrogate_train(plum_ctx* ctx, int steps, double* loss);
+/* Network outputs of `rows` descriptor rows (64 floats each): bf16 = 1 the
+   matrix-core inference kernel, 0 the FP32 training forward pass. */
+int plum_surrogate_forward(plum_ctx* ctx, const float* x, int rows, int bf16, float* f);
+/* Loss and parameter gradient (FP32 kernels) of `samples` samples: sample i
+   has rows [2 i maxm, 2 (i + 1) maxm) of x (its n[i] old beads first, its
+   n[i] new beads from row maxm on), target[i]. */
+int plum_surrogate_gradient(plum_ctx* ctx, const float* x, const double* target, const int* n, int samples,
+                            double* loss, float* grad);
+/* Descriptor rows (2 maxm x 64), old/new positions (2 maxm x 4) and moved
+   count of proposal slot `slot` from the last batch (for checks). */
+int plum_surrogate_debug_slot(plum_ctx* ctx, int slot, float* rows, double* pos, int* n);
+int plum_max_mol_len(const plum_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,11 +31,44 @@ struct plum_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int g_batches = 0, g_grid = 0, g_pmax = 0;  // shape the graph was captured with
+  bool da = false;        // delayed acceptance (one system with its surrogate on)
   bool profiling = false;
   std::vector<void*> allocs;
 };
 
+// Surrogate state of one context (delayed acceptance), defined with its entry
+// points in surrogate_api.h.
+struct SurHost {
+  plum_surrogate_params p{};
+  bool on = false;
+  NetBF16* net = nullptr;
+  float *w32 = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+  float *sur_x = nullptr, *sur_f = nullptr;
+  float* tr_x = nullptr;
+  double* tr_target = nullptr;
+  int *tr_n = nullptr, *tr_ctl = nullptr;
+  int* h_ctl = nullptr;   // pinned copy of tr_ctl
+  int tb_cap = 0;         // minibatch capacity (samples)
+  float *X = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *F = nullptr, *G = nullptr;
+  float *D3 = nullptr, *D2 = nullptr, *D1 = nullptr;
+  double *T = nullptr, *loss = nullptr;
+  int* N = nullptr;
+  int step = 0;
+  long long train_steps = 0;
+  double last_loss = 0;
+  std::vector<void*> allocs;
+};
+
+void sur_free(SurHost* s) {
+  if (!s) return;
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+  delete s;
+}
+
 namespace {
+
+int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool update);
 
 constexpr int kSpecKernels = 5;
 const char* kSpecKernelNames[kSpecKernels] = {"k_propose_spec", "k_plan_spec", "k_compute_spec", "k_reduce_spec",
@@ -60,6 +93,7 @@ int batch_upload_state(plum_batch* b) {
   std::vector<Dev> D(b->R);
   std::vector<RepCfg> RCv(b->R);
   b->max_nkb_full = 1;
+  b->da = b->R == 1 && b->reps[0]->sur && b->reps[0]->sur->on;
   for (int r = 0; r < b->R; r++) {
     plum_ctx* c = b->reps[r];
     c->nsplit = split_for(c->cand, b->tune.split_div);
@@ -74,6 +108,10 @@ int batch_upload_state(plum_batch* b) {
     q.nmmb = c->nmmb;
     q.rstride = 2 * c->maxm * kMaxSplit;
     q.xcap = c->xcap;
+    q.da = (b->R == 1 && c->sur && c->sur->on) ? 1 : 0;
+    q.fill = q.da ? c->sur->p.fill : 0;
+    q.r_d = q.da ? (float)c->sur->p.r_d : 0.f;
+    q.pad = 0;
     b->max_nkb_full = std::max(b->max_nkb_full, c->nkb_full);
   }
   HIPCHK(hipMemcpyAsync(b->dG, G.data(), sizeof(Geo) * b->R, hipMemcpyHostToDevice, b->st));
@@ -91,9 +129,17 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
   const int R = b->R;
   if (ev) (void)hipEventRecord(ev[0], b->st);
   hipLaunchKernelGGL(k_propose_spec, dim3(b->tune.pmax, R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC,
-                     b->d_vb, R);
+                     R);
+  if (b->da) {
+    // surrogate: descriptors of every moved bead (old, new), network outputs
+    plum_ctx* c = b->reps[0];
+    hipLaunchKernelGGL(k_descr_spec, dim3(2 * c->maxm, b->tune.pmax), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC);
+    const int rows = b->tune.pmax * 2 * c->maxm;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(b->n_cu, (rows + 127) / 128))), dim3(kBlock), 0, b->st,
+                       c->sur->net, c->sur->sur_x, rows, c->sur->sur_f);
+  }
   if (ev) (void)hipEventRecord(ev[1], b->st);
-  hipLaunchKernelGGL(k_plan_spec, dim3(R), dim3(kWave), 0, b->st, b->dG, b->dD, b->dRC, b->d_vb, R);
+  hipLaunchKernelGGL(k_plan_spec, dim3(R), dim3(kWave), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, b->d_vb, R);
   if (ev) (void)hipEventRecord(ev[2], b->st);
   if (b->precision == PLUM_FP32_PAIR)
     hipLaunchKernelGGL(k_compute_spec<1>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
@@ -152,7 +198,8 @@ int batch_replay(plum_batch* b, int nb, bool use_graph, plum_ctx* prof) {
     HIPCHK(hipGetLastError());
     return PLUM_OK;
   }
-  if (!b->gexec || b->g_batches != nb || b->g_grid != batch_grid(b) || b->g_pmax != b->tune.pmax) {
+  const int key_pmax = b->tune.pmax + (b->da ? 1000 : 0);
+  if (!b->gexec || b->g_batches != nb || b->g_grid != batch_grid(b) || b->g_pmax != key_pmax) {
     batch_drop_graph(b);
     HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < nb; i++) launch_batch(b);
@@ -163,7 +210,7 @@ int batch_replay(plum_batch* b, int nb, bool use_graph, plum_ctx* prof) {
     HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     b->g_batches = nb;
     b->g_grid = batch_grid(b);
-    b->g_pmax = b->tune.pmax;
+    b->g_pmax = key_pmax;
   }
   HIPCHK(hipGraphLaunch(b->gexec, b->st));
   return PLUM_OK;
@@ -287,7 +334,15 @@ int batch_run(plum_batch* b, const plum_mc_params* mcp, long long n_moves, bool 
   for (;;) {
     CHK(batch_replay(b, b->tune.batches_per_graph, use_graph, prof));
     HIPCHK(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int2) * R, hipMemcpyDeviceToHost, b->st));
+    if (b->da) HIPCHK(hipMemcpyAsync(b->reps[0]->sur->h_ctl, b->reps[0]->sur->tr_ctl, 2 * sizeof(int),
+                                     hipMemcpyDeviceToHost, b->st));
     HIPCHK(hipStreamSynchronize(b->st));
+    if (b->da) {
+      // online training on the newest exact samples
+      SurHost* su = b->reps[0]->sur;
+      if (su->p.train_every > 0 && su->h_ctl[0] >= su->p.train_samples)
+        CHK(sur_train_step(b->reps[0], b->st, su->p.train_samples, true, true));
+    }
     since_resync += (long long)b->tune.batches_per_graph * b->tune.pmax;
     bool busy = false, fix = false;
     for (int r = 0; r < R; r++) {
@@ -365,7 +420,18 @@ int check_tuning(const plum_batch_tuning* t) {
   if (t->split_div < 16) return fail(PLUM_E_ARG, "split_div must be >= 16");
   if (t->batches_per_graph < 1 || t->batches_per_graph > 1024)
     return fail(PLUM_E_ARG, "batches_per_graph must be 1..1024");
-  if (t->pmax < 1 || t->pmax > kMaxProp) return fail(PLUM_E_ARG, "pmax must be 1..8");
+  if (t->pmax < 1 || t->pmax > kMaxScreen) return fail(PLUM_E_ARG, "pmax must be 1..32");
+  return PLUM_OK;
+}
+
+// Proposals per batch: at most the exact slots without the surrogate, at
+// most the allocated proposal slots with it.
+int check_pmax(const plum_batch* b, int pmax) {
+  for (plum_ctx* c : b->reps) {
+    const bool da = b->R == 1 && c->sur && c->sur->on;
+    if (pmax > (da ? c->screen_cap : kMaxProp))
+      return fail(PLUM_E_ARG, da ? "pmax exceeds the screened proposals of the surrogate" : "pmax must be 1..8");
+  }
   return PLUM_OK;
 }
 
@@ -409,6 +475,7 @@ int plum_mc_set_tuning(plum_ctx* c, const plum_batch_tuning* t) {
     plum_ctx* one[1] = {c};
     CHK(batch_new(one, 1, &c->self));
   }
+  CHK(check_pmax(c->self, t->pmax));
   const double us = c->self->tune.sweep_us;
   c->self->tune = *t;
   c->self->tune.sweep_us = us;
@@ -451,6 +518,7 @@ int plum_batch_get_tuning(const plum_batch* b, plum_batch_tuning* out) {
 int plum_batch_set_tuning(plum_batch* b, const plum_batch_tuning* t) {
   if (!b || !t) return fail(PLUM_E_ARG, "null argument");
   CHK(check_tuning(t));
+  CHK(check_pmax(b, t->pmax));
   const double us = b->tune.sweep_us;
   b->tune = *t;
   b->tune.sweep_us = us;

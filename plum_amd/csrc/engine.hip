@@ -17,18 +17,22 @@
 #include "kernels_full.h"
 #include "kernels_move.h"
 #include "plum_gpu.h"
+#include "surrogate.h"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
 using namespace plum;
 
 struct plum_batch;
+struct SurHost;
 extern "C" int plum_batch_destroy(plum_batch* batch);
+void sur_free(SurHost* s);
 
 struct plum_ctx {
   plum_params p{};
@@ -48,6 +52,8 @@ struct plum_ctx {
   int nkb_s = 0;         // k-space blocks of the speculative loop
   int xcap = 0;          // cross-term partial capacity
   plum_batch* self = nullptr;  // this system as a batch of one (device loop)
+  SurHost* sur = nullptr;      // delayed-acceptance surrogate (null when off)
+  int screen_cap = kMaxProp;   // proposal slots allocated
   double* epart_k = nullptr;
   std::vector<int> mol_start;
   std::vector<int> type_h;
@@ -457,8 +463,8 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   ALLOC(d.bond_cur, n_mols);
   ALLOC(d.mv_old, (size_t)kMaxProp * maxm); ALLOC(d.mv_new, (size_t)kMaxProp * maxm);
   ALLOC(d.mv_type, (size_t)kMaxProp * maxm);
-  ALLOC(d.recs, kMaxProp); ALLOC(d.smv_idx, (size_t)kMaxProp * maxm);
-  ALLOC(d.prop, kMaxProp); ALLOC(d.plan, 1);
+  ALLOC(d.recs, kMaxScreen); ALLOC(d.smv_idx, (size_t)kMaxProp * maxm);
+  ALLOC(d.prop, kMaxScreen); ALLOC(d.plan, 1);
   c->nkb_s = (g.use_ewald && g.n_kh > 0) ? (g.n_kh + kKSpec - 1) / kKSpec : 0;
   ALLOC(d.part_ks, (size_t)kKVals * std::max(1, c->nkb_s));
   c->xcap = kMaxPairs * cross_blocks(maxm, maxm);
@@ -512,7 +518,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
       hipMemsetAsync(d.mv_slot, 0xff, sizeof(int) * n, c->st) != hipSuccess ||
       hipMemsetAsync(d.ctl, 0, sizeof(Ctl), c->st) != hipSuccess ||
       hipMemsetAsync(d.plan, 0, sizeof(SpecPlan), c->st) != hipSuccess ||
-      hipMemsetAsync(d.recs, 0, sizeof(MoveRec) * kMaxProp, c->st) != hipSuccess ||
+      hipMemsetAsync(d.recs, 0, sizeof(MoveRec) * kMaxScreen, c->st) != hipSuccess ||
       hipMemsetAsync(c->d_in, 0, c->in_bytes, c->st) != hipSuccess)
     return bail(fail(PLUM_E_HIP, "hipMemsetAsync failed"));
   if ((rc = build_cells(c, 0)) != PLUM_OK) return bail(rc);
@@ -525,6 +531,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
 int plum_destroy(plum_ctx* c) {
   if (!c) return PLUM_OK;
   if (c->self) plum_batch_destroy(c->self);
+  if (c->sur) sur_free(c->sur);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void* p : c->dev_allocs) (void)hipFree(p);
   if (c->h_in) (void)hipHostFree(c->h_in);
@@ -688,3 +695,4 @@ int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, 
 }  // extern "C"
 
 #include "batch_api.h"
+#include "surrogate_api.h"

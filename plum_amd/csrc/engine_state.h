@@ -17,12 +17,17 @@ struct MoveRec {
   int pad;
 };
 
-constexpr int kMaxProp = 8;                                   // proposals per speculative batch
-constexpr int kMaxPairs = kMaxProp * (kMaxProp - 1) / 2;       // proposal pairs
+constexpr int kMaxProp = 8;                                   // exactly evaluated proposals per batch
+constexpr int kMaxScreen = 32;                                // screened proposals per batch
+constexpr int kMaxPairs = kMaxProp * (kMaxProp - 1) / 2;       // exact-slot pairs
 constexpr int kKVals = kMaxProp + kMaxPairs;                   // k-space sums per batch
+constexpr int kDescr = 64;                                     // surrogate descriptor length
 
-// Pair slot of proposals p < q.
-__host__ __device__ __forceinline__ int pair_index(int p, int q) { return q * (q - 1) / 2 + p; }
+// Pair slot of exact slots p != q.
+__host__ __device__ __forceinline__ int pair_index(int p, int q) {
+  const int a = p < q ? p : q, b = p < q ? q : p;
+  return b * (b - 1) / 2 + a;
+}
 
 // One proposal of a speculative batch (written by the proposal kernel).
 struct PropInfo {
@@ -35,17 +40,25 @@ struct PropInfo {
   int valid, skipped;  // attempted move / dropped after a cell overflow
 };
 
-// Plan and reduced sums of a speculative batch.
+// Plan and reduced sums of a speculative batch.  Proposals live in screen
+// slots (up to kMaxScreen); the ones evaluated exactly get exact slots (up
+// to kMaxProp), which carry dS(k), the pair partial sums and the cross terms.
 struct SpecPlan {
-  int n_prop;                    // proposals in the batch
-  int n_near;                    // proposal pairs within interaction range
+  int n_prop;                    // proposals made this batch
+  int n_decide;                  // proposals that can be decided (stops before a pass with no exact slot)
+  int n_exact;                   // exact slots in use
+  int n_near;                    // exact-slot pairs within interaction range
+  int exact_of[kMaxScreen];      // exact slot of a proposal, -1 if none
+  int screen_of[kMaxProp];       // proposal of an exact slot
+  int pass1[kMaxScreen];         // first stage of delayed acceptance passed (1 without it)
   int pair_p[kMaxPairs], pair_q[kMaxPairs];
   int pair_off[kMaxPairs + 1];   // cross-term blocks per near pair (prefix)
-  int prop_off[kMaxProp + 1];    // real-space + moved-moved blocks per proposal (prefix)
-  unsigned acc_mask;             // accepted by the last resolve: S(k) += dS_p pending
+  int prop_off[kMaxProp + 1];    // real-space + moved-moved blocks per exact slot (prefix)
+  unsigned acc_mask;             // exact slots accepted by the last resolve: S(k) += dS pending
   int pad;
-  double ksum[kKVals];           // k space: dE per proposal, then 2 sum c_k Re(dS_p* dS_q)
-  double2 rsum[kMaxProp];        // (LJ, real) dE per proposal against the unmoved state
+  double de_s[kMaxScreen];       // surrogate dE of each proposal (0 without it)
+  double ksum[kKVals];           // k space: dE per exact slot, then 2 sum c_k Re(dS_p* dS_q)
+  double2 rsum[kMaxProp];        // (LJ, real) dE per exact slot against the unmoved state
   double2 xsum[kMaxPairs];       // (LJ, real) cross term per near pair
 };
 
@@ -69,6 +82,11 @@ struct Ctl {
   long long accepted[5];
   long long valid_moves;
   long long skipped;    // device-loop moves skipped after a cell overflow
+  // delayed acceptance
+  long long s1_rejected;   // first-stage rejections (no exact evaluation needed)
+  long long s1_passed;     // first-stage passes (exact evaluation, second stage)
+  long long train_evals;   // first-stage rejections evaluated for training
+  long long stale_cuts;    // batches ended early by a stale surrogate value
 };
 
 struct Dev {
@@ -109,15 +127,24 @@ struct Dev {
   double* fpart;
   // device Monte Carlo
   int *chain_ids, *ion_ids;
-  // speculative batches: kMaxProp proposals, proposal p's moved beads at
-  // p * maxm in mv_old / mv_new / mv_type / smv_idx, its phase tables at
-  // p * maxm * 2 * tab_w, its dS(k) at p * n_kh
+  // speculative batches: proposal s's moved beads at s * maxm in mv_old /
+  // mv_new / mv_type / smv_idx, its phase tables at s * maxm * 2 * tab_w;
+  // exact slot e's dS(k) at e * n_kh
   MoveRec* recs;
   int* smv_idx;
   PropInfo* prop;
   SpecPlan* plan;
   double* part_ks;        // [kKVals][k blocks]
   double2* part_x;        // cross-term block partials
+  // delayed acceptance (null without the surrogate): descriptor rows and
+  // network outputs of every proposal (row (2 s + on) * maxm + m, on = 0 old
+  // / 1 new), and the ring of training samples (rows, target, bead count)
+  float* sur_x;
+  float* sur_f;
+  float* tr_x;
+  double* tr_target;
+  int* tr_n;
+  int* tr_ctl;            // [0] samples written (ring head), [1] ring capacity
 };
 
 // Parameters of the device-resident loop, by value.

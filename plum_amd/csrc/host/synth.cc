@@ -128,32 +128,56 @@ int plum_synth_system(const plum_synth_spec* s, double* xyz_out, double* q, int*
     grid.add(n);
     n++;
   };
+  // a chain that walks into a dead end is regrown from a new start
+  std::vector<double> ch;
   for (int c = 0; c < s->n_chains; c++) {
-    double p[3];
-    int tries = 0;
-    for (;;) {
-      p[0] = u(g) * s->box[0];
-      p[1] = u(g) * s->box[1];
-      p[2] = s->graft_z >= 0 ? s->graft_z : zlo + u(g) * (zhi - zlo);
-      if (!grid.too_close(p, s->min_sep)) break;
-      if (++tries > 100000) return PLUM_E_CAPACITY;
-    }
-    push(p, s->bead_q, s->graft_z >= 0 ? 2 : 0);
-    for (int k = 1; k < s->chain_len; k++) {
-      tries = 0;
+    bool grown = false;
+    for (int attempt = 0; attempt < 1000 && !grown; attempt++) {
+      ch.clear();
+      double p[3];
+      int tries = 0;
       for (;;) {
-        double v[3], t[3];
-        sphere(g, v);
-        const double* last = &xyz[3 * (size_t)(n - 1)];
-        for (int d = 0; d < 3; d++) t[d] = last[d] + s->r0 * v[d];
-        const bool zok = !zr || (t[2] > zlo && t[2] < zhi);
-        if (zok && !grid.too_close(t, s->min_sep)) {
-          push(t, s->bead_q, 0);
-          break;
-        }
-        if (++tries > 10000) return PLUM_E_CAPACITY;
+        p[0] = u(g) * s->box[0];
+        p[1] = u(g) * s->box[1];
+        p[2] = s->graft_z >= 0 ? s->graft_z : zlo + u(g) * (zhi - zlo);
+        if (!grid.too_close(p, s->min_sep)) break;
+        if (++tries > 100000) return PLUM_E_CAPACITY;
       }
+      ch.insert(ch.end(), p, p + 3);
+      bool stuck = false;
+      for (int k = 1; k < s->chain_len && !stuck; k++) {
+        tries = 0;
+        for (;;) {
+          double v[3], t[3];
+          sphere(g, v);
+          const double* last = &ch[ch.size() - 3];
+          for (int d = 0; d < 3; d++) t[d] = last[d] + s->r0 * v[d];
+          bool ok = !zr || (t[2] > zlo && t[2] < zhi);
+          ok = ok && !grid.too_close(t, s->min_sep);
+          for (size_t b = 0; ok && b + 3 <= ch.size(); b += 3) {
+            double d2 = 0;
+            for (int d = 0; d < 3; d++) {
+              double dd = t[d] - ch[b + d];
+              if (grid.periodic[d]) dd -= s->box[d] * std::round(dd / s->box[d]);
+              d2 += dd * dd;
+            }
+            ok = d2 >= s->min_sep * s->min_sep;
+          }
+          if (ok) {
+            ch.insert(ch.end(), t, t + 3);
+            break;
+          }
+          if (++tries > 2000) {
+            stuck = true;
+            break;
+          }
+        }
+      }
+      if (stuck) continue;
+      for (int k = 0; k < s->chain_len; k++) push(&ch[3 * (size_t)k], s->bead_q, (k == 0 && s->graft_z >= 0) ? 2 : 0);
+      grown = true;
     }
+    if (!grown) return PLUM_E_CAPACITY;
     mol_start[++m] = n;
   }
   for (int i = 0; i < s->n_counterions + 2 * s->n_salt_pairs; i++) {

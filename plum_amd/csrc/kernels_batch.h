@@ -1,28 +1,39 @@
 // Device-resident Monte Carlo over R independent systems (R = 1 for a single
-// system), in speculative batches of up to kMaxProp trial moves per system.
+// system), in speculative batches.
+//
+// A batch makes up to `pmax` proposals (screen slots) per system; the ones to
+// be evaluated exactly get one of kMaxProp exact slots.  Without delayed
+// acceptance every proposal is evaluated exactly (pmax <= kMaxProp).  With
+// it, a surrogate screens up to kMaxScreen proposals and only those passing
+// the first stage (plus, as training data, a few that did not) are evaluated.
 //
 // One batch = five launches (kernel boundaries order them; no grid-wide
-// fences, which on a multi-XCD part would write back and invalidate L2):
-//   k_propose_spec   grid (Pmax, R): proposal p of system r (move number
-//                    counter + p), its phase tables, spring / wall / dipole
+// fences, which on a multi-XCD part would write back and invalidate L2),
+// plus the surrogate's two between proposal and plan:
+//   k_propose_spec   grid (pmax, R): proposal s of system r (move number
+//                    counter + s), its phase tables, spring / wall / dipole
 //                    parts and bounding sphere (proposals stop at the first
 //                    molecule already proposed).
-//   k_plan_spec      grid R: block counts; pairs of proposals within
-//                    interaction range get cross-term blocks.
-//   k_compute_spec   all systems' k-space blocks (dS_p(k) of every proposal,
-//                    dE_k per proposal and the cross sums 2 c_k Re(dS_p* dS_q)),
+//   k_plan_spec      grid R: first-stage decisions, exact slots, block
+//                    counts; pairs of exact slots within interaction range
+//                    get cross-term blocks.
+//   k_compute_spec   all systems' k-space blocks (dS_e(k) of every exact slot,
+//                    dE_k per slot and the cross sums 2 c_k Re(dS_p* dS_q)),
 //                    real-space query blocks and moved-moved blocks per
-//                    proposal, and cross blocks (LJ + real pairs between two
-//                    proposals' moved beads, old/new combinations), spread over
+//                    slot, and cross blocks (LJ + real pairs between two
+//                    slots' moved beads, old/new combinations), spread over
 //                    one grid.
 //   k_reduce_spec    one block per sum and system, fixed order.
 //   k_resolve_spec   grid R: the reference's sequential Metropolis chain over
 //                    the batch (simulation.cc:241-355), each dE corrected by
-//                    the cross terms of the proposals accepted before it, then
-//                    the commits.
+//                    the cross terms of the proposals accepted before it,
+//                    then the commits.
 // Every proposal is evaluated against the state before the batch; accepting
 // p changes q's dE by exactly the cross terms, so the chain is the sequential
-// one (the random numbers depend only on the move number).
+// one (the random numbers depend only on the move number).  A surrogate value
+// has no such correction: the batch ends before the first proposal whose
+// descriptors could see a bead accepted earlier in the batch, and that
+// proposal is made again, from the same move number, in the next batch.
 #pragma once
 #include "kernels_move.h"
 
@@ -39,11 +50,15 @@ struct RepCfg {
   int nkb_s;    // k-space blocks of kKSpec vectors (0 without Ewald)
   int nsplit;   // blocks per real-space query
   int ncell;    // cells
-  int pmax;     // proposals per batch (1..kMaxProp)
+  int pmax;     // proposals per batch (<= kMaxProp, <= kMaxScreen with the surrogate)
   int maxm;     // moved-list capacity per proposal (longest molecule)
-  int nmmb;     // moved-moved block capacity per proposal
-  int rstride;  // real-space partials per proposal
+  int nmmb;     // moved-moved block capacity per exact slot
+  int rstride;  // real-space partials per exact slot
   int xcap;     // cross-term partial capacity
+  int da;       // delayed acceptance on
+  int fill;     // exact slots given to first-stage rejections (training data)
+  float r_d;    // descriptor radius of the surrogate
+  int pad;
 };
 
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
@@ -77,19 +92,19 @@ __device__ __forceinline__ int find_replica(const int* pre, int R, int v) {
   return lo;
 }
 
-// View of proposal slot p: its move record, moved list, phase tables, dS(k)
-// and partial sums.
-__device__ __forceinline__ Dev prop_view(const Geo& g, const Dev& d, const RepCfg& rc, int p) {
+// View of proposal s (move record, moved list, phase tables) with the dS(k)
+// and partial sums of exact slot e.
+__device__ __forceinline__ Dev prop_view(const Geo& g, const Dev& d, const RepCfg& rc, int s, int e) {
   Dev v = d;
-  v.rec = d.recs + p;
-  v.mv_idx = d.smv_idx + p * rc.maxm;
-  v.mv_old = d.mv_old + p * rc.maxm;
-  v.mv_new = d.mv_new + p * rc.maxm;
-  v.mv_type = d.mv_type + p * rc.maxm;
-  v.tab = d.tab + (size_t)p * rc.maxm * 2 * g.tab_w;
-  v.dS = d.dS + (size_t)p * g.n_kh;
-  v.part_r = d.part_r + (size_t)p * rc.rstride;
-  v.part_mm = d.part_mm + (size_t)p * rc.nmmb;
+  v.rec = d.recs + s;
+  v.mv_idx = d.smv_idx + s * rc.maxm;
+  v.mv_old = d.mv_old + s * rc.maxm;
+  v.mv_new = d.mv_new + s * rc.maxm;
+  v.mv_type = d.mv_type + s * rc.maxm;
+  v.tab = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
+  v.dS = d.dS + (size_t)e * g.n_kh;
+  v.part_r = d.part_r + (size_t)e * rc.rstride;
+  v.part_mm = d.part_mm + (size_t)e * rc.nmmb;
   return v;
 }
 
@@ -113,12 +128,12 @@ struct PropSmem {
   double sh[kBlock / kWave];
   int s_forbid, s_cut;
   MoveSel s_sel;
-  MoveSel s_sels[kMaxProp];
+  MoveSel s_sels[kMaxScreen];
 };
 
 __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                          const McDev* __restrict__ MC,
-                                                         const RepCfg* __restrict__ RC, int* vb_count, int R) {
+                                                         const RepCfg* __restrict__ RC, int R) {
   __shared__ PropSmem sm;
   const int p = blockIdx.x, r = blockIdx.y;
   if (r >= R) return;
@@ -145,149 +160,197 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   }
   __syncthreads();
   const int cut = sm.s_cut;
-  if (p < cut) {
-    const int overflow = C->overflow;
-    MoveSel sel = sm.s_sel;
-    const int was_valid = sel.valid;
-    if (overflow) sel.valid = 0;   // the cell list is incomplete: skip, redo later
-    const Dev v = prop_view(g, d, rc, p);
-    propose_block(g, v, mc, ctr0 + p, sel, p * rc.maxm, sm.pr);
-    __syncthreads();
-    if (g.use_ewald && g.n_kh > 0) tables_block(g, v);
-    // spring energy of the trial molecule, wall and dipole parts, bounding
-    // sphere: everything of the dE that involves this proposal alone
-    const int nm = v.rec->n_moved;
-    const int mol = v.rec->mol;
-    const int a = d.mol_start[mol], bend = d.mol_start[mol + 1];
-    double ebt = 0;
-    // (a valid move that moves no bead keeps the molecule's current energy)
-    if (v.rec->valid && g.use_bond && bend - a > 1) ebt = molecule_spring(g, d, mol, true, sm.sbuf);
-    if (threadIdx.x == 0) sm.s_forbid = 0;
-    __syncthreads();
-    double ew = 0, dmz = 0, cx = 0, cy = 0, cz = 0;
-    for (int k = threadIdx.x; k < nm; k += kBlock) {
-      const double4 pn = v.mv_new[k], po = v.mv_old[k];
-      if (g.use_ext) {
-        const WallType w = d.wall[v.mv_type[k]];
-        const double en = wall_energy(pn.z, w, g);
-        if (en >= kVeryLargeEnergy) atomicOr(&sm.s_forbid, 1);
-        ew += en - wall_energy(po.z, w, g);
-      }
-      dmz += pn.w * (pn.z - po.z);
-      cx += pn.x + po.x;
-      cy += pn.y + po.y;
-      cz += pn.z + po.z;
-    }
-    const double ewt = block_sum<kBlock>(ew, sm.sh);
-    const double dmzt = block_sum<kBlock>(dmz, sm.sh);
-    const double cxt = block_sum<kBlock>(cx, sm.sh);
-    const double cyt = block_sum<kBlock>(cy, sm.sh);
-    const double czt = block_sum<kBlock>(cz, sm.sh);
-    if (threadIdx.x == 0) {
-      const double inv = nm > 0 ? 1.0 / (2.0 * nm) : 0.0;
-      sm.sbuf[0] = cxt * inv;
-      sm.sbuf[1] = cyt * inv;
-      sm.sbuf[2] = czt * inv;
-    }
-    __syncthreads();
-    const double c0 = sm.sbuf[0], c1 = sm.sbuf[1], c2 = sm.sbuf[2];
-    __syncthreads();
-    double rad = 0;
-    for (int k = threadIdx.x; k < nm; k += kBlock) {
-      const double4 pn = v.mv_new[k], po = v.mv_old[k];
-      const double dn = (pn.x - c0) * (pn.x - c0) + (pn.y - c1) * (pn.y - c1) + (pn.z - c2) * (pn.z - c2);
-      const double dd = (po.x - c0) * (po.x - c0) + (po.y - c1) * (po.y - c1) + (po.z - c2) * (po.z - c2);
-      rad = fmax(rad, fmax(dn, dd));
-    }
-    sm.sbuf[threadIdx.x] = rad;
-    __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off) sm.sbuf[threadIdx.x] = fmax(sm.sbuf[threadIdx.x], sm.sbuf[threadIdx.x + off]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      PropInfo& pi = d.prop[p];
-      pi.bond_trial = ebt;
-      pi.ew = ewt;
-      pi.dmz = dmzt;
-      pi.c[0] = c0;
-      pi.c[1] = c1;
-      pi.c[2] = c2;
-      pi.rad = sqrt(sm.sbuf[0]);
-      pi.forbid = sm.s_forbid;
-      pi.n = nm;
-      pi.mol = mol;
-      pi.type = v.rec->move_type;
-      pi.valid = v.rec->valid;
-      pi.skipped = (overflow && was_valid) ? 1 : 0;
-    }
-  }
   if (p == 0 && threadIdx.x == 0) plan->n_prop = cut;
+  if (p >= cut) return;
+  const int overflow = C->overflow;
+  MoveSel sel = sm.s_sel;
+  const int was_valid = sel.valid;
+  if (overflow) sel.valid = 0;   // the cell list is incomplete: skip, redo later
+  const Dev v = prop_view(g, d, rc, p, 0);
+  propose_block(g, v, mc, ctr0 + p, sel, p * rc.maxm, sm.pr);
+  __syncthreads();
+  if (g.use_ewald && g.n_kh > 0) tables_block(g, v);
+  // spring energy of the trial molecule, wall and dipole parts, bounding
+  // sphere: everything of the dE that involves this proposal alone
+  const int nm = v.rec->n_moved;
+  const int mol = v.rec->mol;
+  const int a = d.mol_start[mol], bend = d.mol_start[mol + 1];
+  double ebt = 0;
+  // (a valid move that moves no bead keeps the molecule's current energy)
+  if (v.rec->valid && g.use_bond && bend - a > 1) ebt = molecule_spring(g, d, mol, true, sm.sbuf);
+  if (threadIdx.x == 0) sm.s_forbid = 0;
+  __syncthreads();
+  double ew = 0, dmz = 0, cx = 0, cy = 0, cz = 0;
+  for (int k = threadIdx.x; k < nm; k += kBlock) {
+    const double4 pn = v.mv_new[k], po = v.mv_old[k];
+    if (g.use_ext) {
+      const WallType w = d.wall[v.mv_type[k]];
+      const double en = wall_energy(pn.z, w, g);
+      if (en >= kVeryLargeEnergy) atomicOr(&sm.s_forbid, 1);
+      ew += en - wall_energy(po.z, w, g);
+    }
+    dmz += pn.w * (pn.z - po.z);
+    cx += pn.x + po.x;
+    cy += pn.y + po.y;
+    cz += pn.z + po.z;
+  }
+  const double ewt = block_sum<kBlock>(ew, sm.sh);
+  const double dmzt = block_sum<kBlock>(dmz, sm.sh);
+  const double cxt = block_sum<kBlock>(cx, sm.sh);
+  const double cyt = block_sum<kBlock>(cy, sm.sh);
+  const double czt = block_sum<kBlock>(cz, sm.sh);
+  if (threadIdx.x == 0) {
+    const double inv = nm > 0 ? 1.0 / (2.0 * nm) : 0.0;
+    sm.sbuf[0] = cxt * inv;
+    sm.sbuf[1] = cyt * inv;
+    sm.sbuf[2] = czt * inv;
+  }
+  __syncthreads();
+  const double c0 = sm.sbuf[0], c1 = sm.sbuf[1], c2 = sm.sbuf[2];
+  __syncthreads();
+  double rad = 0;
+  for (int k = threadIdx.x; k < nm; k += kBlock) {
+    const double4 pn = v.mv_new[k], po = v.mv_old[k];
+    const double dn = (pn.x - c0) * (pn.x - c0) + (pn.y - c1) * (pn.y - c1) + (pn.z - c2) * (pn.z - c2);
+    const double dd = (po.x - c0) * (po.x - c0) + (po.y - c1) * (po.y - c1) + (po.z - c2) * (po.z - c2);
+    rad = fmax(rad, fmax(dn, dd));
+  }
+  sm.sbuf[threadIdx.x] = rad;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm.sbuf[threadIdx.x] = fmax(sm.sbuf[threadIdx.x], sm.sbuf[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    PropInfo& pi = d.prop[p];
+    pi.bond_trial = ebt;
+    pi.ew = ewt;
+    pi.dmz = dmzt;
+    pi.c[0] = c0;
+    pi.c[1] = c1;
+    pi.c[2] = c2;
+    pi.rad = sqrt(sm.sbuf[0]);
+    pi.forbid = sm.s_forbid;
+    pi.n = nm;
+    pi.mol = mol;
+    pi.type = v.rec->move_type;
+    pi.valid = v.rec->valid;
+    pi.skipped = (overflow && was_valid) ? 1 : 0;
+  }
 }
 
-// Plan of a batch, one block per system: real-space + moved-moved blocks of
-// every proposal, the proposal pairs within interaction range (bounding
-// spheres closer than the search radius) and their cross-term blocks, and
-// the system's virtual-block count.
+// Plan of a batch, one block per system: first-stage decisions (delayed
+// acceptance), exact slots, real-space + moved-moved blocks per exact slot,
+// the exact-slot pairs within interaction range (bounding spheres closer than
+// the search radius) and their cross-term blocks, the virtual-block count.
 struct PlanSmem {
-  PropInfo s_prop[kMaxProp];
+  PropInfo s_prop[kMaxScreen];
+  double s_nn[kMaxScreen];
+  double s_bcur[kMaxScreen];
 };
 
-__global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
-                            int* vb_count, int R) {
+__global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D, const McDev* __restrict__ MC,
+                            const RepCfg* __restrict__ RC, int* vb_count, int R) {
   __shared__ PlanSmem sm;
   const int r = blockIdx.x;
   if (r >= R) return;
   const Geo& g = G[r];
   const Dev& d = D[r];
+  const McDev& mc = MC[r];
   const RepCfg& rc = RC[r];
   SpecPlan* plan = d.plan;
-  const int cut = plan->n_prop;
-  if (threadIdx.x < cut) sm.s_prop[threadIdx.x] = d.prop[threadIdx.x];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int off = 0;
-    plan->prop_off[0] = 0;
-    for (int q = 0; q < cut; q++) {
-      const int n = sm.s_prop[q].n;
-      off += n > 0 ? 2 * n * rc.nsplit + nmm_blocks(n) : 0;
-      plan->prop_off[q + 1] = off;
+  const int np = plan->n_prop;
+  const int t = threadIdx.x;
+  if (t < np) {
+    const PropInfo P = d.prop[t];
+    sm.s_prop[t] = P;
+    sm.s_bcur[t] = d.bond_cur[P.mol];
+    double nn = 0;
+    if (rc.da && P.valid) {
+      // network part of the surrogate: sum over moved beads of f(new) - f(old)
+      const float* f = d.sur_f + (size_t)(2 * t) * rc.maxm;
+      for (int m = 0; m < P.n; m++) nn += (double)f[rc.maxm + m] - (double)f[m];
     }
-    int nn = 0, xo = 0;
-    plan->pair_off[0] = 0;
-    for (int q = 1; q < cut; q++) {
-      const PropInfo& pq = sm.s_prop[q];
-      if (pq.n == 0) continue;
-      for (int q0 = 0; q0 < q; q0++) {
-        const PropInfo& pp = sm.s_prop[q0];
-        if (pp.n == 0) continue;
-        if (center_dist(pp.c, pq.c, g) > pp.rad + pq.rad + g.rs + 1e-9) continue;
-        plan->pair_p[nn] = q0;
-        plan->pair_q[nn] = q;
-        xo += cross_blocks(pp.n, pq.n);
-        nn++;
-        plan->pair_off[nn] = xo;
-      }
-    }
-    plan->n_near = nn;
-    vb_count[r] = cut > 0 ? rc.nkb_s + off + xo : 0;
+    sm.s_nn[t] = nn;
   }
+  __syncthreads();
+  if (t != 0) return;
+  const unsigned long long ctr0 = d.ctl->move_counter;
+  int ne = 0, ndec = np;
+  for (int s = 0; s < np; s++) {
+    const PropInfo& P = sm.s_prop[s];
+    plan->exact_of[s] = -1;
+    int pass = 1;
+    double des = 0;
+    if (rc.da && P.valid) {
+      // surrogate dE: exact spring and wall parts plus the network's estimate
+      // of the pair and k-space parts; first stage min(1, exp(-beta dE_s))
+      const int len = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
+      if (g.use_bond && len > 1) des += P.bond_trial - sm.s_bcur[s];
+      if (g.use_ext) des += P.forbid ? kVeryLargeEnergy : P.ew;
+      des += sm.s_nn[s];
+      const unsigned long long ctr = ctr0 + s;
+      const uint4 u = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xDA1u, 0u),
+                                  make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
+      pass = des < kVeryLargeEnergy && u01_32(u.x) < exp(-g.beta * des);
+    }
+    plan->pass1[s] = pass;
+    plan->de_s[s] = des;
+    if (pass && P.valid) {
+      if (ne == kMaxProp) {
+        ndec = s;   // no exact slot left: this proposal is decided next batch
+        break;
+      }
+      plan->exact_of[s] = ne;
+      plan->screen_of[ne] = s;
+      ne++;
+    }
+  }
+  // spare exact slots: first-stage rejections, evaluated for training only
+  for (int s = 0; s < ndec && ne < rc.fill; s++)
+    if (plan->exact_of[s] < 0 && !plan->pass1[s] && sm.s_prop[s].valid && sm.s_prop[s].n > 0) {
+      plan->exact_of[s] = ne;
+      plan->screen_of[ne] = s;
+      ne++;
+    }
+  plan->n_decide = ndec;
+  plan->n_exact = ne;
+  int off = 0;
+  plan->prop_off[0] = 0;
+  for (int e = 0; e < ne; e++) {
+    const int n = sm.s_prop[plan->screen_of[e]].n;
+    off += n > 0 ? 2 * n * rc.nsplit + nmm_blocks(n) : 0;
+    plan->prop_off[e + 1] = off;
+  }
+  int nn = 0, xo = 0;
+  plan->pair_off[0] = 0;
+  for (int e = 1; e < ne; e++) {
+    const PropInfo& pq = sm.s_prop[plan->screen_of[e]];
+    if (pq.n == 0) continue;
+    for (int e0 = 0; e0 < e; e0++) {
+      const PropInfo& pp = sm.s_prop[plan->screen_of[e0]];
+      if (pp.n == 0) continue;
+      if (center_dist(pp.c, pq.c, g) > pp.rad + pq.rad + g.rs + 1e-9) continue;
+      plan->pair_p[nn] = e0;
+      plan->pair_q[nn] = e;
+      xo += cross_blocks(pp.n, pq.n);
+      nn++;
+      plan->pair_off[nn] = xo;
+    }
+  }
+  plan->n_near = nn;
+  // k-space blocks also fold in the pending S(k) of the last batch
+  vb_count[r] = (np > 0 && (ne > 0 || plan->acc_mask)) ? rc.nkb_s + off + xo : 0;
 }
 
 // ------------------------------------------------------------ k space
-// 16 k vectors x 16 bead groups per block; every proposal's dS_p(k)
+// 16 k vectors x 16 bead groups per block; every exact slot's dS_e(k)
 // accumulates in registers, the groups meet in LDS, and the 16 threads of
-// group 0 form dE_p = c_k (2 Re(S* dS_p) + |dS_p|^2) and the cross sums.  The S(k) of the
-// proposals accepted by the previous batch is folded in first.
+// group 0 form dE_e = c_k (2 Re(S* dS_e) + |dS_e|^2) and the cross sums.  The
+// S(k) of the slots accepted by the previous batch is folded in first.
 struct KSpecSmem {
   double2 s_ds[kMaxProp][kKSpecGroups][kKSpec];
 };
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-  return v;  // valid in lane 0
-}
 
 __device__ __forceinline__ double sum16(double v) {
 #pragma unroll
@@ -295,8 +358,8 @@ __device__ __forceinline__ double sum16(double v) {
   return v;  // valid in lane 0 of each 16-lane group
 }
 
-__device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, int np, unsigned acc_mask, int blk,
-                                  KSpecSmem& sm) {
+__device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, const SpecPlan* plan, int ne,
+                                  unsigned acc_mask, int blk, KSpecSmem& sm) {
   const int kk = threadIdx.x % kKSpec, grp = threadIdx.x / kKSpec;
   const int k = blk * kKSpec + kk;
   int ox = 0, oy = 0, oz = 0;
@@ -307,13 +370,14 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
     oz = g.nkx + 1 + 2 * g.nky + 1 + l.z + g.nkz;
   }
 #pragma unroll
-  for (int p = 0; p < kMaxProp; p++) {
-    if (p < np) {
+  for (int e = 0; e < kMaxProp; e++) {
+    if (e < ne) {
       double2 acc = make_double2(0, 0);
-      const int n = __builtin_amdgcn_readfirstlane(d.recs[p].n_moved);
+      const int s = uload(&plan->screen_of[e]);
+      const int n = uload(&d.recs[s].n_moved);
       if (k < g.n_kh) {
-        const double4* mo = d.mv_old + p * rc.maxm;
-        const double2* base = d.tab + (size_t)p * rc.maxm * 2 * g.tab_w;
+        const double4* mo = d.mv_old + s * rc.maxm;
+        const double2* base = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
         for (int m = grp; m < n; m += kKSpecGroups) {
           const double qm = mo[m].w;
           const double2* to = base + (size_t)(2 * m) * g.tab_w;
@@ -324,23 +388,23 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
           acc.y += qm * (en.y - eo.y);
         }
       }
-      sm.s_ds[p][grp][kk] = acc;
+      sm.s_ds[e][grp][kk] = acc;
     }
   }
   __syncthreads();
   if (grp != 0) return;
   double2 t[kMaxProp];
 #pragma unroll
-  for (int p = 0; p < kMaxProp; p++) t[p] = make_double2(0, 0);
+  for (int e = 0; e < kMaxProp; e++) t[e] = make_double2(0, 0);
   double2 S = make_double2(0, 0);
   double ck = 0;
   if (k < g.n_kh) {
     S = d.Sk[k];
     if (acc_mask) {
 #pragma unroll
-      for (int p = 0; p < kMaxProp; p++)
-        if (acc_mask & (1u << p)) {
-          const double2 prev = d.dS[(size_t)p * g.n_kh + k];
+      for (int e = 0; e < kMaxProp; e++)
+        if (acc_mask & (1u << e)) {
+          const double2 prev = d.dS[(size_t)e * g.n_kh + k];
           S.x += prev.x;
           S.y += prev.y;
         }
@@ -348,29 +412,29 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
     }
     ck = d.ck[k];
 #pragma unroll
-    for (int p = 0; p < kMaxProp; p++) {
-      if (p < np) {
+    for (int e = 0; e < kMaxProp; e++) {
+      if (e < ne) {
 #pragma unroll
-        for (int r = 0; r < kKSpecGroups; r++) {
-          t[p].x += sm.s_ds[p][r][kk].x;
-          t[p].y += sm.s_ds[p][r][kk].y;
+        for (int q = 0; q < kKSpecGroups; q++) {
+          t[e].x += sm.s_ds[e][q][kk].x;
+          t[e].y += sm.s_ds[e][q][kk].y;
         }
-        d.dS[(size_t)p * g.n_kh + k] = t[p];
+        d.dS[(size_t)e * g.n_kh + k] = t[e];
       }
     }
   }
   // sums over the block's 16 k vectors (t = 0 past n_kh)
   const int nkb = rc.nkb_s;
 #pragma unroll
-  for (int p = 0; p < kMaxProp; p++) {
-    if (p >= np) break;
-    const double e = ck * (2.0 * (S.x * t[p].x + S.y * t[p].y) + t[p].x * t[p].x + t[p].y * t[p].y);
-    const double sv = sum16(e);
-    if (threadIdx.x == 0) d.part_ks[(size_t)p * nkb + blk] = sv;
+  for (int e = 0; e < kMaxProp; e++) {
+    if (e >= ne) break;
+    const double v = ck * (2.0 * (S.x * t[e].x + S.y * t[e].y) + t[e].x * t[e].x + t[e].y * t[e].y);
+    const double sv = sum16(v);
+    if (threadIdx.x == 0) d.part_ks[(size_t)e * nkb + blk] = sv;
   }
 #pragma unroll
   for (int q = 1; q < kMaxProp; q++) {
-    if (q >= np) break;
+    if (q >= ne) break;
 #pragma unroll
     for (int p = 0; p < q; p++) {
       const double x = 2.0 * ck * (t[p].x * t[q].x + t[p].y * t[q].y);
@@ -380,16 +444,16 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
   }
 }
 
-// Cross block: LJ + real pairs between proposal q's and proposal p's moved
-// beads, u(i',j') - u(i,j') - u(i',j) + u(i,j): the change of q's dE when p
-// has been accepted (same pair functions as the moved-moved pairs).
-__device__ void cross_block(const Geo& g, const Dev& d, const RepCfg& rc, int p, int q, int np_, int nq, int local,
-                            double* sh, double2* out) {
+// Cross block: LJ + real pairs between slot q's and slot p's moved beads,
+// u(i',j') - u(i,j') - u(i',j) + u(i,j): the change of either one's dE when
+// the other has been accepted (same pair functions as the moved-moved pairs).
+__device__ void cross_block(const Geo& g, const Dev& d, const RepCfg& rc, int sp, int sq, int np_, int nq,
+                            int local, double* sh, double2* out) {
   const int t = local * kBlock + threadIdx.x;
   double elj = 0, ere = 0;
   if (t < np_ * nq) {
     const int i = t / np_, j = t - (t / np_) * np_;
-    const int si = q * rc.maxm + i, sj = p * rc.maxm + j;
+    const int si = sq * rc.maxm + i, sj = sp * rc.maxm + j;
     const double4 a0 = d.mv_old[si], a1 = d.mv_new[si], b0 = d.mv_old[sj], b1 = d.mv_new[sj];
     const double ao[3] = {a0.x, a0.y, a0.z}, an[3] = {a1.x, a1.y, a1.z};
     const double bo[3] = {b0.x, b0.y, b0.z}, bn[3] = {b1.x, b1.y, b1.z};
@@ -431,18 +495,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const Geo g = load_const(G, r);
     const Dev d = load_const(D, r);
     const RepCfg rc = load_const(RC, r);
-    SpecPlan* plan = d.plan;
-    const int np = uload(&plan->n_prop);
+    const SpecPlan* plan = d.plan;
+    const int ne = uload(&plan->n_exact);
     if (b < rc.nkb_s) {
-      kspace_spec_block(g, d, rc, np, (unsigned)uload((const int*)&plan->acc_mask), b, sm.k);
+      kspace_spec_block(g, d, rc, plan, ne, (unsigned)uload((const int*)&plan->acc_mask), b, sm.k);
     } else {
       b -= rc.nkb_s;
-      const int nprop_blocks = uload(&plan->prop_off[np]);
+      const int nprop_blocks = uload(&plan->prop_off[ne]);
       if (b < nprop_blocks) {
-        int p = 0;
-        while (p + 1 < np && uload(&plan->prop_off[p + 1]) <= b) p++;
-        const int local = b - uload(&plan->prop_off[p]);
-        const Dev v = prop_view(g, d, rc, p);
+        int e = 0;
+        while (e + 1 < ne && uload(&plan->prop_off[e + 1]) <= b) e++;
+        const int local = b - uload(&plan->prop_off[e]);
+        const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
         const int nm = uload(&v.rec->n_moved);
         real_block<P>(g, v, local, 2 * nm * rc.nsplit, nmm_blocks(nm), rc.nsplit, sm.r);
       } else {
@@ -451,9 +515,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         int j = 0;
         while (j + 1 < nn && uload(&plan->pair_off[j + 1]) <= b) j++;
         const int off = uload(&plan->pair_off[j]);
-        const int p = uload(&plan->pair_p[j]), q = uload(&plan->pair_q[j]);
-        const int np_ = uload(&d.recs[p].n_moved), nq = uload(&d.recs[q].n_moved);
-        cross_block(g, d, rc, p, q, np_, nq, b - off, cs.sh, d.part_x + b);
+        const int sp = uload(&plan->screen_of[uload(&plan->pair_p[j])]);
+        const int sq = uload(&plan->screen_of[uload(&plan->pair_q[j])]);
+        const int np_ = uload(&d.recs[sp].n_moved), nq = uload(&d.recs[sq].n_moved);
+        cross_block(g, d, rc, sp, sq, np_, nq, b - off, cs.sh, d.part_x + b);
       }
     }
     __syncthreads();  // the LDS union is reused by the next virtual block
@@ -461,7 +526,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 }
 
 // Sums of a batch, one block per (sum, system), fixed order: blockIdx.x in
-// [0, kKVals) k-space sums, then kMaxProp (LJ, real) sums per proposal
+// [0, kKVals) k-space sums, then kMaxProp (LJ, real) sums per exact slot
 // (real-space query blocks + moved-moved blocks), then kMaxPairs cross sums.
 __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                         const RepCfg* __restrict__ RC, int R) {
@@ -471,10 +536,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
   const Dev& d = D[r];
   const RepCfg& rc = RC[r];
   SpecPlan* plan = d.plan;
-  const int np = plan->n_prop;
-  if (np == 0) return;
+  if (plan->n_prop == 0) return;
+  const int ne = plan->n_exact;
   if (j < kKVals) {
-    if (j >= np && (j < kMaxProp || j >= kMaxProp + np * (np - 1) / 2)) return;
+    if (j >= ne && (j < kMaxProp || j >= kMaxProp + ne * (ne - 1) / 2)) return;
     if (rc.nkb_s == 0) {
       if (threadIdx.x == 0) plan->ksum[j] = 0;
       return;
@@ -484,13 +549,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     const double t = block_sum<kBlock>(s, sh);
     if (threadIdx.x == 0) plan->ksum[j] = t;
   } else if (j < kKVals + kMaxProp) {
-    const int p = j - kKVals;
-    if (p >= np) return;
-    const int nm = d.recs[p].n_moved;
+    const int e = j - kKVals;
+    if (e >= ne) return;
+    const int nm = d.recs[plan->screen_of[e]].n_moved;
     if (nm == 0) return;
     const int nreal = 2 * nm * rc.nsplit, nmm = nmm_blocks(nm);
-    const double2* pr = d.part_r + (size_t)p * rc.rstride;
-    const double2* pm = d.part_mm + (size_t)p * rc.nmmb;
+    const double2* pr = d.part_r + (size_t)e * rc.rstride;
+    const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;
     for (int i = threadIdx.x; i < nreal; i += kBlock) {
       lj += pr[i].x;
@@ -502,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     }
     const double a = block_sum<kBlock>(lj, sh);
     const double c = block_sum<kBlock>(re, sh);
-    if (threadIdx.x == 0) plan->rsum[p] = make_double2(a, c);
+    if (threadIdx.x == 0) plan->rsum[e] = make_double2(a, c);
   } else {
     const int jj = j - kKVals - kMaxProp;
     if (jj >= plan->n_near) return;
@@ -521,15 +586,20 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
 // ------------------------------------------------------------ resolve
 struct ResSmem {
   int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
-  PropInfo s_prop[kMaxProp];
+  PropInfo s_prop[kMaxScreen];
+  double s_bcur[kMaxScreen];
+  int s_len[kMaxScreen];
+  double s_u[kMaxScreen];
+  double s_des[kMaxScreen];
+  int s_pass[kMaxScreen];
+  int s_exact[kMaxScreen];
+  int s_screen[kMaxProp];
   double2 s_rsum[kMaxProp];
   double s_ksum[kKVals];
   double2 s_xsum[kMaxPairs];
   int s_near[kMaxPairs];
-  double s_bcur[kMaxProp];
-  int s_len[kMaxProp];
-  double s_u[kMaxProp];
-  int s_mask, s_np;
+  int s_train[kMaxProp];         // ring slot of a training sample per exact slot, -1 none
+  int s_mask, s_ndone;
 };
 
 __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
@@ -546,15 +616,18 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
   SpecPlan* plan = d.plan;
   const int np = plan->n_prop;
   if (np == 0) return;
+  const int ndec = plan->n_decide;
   const unsigned long long ctr0 = C->move_counter;
   // everything the sequential decisions read, fetched in parallel
   const int t = threadIdx.x;
   if (t < np) {
     const PropInfo P = d.prop[t];
     sm.s_prop[t] = P;
-    sm.s_rsum[t] = plan->rsum[t];
     sm.s_bcur[t] = d.bond_cur[P.mol];
     sm.s_len[t] = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
+    sm.s_des[t] = plan->de_s[t];
+    sm.s_pass[t] = plan->pass1[t];
+    sm.s_exact[t] = plan->exact_of[t];
     const unsigned long long ctr = ctr0 + t;
     const uint4 u = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xACCu, 0u),
                                 make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
@@ -566,6 +639,11 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     sm.s_near[j] = -1;
     sm.s_xsum[j] = plan->xsum[j];
   }
+  if (t >= 192 && t < 192 + kMaxProp) {
+    sm.s_rsum[t - 192] = plan->rsum[t - 192];
+    sm.s_screen[t - 192] = plan->screen_of[t - 192];
+    sm.s_train[t - 192] = -1;
+  }
   __syncthreads();
   if (t == 0) {
     const int nn = plan->n_near;
@@ -573,26 +651,54 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
   }
   __syncthreads();
   if (t == 0) {
-    unsigned mask = 0;
+    unsigned mask = 0;            // accepted exact slots
+    unsigned long long acc_screen = 0;  // accepted proposals
     double Mz = C->Mz, cur_dipl = C->cur_dipl;
     double run[8], last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 8; k++) run[k] = C->running[k];
     long long att[5] = {0, 0, 0, 0, 0}, acc5[5] = {0, 0, 0, 0, 0};
-    long long skipped = 0, valid = 0;
+    long long skipped = 0, valid = 0, s1r = 0, s1p = 0, tev = 0, cuts = 0;
     double last_total = C->total;
-    for (int q = 0; q < np; q++) {
-      const PropInfo& P = sm.s_prop[q];
+    int done = ndec;
+    int head = rc.da ? d.tr_ctl[0] : 0;
+    const int ring = rc.da ? d.tr_ctl[1] : 0;
+    for (int s = 0; s < ndec; s++) {
+      const PropInfo& P = sm.s_prop[s];
+      if (rc.da && acc_screen) {
+        // the surrogate value of this proposal was taken before an earlier
+        // acceptance; if its descriptors could see that molecule, stop here
+        bool stale = false;
+        for (int q = 0; q < s && !stale; q++)
+          if ((acc_screen >> q) & 1ull) {
+            const PropInfo& A = sm.s_prop[q];
+            stale = center_dist(A.c, P.c, g) <= A.rad + P.rad + (double)rc.r_d + 1e-9;
+          }
+        if (stale) {
+          done = s;
+          cuts++;
+          break;
+        }
+      }
       if (!P.valid) {
         if (P.skipped) skipped++;
         continue;
       }
-      // a proposal that moves no bead (crankshaft between neighbours) has no
-      // real-space blocks: its pair terms are zero
-      double lj = P.n > 0 ? sm.s_rsum[q].x : 0.0, re = P.n > 0 ? sm.s_rsum[q].y : 0.0;
-      double kk = sm.s_ksum[q];
-      for (int p = 0; p < q; p++) {
+      const int e = sm.s_exact[s];
+      if (!sm.s_pass[s] && e < 0) {
+        // first-stage rejection: no exact evaluation
+        if (P.type >= 0 && P.type < 5) att[P.type] += 1;
+        valid++;
+        s1r++;
+        continue;
+      }
+      // exact dE against the state before the batch, plus the cross terms of
+      // the exact slots accepted before this proposal (a proposal that moves
+      // no bead has no real-space blocks: its pair terms are zero)
+      double lj = P.n > 0 ? sm.s_rsum[e].x : 0.0, re = P.n > 0 ? sm.s_rsum[e].y : 0.0;
+      double kk = sm.s_ksum[e];
+      for (int p = 0; p < kMaxProp; p++) {
         if (!(mask & (1u << p))) continue;
-        const int pi = pair_index(p, q);
+        const int pi = pair_index(p, e);
         kk += sm.s_ksum[kMaxProp + pi];
         const int j = sm.s_near[pi];
         if (j >= 0) {
@@ -633,13 +739,31 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         tt[T_DIPOLE] = dd;
         dE += re + kk + dd;
       }
-      const int len = sm.s_len[q];
+      const int len = sm.s_len[s];
       if (!early && g.use_bond && len > 1) {
-        tt[T_BOND] = P.bond_trial - sm.s_bcur[q];
+        tt[T_BOND] = P.bond_trial - sm.s_bcur[s];
         dE += tt[T_BOND];
       }
-      // Metropolis (simulation.cc:327-332): no draw for a forbidden move
-      const int acc = (dE < kVeryLargeEnergy) && (sm.s_u[q] < exp(-g.beta * dE));
+      if (rc.da && !early && P.n > 0 && ring > 0) {
+        // training sample: the pair + k-space part the network estimates
+        const int slot = head % ring;
+        d.tr_target[slot] = lj + re + kk;
+        d.tr_n[slot] = P.n;
+        sm.s_train[e] = slot;
+        head++;
+      }
+      if (!sm.s_pass[s]) {
+        // evaluated for training only; the first stage rejected it
+        if (P.type >= 0 && P.type < 5) att[P.type] += 1;
+        valid++;
+        s1r++;
+        tev++;
+        continue;
+      }
+      if (rc.da) s1p++;
+      // Metropolis (simulation.cc:327-332), no draw for a forbidden move; with
+      // the surrogate, the second stage min(1, exp(-beta (dE - dE_s)))
+      const int acc = (dE < kVeryLargeEnergy) && (sm.s_u[s] < exp(-g.beta * (dE - sm.s_des[s])));
       for (int k = 0; k < 8; k++) last[k] = tt[k];
       last_total = dE;
       if (P.type >= 0 && P.type < 5) {
@@ -648,7 +772,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
       }
       valid++;
       if (acc) {
-        mask |= 1u << q;
+        mask |= 1u << e;
+        acc_screen |= 1ull << s;
         Mz += P.dmz;
         if (g.dipole) cur_dipl = trial_dipl;
         for (int k = 0; k < 8; k++) run[k] += tt[k];
@@ -669,27 +794,48 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     }
     C->skipped += skipped;
     C->valid_moves += valid;
-    C->move_counter = ctr0 + np;
+    C->s1_rejected += s1r;
+    C->s1_passed += s1p;
+    C->train_evals += tev;
+    C->stale_cuts += cuts;
+    C->move_counter = ctr0 + done;
+    if (rc.da) d.tr_ctl[0] = head;
     plan->acc_mask = mask;
     sm.s_mask = mask;
+    sm.s_ndone = done;
   }
   __syncthreads();
   const unsigned mask = sm.s_mask;
-  for (int q = 0; q < np; q++) {
-    if (!(mask & (1u << q))) continue;
-    commit_cells(g, prop_view(g, d, rc, q), sm.s_nc, sm.s_flag);
+  for (int e = 0; e < kMaxProp; e++) {
+    if (!(mask & (1u << e))) continue;
+    commit_cells(g, prop_view(g, d, rc, sm.s_screen[e], e), sm.s_nc, sm.s_flag);
   }
-  for (int q = 0; q < np; q++) {
-    const int n = sm.s_prop[q].n;
+  if (rc.da) {
+    // descriptor rows of the training samples (old rows, then new rows)
+    for (int e = 0; e < kMaxProp; e++) {
+      const int slot = sm.s_train[e];
+      if (slot < 0) continue;
+      const int s = sm.s_screen[e];
+      const int n = sm.s_prop[s].n;
+      const float* src = d.sur_x + (size_t)(2 * s) * rc.maxm * kDescr;
+      float* dst = d.tr_x + (size_t)slot * 2 * rc.maxm * kDescr;
+      for (int i = threadIdx.x; i < n * kDescr; i += kBlock) {
+        dst[i] = src[i];
+        dst[(size_t)rc.maxm * kDescr + i] = src[(size_t)rc.maxm * kDescr + i];
+      }
+    }
+  }
+  for (int s = 0; s < np; s++) {
+    const int n = sm.s_prop[s].n;
     for (int k = threadIdx.x; k < n; k += kBlock) {
-      const int i = d.smv_idx[q * rc.maxm + k];
+      const int i = d.smv_idx[s * rc.maxm + k];
       d.moved[i] = 0;
       d.mv_slot[i] = -1;
     }
   }
 }
 
-// Folds the pending accepted dS_p(k) into S(k) (end of a run); one block row
+// Folds the pending accepted dS_e(k) into S(k) (end of a run); one block row
 // per system, grid-stride over its k vectors.
 __global__ void k_flush_spec(const Geo* __restrict__ G, const Dev* __restrict__ D, int R) {
   const int r = blockIdx.y;
@@ -700,9 +846,9 @@ __global__ void k_flush_spec(const Geo* __restrict__ G, const Dev* __restrict__ 
   if (!mask) return;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g.n_kh; k += gridDim.x * blockDim.x) {
     double2 s = d.Sk[k];
-    for (int p = 0; p < kMaxProp; p++)
-      if (mask & (1u << p)) {
-        const double2 ds = d.dS[(size_t)p * g.n_kh + k];
+    for (int e = 0; e < kMaxProp; e++)
+      if (mask & (1u << e)) {
+        const double2 ds = d.dS[(size_t)e * g.n_kh + k];
         s.x += ds.x;
         s.y += ds.y;
       }

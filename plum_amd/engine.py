@@ -23,8 +23,13 @@ ABI_SYMBOLS = (
     "plum_get_positions", "plum_last_error", "plum_mc_run", "plum_set_profiling",
     "plum_kernel_times", "plum_batch_create", "plum_batch_destroy", "plum_batch_mc_run",
     "plum_batch_tune", "plum_batch_get_tuning", "plum_batch_set_tuning", "plum_mc_get_tuning",
-    "plum_mc_set_tuning",
+    "plum_mc_set_tuning", "plum_surrogate_enable", "plum_surrogate_disable",
+    "plum_surrogate_stats_get", "plum_surrogate_get_weights", "plum_surrogate_set_weights",
+    "plum_surrogate_train", "plum_surrogate_forward", "plum_surrogate_gradient",
+    "plum_surrogate_debug_slot", "plum_max_mol_len",
 )
+SURROGATE_PARAMS = 64 * 128 + 128 + 2 * (128 * 128 + 128) + 128 + 1
+DESCR = 64
 
 
 class PlumError(RuntimeError):
@@ -93,6 +98,22 @@ class BatchStats(C.Structure):
     ]
 
 
+class SurrogateParams(C.Structure):
+    _fields_ = [("screen", C.c_int), ("fill", C.c_int), ("train_every", C.c_int),
+                ("train_samples", C.c_int), ("ring", C.c_int), ("lr", C.c_double),
+                ("r_d", C.c_double), ("target_clip", C.c_double), ("seed", C.c_ulonglong)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("s1_rejected", C.c_longlong), ("s1_passed", C.c_longlong),
+                ("train_evals", C.c_longlong), ("stale_cuts", C.c_longlong),
+                ("samples", C.c_longlong), ("train_steps", C.c_longlong),
+                ("last_loss", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -128,6 +149,17 @@ def lib():
         L.plum_batch_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
         L.plum_mc_get_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
         L.plum_mc_set_tuning.argtypes = [vp, C.POINTER(BatchTuning)]
+        fp = C.POINTER(C.c_float)
+        L.plum_surrogate_enable.argtypes = [vp, C.POINTER(SurrogateParams)]
+        L.plum_surrogate_disable.argtypes = [vp]
+        L.plum_surrogate_stats_get.argtypes = [vp, C.POINTER(SurrogateStats)]
+        L.plum_surrogate_get_weights.argtypes = [vp, fp, C.c_int]
+        L.plum_surrogate_set_weights.argtypes = [vp, fp, C.c_int]
+        L.plum_surrogate_train.argtypes = [vp, C.c_int, dp]
+        L.plum_surrogate_forward.argtypes = [vp, fp, C.c_int, C.c_int, fp]
+        L.plum_surrogate_gradient.argtypes = [vp, fp, dp, ip, C.c_int, dp, fp]
+        L.plum_surrogate_debug_slot.argtypes = [vp, C.c_int, fp, dp, ip]
+        L.plum_max_mol_len.argtypes = [vp]
         for f in ABI_SYMBOLS:
             if f != "plum_last_error":
                 getattr(L, f).restype = C.c_int
@@ -272,6 +304,67 @@ class Engine:
 
     def set_profiling(self, on):
         _check(lib().plum_set_profiling(self.h, int(on)))
+
+    # ---- delayed acceptance with the learned surrogate
+    def surrogate_enable(self, screen=32, fill=4, train_every=16, train_samples=64, ring=1024,
+                         lr=1e-3, r_d=4.0, target_clip=20.0, seed=1):
+        p = SurrogateParams(screen=screen, fill=fill, train_every=train_every,
+                            train_samples=train_samples, ring=ring, lr=lr, r_d=r_d,
+                            target_clip=target_clip, seed=seed)
+        _check(lib().plum_surrogate_enable(self.h, C.byref(p)))
+
+    def surrogate_disable(self):
+        _check(lib().plum_surrogate_disable(self.h))
+
+    def surrogate_stats(self):
+        st = SurrogateStats()
+        _check(lib().plum_surrogate_stats_get(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def surrogate_weights(self, w=None):
+        fp = C.POINTER(C.c_float)
+        if w is None:
+            out = np.zeros(SURROGATE_PARAMS, dtype=np.float32)
+            _check(lib().plum_surrogate_get_weights(self.h, out.ctypes.data_as(fp), len(out)))
+            return out
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().plum_surrogate_set_weights(self.h, w.ctypes.data_as(fp), len(w)))
+
+    def surrogate_train(self, steps):
+        loss = C.c_double()
+        _check(lib().plum_surrogate_train(self.h, steps, C.byref(loss)))
+        return loss.value
+
+    def surrogate_forward(self, x, bf16=True):
+        fp = C.POINTER(C.c_float)
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, DESCR)
+        out = np.zeros(len(x), dtype=np.float32)
+        _check(lib().plum_surrogate_forward(self.h, x.ctypes.data_as(fp), len(x), int(bf16),
+                                            out.ctypes.data_as(fp)))
+        return out
+
+    def surrogate_gradient(self, x, target, n):
+        fp = C.POINTER(C.c_float)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        grad = np.zeros(SURROGATE_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().plum_surrogate_gradient(self.h, x.ctypes.data_as(fp), _dp(target), _ip(n),
+                                             len(target), C.byref(loss), grad.ctypes.data_as(fp)))
+        return loss.value, grad
+
+    def max_mol_len(self):
+        return lib().plum_max_mol_len(self.h)
+
+    def surrogate_debug_slot(self, slot):
+        m = self.max_mol_len()
+        rows = np.zeros((2 * m, DESCR), dtype=np.float32)
+        pos = np.zeros((2 * m, 4))
+        n = C.c_int()
+        _check(lib().plum_surrogate_debug_slot(self.h, slot, rows.ctypes.data_as(C.POINTER(C.c_float)),
+                                               _dp(pos), C.byref(n)))
+        return rows, pos, n.value
 
     def tuning(self):
         t = BatchTuning()

@@ -182,9 +182,16 @@ def config(name, seed=None, **over):
                              "wall_eps": 1.0},
                             {"name": "Z", "sigma": 1.0, "eps": 0.0, "wall_sigma": 1.0,
                              "wall_eps": 0.0}])
-        xyz, q, ty, ms = build_bulk(n_ch, clen, n_ch * clen, None, seed,
-                                    box=[lxy, lxy, lz], z_range=(0.8, lz - 0.8),
-                                    npbc=2, graft=1.0, phantom_sites=sites)
+        # grafted chains + counterions from the native generator (chains that
+        # walk into a dead end are regrown), surface sites first
+        from . import host
+        cx, cq, cty, cms = host.synth_system(n_ch, clen, n_ch * clen, 0, [lxy, lxy, lz], seed,
+                                             npbc=2, graft_z=1.0, z_range=(0.8, lz - 0.8))
+        sx = np.array([p for (p, _, _) in sites])
+        xyz = np.concatenate([sx, cx])
+        q = np.concatenate([[qq for (_, qq, _) in sites], cq])
+        ty = np.concatenate([[t for (_, _, t) in sites], cty]).astype(np.int32)
+        ms = np.concatenate([np.arange(len(sites)), cms + len(sites)]).astype(np.int32)
         base.update(box=[lxy, lxy, lz], npbc=2, alpha=(5.0 / lxy) ** 2, dipole=1,
                     use_ext=1, types=types, phantom=400, grafted=n_ch,
                     move_prob=[0.0, 0.2, 0.5, 0.3, 0.0])

@@ -1,0 +1,179 @@
+"""Delayed acceptance with the learned surrogate on the GPU: the inference
+(BF16 matrix cores) and training (FP32) kernels against the float64
+reference in oracle/surrogate.py, the descriptors against the reference, the
+ensemble sampled with delayed acceptance against the plain chain, and what
+training the network gains."""
+import numpy as np
+import pytest
+
+from oracle import surrogate as S
+from plum_amd import engine as E
+from plum_amd import systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _mc(sd, seed):
+    return dict(move_prob=sd["move_prob"], move_size=sd["move_size"], seed=seed,
+                phantom=sd.get("phantom", 0), grafted=sd.get("grafted", 0), bond_len=1.0)
+
+
+def _engine(name="small", **kw):
+    sd = systems.config(name)
+    eng = E.Engine(sd)
+    eng.surrogate_enable(**kw)
+    return sd, eng
+
+
+def test_inference_and_training_forward_match_reference():
+    sd, eng = _engine(train_every=0, seed=4)
+    w = eng.surrogate_weights()
+    rng = np.random.default_rng(1)
+    x = rng.normal(0, 1.0, (300, S.DESCR)).astype(np.float32)
+    f32 = eng.surrogate_forward(x, bf16=False)
+    ref = S.forward(w, x)
+    scale = np.abs(ref).max()
+    err32 = np.abs(f32 - ref).max() / scale
+    fbf = eng.surrogate_forward(x, bf16=True)
+    refbf = S.forward(w, x, round_bf16=True)
+    errbf = np.abs(fbf - refbf).max() / scale
+    errbf_full = np.abs(fbf - ref).max() / scale
+    print("forward: fp32 rel %.2e, bf16 vs bf16-rounded reference %.2e, vs fp64 %.2e" % (
+        err32, errbf, errbf_full), flush=True)
+    assert err32 < 1e-5
+    # matrix-core layout and BF16 rounding reproduced: only activations that
+    # round differently at a BF16 tie boundary can differ
+    assert errbf < 2e-3
+    assert errbf_full < 5e-2
+    eng.close()
+
+
+def test_training_gradient_matches_reference():
+    sd, eng = _engine(train_every=0, seed=6)
+    maxm = eng.max_mol_len()
+    w = eng.surrogate_weights()
+    rng = np.random.default_rng(2)
+    samples = 5
+    x = rng.normal(0, 1.0, (samples * 2 * maxm, S.DESCR)).astype(np.float32)
+    n = rng.integers(1, maxm + 1, samples).astype(np.int32)
+    target = rng.normal(0, 3.0, samples)
+    loss, grad = eng.surrogate_gradient(x, target, n)
+    rloss, rgrad = S.loss_and_grad(w, x, target, n, maxm, clip=20.0)
+    gerr = np.abs(grad - rgrad).max() / np.abs(rgrad).max()
+    print("gradient: loss %.6e vs %.6e, grad max rel %.2e" % (loss, rloss, gerr), flush=True)
+    assert abs(loss - rloss) <= 1e-4 * rloss
+    assert gerr < 1e-4
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["small", "slab_small"])
+def test_descriptors_match_reference(name):
+    sd, eng = _engine(name, screen=1, fill=0, train_every=0, seed=2)
+    before = eng.positions()
+    eng.mc_run(1, **_mc(sd, 77))
+    rows, pos, n = eng.surrogate_debug_slot(0)
+    maxm = eng.max_mol_len()
+    assert n > 0
+    # the moved beads: their old positions are the pre-batch positions
+    lo = int(np.flatnonzero((before == pos[0, :3]).all(1))[0])
+    excl = (lo, lo + n - 1)
+    box = sd["box"]
+    worst = 0.0
+    for m in range(n):
+        for on in (0, 1):
+            p = pos[on * maxm + m]
+            ref = S.descriptor(p[:3], p[3], before, sd["q"], excl, box, sd["npbc"], 4.0)
+            got = rows[on * maxm + m]
+            worst = max(worst, np.abs(got - ref).max() / max(1.0, np.abs(ref).max()))
+    print("%s descriptors (%d beads): worst rel %.2e" % (name, n, worst), flush=True)
+    assert worst < 2e-4
+    eng.close()
+
+
+def _observables(eng, sd):
+    xyz = eng.positions()
+    ms = sd["mol_start"]
+    rg = []
+    for m in range(len(ms) - 1):
+        a, b = ms[m], ms[m + 1]
+        if b - a > 1:
+            c = xyz[a:b] - xyz[a:b].mean(0)
+            rg.append((c * c).sum(1).mean())
+    t = eng.totals()
+    return sum(t[k] for k in E.TERMS if k != "dipole"), float(np.mean(rg))
+
+
+def _block_stats(v, blocks=10):
+    v = np.asarray(v)
+    b = v[: len(v) // blocks * blocks].reshape(blocks, -1).mean(1)
+    return b.mean(), b.std(ddof=1) / np.sqrt(blocks)
+
+
+def test_delayed_acceptance_samples_the_plain_ensemble():
+    """Energy and radius of gyration averages of the delayed-acceptance chain
+    agree with the plain chain's within their statistical errors, both chains
+    starting from a configuration equilibrated by a long plain run; the
+    delayed-acceptance run keeps its running totals equal to the from-scratch
+    energy."""
+    name = "small"
+    sd0 = systems.config(name)
+    eq = E.Engine(sd0)
+    eq.mc_run(30000, **_mc(sd0, 41))
+    sd = dict(sd0, xyz=eq.positions())
+    eq.close()
+    res = {}
+    for mode in ("plain", "da"):
+        eng = E.Engine(sd)
+        if mode == "da":
+            eng.surrogate_enable(screen=16, fill=2, train_every=8, train_samples=32, seed=3)
+            eng.mc_run(3000, **_mc(sd, 90))      # first training
+        obs = []
+        for i in range(300):
+            eng.mc_run(150, **_mc(sd, 1000 + i))
+            obs.append(_observables(eng, sd))
+        obs = np.array(obs)
+        res[mode] = [_block_stats(obs[:, j]) for j in range(2)]
+        if mode == "da":
+            print("da stats", eng.surrogate_stats(), flush=True)
+            run, full = eng.totals(), eng.energy_full()
+            for k in E.TERMS:
+                assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, run[k], full[k])
+        eng.close()
+    zs = []
+    for j, label in enumerate(("energy", "Rg^2")):
+        (m1, s1), (m2, s2) = res["plain"][j], res["da"][j]
+        z = abs(m1 - m2) / np.hypot(s1, s2)
+        zs.append(z)
+        print("%s: plain %.4f +- %.4f, DA %.4f +- %.4f, |z| = %.2f" % (label, m1, s1, m2, s2, z),
+              flush=True)
+    assert max(zs) < 3.0
+
+
+def test_training_reduces_the_loss_and_the_trained_network_gains():
+    """Collect exact samples, train, and compare the delayed-acceptance chain
+    with the untrained and the trained network: exact evaluations per
+    accepted move (the cost the surrogate exists to cut)."""
+    sd = systems.config("slab_small")
+    eng = E.Engine(sd)
+    eng.surrogate_enable(screen=16, fill=8, train_every=0, train_samples=64, ring=512, seed=8)
+    st0 = eng.mc_run(1500, **_mc(sd, 5))
+    s0 = eng.surrogate_stats()
+    loss0 = eng.surrogate_train(1)
+    loss1 = eng.surrogate_train(300)
+    print("training loss %.3f -> %.3f" % (loss0, loss1), flush=True)
+    assert loss1 < 0.5 * loss0
+    st1 = eng.mc_run(1500, **_mc(sd, 6))
+    s1 = eng.surrogate_stats()
+
+    def per_accept(st, a, b):
+        acc = sum(st.accepted)
+        exact = (b["s1_passed"] - a["s1_passed"])
+        return exact / max(acc, 1), acc
+
+    base = {k: 0 for k in s0}
+    e0, a0 = per_accept(st0, base, s0)
+    e1, a1 = per_accept(st1, s0, s1)
+    print("exact evaluations per accepted move: untrained %.2f (%d accepted), trained %.2f "
+          "(%d accepted)" % (e0, a0, e1, a1), flush=True)
+    assert a1 > 0 and e1 < e0
+    eng.close()
