@@ -6,8 +6,9 @@ A "step" is one device-resident sweep of `--moves-per-step` trial moves
 counterions, density 0.05, Ewald alpha = 0.08 (r_c = 12, |n| <= 18), LJ
 eps = sigma = 1 cut at 2.5, springs k = 400, r0 = 1, move mix 10% COM, 40%
 pivot, 30% crankshaft, 20% reptation.  Inputs are resident in HBM before the
-timed region.  `value` is the FP64 path; the FP32-pair path is reported
-beside it.
+timed region.  `value` is the FP64 path; beside it: the FP32-pair path and
+its per-term dE error, the confined brush with plain vs surrogate-screened
+(delayed acceptance) moves, and the 256-replica batch.
 
 Prints one JSON line (see the module docstring of each field below).
 """
@@ -41,11 +42,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--moves-per-step", type=int, default=1000)
+    ap.add_argument("--moves-per-step", type=int, default=2000)
     ap.add_argument("--config", default="c1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--profile-moves", type=int, default=300)
+    ap.add_argument("--no-brush", action="store_true")
+    ap.add_argument("--no-replicas", action="store_true")
+    ap.add_argument("--brush-moves", type=int, default=20000)
+    ap.add_argument("--replica-sweeps", type=int, default=512)
     return ap.parse_args()
 
 
@@ -126,6 +131,112 @@ def cpu_baseline(sd, target_s=15.0):
     }
 
 
+def fp32_term_errors(sd, moves=20):
+    """Per-term dE of the FP32-pair path against the FP64 path on identical
+    chain moves (host path: the caller's trial coordinates)."""
+    e64 = E.Engine(sd, precision=E.PLUM_FP64)
+    e32 = E.Engine(sd, precision=E.PLUM_FP32_PAIR)
+    full = e64.energy_full()
+    rng = np.random.default_rng(7)
+    ms = sd["mol_start"]
+    chains = [m for m in range(len(ms) - 1) if ms[m + 1] - ms[m] > 1]
+    worst = {k: 0.0 for k in E.TERMS}
+    for _ in range(moves):
+        mol = chains[rng.integers(len(chains))]
+        a, b = ms[mol], ms[mol + 1]
+        idx = np.arange(a, b, dtype=np.int32)
+        tr = e64.positions()[a:b] + rng.normal(0, 0.05, size=(b - a, 3))
+        t64, d64 = e64.delta(mol, idx, tr)
+        _, d32 = e32.delta(mol, idx, tr)
+        for k in E.TERMS:
+            den = max(abs(d64[k]), 1e-3 * abs(full[k]), 1e-2)
+            worst[k] = max(worst[k], abs(d32[k] - d64[k]) / den)
+        acc = t64 < 1e8 and rng.random() < np.exp(-t64)
+        e64.finalize(acc)
+        e32.finalize(acc)
+    e64.close()
+    e32.close()
+    return {k: float("%.3g" % v) for k, v in worst.items()}
+
+
+def brush_leg(moves):
+    """Confined brush (64 x 50 grafted chains + counterions, 400 charged
+    surface sites, slab Ewald): the plain speculative path against delayed
+    acceptance with the online-trained surrogate, in accepted moves per
+    second."""
+    sd = systems.config("c3")
+    kw = dict(move_prob=sd["move_prob"], move_size=sd["move_size"], bond_len=sd["r0"],
+              phantom=sd["phantom"], grafted=sd["grafted"])
+    res = {"workload": "brush 64x50 grafted at 0.1/sigma^2 + 3200 ions + 400 surface sites, "
+                       "walls z=0,40, slab Ewald (Lz 190), moves 20/50/30 COM/pivot/crankshaft",
+           "particles": int(len(sd["q"])), "moves": moves}
+    out = {}
+    for label, pmax in (("plain_p1", 1), ("plain", 8)):
+        eng = E.Engine(sd)
+        eng.set_tuning(pmax=pmax)
+        eng.mc_run(2000, seed=1, **kw)
+        st = eng.mc_run(moves, seed=2, **kw)
+        acc = sum(st.accepted)
+        out[label] = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
+        log("brush %s: %.0f moves/s, %.0f accepted/s" % (label, out[label][0], out[label][1]))
+        eng.close()
+    eng = E.Engine(sd)
+    eng.surrogate_enable(screen=32, fill=4, train_every=16, train_samples=128, ring=2048, lr=1e-3,
+                         r_d=4.0, seed=1)
+    eng.mc_run(3 * moves, seed=1, **kw)          # online training from scratch
+    s0 = eng.surrogate_stats()
+    st = eng.mc_run(moves, seed=2, **kw)
+    s1 = eng.surrogate_stats()
+    acc = sum(st.accepted)
+    da = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
+    log("brush DA: %.0f moves/s, %.0f accepted/s" % (da[0], da[1]))
+    eng.close()
+    res.update({
+        "plain_fp64_moves_per_s": out["plain"][0], "plain_fp64_accepted_per_s": out["plain"][1],
+        "plain_fp64_accept_ratio": out["plain"][2],
+        "plain_one_proposal_moves_per_s": out["plain_p1"][0],
+        "plain_one_proposal_accepted_per_s": out["plain_p1"][1],
+        "surrogate_moves_per_s": da[0], "surrogate_accepted_per_s": da[1],
+        "surrogate_accept_ratio": da[2],
+        "accepted_per_s_vs_plain": da[1] / out["plain"][1],
+        "accepted_per_s_vs_one_proposal_plain": da[1] / out["plain_p1"][1],
+        "first_stage_pass_fraction": (s1["s1_passed"] - s0["s1_passed"]) / max(st.moves, 1),
+        "exact_evals_per_accepted": ((s1["s1_passed"] - s0["s1_passed"]) +
+                                     (s1["train_evals"] - s0["train_evals"])) / max(acc, 1),
+        "stale_cuts": s1["stale_cuts"] - s0["stale_cuts"],
+        "training_loss_kT2": s1["last_loss"], "training_steps": s1["train_steps"],
+    })
+    return res
+
+
+def replica_leg(sweeps):
+    """256 independent bulk systems (chain length 10..200, 5..100 chains,
+    0..2000 salt pairs, half of them charged, density 0.02..0.2), FP32 pair
+    math, advanced together by the batched kernels after the launch-shape
+    autotuner."""
+    t0 = time.perf_counter()
+    sds = systems.replica_set(256)
+    engs = [E.Engine(sd, precision=E.PLUM_FP32_PAIR) for sd in sds]
+    log("replicas: %d particles ready in %.1f s" % (sum(len(s["q"]) for s in sds),
+                                                    time.perf_counter() - t0))
+    b = E.Batch(engs)
+    mc = [dict(move_prob=sd["move_prob"], move_size=1.0, seed=1000 + r, bond_len=1.0)
+          for r, sd in enumerate(sds)]
+    t = b.tune(mc, moves_per_trial=16)
+    st = b.mc_run(sweeps, mc)
+    log("replicas: %d moves in %.2f s" % (st.moves, st.seconds))
+    res = {"workload": "256 bulk replicas, chain length U[10,200] x U[5,100] chains, salt "
+                       "U[0,2000] pairs, charged with p=0.5, density U[0.02,0.2]; FP32 pairs",
+           "particles": int(sum(len(s["q"]) for s in sds)), "moves_per_replica": sweeps,
+           "moves_per_s": st.moves / st.seconds,
+           "accept_ratio": sum(st.accepted) / max(sum(st.attempted), 1),
+           "tuning": {"blocks_per_cu": t.blocks_per_cu, "split_div": t.split_div, "pmax": t.pmax}}
+    b.close()
+    for e in engs:
+        e.close()
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -200,9 +311,14 @@ def main():
         log("FP32-pair: %d moves in %.3f s" % (st32["moves"], s32))
         out["fp32pair_kernel_avg_us"] = {k: round(v[0], 3) for k, v in kt32.items()}
         e32.close()
+    eng.close()
+    out["dE_rel_err_fp32pair_vs_fp64"] = fp32_term_errors(sd)
+    if not a.no_brush:
+        out["brush_delayed_acceptance"] = brush_leg(a.brush_moves)
+    if not a.no_replicas:
+        out["replicas"] = replica_leg(a.replica_sweeps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sd)
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

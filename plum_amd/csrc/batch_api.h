@@ -48,11 +48,13 @@ struct SurHost {
   double* tr_target = nullptr;
   int *tr_n = nullptr, *tr_ctl = nullptr;
   int* h_ctl = nullptr;   // pinned copy of tr_ctl
+  double* h_loss = nullptr;  // pinned copy of the last training loss
   int tb_cap = 0;         // minibatch capacity (samples)
   float *X = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *F = nullptr, *G = nullptr;
   float *D3 = nullptr, *D2 = nullptr, *D1 = nullptr;
   double *T = nullptr, *loss = nullptr;
   int* N = nullptr;
+  float* part = nullptr;  // chunk partials of the weight gradients
   int step = 0;
   long long train_steps = 0;
   double last_loss = 0;
@@ -63,16 +65,17 @@ void sur_free(SurHost* s) {
   if (!s) return;
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+  if (s->h_loss) (void)hipHostFree(s->h_loss);
   delete s;
 }
 
 namespace {
 
-int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool update);
+int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool update, bool sync);
 
-constexpr int kSpecKernels = 5;
-const char* kSpecKernelNames[kSpecKernels] = {"k_propose_spec", "k_plan_spec", "k_compute_spec", "k_reduce_spec",
-                                              "k_resolve_spec"};
+constexpr int kSpecKernels = 7;
+const char* kSpecKernelNames[kSpecKernels] = {"k_propose_spec", "k_descr_spec", "k_mlp_infer", "k_plan_spec",
+                                              "k_compute_spec", "k_reduce_spec", "k_resolve_spec"};
 
 template <typename T>
 int balloc(plum_batch* b, T** ptr, size_t count) {
@@ -130,29 +133,33 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
   if (ev) (void)hipEventRecord(ev[0], b->st);
   hipLaunchKernelGGL(k_propose_spec, dim3(b->tune.pmax, R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC,
                      R);
+  if (ev) (void)hipEventRecord(ev[1], b->st);
   if (b->da) {
     // surrogate: descriptors of every moved bead (old, new), network outputs
     plum_ctx* c = b->reps[0];
     hipLaunchKernelGGL(k_descr_spec, dim3(2 * c->maxm, b->tune.pmax), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC);
+    if (ev) (void)hipEventRecord(ev[2], b->st);
     const int rows = b->tune.pmax * 2 * c->maxm;
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(b->n_cu, (rows + 127) / 128))), dim3(kBlock), 0, b->st,
                        c->sur->net, c->sur->sur_x, rows, c->sur->sur_f);
+  } else if (ev) {
+    (void)hipEventRecord(ev[2], b->st);
   }
-  if (ev) (void)hipEventRecord(ev[1], b->st);
-  hipLaunchKernelGGL(k_plan_spec, dim3(R), dim3(kWave), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, b->d_vb, R);
-  if (ev) (void)hipEventRecord(ev[2], b->st);
+  if (ev) (void)hipEventRecord(ev[3], b->st);
+  hipLaunchKernelGGL(k_plan_spec, dim3(R), dim3(kPlanThreads), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, b->d_vb, R);
+  if (ev) (void)hipEventRecord(ev[4], b->st);
   if (b->precision == PLUM_FP32_PAIR)
     hipLaunchKernelGGL(k_compute_spec<1>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
                        b->d_vb, R);
   else
     hipLaunchKernelGGL(k_compute_spec<0>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
                        b->d_vb, R);
-  if (ev) (void)hipEventRecord(ev[3], b->st);
+  if (ev) (void)hipEventRecord(ev[5], b->st);
   hipLaunchKernelGGL(k_reduce_spec, dim3(kKVals + kMaxProp + kMaxPairs, R), dim3(kBlock), 0, b->st, b->dG, b->dD,
                      b->dRC, R);
-  if (ev) (void)hipEventRecord(ev[4], b->st);
+  if (ev) (void)hipEventRecord(ev[6], b->st);
   hipLaunchKernelGGL(k_resolve_spec, dim3(R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, R);
-  if (ev) (void)hipEventRecord(ev[5], b->st);
+  if (ev) (void)hipEventRecord(ev[7], b->st);
 }
 
 void launch_tail(plum_batch* b) {
@@ -180,12 +187,21 @@ int batch_replay(plum_batch* b, int nb, bool use_graph, plum_ctx* prof) {
       launch_batch(b, ev.data());
       HIPCHK(hipEventSynchronize(ev[kSpecKernels]));
       for (int k = 0; k < kSpecKernels; k++) {
+        if (!b->da && (k == 1 || k == 2)) continue;   // no surrogate kernels
         float ms = 0;
         if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) {
           prof->ksum_us[k] += 1000.0 * ms;
           prof->kcount[k] += 1;
         }
       }
+      // stop once every system has made its moves (no idle batches timed)
+      hipLaunchKernelGGL(k_batch_status, dim3((b->R + kBlock - 1) / kBlock), dim3(kBlock), 0, b->st, b->dD, b->dMC,
+                         b->R, b->d_status);
+      HIPCHK(hipMemcpyAsync(b->h_status, b->d_status, sizeof(int2) * b->R, hipMemcpyDeviceToHost, b->st));
+      HIPCHK(hipStreamSynchronize(b->st));
+      bool busy = false;
+      for (int r = 0; r < b->R; r++) busy |= b->h_status[r].y > 0 && !b->h_status[r].x;
+      if (!busy) break;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     launch_tail(b);
@@ -341,7 +357,7 @@ int batch_run(plum_batch* b, const plum_mc_params* mcp, long long n_moves, bool 
       // online training on the newest exact samples
       SurHost* su = b->reps[0]->sur;
       if (su->p.train_every > 0 && su->h_ctl[0] >= su->p.train_samples)
-        CHK(sur_train_step(b->reps[0], b->st, su->p.train_samples, true, true));
+        CHK(sur_train_step(b->reps[0], b->st, su->p.train_samples, true, true, false));
     }
     since_resync += (long long)b->tune.batches_per_graph * b->tune.pmax;
     bool busy = false, fix = false;

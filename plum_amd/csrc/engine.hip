@@ -683,11 +683,13 @@ int plum_set_profiling(plum_ctx* c, int on) {
 
 int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, long long* count) {
   if (!c) return fail(PLUM_E_ARG, "null context");
-  const int n = std::min(cap, (int)c->kname.size());
-  for (int i = 0; i < n; i++) {
-    if (names) names[i] = c->kname[i].c_str();
-    if (avg_us) avg_us[i] = c->kcount[i] ? c->ksum_us[i] / c->kcount[i] : 0.0;
-    if (count) count[i] = c->kcount[i];
+  int n = 0;
+  for (size_t i = 0; i < c->kname.size() && n < cap; i++) {
+    if (!c->kcount[i]) continue;   // kernels that did not run
+    if (names) names[n] = c->kname[i].c_str();
+    if (avg_us) avg_us[n] = c->ksum_us[i] / c->kcount[i];
+    if (count) count[n] = c->kcount[i];
+    n++;
   }
   return n;
 }

@@ -38,6 +38,8 @@ struct PropInfo {
   int forbid;          // a moved bead entered a wall (1e8)
   int n, mol, type;    // moved beads, molecule, move type
   int valid, skipped;  // attempted move / dropped after a cell overflow
+  int prev;            // earlier proposal of the same molecule in the batch, -1 none
+  int pad;
 };
 
 // Plan and reduced sums of a speculative batch.  Proposals live in screen

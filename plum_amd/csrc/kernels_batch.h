@@ -12,8 +12,8 @@
 // plus the surrogate's two between proposal and plan:
 //   k_propose_spec   grid (pmax, R): proposal s of system r (move number
 //                    counter + s), its phase tables, spring / wall / dipole
-//                    parts and bounding sphere (proposals stop at the first
-//                    molecule already proposed).
+//                    parts and bounding sphere (a molecule proposed twice is
+//                    proposed from its current coordinates the second time).
 //   k_plan_spec      grid R: first-stage decisions, exact slots, block
 //                    counts; pairs of exact slots within interaction range
 //                    get cross-term blocks.
@@ -30,7 +30,9 @@
 //                    then the commits.
 // Every proposal is evaluated against the state before the batch; accepting
 // p changes q's dE by exactly the cross terms, so the chain is the sequential
-// one (the random numbers depend only on the move number).  A surrogate value
+// one (the random numbers depend only on the move number).  A second
+// proposal of one molecule assumes the first is rejected; if it was
+// accepted, the batch ends before the second.  A surrogate value
 // has no such correction: the batch ends before the first proposal whose
 // descriptors could see a bead accepted earlier in the batch, and that
 // proposal is made again, from the same move number, in the next batch.
@@ -126,7 +128,7 @@ struct PropSmem {
   PrSmem pr;
   double sbuf[kBlock];
   double sh[kBlock / kWave];
-  int s_forbid, s_cut;
+  int s_forbid, s_cut, s_prev;
   MoveSel s_sel;
   MoveSel s_sels[kMaxScreen];
 };
@@ -147,12 +149,14 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   if (threadIdx.x < rc.pmax) sm.s_sels[threadIdx.x] = select_move(d, mc, ctr0 + threadIdx.x);
   __syncthreads();
   if (threadIdx.x == 0) {
-    // proposals stop at the first molecule already in the batch (its
-    // coordinates would depend on an earlier decision) and at the run's end
+    // a molecule proposed again in the batch is proposed from its current
+    // coordinates, i.e. on the assumption that its earlier proposal is
+    // rejected; the resolve ends the batch there if it was accepted
     int cut = rc.pmax;
-    for (int q = 1; q < rc.pmax && cut == rc.pmax; q++)
-      for (int q2 = 0; q2 < q; q2++)
-        if (sm.s_sels[q2].mol == sm.s_sels[q].mol) cut = q;
+    int prev = -1;
+    for (int q2 = 0; q2 < p; q2++)
+      if (sm.s_sels[q2].mol == sm.s_sels[p].mol) prev = q2;
+    sm.s_prev = prev;
     sm.s_sel = sm.s_sels[p];
     if (mc.ctr_limit <= ctr0) cut = 0;
     else if ((long long)(mc.ctr_limit - ctr0) < cut) cut = (int)(mc.ctr_limit - ctr0);
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   const int was_valid = sel.valid;
   if (overflow) sel.valid = 0;   // the cell list is incomplete: skip, redo later
   const Dev v = prop_view(g, d, rc, p, 0);
-  propose_block(g, v, mc, ctr0 + p, sel, p * rc.maxm, sm.pr);
+  propose_block(g, v, mc, ctr0 + p, sel, sm.pr);
   __syncthreads();
   if (g.use_ewald && g.n_kh > 0) tables_block(g, v);
   // spring energy of the trial molecule, wall and dipole parts, bounding
@@ -177,7 +181,8 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   const int a = d.mol_start[mol], bend = d.mol_start[mol + 1];
   double ebt = 0;
   // (a valid move that moves no bead keeps the molecule's current energy)
-  if (v.rec->valid && g.use_bond && bend - a > 1) ebt = molecule_spring(g, d, mol, true, sm.sbuf);
+  if (v.rec->valid && g.use_bond && bend - a > 1)
+    ebt = molecule_spring_range(g, d, mol, v.rec->lo, v.rec->hi, v.mv_new, sm.sbuf);
   if (threadIdx.x == 0) sm.s_forbid = 0;
   __syncthreads();
   double ew = 0, dmz = 0, cx = 0, cy = 0, cz = 0;
@@ -236,6 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
     pi.type = v.rec->move_type;
     pi.valid = v.rec->valid;
     pi.skipped = (overflow && was_valid) ? 1 : 0;
+    pi.prev = sm.s_prev;
   }
 }
 
@@ -243,13 +249,18 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
 // acceptance), exact slots, real-space + moved-moved blocks per exact slot,
 // the exact-slot pairs within interaction range (bounding spheres closer than
 // the search radius) and their cross-term blocks, the virtual-block count.
+constexpr int kPlanThreads = 512;
+
 struct PlanSmem {
   PropInfo s_prop[kMaxScreen];
   double s_nn[kMaxScreen];
   double s_bcur[kMaxScreen];
+  double s_u1[kMaxScreen];
+  int s_len[kMaxScreen];
 };
 
-__global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D, const McDev* __restrict__ MC,
+__global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
+                                                            const McDev* __restrict__ MC,
                             const RepCfg* __restrict__ RC, int* vb_count, int R) {
   __shared__ PlanSmem sm;
   const int r = blockIdx.x;
@@ -261,21 +272,35 @@ __global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D
   SpecPlan* plan = d.plan;
   const int np = plan->n_prop;
   const int t = threadIdx.x;
+  const unsigned long long ctr0 = d.ctl->move_counter;
   if (t < np) {
     const PropInfo P = d.prop[t];
     sm.s_prop[t] = P;
     sm.s_bcur[t] = d.bond_cur[P.mol];
-    double nn = 0;
-    if (rc.da && P.valid) {
-      // network part of the surrogate: sum over moved beads of f(new) - f(old)
-      const float* f = d.sur_f + (size_t)(2 * t) * rc.maxm;
-      for (int m = 0; m < P.n; m++) nn += (double)f[rc.maxm + m] - (double)f[m];
-    }
-    sm.s_nn[t] = nn;
+    sm.s_len[t] = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
+    const unsigned long long ctr = ctr0 + t;
+    const uint4 u = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xDA1u, 0u),
+                                make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
+    sm.s_u1[t] = u01_32(u.x);
   }
   __syncthreads();
+  if (rc.da) {
+    // network part of the surrogate, one wave per proposal: sum over moved
+    // beads of f(new) - f(old), fixed order
+    const int w = t / kWave, lane = t % kWave;
+    for (int s = w; s < np; s += kPlanThreads / kWave) {
+      const PropInfo& P = sm.s_prop[s];
+      const float* f = d.sur_f + (size_t)(2 * s) * rc.maxm;
+      double nn = 0;
+      if (P.valid)
+        for (int m = lane; m < P.n; m += kWave) nn += (double)f[rc.maxm + m] - (double)f[m];
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) nn += __shfl_down(nn, off, kWave);
+      if (lane == 0) sm.s_nn[s] = nn;
+    }
+    __syncthreads();
+  }
   if (t != 0) return;
-  const unsigned long long ctr0 = d.ctl->move_counter;
   int ne = 0, ndec = np;
   for (int s = 0; s < np; s++) {
     const PropInfo& P = sm.s_prop[s];
@@ -285,14 +310,10 @@ __global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D
     if (rc.da && P.valid) {
       // surrogate dE: exact spring and wall parts plus the network's estimate
       // of the pair and k-space parts; first stage min(1, exp(-beta dE_s))
-      const int len = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
-      if (g.use_bond && len > 1) des += P.bond_trial - sm.s_bcur[s];
+      if (g.use_bond && sm.s_len[s] > 1) des += P.bond_trial - sm.s_bcur[s];
       if (g.use_ext) des += P.forbid ? kVeryLargeEnergy : P.ew;
       des += sm.s_nn[s];
-      const unsigned long long ctr = ctr0 + s;
-      const uint4 u = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xDA1u, 0u),
-                                  make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
-      pass = des < kVeryLargeEnergy && u01_32(u.x) < exp(-g.beta * des);
+      pass = des < kVeryLargeEnergy && sm.s_u1[s] < exp(-g.beta * des);
     }
     plan->pass1[s] = pass;
     plan->de_s[s] = des;
@@ -329,7 +350,7 @@ __global__ void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D
     if (pq.n == 0) continue;
     for (int e0 = 0; e0 < e; e0++) {
       const PropInfo& pp = sm.s_prop[plan->screen_of[e0]];
-      if (pp.n == 0) continue;
+      if (pp.n == 0 || pp.mol == pq.mol) continue;
       if (center_dist(pp.c, pq.c, g) > pp.rad + pq.rad + g.rs + 1e-9) continue;
       plan->pair_p[nn] = e0;
       plan->pair_q[nn] = e;
@@ -664,6 +685,12 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     const int ring = rc.da ? d.tr_ctl[1] : 0;
     for (int s = 0; s < ndec; s++) {
       const PropInfo& P = sm.s_prop[s];
+      if (P.prev >= 0 && ((acc_screen >> P.prev) & 1ull)) {
+        // proposed on the assumption that the molecule's earlier proposal in
+        // the batch is rejected; it was accepted: propose again next batch
+        done = s;
+        break;
+      }
       if (rc.da && acc_screen) {
         // the surrogate value of this proposal was taken before an earlier
         // acceptance; if its descriptors could see that molecule, stop here
@@ -823,14 +850,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         dst[i] = src[i];
         dst[(size_t)rc.maxm * kDescr + i] = src[(size_t)rc.maxm * kDescr + i];
       }
-    }
-  }
-  for (int s = 0; s < np; s++) {
-    const int n = sm.s_prop[s].n;
-    for (int k = threadIdx.x; k < n; k += kBlock) {
-      const int i = d.smv_idx[s * rc.maxm + k];
-      d.moved[i] = 0;
-      d.mv_slot[i] = -1;
     }
   }
 }

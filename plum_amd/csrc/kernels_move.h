@@ -377,6 +377,30 @@ __device__ double molecule_spring(const Geo& g, const Dev& d, int mol, bool tria
   return acc;  // valid in thread 0
 }
 
+// Spring energy of molecule `mol` with the beads lo..hi at mv_new[i - lo]
+// (a proposal's contiguous moved range) and the others at their current
+// positions; same fixed-order block tree as molecule_spring.
+__device__ double molecule_spring_range(const Geo& g, const Dev& d, int mol, int lo, int hi,
+                                        const double4* mv_new, double* sbuf) {
+  const int a = d.mol_start[mol], b = d.mol_start[mol + 1];
+  double acc = 0;
+  for (int base = a; base < b - 1; base += kBlock) {
+    const int i = base + threadIdx.x;
+    double e = 0;
+    if (i < b - 1) {
+      double p0[3] = {d.x[i], d.y[i], d.z[i]};
+      double p1[3] = {d.x[i + 1], d.y[i + 1], d.z[i + 1]};
+      if (i >= lo && i <= hi) { const double4 v = mv_new[i - lo]; p0[0] = v.x; p0[1] = v.y; p0[2] = v.z; }
+      if (i + 1 >= lo && i + 1 <= hi) { const double4 v = mv_new[i + 1 - lo]; p1[0] = v.x; p1[1] = v.y; p1[2] = v.z; }
+      const double dx = p0[0] - p1[0], dy = p0[1] - p1[1], dz = p0[2] - p1[2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      e = 0.5 * g.k_bond * (r - g.r0) * (r - g.r0);
+    }
+    acc += block_sum<kBlock>(e, sbuf);
+  }
+  return acc;  // valid in thread 0
+}
+
 // Removes the tombstones of one cell, keeping slot order (one thread).
 __device__ void compact_cell(const Geo& g, const Dev& d, int cell) {
   const int cnt = d.ccount[cell];
@@ -687,8 +711,8 @@ __device__ MoveSel select_move(const Dev& d, const McDev& mc, unsigned long long
 // One proposal of Simulation::TranslationalMove (simulation.cc:241-312) with
 // the move generators of molecule.cc:103-312, drawing from Philox4x32-10
 // (counter = move number, streams: 1 move parameters, 2+i bead i).  `d` is
-// the view of one proposal slot (its rec / mv_* arrays); moved-list indices
-// are stored in mv_slot as slot_base + k.
+// the view of one proposal slot (its rec / mv_* arrays); nothing shared per
+// bead is written, so two proposals of one molecule can coexist in a batch.
 struct PrSmem {
   double s_b[kMaxChainDev][3];
   int s_mol, s_type, s_valid, s_a, s_len;
@@ -697,7 +721,7 @@ struct PrSmem {
 };
 
 __device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, unsigned long long ctr, MoveSel sel,
-                              int slot_base, PrSmem& sm) {
+                              PrSmem& sm) {
   double (&s_b)[kMaxChainDev][3] = sm.s_b;
   int& s_mol = sm.s_mol;
   int& s_type = sm.s_type;
@@ -851,8 +875,6 @@ __device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, unsig
   for (int k = threadIdx.x; k < nmv; k += kBlock) {
     const int i = (mt == 3) ? d.mv_idx[k] : a + k;
     if (mt != 3) d.mv_idx[k] = i;
-    d.moved[i] = 1;
-    d.mv_slot[i] = slot_base + k;
     d.mv_old[k] = make_double4(d.x[i], d.y[i], d.z[i], d.q[i]);
     d.mv_type[k] = d.type[i];
   }

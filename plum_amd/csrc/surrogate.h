@@ -298,22 +298,39 @@ __global__ void k_gemm_abt_dtanh(const float* __restrict__ A, const float* __res
   Cm[idx] = s * (1.f - h * h);
 }
 
-// G[K1 x K2] = A^T B for A[M x K1], B[M x K2] (weight gradients), summed over
-// rows in order; with bias_g, also the column sums of B into bias_g (K2).
-__global__ void k_gemm_atb(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ Gm,
-                           float* __restrict__ bias_g, int M, int K1, int K2) {
+// G[K1 x K2] = A^T B for A[M x K1], B[M x K2] (weight gradients) in two
+// passes: k_gemm_atb_part sums rows [c R, (c + 1) R) of chunk c into
+// part[c][K1 x K2 (+ K2 column sums of B)], k_gemm_atb_sum adds the chunks in
+// order.
+constexpr int kAtbRows = 128;
+
+__global__ void k_gemm_atb_part(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ part,
+                                int M, int K1, int K2) {
+  const int c = blockIdx.y;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nout = K1 * K2 + K2;
+  if (idx >= nout) return;
+  const int m0 = c * kAtbRows, m1 = min(M, m0 + kAtbRows);
+  float s = 0.f;
   if (idx < K1 * K2) {
     const int i = idx / K2, j = idx - (idx / K2) * K2;
-    float s = 0.f;
-    for (int m = 0; m < M; m++) s += A[(size_t)m * K1 + i] * B[(size_t)m * K2 + j];
-    Gm[idx] = s;
-  } else if (bias_g && idx < K1 * K2 + K2) {
+    for (int m = m0; m < m1; m++) s += A[(size_t)m * K1 + i] * B[(size_t)m * K2 + j];
+  } else {
     const int j = idx - K1 * K2;
-    float s = 0.f;
-    for (int m = 0; m < M; m++) s += B[(size_t)m * K2 + j];
-    bias_g[j] = s;
+    for (int m = m0; m < m1; m++) s += B[(size_t)m * K2 + j];
   }
+  part[(size_t)c * nout + idx] = s;
+}
+
+__global__ void k_gemm_atb_sum(const float* __restrict__ part, int chunks, int K1, int K2, float* __restrict__ Gm,
+                               float* __restrict__ bias_g) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nout = K1 * K2 + K2;
+  if (idx >= nout) return;
+  float s = 0.f;
+  for (int c = 0; c < chunks; c++) s += part[(size_t)c * nout + idx];
+  if (idx < K1 * K2) Gm[idx] = s;
+  else if (bias_g) bias_g[idx - K1 * K2] = s;
 }
 
 // Minibatch loss: sample i owns rows [i R, (i + 1) R) of f (R = 2 maxm: old

@@ -32,6 +32,7 @@ int sur_workspace(plum_ctx* c, int samples) {
   CHK(salloc(s, &s->G, M));
   CHK(salloc(s, &s->T, samples));
   CHK(salloc(s, &s->N, samples));
+  CHK(salloc(s, &s->part, ((M + kAtbRows - 1) / kAtbRows) * (size_t)(kHid * kHid + kHid)));
   s->tb_cap = samples;
   return PLUM_OK;
 }
@@ -51,9 +52,18 @@ void sur_forward32(plum_ctx* c, hipStream_t st, size_t M) {
                      kHid, 0);
 }
 
+// dW = A^T B (and the bias gradient, column sums of B) over M rows.
+void sur_atb(SurHost* s, hipStream_t st, const float* A, const float* B, float* Gm, float* bias_g, int M, int K1,
+             int K2) {
+  const int chunks = (M + kAtbRows - 1) / kAtbRows;
+  const int nout = K1 * K2 + K2;
+  hipLaunchKernelGGL(k_gemm_atb_part, dim3(nblocks(nout), chunks), dim3(kBlock), 0, st, A, B, s->part, M, K1, K2);
+  hipLaunchKernelGGL(k_gemm_atb_sum, dim3(nblocks(nout)), dim3(kBlock), 0, st, s->part, chunks, K1, K2, Gm, bias_g);
+}
+
 // Loss and gradient of the samples in s->X / s->T / s->N, then (update) one
 // Adam step and the BF16 repack.
-int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool update) {
+int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool update, bool sync) {
   SurHost* s = c->sur;
   CHK(sur_workspace(c, samples));
   const size_t M = (size_t)samples * 2 * c->maxm;
@@ -66,19 +76,15 @@ int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool u
   float* g = s->grad;
   const float* w = s->w32;
   const int m = (int)M;
-  hipLaunchKernelGGL(k_gemm_atb, dim3(nblocks(kHid + 1)), dim3(kBlock), 0, st, s->H3, s->G, g + kOffW4, g + kOffB4,
-                     m, kHid, 1);
+  sur_atb(s, st, s->H3, s->G, g + kOffW4, g + kOffB4, m, kHid, 1);
   hipLaunchKernelGGL(k_out_delta, dim3(nblocks(M * kHid)), dim3(kBlock), 0, st, s->G, w + kOffW4, s->H3, s->D3, m);
-  hipLaunchKernelGGL(k_gemm_atb, dim3(nblocks(kHid * kHid + kHid)), dim3(kBlock), 0, st, s->H2, s->D3, g + kOffW3,
-                     g + kOffB3, m, kHid, kHid);
+  sur_atb(s, st, s->H2, s->D3, g + kOffW3, g + kOffB3, m, kHid, kHid);
   hipLaunchKernelGGL(k_gemm_abt_dtanh, dim3(nblocks(M * kHid)), dim3(kBlock), 0, st, s->D3, w + kOffW3, s->H2, s->D2,
                      m, kHid, kHid);
-  hipLaunchKernelGGL(k_gemm_atb, dim3(nblocks(kHid * kHid + kHid)), dim3(kBlock), 0, st, s->H1, s->D2, g + kOffW2,
-                     g + kOffB2, m, kHid, kHid);
+  sur_atb(s, st, s->H1, s->D2, g + kOffW2, g + kOffB2, m, kHid, kHid);
   hipLaunchKernelGGL(k_gemm_abt_dtanh, dim3(nblocks(M * kHid)), dim3(kBlock), 0, st, s->D2, w + kOffW2, s->H1, s->D1,
                      m, kHid, kHid);
-  hipLaunchKernelGGL(k_gemm_atb, dim3(nblocks(kDescr * kHid + kHid)), dim3(kBlock), 0, st, s->X, s->D1, g + kOffW1,
-                     g + kOffB1, m, kDescr, kHid);
+  sur_atb(s, st, s->X, s->D1, g + kOffW1, g + kOffB1, m, kDescr, kHid);
   if (update) {
     s->step++;
     hipLaunchKernelGGL(k_adam, dim3(nblocks(kNParams)), dim3(kBlock), 0, st, s->w32, s->grad, s->m1, s->m2, kNParams,
@@ -87,8 +93,11 @@ int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool u
     s->train_steps++;
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(&s->last_loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpyAsync(s->h_loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+  if (sync) {
+    HIPCHK(hipStreamSynchronize(st));
+    s->last_loss = *s->h_loss;
+  }
   return PLUM_OK;
 }
 
@@ -142,8 +151,10 @@ int plum_surrogate_enable(plum_ctx* c, const plum_surrogate_params* p) {
       (rc = salloc(s, &s->tr_target, p->ring)) || (rc = salloc(s, &s->tr_n, p->ring)) ||
       (rc = salloc(s, &s->tr_ctl, 2)) || (rc = salloc(s, &s->loss, 1)))
     return bail(rc);
-  if (hipHostMalloc((void**)&s->h_ctl, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void**)&s->h_ctl, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_loss, sizeof(double), hipHostMallocDefault) != hipSuccess)
     return bail(fail(PLUM_E_NOMEM, "hipHostMalloc failed"));
+  *s->h_loss = 0;
   // weights: uniform(-a, a), a = sqrt(6 / (fan_in + fan_out)) (Glorot &
   // Bengio 2010); output row scaled by 0.1; biases zero
   std::vector<float> w(kNParams, 0.f);
@@ -211,6 +222,7 @@ int plum_surrogate_stats_get(plum_ctx* c, plum_surrogate_stats* out) {
   HIPCHK(hipMemcpyAsync(c->h_ctl, c->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(c->sur->h_ctl, c->sur->tr_ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  c->sur->last_loss = *c->sur->h_loss;
   out->s1_rejected = c->h_ctl->s1_rejected;
   out->s1_passed = c->h_ctl->s1_passed;
   out->train_evals = c->h_ctl->train_evals;
@@ -255,7 +267,7 @@ int plum_surrogate_train(plum_ctx* c, int steps, double* loss) {
   const int have = std::min(c->sur->h_ctl[0], c->sur->p.ring);
   const int samples = std::min(have, c->sur->p.train_samples);
   if (samples < 1) return fail(PLUM_E_STATE, "no training samples collected yet");
-  for (int i = 0; i < steps; i++) CHK(sur_train_step(c, st, samples, true, true));
+  for (int i = 0; i < steps; i++) CHK(sur_train_step(c, st, samples, true, true, i == steps - 1));
   if (loss) *loss = c->sur->last_loss;
   return PLUM_OK;
 }
@@ -294,7 +306,7 @@ int plum_surrogate_gradient(plum_ctx* c, const float* x, const double* target, c
   HIPCHK(hipMemcpyAsync(s->X, x, sizeof(float) * M * kDescr, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->T, target, sizeof(double) * samples, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(s->N, n, sizeof(int) * samples, hipMemcpyHostToDevice, st));
-  CHK(sur_train_step(c, st, samples, false, false));
+  CHK(sur_train_step(c, st, samples, false, false, true));
   HIPCHK(hipMemcpyAsync(grad, s->grad, sizeof(float) * kNParams, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (loss) *loss = s->last_loss;
