@@ -279,6 +279,28 @@ int plum_surrogate_gradient(plum_ctx* ctx, const float* x, const double* target,
 int plum_surrogate_debug_slot(plum_ctx* ctx, int slot, float* rows, double* pos, int* n);
 int plum_max_mol_len(const plum_ctx* ctx);
 
+/* ---- configurational-bias chain growth (grand-canonical moves) --------- */
+
+typedef struct {
+  int n_trials;          /* K trial positions of this growth step */
+  int grown;             /* beads of the trial chain placed so far */
+  int exclude_lo, exclude_hi;  /* bead index range left out of the existing
+                                  system (the chain being deleted and its
+                                  counterions); exclude_lo > exclude_hi: none */
+  int type_chain, type_ion;
+  int charged;           /* each chain bead comes with a counterion */
+  double q_chain, q_ion;
+} plum_cbmc_request;
+
+/* Replaces ForceField::BeadsEnergy (cbmc.cc:5-151) for all K trials of one
+   growth step (the loop of CBMCFGenTrialBeads, cbmc.cc:161-212) in one
+   pass on the GPU: trials = K chain-bead positions then, if charged, K
+   counterion positions; grown = the chain beads placed so far then their
+   counterions; energies[K] receives each trial's energy (1e8 when LJ or a
+   wall forbids it).  The slab dipole term is not supported. */
+int plum_cbmc_trials(plum_ctx* ctx, const plum_cbmc_request* req, const double* trials, const double* grown,
+                     double* energies);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,10 @@ def lib():
         L.orc_radius_of_gyration.restype = C.c_double
         L.orc_end_to_end.argtypes = [C.c_void_p]
         L.orc_end_to_end.restype = C.c_double
+        L.orc_cbmc_beads_energy.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, dp, C.c_int,
+                                            C.c_double, C.c_int, dp, dp, C.c_int, C.c_int,
+                                            C.c_int]
+        L.orc_cbmc_beads_energy.restype = C.c_double
         _lib = L
     return _lib
 
@@ -245,6 +249,23 @@ class Oracle:
     def bead_energy_sample(self, xyz, q, t, n_partners):
         p = np.ascontiguousarray(xyz, dtype=np.float64)
         return lib().orc_bead_energy_sample(self.h, _dp(p), float(q), int(t), int(n_partners))
+
+    def cbmc_beads_energy(self, b1, t1, q1, grown=None, b2=None, t2=0, q2=0.0, grown_ions=None,
+                          exclude_mols=None):
+        """ForceField::BeadsEnergy (cbmc.cc:5-151) of a trial chain bead b1 and,
+        for a charged chain, its counterion b2, against the system minus the
+        molecules exclude_mols = (lo, hi) and the chain grown so far."""
+        ch = b2 is not None
+        p1 = np.ascontiguousarray(b1, dtype=np.float64)
+        p2 = np.ascontiguousarray(b2 if ch else np.zeros(3), dtype=np.float64)
+        g = np.zeros((0, 3)) if grown is None else np.asarray(grown, dtype=np.float64).reshape(-1, 3)
+        cur = len(g)
+        gc = np.ascontiguousarray(g) if cur else np.zeros(3)
+        gi = (np.ascontiguousarray(np.asarray(grown_ions, dtype=np.float64).reshape(cur, 3))
+              if ch and cur else np.zeros(max(cur, 1) * 3))
+        lo, hi = exclude_mols if exclude_mols is not None else (0, -1)
+        return lib().orc_cbmc_beads_energy(self.h, _dp(p1), int(t1), float(q1), _dp(p2), int(t2),
+                                           float(q2), int(ch), _dp(gc), _dp(gi), cur, lo, hi)
 
     def rg(self):
         return lib().orc_radius_of_gyration(self.h)

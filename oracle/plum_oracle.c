@@ -942,6 +942,92 @@ double orc_bead_energy_sample(orc_sys* s, const double xyz[3], double q, int t,
   return e;
 }
 
+/* ForceField::BeadsEnergy (cbmc.cc:5-151): trial chain bead b1 (type t1,
+   charge q1) and, for charged chains, its trial counterion b2, against the
+   molecules outside [ex_lo, ex_hi] (cbmc.cc:23; none when ex_lo > ex_hi),
+   the grown chain beads grown_c[0..cur) and counterions grown_i[0..cur),
+   with the reference's break on LJ >= 1e8.  The slab dipole term
+   (DipoleEDiff) is not restated.  Trial coordinates as the reference reads
+   them (equal to the current ones outside a pending move). */
+double orc_cbmc_beads_energy(orc_sys* s, const double b1[3], int t1, double q1, const double b2[3],
+                             int t2, double q2, int charged, const double* grown_c,
+                             const double* grown_i, int cur, int ex_lo, int ex_hi) {
+  double pair_e = 0, ewald_e = 0;
+  double pair_e1 = 0, pair_e2 = 0, ewald_r1 = 0, ewald_r2 = 0, ewald_k1 = 0, ewald_k2 = 0;
+  const int use_pair = s->p.use_pair, use_ewald = s->p.use_ewald;
+  /* 1. other molecules */
+  for (int m = 0; m < s->n_mols; m++) {
+    if (ex_lo <= ex_hi && m >= ex_lo && m <= ex_hi) continue;
+    for (int j = s->mol_start[m]; j < s->mol_start[m + 1]; j++) {
+      const double* bj = &s->trial[3 * j];
+      if (use_pair) {
+        pair_e1 = lj_pair_xyz(s, b1, t1, bj, s->type[j]);
+        if (charged) pair_e2 = lj_pair_xyz(s, b2, t2, bj, s->type[j]);
+      }
+      pair_e += pair_e1 + pair_e2;
+      if (pair_e >= kVeryLargeEnergy) break;
+      if (use_ewald) {
+        ewald_r1 = real_pair_xyz(s, b1, q1, bj, s->q[j]);
+        ewald_k1 = repl_pair_xyz(s, b1, q1, bj, s->q[j]);
+        if (charged) {
+          ewald_r2 = real_pair_xyz(s, b2, q2, bj, s->q[j]);
+          ewald_k2 = repl_pair_xyz(s, b2, q2, bj, s->q[j]);
+        }
+      }
+      ewald_e += ewald_r1 + ewald_k1 + ewald_r2 + ewald_k2;
+    }
+    if (pair_e >= kVeryLargeEnergy) break;
+  }
+  /* 2. the grown part of the trial chain */
+  for (int i = 0; i < cur; i++) {
+    if (use_pair) {
+      pair_e1 = 0;
+      if (i < cur - 1) pair_e1 += lj_pair_xyz(s, b1, t1, &grown_c[3 * i], t1);
+      if (charged) {
+        pair_e1 += lj_pair_xyz(s, b1, t1, &grown_i[3 * i], t2);
+        pair_e2 = lj_pair_xyz(s, b2, t2, &grown_c[3 * i], t1);
+        pair_e2 += lj_pair_xyz(s, b2, t2, &grown_i[3 * i], t2);
+      }
+    }
+    pair_e += pair_e1 + pair_e2;
+    if (pair_e >= kVeryLargeEnergy) break;
+    if (use_ewald) {
+      ewald_r1 = real_pair_xyz(s, b1, q1, &grown_c[3 * i], q1);
+      ewald_k1 = repl_pair_xyz(s, b1, q1, &grown_c[3 * i], q1);
+      if (charged) {
+        ewald_r1 += real_pair_xyz(s, b1, q1, &grown_i[3 * i], q2);
+        ewald_k1 += repl_pair_xyz(s, b1, q1, &grown_i[3 * i], q2);
+        ewald_r2 = real_pair_xyz(s, b2, q2, &grown_c[3 * i], q1);
+        ewald_k2 = repl_pair_xyz(s, b2, q2, &grown_c[3 * i], q1);
+        ewald_r2 += real_pair_xyz(s, b2, q2, &grown_i[3 * i], q2);
+        ewald_k2 += repl_pair_xyz(s, b2, q2, &grown_i[3 * i], q2);
+      }
+    }
+    ewald_e += ewald_r1 + ewald_k1 + ewald_r2 + ewald_k2;
+  }
+  /* 3. the pair (b1, b2), self energies and self images */
+  if (use_pair && charged) pair_e += lj_pair_xyz(s, b1, t1, b2, t2);
+  if (use_ewald && pair_e < kVeryLargeEnergy) {
+    ewald_e += self_energy(s, q1);
+    ewald_e += 0.5 * real_pair_xyz(s, b1, q1, b1, q1);
+    ewald_e += 0.5 * repl_pair_xyz(s, b1, q1, b1, q1);
+    if (charged) {
+      ewald_e += self_energy(s, q2);
+      ewald_e += real_pair_xyz(s, b1, q1, b2, q2);
+      ewald_e += repl_pair_xyz(s, b1, q1, b2, q2);
+      ewald_e += 0.5 * real_pair_xyz(s, b2, q2, b2, q2);
+      ewald_e += 0.5 * repl_pair_xyz(s, b2, q2, b2, q2);
+    }
+  }
+  /* 4. walls */
+  if (s->p.use_ext) {
+    pair_e += wall_bead_z(s, b1[2], t1);
+    if (charged) pair_e += wall_bead_z(s, b2[2], t2);
+  }
+  if (pair_e >= kVeryLargeEnergy) return kVeryLargeEnergy;
+  return pair_e + ewald_e;
+}
+
 /* Simulation::RadiusOfGyration (simulation.cc:406-434) over all chains. */
 double orc_radius_of_gyration(const orc_sys* s) {
   static const double nobox[3] = {0, 0, 0};

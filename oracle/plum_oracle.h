@@ -148,6 +148,15 @@ double orc_bead_energy_sample(orc_sys* s, const double xyz[3], double q, int t,
 double orc_radius_of_gyration(const orc_sys* s);
 double orc_end_to_end(const orc_sys* s);
 
+/* ForceField::BeadsEnergy (cbmc.cc:5-151) for one trial (chain bead b1 and,
+   if charged, its counterion b2); molecules ex_lo..ex_hi left out (none if
+   ex_lo > ex_hi).  Pinned by the GPU trial energies only through this
+   restatement of the reference loops (parity unpinned beyond the pair
+   functions it calls, which are pinned). */
+double orc_cbmc_beads_energy(orc_sys* s, const double b1[3], int t1, double q1, const double b2[3],
+                             int t2, double q2, int charged, const double* grown_c,
+                             const double* grown_i, int cur, int ex_lo, int ex_hi);
+
 #ifdef __cplusplus
 }
 #endif

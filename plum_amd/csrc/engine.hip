@@ -14,6 +14,7 @@
 //   control block:        dipole moment, dipole lag state, running totals,
 //                         the last dE per term, status flags, move counters
 #include "kernels_batch.h"
+#include "kernels_cbmc.h"
 #include "kernels_full.h"
 #include "kernels_move.h"
 #include "plum_gpu.h"
@@ -31,6 +32,8 @@ using namespace plum;
 
 struct plum_batch;
 struct SurHost;
+struct CbmcHost;
+void cbmc_free(CbmcHost* h);
 extern "C" int plum_batch_destroy(plum_batch* batch);
 void sur_free(SurHost* s);
 
@@ -53,6 +56,7 @@ struct plum_ctx {
   int xcap = 0;          // cross-term partial capacity
   plum_batch* self = nullptr;  // this system as a batch of one (device loop)
   SurHost* sur = nullptr;      // delayed-acceptance surrogate (null when off)
+  CbmcHost* cbmc = nullptr;    // configurational-bias trial buffers
   int screen_cap = kMaxProp;   // proposal slots allocated
   double* epart_k = nullptr;
   std::vector<int> mol_start;
@@ -532,6 +536,7 @@ int plum_destroy(plum_ctx* c) {
   if (!c) return PLUM_OK;
   if (c->self) plum_batch_destroy(c->self);
   if (c->sur) sur_free(c->sur);
+  if (c->cbmc) cbmc_free(c->cbmc);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void* p : c->dev_allocs) (void)hipFree(p);
   if (c->h_in) (void)hipHostFree(c->h_in);
@@ -698,3 +703,4 @@ int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, 
 
 #include "batch_api.h"
 #include "surrogate_api.h"
+#include "cbmc_api.h"

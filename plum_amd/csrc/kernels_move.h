@@ -198,27 +198,18 @@ struct RSmem {
   double sh[kBlock / kWave];
 };
 
+// LJ + real-space energy of one point p (charge p.w, type tm) against every
+// bead image in the cell list within the search radius, skipping the index
+// range [mlo, mhi] (contig) or the moved flags; candidate slots split over
+// `nsplit` blocks (this block takes share `part`).  Result valid in thread 0.
 template <int P>
-__device__ void real_block(const Geo& g, const Dev& d, int b, int nreal, int nmm, int nsplit, RSmem<P>& sm) {
+__device__ double2 real_query(const Geo& g, const Dev& d, double4 p, int tm, int contig, int mlo, int mhi, int part,
+                              int nsplit, RSmem<P>& sm) {
   int* s_cell = sm.s_cell;
   int* s_img = sm.s_img;
   int* s_end = sm.s_end;
   float (&s_rel)[P ? kMaxQueryCells : 1][3] = sm.s_rel;
   double* sh = sm.sh;
-  if (b >= nreal) {
-    mm_block(g, d, b - nreal, nmm, sh);
-    return;
-  }
-  const int q2 = b / nsplit, part = b - q2 * nsplit;
-  const int m = q2 >> 1, on = q2 & 1;
-  const int nm = d.rec->n_moved;
-  if (m >= nm) {
-    if (threadIdx.x == 0) d.part_r[b] = make_double2(0, 0);
-    return;
-  }
-  const int contig = d.rec->contig, mlo = d.rec->lo, mhi = d.rec->hi;
-  const double4 p = on ? d.mv_new[m] : d.mv_old[m];
-  const int tm = d.mv_type[m];
   const double pw[3] = {wrap_coord(p.x, g.L[0], g.inv_L[0]), wrap_coord(p.y, g.L[1], g.inv_L[1]),
                         wrap_coord(p.z, g.L[2], g.inv_L[2])};
   int lo[3], nd[3];
@@ -253,7 +244,16 @@ __device__ void real_block(const Geo& g, const Dev& d, int b, int nreal, int nmm
       }
       s_cell[t] = cell;
       s_img[t] = img;
-      s_end[t] = d.ccount[cell];
+      // cells whose box (widened by a rounding margin) lies beyond the search
+      // radius hold no candidate
+      double dmin2 = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double b0 = (double)u[k] * g.h[k] - 1e-9 * g.L[k], b1 = b0 + g.h[k] + 2e-9 * g.L[k];
+        const double dd = fmax(0.0, fmax(b0 - pw[k], pw[k] - b1));
+        dmin2 += dd * dd;
+      }
+      s_end[t] = dmin2 > g.rs2 ? 0 : d.ccount[cell];
     }
     __syncthreads();
     lds_scan(s_end, nloc);
@@ -331,10 +331,28 @@ __device__ void real_block(const Geo& g, const Dev& d, int b, int nreal, int nmm
     }
     __syncthreads();
   }
-  const double sgn = on ? 1.0 : -1.0;
   const double a = block_sum<kBlock>(elj, sh);
   const double c = block_sum<kBlock>(ere, sh);
-  if (threadIdx.x == 0) d.part_r[b] = make_double2(sgn * a, sgn * c);
+  return make_double2(a, c);
+}
+
+template <int P>
+__device__ void real_block(const Geo& g, const Dev& d, int b, int nreal, int nmm, int nsplit, RSmem<P>& sm) {
+  if (b >= nreal) {
+    mm_block(g, d, b - nreal, nmm, sm.sh);
+    return;
+  }
+  const int q2 = b / nsplit, part = b - q2 * nsplit;
+  const int m = q2 >> 1, on = q2 & 1;
+  const int nm = d.rec->n_moved;
+  if (m >= nm) {
+    if (threadIdx.x == 0) d.part_r[b] = make_double2(0, 0);
+    return;
+  }
+  const double4 p = on ? d.mv_new[m] : d.mv_old[m];
+  const double2 e = real_query<P>(g, d, p, d.mv_type[m], d.rec->contig, d.rec->lo, d.rec->hi, part, nsplit, sm);
+  const double sgn = on ? 1.0 : -1.0;
+  if (threadIdx.x == 0) d.part_r[b] = make_double2(sgn * e.x, sgn * e.y);
 }
 
 // All dE work of one trial move in one launch: blocks [0, nkb) k space,

@@ -26,7 +26,7 @@ ABI_SYMBOLS = (
     "plum_mc_set_tuning", "plum_surrogate_enable", "plum_surrogate_disable",
     "plum_surrogate_stats_get", "plum_surrogate_get_weights", "plum_surrogate_set_weights",
     "plum_surrogate_train", "plum_surrogate_forward", "plum_surrogate_gradient",
-    "plum_surrogate_debug_slot", "plum_max_mol_len",
+    "plum_surrogate_debug_slot", "plum_max_mol_len", "plum_cbmc_trials",
 )
 SURROGATE_PARAMS = 64 * 128 + 128 + 2 * (128 * 128 + 128) + 128 + 1
 DESCR = 64
@@ -104,6 +104,12 @@ class SurrogateParams(C.Structure):
                 ("r_d", C.c_double), ("target_clip", C.c_double), ("seed", C.c_ulonglong)]
 
 
+class CbmcRequest(C.Structure):
+    _fields_ = [("n_trials", C.c_int), ("grown", C.c_int), ("exclude_lo", C.c_int),
+                ("exclude_hi", C.c_int), ("type_chain", C.c_int), ("type_ion", C.c_int),
+                ("charged", C.c_int), ("q_chain", C.c_double), ("q_ion", C.c_double)]
+
+
 class SurrogateStats(C.Structure):
     _fields_ = [("s1_rejected", C.c_longlong), ("s1_passed", C.c_longlong),
                 ("train_evals", C.c_longlong), ("stale_cuts", C.c_longlong),
@@ -160,6 +166,7 @@ def lib():
         L.plum_surrogate_gradient.argtypes = [vp, fp, dp, ip, C.c_int, dp, fp]
         L.plum_surrogate_debug_slot.argtypes = [vp, C.c_int, fp, dp, ip]
         L.plum_max_mol_len.argtypes = [vp]
+        L.plum_cbmc_trials.argtypes = [vp, C.POINTER(CbmcRequest), dp, dp, dp]
         for f in ABI_SYMBOLS:
             if f != "plum_last_error":
                 getattr(L, f).restype = C.c_int
@@ -334,6 +341,29 @@ class Engine:
         loss = C.c_double()
         _check(lib().plum_surrogate_train(self.h, steps, C.byref(loss)))
         return loss.value
+
+    def cbmc_trials(self, trials, type_chain, q_chain, grown=None, ions=None, type_ion=0,
+                    q_ion=0.0, grown_ions=None, exclude=None):
+        """Energies of K trial positions of one configurational-bias growth
+        step (ForceField::BeadsEnergy, cbmc.cc:5-151): trials (K, 3) chain
+        beads, ions (K, 3) their counterions for a charged chain, grown (cur,
+        3) the chain beads placed so far and grown_ions their counterions,
+        exclude = (lo, hi) a bead index range left out of the system."""
+        t = np.ascontiguousarray(trials, dtype=np.float64).reshape(-1, 3)
+        K = len(t)
+        ch = ions is not None
+        tri = np.concatenate([t, np.ascontiguousarray(ions, dtype=np.float64).reshape(K, 3)]) if ch else t
+        g = np.zeros((0, 3)) if grown is None else np.asarray(grown, dtype=np.float64).reshape(-1, 3)
+        cur = len(g)
+        if ch and cur:
+            g = np.concatenate([g, np.asarray(grown_ions, dtype=np.float64).reshape(cur, 3)])
+        g = np.ascontiguousarray(g) if cur else np.zeros(3)
+        lo, hi = exclude if exclude is not None else (0, -1)
+        rq = CbmcRequest(K, cur, lo, hi, type_chain, type_ion, int(ch), q_chain, q_ion)
+        tri = np.ascontiguousarray(tri)
+        out = np.zeros(K)
+        _check(lib().plum_cbmc_trials(self.h, C.byref(rq), _dp(tri), _dp(g), _dp(out)))
+        return out
 
     def surrogate_forward(self, x, bf16=True):
         fp = C.POINTER(C.c_float)

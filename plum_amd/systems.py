@@ -20,6 +20,9 @@ import os
 import numpy as np
 
 CONFIG_SEEDS = {"c0": 1000, "c1": 1001, "c2": 1002, "c3": 1003, "c4": 1004}
+# chemical potential (kT) of the exchanged chain + counterions of the
+# grand-canonical workload
+GC_CHEM_POT = 0.0
 
 
 class _Grid:
@@ -152,6 +155,25 @@ def build_bulk(n_chains, chain_len, n_ions, box_l, seed, bead_q=-1.0, ion_q=1.0,
             np.array(ty, dtype=np.int32), np.array(ms, dtype=np.int32))
 
 
+def gc_layout(xyz, q, ty, ms, n_chains, chain_len):
+    """Reorder a bulk system of n_chains chains followed by n_chains *
+    chain_len single-bead counterions into the grand-canonical layout of the
+    reference (cbmc.cc:367-373: each exchanged chain is followed by its
+    chain_len counterions, so chain c is molecule c (1 + chain_len))."""
+    order = []
+    n_ch_mols = n_chains
+    for c in range(n_chains):
+        order.append(c)
+        order.extend(n_ch_mols + c * chain_len + k for k in range(chain_len))
+    nx, nq, nt, nms = [], [], [], [0]
+    for m in order:
+        a, b = ms[m], ms[m + 1]
+        nx.append(xyz[a:b]); nq.append(q[a:b]); nt.append(ty[a:b])
+        nms.append(nms[-1] + b - a)
+    return (np.concatenate(nx), np.concatenate(nq), np.concatenate(nt).astype(np.int32),
+            np.array(nms, dtype=np.int32))
+
+
 def config(name, seed=None, **over):
     """Named workloads.  c0/c1/c3 follow the stated configurations; 'tiny' and
     'small' are quick test systems."""
@@ -195,6 +217,17 @@ def config(name, seed=None, **over):
         base.update(box=[lxy, lxy, lz], npbc=2, alpha=(5.0 / lxy) ** 2, dipole=1,
                     use_ext=1, types=types, phantom=400, grafted=n_ch,
                     move_prob=[0.0, 0.2, 0.5, 0.3, 0.0])
+    elif name == "c4":
+        L, n_ch, clen = 60.0, 50, 32
+        xyz, q, ty, ms = gc_layout(*build_bulk(n_ch, clen, n_ch * clen, L, seed), n_ch, clen)
+        base.update(box=[L, L, L], alpha=(5.0 / L) ** 2, gc=dict(
+            chain_len=clen, bead_charge=-1.0, trials=16, freq=10, chem_pot=GC_CHEM_POT,
+            debroglie=1.0))
+    elif name == "gc_small":
+        L, n_ch, clen = 12.0, 4, 6
+        xyz, q, ty, ms = gc_layout(*build_bulk(n_ch, clen, n_ch * clen, L, seed), n_ch, clen)
+        base.update(box=[L, L, L], alpha=(5.0 / L) ** 2, gc=dict(
+            chain_len=clen, bead_charge=-1.0, trials=8, freq=4, chem_pot=0.0, debroglie=1.0))
     elif name == "tiny":
         L = 8.0
         xyz, q, ty, ms = build_bulk(2, 5, 10, L, seed)
