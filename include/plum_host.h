@@ -33,10 +33,19 @@ int plum_runin_parse(const char* run_in_path, plum_runin_summary* out);
 /* Simulation built from a run.in path (crd/top resolved next to it), seeded
    explicitly (the reference seeds from the clock, simulation.cc:133-136). */
 int plum_sim_create(const char* run_in_path, unsigned seed, plum_sim** out);
-/* Simulation::Run without grand-canonical moves and without output files
-   (simulation.cc:216-355): `steps` trial moves on the host RNG stream, each
-   energy difference evaluated on the GPU. */
+/* Simulation::Run without output files (simulation.cc:216-404): `steps`
+   steps on the host RNG stream, each a translational trial move (energy
+   difference on the GPU) or, with grand-canonical moves on, every gc_freq-th
+   draw on average a chain insertion or deletion by configurational bias
+   (cbmc.cc:213-441; each growth step's trial energies in one GPU pass). */
 int plum_sim_run(plum_sim* sim, long long steps, long long attempted[5], long long accepted[5]);
+/* Grand-canonical counters: [0] insertions, [1] deletions; and the current
+   number of beads and molecules (they change with accepted exchanges). */
+int plum_sim_gc_stats(plum_sim* sim, long long attempted[2], long long accepted[2], int* n_beads, int* n_mols);
+/* ForceField::CalcChemicalPotentialF (cbmc.cc:444-533): the mean weight of
+   50 Widom test insertions of the exchanged chain species (or, without
+   grand-canonical moves, of the first chain species, 30 trials per bead). */
+int plum_sim_widom(plum_sim* sim, double* mean_weight);
 int plum_sim_energy(plum_sim* sim, plum_terms* running, plum_terms* from_scratch);
 int plum_sim_positions(plum_sim* sim, double* xyz, int n_beads);
 /* Simulation::RadiusOfGyration / EndToEndDistance (simulation.cc:406-501). */

@@ -48,6 +48,17 @@ class OrcMT(C.Structure):
     _fields_ = [("mt", C.c_uint32 * 624), ("mti", C.c_int)]
 
 
+class OrcGC(C.Structure):
+    _fields_ = [("chem_pot", C.c_double), ("debroglie", C.c_double), ("bead_charge", C.c_double),
+                ("rigid_bond", C.c_double), ("chain_len", C.c_int), ("trials", C.c_int),
+                ("type_c", C.c_int), ("bond", C.c_int), ("phantom", C.c_int), ("coion", C.c_int),
+                ("grafted", C.c_int), ("grafted_counterion", C.c_int)]
+
+
+class OrcGauss(C.Structure):
+    _fields_ = [("iset", C.c_int), ("gset", C.c_double)]
+
+
 class OrcStepInfo(C.Structure):
     _fields_ = [
         ("attempted", C.c_int), ("move_type", C.c_int), ("mol", C.c_int),
@@ -118,6 +129,18 @@ def lib():
                                             C.c_double, C.c_int, dp, dp, C.c_int, C.c_int,
                                             C.c_int]
         L.orc_cbmc_beads_energy.restype = C.c_double
+        L.orc_gc_insertion.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
+                                       C.POINTER(OrcGC), dp, dp]
+        L.orc_gc_deletion.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
+                                      C.POINTER(OrcGC), dp]
+        L.orc_gc_widom.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
+                                   C.POINTER(OrcGC), C.c_int]
+        L.orc_gc_widom.restype = C.c_double
+        L.orc_gasdev.argtypes = [C.c_double, C.c_double, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
+        L.orc_gasdev.restype = C.c_double
+        L.orc_translational_move.argtypes = [C.c_void_p, C.POINTER(OrcMT), dp, C.c_double,
+                                             C.c_int, C.c_int, C.c_double, C.c_int,
+                                             C.POINTER(OrcStepInfo)]
         _lib = L
     return _lib
 
@@ -314,3 +337,99 @@ def mc_step(orc, rng, move_prob, move_size, phantom=0, grafted=0, bond_len=1.0,
     lib().orc_mc_step(orc.h, C.byref(rng.s), _dp(mp), move_size, phantom, grafted,
                       bond_len, int(vary_bond), C.byref(info))
     return info
+
+
+class GCSim:
+    """Simulation::Run with grand-canonical chain exchange (simulation.cc:
+    216-404) on the oracle: one MT19937 stream, translational moves through
+    orc_translational_move, insertions and deletions through
+    orc_gc_insertion / orc_gc_deletion; the system is rebuilt after each
+    accepted exchange.  The exchanged species follows the reference: chain
+    beads and counterions both of the exchanged bead type."""
+
+    def __init__(self, sd, seed, rigid_bond=None):
+        self.sd = dict(sd)
+        g = sd["gc"]
+        self.freq = g["freq"]
+        self.gc = OrcGC(chem_pot=g["chem_pot"], debroglie=g["debroglie"],
+                        bead_charge=g["bead_charge"],
+                        rigid_bond=sd["r0"] if rigid_bond is None else rigid_bond,
+                        chain_len=g["chain_len"], trials=g["trials"], type_c=g.get("type", 0),
+                        bond=1 if sd["use_bond"] else 2, phantom=sd.get("phantom", 0), coion=0,
+                        grafted=sd.get("grafted", 0), grafted_counterion=0)
+        self.rng = MT19937(seed)
+        self.gauss = OrcGauss()
+        self.orc = Oracle(self.sd)
+        self.attempted = [0] * 5
+        self.accepted = [0] * 5
+        self.gc_attempted = [0, 0]
+        self.gc_accepted = [0, 0]
+
+    def _rebuild(self, xyz, q, ty, ms):
+        self.orc.close()
+        self.sd.update(xyz=xyz, q=q, type=ty, mol_start=ms)
+        self.orc = Oracle(self.sd)
+
+    def _gc_move(self):
+        L = self.gc.chain_len
+        if self.rng.next() % 2:
+            self.gc_attempted[0] += 1
+            chain = np.zeros(6 * L)
+            w = C.c_double()
+            if lib().orc_gc_insertion(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                      C.byref(self.gc), _dp(chain), C.byref(w)):
+                self.gc_accepted[0] += 1
+                chain = chain.reshape(-1, 3)
+                ch = self.gc.bead_charge != 0
+                add = chain if ch else chain[:L]
+                xyz = np.concatenate([self.orc.positions(), add])
+                qb = self.gc.bead_charge
+                q = np.concatenate([self.sd["q"], np.full(L, qb)] + ([np.full(L, -qb)] if ch else []))
+                ty = np.concatenate([self.sd["type"], np.full(len(add), self.gc.type_c)])
+                ms = list(self.sd["mol_start"])
+                n = ms[-1]
+                ms.append(n + L)
+                if ch:
+                    ms.extend(n + L + 1 + np.arange(L))
+                self._rebuild(xyz, q, ty.astype(np.int32), np.array(ms, dtype=np.int32))
+        else:
+            self.gc_attempted[1] += 1
+            n_mol = len(self.sd["mol_start"]) - 1
+            if n_mol - self.gc.phantom - self.gc.coion <= 0:
+                return
+            w = C.c_double()
+            d = lib().orc_gc_deletion(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                      C.byref(self.gc), C.byref(w))
+            if d >= self.gc.phantom + self.gc.coion:
+                self.gc_accepted[1] += 1
+                ms = self.sd["mol_start"]
+                ions = L if self.gc.bead_charge != 0 else 0
+                lo, hi = ms[d], ms[d + ions + 1]
+                keep = np.r_[0:lo, hi:ms[-1]]
+                nms = np.r_[ms[:d + 1], ms[d + ions + 2:] - (hi - lo)]
+                self._rebuild(self.orc.positions()[keep], self.sd["q"][keep],
+                              self.sd["type"][keep], nms.astype(np.int32))
+
+    def run(self, steps):
+        sd = self.sd
+        mp = np.ascontiguousarray(sd["move_prob"], dtype=np.float64)
+        for _ in range(steps):
+            x = self.rng.next()
+            r = x - (1 << 32) if x >= (1 << 31) else x
+            if r % self.freq == 0:
+                self._gc_move()
+            else:
+                info = OrcStepInfo()
+                lib().orc_translational_move(self.orc.h, C.byref(self.rng.s), _dp(mp),
+                                             sd["move_size"], sd.get("phantom", 0),
+                                             sd.get("grafted", 0), sd["r0"], 1, C.byref(info))
+                if info.attempted:
+                    self.attempted[info.move_type] += 1
+                    self.accepted[info.move_type] += info.accepted
+
+    def widom(self, n_ins=50):
+        return lib().orc_gc_widom(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                  C.byref(self.gc), n_ins)
+
+    def close(self):
+        self.orc.close()

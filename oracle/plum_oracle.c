@@ -865,9 +865,15 @@ int orc_move_reptation(orc_sys* s, int mol, orc_mt19937* r, double eq_bond_len,
 void orc_mc_step(orc_sys* s, orc_mt19937* r, const double move_prob[5],
                  double move_size, int phantom, int grafted, double bond_len,
                  int vary_bond, orc_step_info* info) {
+  (void)orc_mt_next(r);  /* simulation.cc:221, the per-step draw */
+  orc_translational_move(s, r, move_prob, move_size, phantom, grafted, bond_len, vary_bond, info);
+}
+
+void orc_translational_move(orc_sys* s, orc_mt19937* r, const double move_prob[5],
+                            double move_size, int phantom, int grafted, double bond_len,
+                            int vary_bond, orc_step_info* info) {
   memset(info, 0, sizeof(*info));
   info->mol = -1;
-  (void)orc_mt_next(r);  /* simulation.cc:221, the per-step draw */
   int n_mol = s->n_mols;
   /* Simulation::UpdateMolCounts (simulation.cc:1066-1094) */
   int n_chain = 0, n_cion = 0, n_aion = 0, n_nion = 0;
@@ -1062,4 +1068,232 @@ double orc_end_to_end(const orc_sys* s) {
     nc++;
   }
   return nc ? e / nc : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* grand-canonical chain exchange (cbmc.cc:152-533)                    */
+/* ------------------------------------------------------------------ */
+
+/* gasdev (misc.cc:111-133): Marsaglia's polar method, the second deviate
+   of each pair cached for the next call. */
+double orc_gasdev(double mean, double stdev, orc_mt19937* r, orc_gauss* gs) {
+  if (gs->iset == 0) {
+    double v1, v2, rsq;
+    do {
+      v1 = 2.0 * u01(r) - 1.0;
+      v2 = 2.0 * u01(r) - 1.0;
+      rsq = v1 * v1 + v2 * v2;
+    } while (rsq >= 1.0 || rsq == 0.0);
+    const double fac = sqrt(-2.0 * log(rsq) / rsq);
+    gs->gset = v1 * fac;
+    gs->iset = 1;
+    return v2 * fac * stdev + mean;
+  }
+  gs->iset = 0;
+  return gs->gset * stdev + mean;
+}
+
+/* PotentialSpring::RandomBondLen (potential_spring.cc:50-61): Gaussian
+   around r0 of width 1/sqrt(beta k), thinned by len^2 / (r0 + 3 sigma)^2. */
+static double random_bond_len(const orc_sys* s, orc_mt19937* r, orc_gauss* gs) {
+  const double sigma = sqrt(1 / (s->p.beta * s->p.k_bond));
+  const double a = (s->p.r0 + 3 * sigma) * (s->p.r0 + 3 * sigma);
+  double len = 0;
+  int ready = 0;
+  while (!ready) {
+    len = orc_gasdev(s->p.r0, sigma, r, gs);
+    ready = u01(r) < (len * len / a);
+  }
+  return len;
+}
+
+/* ForceField::UpdateMolCounts (force_field.cc:1258-1277) */
+static void gc_counts(const orc_sys* s, const orc_gc* gc, int* n_chain, int* n_cion, int* n_aion) {
+  int c = 0, ci = 0, ai = 0;
+  for (int m = gc->phantom; m < s->n_mols; m++) {
+    const int a = s->mol_start[m], len = s->mol_start[m + 1] - a;
+    if (len > 1) c++;
+    else if (s->q[a] >= 0) ci++;
+    else ai++;
+  }
+  *n_chain = c - gc->grafted;
+  *n_cion = ci;
+  *n_aion = ai;
+}
+
+static int gc_spc1(const orc_gc* gc, int n_chain, int n_cion, int n_aion) {
+  if (gc->chain_len > 1) return n_chain;
+  if (gc->bead_charge >= 0) return n_cion - gc->coion;
+  return n_aion - gc->coion;
+}
+
+static int gc_spc2(const orc_gc* gc, int n_cion, int n_aion) {
+  if (gc->bead_charge == 0) return 0;
+  return gc->bead_charge > 0 ? n_aion : n_cion;
+}
+
+/* ForceField::CBMCFGenTrialBeads (cbmc.cc:161-212): K trial chain beads at a
+   sampled bond length in a random direction from `end`, each with a
+   counterion uniform in the box for a charged chain; tri holds the chain
+   trials then the counterion trials.  Returns W = sum of the weights. */
+static double gen_trials(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, const double* end,
+                         const double* chain, int cur, int ex_lo, int ex_hi, double* tri, double* wts) {
+  const int K = gc->trials, L = gc->chain_len, ch = gc->bead_charge != 0;
+  double W = 0;
+  for (int i = 0; i < K; i++) {
+    double bond_len = 0;
+    if (gc->bond == 1) bond_len = random_bond_len(s, r, gs);
+    else if (gc->bond == 2) bond_len = gc->rigid_bond;
+    double v[3];
+    orc_rand_sphere(r, v);
+    for (int j = 0; j < 3; j++) tri[3 * i + j] = v[j] * bond_len + end[j];
+    if (ch)
+      for (int j = 0; j < 3; j++) tri[3 * (K + i) + j] = u01(r) * s->p.box[j];
+    const double* b2 = ch ? &tri[3 * (K + i)] : &tri[3 * i];
+    const double e = orc_cbmc_beads_energy(s, &tri[3 * i], gc->type_c, gc->bead_charge, b2, gc->type_c,
+                                           -gc->bead_charge, ch, chain, chain + 3 * L, cur, ex_lo, ex_hi);
+    wts[i] = exp(-s->p.beta * e);
+    W += wts[i];
+  }
+  return W;
+}
+
+/* The growth of a new chain shared by the insertion and the test insertion
+   of the chemical potential: first bead(s) uniform in the box, the rest
+   chosen among the trials by their weights.  Returns the Rosenbluth weight;
+   widom selects the chemical-potential variant (no early return after the
+   first bead, cbmc.cc:489-500). */
+static double grow_chain(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain, int widom) {
+  const int K = gc->trials, L = gc->chain_len, ch = gc->bead_charge != 0;
+  double* tri = (double*)malloc(sizeof(double) * 6 * K);
+  double* wts = (double*)malloc(sizeof(double) * K);
+  double weight = 1.0;
+  for (int i = 0; i < (ch ? 2 : 1); i++)
+    for (int j = 0; j < 3; j++) chain[3 * (i * L) + j] = u01(r) * s->p.box[j];
+  const double e0 = orc_cbmc_beads_energy(s, chain, gc->type_c, gc->bead_charge, ch ? &chain[3 * L] : chain,
+                                          gc->type_c, -gc->bead_charge, ch, chain, chain + 3 * L, 0, 0, -1);
+  if (widom) {
+    if (e0 == 1e8) weight *= 0;
+    else weight *= exp(-s->p.beta * e0);
+  } else {
+    weight *= exp(-s->p.beta * e0);
+    if (weight <= 0) goto done;
+  }
+  for (int i = 1; i < L; i++) {
+    const double Wi = gen_trials(s, r, gs, gc, &chain[3 * (i - 1)], chain, i, 0, -1, tri, wts);
+    weight *= Wi / K;
+    if (weight <= 0) break;
+    const double rn = u01(r) * Wi;
+    int cb = 0;
+    double cum = wts[0];
+    while (cum < rn && cb < K - 1) {
+      cb++;
+      cum += wts[cb];
+    }
+    for (int j = 0; j < 3; j++) chain[3 * i + j] = tri[3 * cb + j];
+    if (ch)
+      for (int j = 0; j < 3; j++) chain[3 * (i + L) + j] = tri[3 * (K + cb) + j];
+  }
+done:
+  free(tri);
+  free(wts);
+  return weight;
+}
+
+int orc_gc_insertion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain_out,
+                     double* weight_out) {
+  const int L = gc->chain_len;
+  memset(chain_out, 0, sizeof(double) * 6 * L);
+  const double weight = grow_chain(s, r, gs, gc, chain_out, 0);
+  if (weight_out) *weight_out = weight;
+  if (weight <= 0) return 0;
+  int n_chain, n_cion, n_aion;
+  gc_counts(s, gc, &n_chain, &n_cion, &n_aion);
+  const int spc1 = gc_spc1(gc, n_chain, n_cion, n_aion), spc2 = gc_spc2(gc, n_cion, n_aion);
+  const int spc1_add = 1, spc2_add = gc->bead_charge == 0 ? 0 : L;
+  double factorial = 1;
+  for (int i = spc1 + 1; i <= spc1 + spc1_add; i++) factorial *= i;
+  for (int i = spc2 + 1; i <= spc2 + spc2_add; i++) factorial *= i;
+  const double vol = s->p.box[0] * s->p.box[1] * s->p.box[2];
+  double vol_over_lam = 1;
+  vol_over_lam *= pow(vol / (gc->debroglie / pow((double)L, 1.5)), spc1_add);
+  vol_over_lam *= pow(vol / (gc->debroglie / pow(1.0, 1.5)), spc2_add);
+  const double C = vol_over_lam * (1.0 / factorial);
+  const double rn = u01(r);
+  return rn < (exp(s->p.beta * gc->chem_pot) * weight) * C;
+}
+
+int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* weight_out) {
+  const int K = gc->trials, L = gc->chain_len, ch = gc->bead_charge != 0;
+  int n_chain, n_cion, n_aion;
+  gc_counts(s, gc, &n_chain, &n_cion, &n_aion);
+  const int spc1 = gc_spc1(gc, n_chain, n_cion, n_aion);
+  int del = (int)floor(u01(r) * spc1);
+  if (del >= spc1) del = spc1 - 1;
+  /* nothing to delete: the reference would index molecule -(1 + L) */
+  if (del < 0) return -1;
+  del = gc->phantom + gc->coion + gc->grafted + gc->grafted_counterion + del * (ch ? 1 + L : 1);
+  const int counterion = ch ? L : 0;
+  if (del + counterion >= s->n_mols || s->mol_start[del + 1] - s->mol_start[del] != L) return -1;
+  const int second = ch ? del + 1 : del;
+  double* chain = (double*)calloc(6 * (size_t)L, sizeof(double));
+  double* tri = (double*)malloc(sizeof(double) * 6 * K);
+  double* wts = (double*)malloc(sizeof(double) * K);
+  const double* b0 = &s->cur[3 * s->mol_start[del]];
+  const double* i0 = &s->cur[3 * s->mol_start[second]];
+  double weight = 1.0;
+  weight *= exp(-s->p.beta * orc_cbmc_beads_energy(s, b0, gc->type_c, gc->bead_charge, i0, gc->type_c,
+                                                   -gc->bead_charge, ch, chain, chain + 3 * L, 0, del,
+                                                   del + counterion));
+  memcpy(chain, b0, sizeof(double) * 3);
+  if (ch) memcpy(&chain[3 * L], i0, sizeof(double) * 3);
+  for (int i = 1; i < L; i++) {
+    memcpy(&chain[3 * i], &s->cur[3 * (s->mol_start[del] + i)], sizeof(double) * 3);
+    if (ch) memcpy(&chain[3 * (i + L)], &s->cur[3 * s->mol_start[del + i + 1]], sizeof(double) * 3);
+    gen_trials(s, r, gs, gc, &chain[3 * (i - 1)], chain, i, del, del + counterion, tri, wts);
+    wts[0] = exp(-s->p.beta * orc_cbmc_beads_energy(s, &chain[3 * i], gc->type_c, gc->bead_charge,
+                                                    &chain[3 * (i + (ch ? L : 0))], gc->type_c,
+                                                    -gc->bead_charge, ch, chain, chain + 3 * L, i, del,
+                                                    del + counterion));
+    double Wi = 0;
+    for (int j = 0; j < K; j++) Wi += wts[j];
+    weight *= Wi / K;
+  }
+  free(chain);
+  free(tri);
+  free(wts);
+  if (weight_out) *weight_out = weight;
+  const int spc2 = gc_spc2(gc, n_cion, n_aion);
+  const int spc1_del = 1, spc2_del = gc->bead_charge == 0 ? 0 : L;
+  double factorial = 1;
+  for (int i = spc1; i > spc1 - spc1_del; i--) factorial *= i;
+  for (int i = spc2; i > spc2 - spc2_del; i--) factorial *= i;
+  const double vol = s->p.box[0] * s->p.box[1] * s->p.box[2];
+  double lam_over_vol = 1;
+  lam_over_vol *= pow((gc->debroglie / pow((double)L, 1.5)) / vol, spc1_del);
+  lam_over_vol *= pow((gc->debroglie / pow(1.0, 1.5)) / vol, spc2_del);
+  const double C = lam_over_vol * factorial;
+  const double rn = u01(r);
+  return rn < C / (exp(s->p.beta * gc->chem_pot) * weight) ? del : -1;
+}
+
+double orc_gc_widom(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, int n_ins) {
+  const int L = gc->chain_len;
+  int n_chain, n_cion, n_aion;
+  gc_counts(s, gc, &n_chain, &n_cion, &n_aion);
+  const int spc1 = gc_spc1(gc, n_chain, n_cion, n_aion), spc2 = gc_spc2(gc, n_cion, n_aion);
+  const int spc1_add = 1, spc2_add = gc->bead_charge == 0 ? 0 : L;
+  double factorial = 1;
+  for (int i = spc1 + 1; i <= spc1 + spc1_add; i++) factorial *= i;
+  for (int i = spc2 + 1; i <= spc2 + spc2_add; i++) factorial *= i;
+  const double vol = s->p.box[0] * s->p.box[1] * s->p.box[2];
+  double vol_over_lam = 1;
+  vol_over_lam *= pow(vol / 15.625 / (gc->debroglie / pow((double)L, 1.5)), spc1_add);
+  vol_over_lam *= pow(vol / 15.625 / (gc->debroglie / pow(1.0, 1.5)), spc2_add);
+  const double C = vol_over_lam * (1.0 / factorial);
+  double* chain = (double*)calloc(6 * (size_t)L, sizeof(double));
+  double total = 0;
+  for (int c = 0; c < n_ins; c++) total += grow_chain(s, r, gs, gc, chain, 1) * C;
+  free(chain);
+  return total / (double)n_ins;
 }

@@ -134,6 +134,10 @@ typedef struct {
 void orc_mc_step(orc_sys* s, orc_mt19937* r, const double move_prob[5],
                  double move_size, int phantom, int grafted, double bond_len,
                  int vary_bond, orc_step_info* info);
+/* Simulation::TranslationalMove (simulation.cc:241-355) alone. */
+void orc_translational_move(orc_sys* s, orc_mt19937* r, const double move_prob[5],
+                            double move_size, int phantom, int grafted, double bond_len,
+                            int vary_bond, orc_step_info* info);
 
 /* Pairwise energy (LJ + real + k space, reference pair functions) of a bead
    of charge q and type t at position xyz against beads [0, n_partners):
@@ -156,6 +160,33 @@ double orc_end_to_end(const orc_sys* s);
 double orc_cbmc_beads_energy(orc_sys* s, const double b1[3], int t1, double q1, const double b2[3],
                              int t2, double q2, int charged, const double* grown_c,
                              const double* grown_i, int cur, int ex_lo, int ex_hi);
+
+/* Grand-canonical chain exchange with configurational bias.
+   The exchanged chain has chain_len beads of charge bead_charge and type
+   type_c; a charged chain comes with chain_len counterions of charge
+   -bead_charge and the same type (force_field.cc:385-400).  bond: 1 harmonic
+   spring (bond lengths from PotentialSpring::RandomBondLen), 2 rigid
+   (rigid_bond), 0 none (length 0). */
+typedef struct {
+  double chem_pot, debroglie, bead_charge, rigid_bond;
+  int chain_len, trials, type_c, bond;
+  int phantom, coion, grafted, grafted_counterion;
+} orc_gc;
+/* state of gasdev's pair cache (misc.cc:111-133 keeps it in statics) */
+typedef struct { int iset; double gset; } orc_gauss;
+
+/* ForceField::CBMCFChainInsertion (cbmc.cc:213-335).  Returns 1 when the
+   insertion is accepted; chain_out (2 chain_len x 3) receives the grown
+   chain then its counterions; *weight the Rosenbluth weight. */
+int orc_gc_insertion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain_out,
+                     double* weight);
+/* ForceField::CBMCFChainDeletion (cbmc.cc:337-441).  Returns the molecule
+   index of the deleted chain (its counterions follow it) or -1. */
+int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* weight);
+/* ForceField::CalcChemicalPotentialF (cbmc.cc:444-533) with n_ins test
+   insertions (mu_tot_ins, 50 in the reference). */
+double orc_gc_widom(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, int n_ins);
+double orc_gasdev(double mean, double stdev, orc_mt19937* r, orc_gauss* gs);
 
 #ifdef __cplusplus
 }

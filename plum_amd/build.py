@@ -17,8 +17,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-I", INC, "-I", CSRC]
 
 ENGINE_SRC = ["engine.hip"]
-HOST_SRC = ["host/runin.cc", "host/force_field.cc", "host/simulation.cc", "host/host_abi.cc",
-            "host/synth.cc"]
+HOST_SRC = ["host/runin.cc", "host/force_field.cc", "host/cbmc.cc", "host/simulation.cc",
+            "host/host_abi.cc", "host/synth.cc"]
 
 
 def run(cmd):

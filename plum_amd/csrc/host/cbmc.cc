@@ -1,0 +1,305 @@
+// Grand-canonical chain exchange with configurational bias: the growth,
+// insertion, deletion and Widom test insertion of the reference
+// (src/force_field/cbmc.cc:152-533) on std::mt19937 with the reference's
+// order of draws, the trial energies of each growth step evaluated together
+// on the GPU (plum_cbmc_trials).
+#include <cmath>
+#include <stdexcept>
+
+#include "force_field.h"
+
+namespace plum {
+
+namespace {
+
+double u01(std::mt19937& g) { return (double)g() / g.max(); }
+
+// randSphere (misc.cc:95-109), Marsaglia 1972
+void rand_sphere(double v[3], std::mt19937& g) {
+  double s = 2, r1 = 0, r2 = 0;
+  while (s > 1) {
+    r1 = 1 - 2 * u01(g);
+    r2 = 1 - 2 * u01(g);
+    s = r1 * r1 + r2 * r2;
+  }
+  const double h = 2 * std::sqrt(1 - s);
+  v[0] = r1 * h;
+  v[1] = r2 * h;
+  v[2] = 1 - 2 * s;
+}
+
+}  // namespace
+
+// gasdev (misc.cc:111-133): Marsaglia's polar method; the second deviate of
+// each pair is kept for the next call (process-wide statics in the
+// reference, one cache per ForceField here).
+double ForceField::gasdev(double mean, double stdev, std::mt19937& g) {
+  if (gauss_iset_ == 0) {
+    double v1, v2, rsq;
+    do {
+      v1 = 2.0 * u01(g) - 1.0;
+      v2 = 2.0 * u01(g) - 1.0;
+      rsq = v1 * v1 + v2 * v2;
+    } while (rsq >= 1.0 || rsq == 0.0);
+    const double fac = std::sqrt(-2.0 * std::log(rsq) / rsq);
+    gauss_gset_ = v1 * fac;
+    gauss_iset_ = 1;
+    return v2 * fac * stdev + mean;
+  }
+  gauss_iset_ = 0;
+  return gauss_gset_ * stdev + mean;
+}
+
+// PotentialSpring::RandomBondLen (potential_spring.cc:50-61)
+double ForceField::random_bond_len(std::mt19937& g) {
+  const double sigma = std::sqrt(1 / (r_.beta * r_.k_bond));
+  const double a = (r_.r0 + 3 * sigma) * (r_.r0 + 3 * sigma);
+  double len = 0;
+  bool ready = false;
+  while (!ready) {
+    len = gasdev(r_.r0, sigma, g);
+    ready = u01(g) < (len * len / a);
+  }
+  return len;
+}
+
+// Energies of K trials (tri: K chain beads then K counterions) against the
+// system less molecules [delete_id, delete_id + counterions] and the chain
+// grown so far (chain_: beads then counterions).
+void ForceField::trial_energies(int K, int cur, int delete_id, const double* tri, double* e) {
+  const bool ch = q_gc_ != 0;
+  std::vector<double> grown;
+  if (cur > 0) {
+    grown.assign(chain_.begin(), chain_.begin() + 3 * cur);
+    if (ch) grown.insert(grown.end(), chain_.begin() + 3 * gc_len_, chain_.begin() + 3 * (gc_len_ + cur));
+  }
+  plum_cbmc_request rq{};
+  rq.n_trials = K;
+  rq.grown = cur;
+  rq.exclude_lo = 0;
+  rq.exclude_hi = -1;
+  if (delete_id >= 0) {
+    const int last = delete_id + (ch ? gc_len_ : 0);
+    rq.exclude_lo = offset_[delete_id];
+    rq.exclude_hi = offset_[last + 1] - 1;
+  }
+  rq.type_chain = gc_type_;
+  rq.type_ion = gc_type_;
+  rq.charged = ch ? 1 : 0;
+  rq.q_chain = q_gc_;
+  rq.q_ion = -q_gc_;
+  check(plum_cbmc_trials(ctx_, &rq, tri, grown.empty() ? nullptr : grown.data(), e), "BeadsEnergy");
+}
+
+// cbmc.cc:161-212.  The trials are drawn first, in the reference's order
+// (bond length, direction, counterion), then evaluated together.
+double ForceField::CBMCFGenTrialBeads(const Bead& end_bead, std::vector<Molecule>& mols, int current_len,
+                                      std::mt19937& g, int delete_id) {
+  sync(mols);
+  const int K = gc_trials_;
+  const bool ch = q_gc_ != 0;
+  tri_.assign(6 * (size_t)K, 0.0);
+  wts_.assign(K, 0.0);
+  en_.assign(K, 0.0);
+  for (int i = 0; i < K; i++) {
+    double bond_len = 0;
+    if (r_.use_bond) bond_len = random_bond_len(g);
+    else if (r_.use_rigid_bond) bond_len = r_.rigid_bond;
+    double v[3];
+    rand_sphere(v, g);
+    for (int j = 0; j < 3; j++) tri_[3 * i + j] = v[j] * bond_len + end_bead.GetCrd(0, j);
+    if (ch)
+      for (int j = 0; j < 3; j++) tri_[3 * (K + i) + j] = u01(g) * box_[j];
+  }
+  trial_energies(K, current_len, delete_id, tri_.data(), en_.data());
+  double Wi = 0;
+  for (int i = 0; i < K; i++) {
+    wts_[i] = std::exp(-r_.beta * en_[i]);
+    Wi += wts_[i];
+  }
+  return Wi;
+}
+
+// Growth of a new chain (cbmc.cc:222-266, and 472-524 for the test
+// insertion, which does not return early after the first bead).  chain_
+// receives the beads then their counterions; returns the Rosenbluth weight.
+double ForceField::grow_chain(std::vector<Molecule>& mols, std::mt19937& g, bool widom) {
+  const int L = gc_len_, K = gc_trials_;
+  const bool ch = q_gc_ != 0;
+  chain_.assign(6 * (size_t)L, 0.0);
+  double weight = 1.0;
+  for (int i = 0; i < (ch ? 2 : 1); i++)
+    for (int j = 0; j < 3; j++) chain_[3 * (size_t)(i * L) + j] = u01(g) * box_[j];
+  double first[6];
+  for (int j = 0; j < 3; j++) {
+    first[j] = chain_[j];
+    first[3 + j] = chain_[3 * (size_t)(ch ? L : 0) + j];
+  }
+  double e0 = 0;
+  trial_energies(1, 0, -1, first, &e0);
+  if (widom) {
+    weight *= e0 == 1e8 ? 0.0 : std::exp(-r_.beta * e0);
+  } else {
+    weight *= std::exp(-r_.beta * e0);
+    if (weight <= 0) return weight;
+  }
+  Bead end;
+  for (int i = 1; i < L; i++) {
+    end.SetAllCrd(&chain_[3 * (size_t)(i - 1)]);
+    const double Wi = CBMCFGenTrialBeads(end, mols, i, g, -1);
+    weight *= Wi / K;
+    if (weight <= 0) break;
+    const double rn = u01(g) * Wi;
+    int cb = 0;
+    double cum = wts_[0];
+    while (cum < rn && cb < K - 1) {  // the reference has no bound here
+      cb++;
+      cum += wts_[cb];
+    }
+    for (int j = 0; j < 3; j++) chain_[3 * (size_t)i + j] = tri_[3 * cb + j];
+    if (ch)
+      for (int j = 0; j < 3; j++) chain_[3 * (size_t)(i + L) + j] = tri_[3 * (K + cb) + j];
+  }
+  return weight;
+}
+
+// spc1: exchanged molecules present; spc2: their counterion species
+// (cbmc.cc:271-279)
+void ForceField::gc_counts(std::vector<Molecule>& mols, int& spc1, int& spc2) {
+  UpdateMolCounts(mols);
+  if (gc_len_ > 1) spc1 = n_chain_;
+  else if (q_gc_ >= 0) spc1 = n_cion_ - r_.coion;
+  else spc1 = n_aion_ - r_.coion;
+  if (q_gc_ == 0) spc2 = 0;
+  else if (q_gc_ > 0) spc2 = n_aion_;
+  else spc2 = n_cion_;
+}
+
+bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& g) {
+  if (!r_.use_gc) throw std::runtime_error("CBMCFChainInsertion without grand-canonical moves");
+  sync(mols);
+  const int L = gc_len_;
+  const bool ch = q_gc_ != 0;
+  const double weight = grow_chain(mols, g, false);
+  if (weight <= 0) return false;
+  // cbmc.cc:268-300
+  int spc1, spc2;
+  gc_counts(mols, spc1, spc2);
+  const int spc1_add = 1, spc2_add = ch ? L : 0;
+  double factorial = 1;
+  for (int i = spc1 + 1; i <= spc1 + spc1_add; i++) factorial *= i;
+  for (int i = spc2 + 1; i <= spc2 + spc2_add; i++) factorial *= i;
+  const double vol = box_[0] * box_[1] * box_[2];
+  double vol_over_lam = 1;
+  vol_over_lam *= std::pow(vol / (r_.debroglie / std::pow((double)L, 1.5)), spc1_add);
+  vol_over_lam *= std::pow(vol / (r_.debroglie / std::pow(1.0, 1.5)), spc2_add);
+  const double C = vol_over_lam * (1.0 / factorial);
+  const double rn = u01(g);
+  if (!(rn < (std::exp(r_.beta * r_.mu) * weight) * C)) return false;
+  // cbmc.cc:309-330: the chain (linear bonds) then one molecule per counterion
+  Molecule m;
+  for (int i = 0; i < L; i++) {
+    Bead b(r_.gc_bead_symbol, -1, -1, q_gc_, chain_[3 * i], chain_[3 * i + 1], chain_[3 * i + 2]);
+    b.SetType(gc_type_);
+    m.AddBead(b);
+  }
+  for (int i = 0; i < L - 1; i++) m.AddBond(i, i + 1);
+  mols.push_back(m);
+  if (ch)
+    for (int i = L; i < 2 * L; i++) {
+      Molecule c;
+      Bead b(r_.gc_bead_symbol, -1, -1, -q_gc_, chain_[3 * i], chain_[3 * i + 1], chain_[3 * i + 2]);
+      b.SetType(gc_type_);
+      c.AddBead(b);
+      mols.push_back(c);
+    }
+  return true;
+}
+
+int ForceField::CBMCFChainDeletion(std::vector<Molecule>& mols, std::mt19937& g) {
+  if (!r_.use_gc) throw std::runtime_error("CBMCFChainDeletion without grand-canonical moves");
+  sync(mols);
+  const int L = gc_len_, K = gc_trials_;
+  const bool ch = q_gc_ != 0;
+  int spc1, spc2;
+  gc_counts(mols, spc1, spc2);
+  int del = (int)std::floor(u01(g) * spc1);
+  if (del >= spc1) del = spc1 - 1;
+  if (del < 0) return -1;  // nothing to delete (the reference indexes out of range)
+  del = r_.phantom + r_.coion + r_.grafted + r_.grafted_counterion + del * (ch ? 1 + L : 1);
+  const int counterion = ch ? L : 0;
+  if (del + counterion >= (int)mols.size() || mols[del].Size() != L)
+    throw std::runtime_error("CBMCFChainDeletion: molecules are not in the exchange layout (chain, then its counterions)");
+  const int second = ch ? del + 1 : del;
+  chain_.assign(6 * (size_t)L, 0.0);
+  double weight = 1.0;
+  {
+    double first[6];
+    for (int j = 0; j < 3; j++) {
+      first[j] = mols[del].bds[0].GetCrd(0, j);
+      first[3 + j] = mols[second].bds[0].GetCrd(0, j);
+    }
+    double e0 = 0;
+    trial_energies(1, 0, del, first, &e0);
+    weight *= std::exp(-r_.beta * e0);
+    for (int j = 0; j < 3; j++) {
+      chain_[j] = first[j];
+      if (ch) chain_[3 * (size_t)L + j] = first[3 + j];
+    }
+  }
+  Bead end;
+  for (int i = 1; i < L; i++) {
+    for (int j = 0; j < 3; j++) {
+      chain_[3 * (size_t)i + j] = mols[del].bds[i].GetCrd(0, j);
+      if (ch) chain_[3 * (size_t)(i + L) + j] = mols[del + i + 1].bds[0].GetCrd(0, j);
+    }
+    end.SetAllCrd(&chain_[3 * (size_t)(i - 1)]);
+    CBMCFGenTrialBeads(end, mols, i, g, del);
+    // the existing bead stands in for trial 0 (cbmc.cc:394-397)
+    double own[6], e = 0;
+    for (int j = 0; j < 3; j++) {
+      own[j] = chain_[3 * (size_t)i + j];
+      own[3 + j] = chain_[3 * (size_t)(i + (ch ? L : 0)) + j];
+    }
+    trial_energies(1, i, del, own, &e);
+    wts_[0] = std::exp(-r_.beta * e);
+    double Wi = 0;
+    for (int j = 0; j < K; j++) Wi += wts_[j];
+    weight *= Wi / K;
+  }
+  // cbmc.cc:403-441
+  const int spc1_del = 1, spc2_del = ch ? L : 0;
+  double factorial = 1;
+  for (int i = spc1; i > spc1 - spc1_del; i--) factorial *= i;
+  for (int i = spc2; i > spc2 - spc2_del; i--) factorial *= i;
+  const double vol = box_[0] * box_[1] * box_[2];
+  double lam_over_vol = 1;
+  lam_over_vol *= std::pow((r_.debroglie / std::pow((double)L, 1.5)) / vol, spc1_del);
+  lam_over_vol *= std::pow((r_.debroglie / std::pow(1.0, 1.5)) / vol, spc2_del);
+  const double C = lam_over_vol * factorial;
+  const double rn = u01(g);
+  return rn < C / (std::exp(r_.beta * r_.mu) * weight) ? del : -1;
+}
+
+double ForceField::CalcChemicalPotentialF(std::vector<Molecule>& mols, std::mt19937& g) {
+  sync(mols);
+  if (gc_len_ < 1) throw std::runtime_error("CalcChemicalPotentialF: no molecule species to insert");
+  const int L = gc_len_, n_ins = 50;  // mu_tot_ins (force_field.cc:236)
+  const bool ch = q_gc_ != 0;
+  int spc1, spc2;
+  gc_counts(mols, spc1, spc2);
+  const int spc1_add = 1, spc2_add = ch ? L : 0;
+  double factorial = 1;
+  for (int i = spc1 + 1; i <= spc1 + spc1_add; i++) factorial *= i;
+  for (int i = spc2 + 1; i <= spc2 + spc2_add; i++) factorial *= i;
+  const double vol = box_[0] * box_[1] * box_[2];
+  double vol_over_lam = 1;
+  vol_over_lam *= std::pow(vol / 15.625 / (r_.debroglie / std::pow((double)L, 1.5)), spc1_add);
+  vol_over_lam *= std::pow(vol / 15.625 / (r_.debroglie / std::pow(1.0, 1.5)), spc2_add);
+  const double C = vol_over_lam * (1.0 / factorial);
+  double total = 0;
+  for (int c = 0; c < n_ins; c++) total += grow_chain(mols, g, true) * C;
+  return total / (double)n_ins;
+}
+
+}  // namespace plum

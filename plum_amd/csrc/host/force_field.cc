@@ -48,7 +48,9 @@ void ForceField::build_params(plum_params& p, const double box_l[3]) {
   p.use_bond = r_.use_bond;
   p.use_ext = r_.use_ext;
   p.lj_cutoff = r_.lj_cutoff;
-  p.n_types = (int)type_names_.size();
+  // the inert bead of an empty system has a type of its own with every
+  // parameter zero
+  p.n_types = (int)type_names_.size() + (ghost_ ? 1 : 0);
   // per-type tables; symbols missing from a table read 0, like the
   // reference's map<string,double>::operator[]
   for (size_t t = 0; t < type_names_.size(); t++) {
@@ -71,26 +73,47 @@ void ForceField::build_params(plum_params& p, const double box_l[3]) {
   p.precision = PLUM_FP64;
 }
 
-void ForceField::Initialize(const RunIn& r, double box_l[3], std::vector<Molecule>& mols) {
-  r_ = r;
-  if (r_.use_gc) throw std::runtime_error("grand-canonical moves are not handled by this ForceField");
-  types_.clear();
-  type_names_.clear();
+void ForceField::assign_types(std::vector<Molecule>& mols) {
   auto add = [&](const std::string& s) {
     if (types_.count(s)) return;
-    if ((int)type_names_.size() >= PLUM_MAX_TYPES) throw std::runtime_error("too many bead types");
+    if ((int)type_names_.size() >= PLUM_MAX_TYPES - 1) throw std::runtime_error("too many bead types");
     types_[s] = (int)type_names_.size();
     type_names_.push_back(s);
   };
   for (const TypeParam& t : r_.lj_types) add(t.symbol);
   for (const TypeParam& t : r_.wall_types) add(t.symbol);
+  if (r_.use_gc) add(r_.gc_bead_symbol);
   for (Molecule& m : mols)
     for (Bead& b : m.bds) {
       add(b.Symbol());
       b.SetType(types_[b.Symbol()]);
     }
-  offset_.assign(mols.size() + 1, 0);
-  for (size_t m = 0; m < mols.size(); m++) offset_[m + 1] = offset_[m] + mols[m].Size();
+}
+
+void ForceField::Initialize(const RunIn& r, double box_l[3], std::vector<Molecule>& mols) {
+  r_ = r;
+  types_.clear();
+  type_names_.clear();
+  if (r_.use_gc) {
+    if (r_.gc_freq <= 0) throw std::runtime_error("GCMC move frequency must be > 0");
+    if (r_.gc_chain_len < 1 || r_.cbmc_trials < 1) throw std::runtime_error("CBMC chain length and trials must be >= 1");
+    gc_len_ = r_.gc_chain_len;
+    q_gc_ = r_.gc_bead_charge;
+    gc_trials_ = r_.cbmc_trials;
+  } else {
+    // force_field.cc:243-255: the chemical potential of the first chain species
+    gc_len_ = -1;
+    for (int i = r_.phantom; i < (int)mols.size(); i++) {
+      gc_len_ = mols[i].Size();
+      q_gc_ = mols[i].bds[0].Charge();
+      if (gc_len_ > 1) break;
+    }
+    r_.debroglie = 1;
+    r_.gc_bead_symbol = "P";
+    gc_trials_ = 30;
+  }
+  assign_types(mols);
+  gc_type_ = TypeOf(r_.use_gc ? r_.gc_bead_symbol : "P");
   for (int k = 0; k < 3; k++) box_[k] = box_l[k];
   InitializeEnergy(mols);
   UpdateMolCounts(mols);
@@ -99,9 +122,13 @@ void ForceField::Initialize(const RunIn& r, double box_l[3], std::vector<Molecul
 void ForceField::InitializeEnergy(std::vector<Molecule>& mols) {
   plum_params p;
   double box[3];
-  const int n = offset_.back();
+  offset_.assign(mols.size() + 1, 0);
+  for (size_t m = 0; m < mols.size(); m++) offset_[m + 1] = offset_[m] + mols[m].Size();
+  n_beads_ = offset_.back();
+  ghost_ = n_beads_ == 0;
+  const int n = ghost_ ? 1 : n_beads_;
   std::vector<double> xyz(3 * (size_t)n), q(n);
-  std::vector<int> type(n), ms(mols.size() + 1);
+  std::vector<int> type(n), ms(ghost_ ? 2 : mols.size() + 1);
   for (size_t m = 0; m < mols.size(); m++) {
     ms[m] = offset_[m];
     for (int k = 0; k < mols[m].Size(); k++) {
@@ -112,7 +139,17 @@ void ForceField::InitializeEnergy(std::vector<Molecule>& mols) {
       type[i] = b.Type();
     }
   }
-  ms[mols.size()] = n;
+  if (ghost_) {
+    // no molecules: one uncharged bead with zero LJ and wall parameters at
+    // the box centre stands in, so that the engine state exists
+    for (int e = 0; e < 3; e++) xyz[e] = 0.5 * box_[e];
+    q[0] = 0;
+    type[0] = (int)type_names_.size();
+    ms[0] = 0;
+    ms[1] = 1;
+  } else {
+    ms[mols.size()] = n;
+  }
   box[0] = box_[0];
   box[1] = box_[1];
   box[2] = box_[2];
@@ -121,11 +158,24 @@ void ForceField::InitializeEnergy(std::vector<Molecule>& mols) {
     plum_destroy(ctx_);
     ctx_ = nullptr;
   }
-  check(plum_create(&p, n, xyz.data(), q.data(), type.data(), (int)mols.size(), ms.data(), &ctx_),
+  check(plum_create(&p, n, xyz.data(), q.data(), type.data(), (int)ms.size() - 1, ms.data(), &ctx_),
         "InitializeEnergy");
 }
 
+// Rebuilds the energy state when the molecule list no longer matches it
+// (after a grand-canonical insertion or deletion).
+void ForceField::sync(std::vector<Molecule>& mols) {
+  int n = 0;
+  for (const Molecule& m : mols) n += m.Size();
+  if (offset_.size() == mols.size() + 1 && n == n_beads_) return;
+  assign_types(mols);
+  InitializeEnergy(mols);
+}
+
+void ForceField::EnergyInitForAddedMolecule(std::vector<Molecule>& mols) { sync(mols); }
+
 double ForceField::EnergyDifference(std::vector<Molecule>& mols, int moved_mol) {
+  sync(mols);
   const Molecule& mol = mols.at(moved_mol);
   std::vector<int> idx;
   std::vector<double> tr;
@@ -169,6 +219,7 @@ double ForceField::TotExtEnergy() { return Totals().wall; }
 
 // force_field.cc:1258-1277
 void ForceField::UpdateMolCounts(std::vector<Molecule>& mols) {
+  sync(mols);
   n_chain_ = n_cion_ = n_aion_ = 0;
   for (int i = r_.phantom; i < (int)mols.size(); i++) {
     if (mols[i].Size() > 1) n_chain_++;

@@ -4,6 +4,7 @@
 #pragma once
 #include <iostream>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,25 @@ class ForceField {
   plum_terms Totals();
   plum_terms FullEnergy();
 
+  // Grand-canonical chain exchange with configurational bias (cbmc.cc);
+  // the trial energies of each growth step are one plum_cbmc_trials call.
+  // cbmc.cc:161: K trials around end_bead; returns their summed weight.
+  double CBMCFGenTrialBeads(const Bead& end_bead, std::vector<Molecule>& mols, int current_len,
+                            std::mt19937& rand_gen, int delete_id);
+  // cbmc.cc:213: on acceptance the chain and its counterions are appended to
+  // mols; the caller then calls EnergyInitForAddedMolecule.
+  bool CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& rand_gen);
+  // cbmc.cc:337: returns the index of the chain to delete (its counterions
+  // follow it) or -1; the caller erases the molecules and then calls
+  // UpdateMolCounts, which rebuilds the energy state.
+  int CBMCFChainDeletion(std::vector<Molecule>& mols, std::mt19937& rand_gen);
+  // cbmc.cc:444: Widom test insertions of the exchanged (or, without GC
+  // moves, the first) chain species.
+  double CalcChemicalPotentialF(std::vector<Molecule>& mols, std::mt19937& rand_gen);
+  // force_field.cc:1088: rebuilds the energy state with the appended molecules.
+  void EnergyInitForAddedMolecule(std::vector<Molecule>& mols);
+  const std::vector<double>& TrialWeights() const { return wts_; }
+
   bool UseGC() const { return r_.use_gc; }
   bool UsePairPot() const { return r_.use_pair; }
   bool UseEwaldPot() const { return r_.use_ewald; }
@@ -67,6 +87,13 @@ class ForceField {
  private:
   void build_params(plum_params& p, const double box_l[3]);
   void check(int rc, const char* what);
+  void assign_types(std::vector<Molecule>& mols);
+  void sync(std::vector<Molecule>& mols);
+  double random_bond_len(std::mt19937& g);
+  double gasdev(double mean, double stdev, std::mt19937& g);
+  void trial_energies(int K, int cur, int delete_id, const double* tri, double* e);
+  double grow_chain(std::vector<Molecule>& mols, std::mt19937& g, bool widom);
+  void gc_counts(std::vector<Molecule>& mols, int& spc1, int& spc2);
 
   RunIn r_;
   double box_[3] = {0, 0, 0};
@@ -76,6 +103,15 @@ class ForceField {
   plum_ctx* ctx_ = nullptr;
   plum_terms last_dE_{};
   int n_chain_ = 0, n_cion_ = 0, n_aion_ = 0;
+  int n_beads_ = 0;        // beads of mols the energy state was built from
+  bool ghost_ = false;     // an empty system carries one inert bead
+  // exchanged species: chain_len beads of charge q_gc_, counterions of
+  // charge -q_gc_, both of type gc_type_ (force_field.cc:385-400)
+  int gc_len_ = 0, gc_trials_ = 0, gc_type_ = 0;
+  double q_gc_ = 0;
+  int gauss_iset_ = 0;     // gasdev's cached deviate (misc.cc:112-113)
+  double gauss_gset_ = 0;
+  std::vector<double> chain_, tri_, wts_, en_;
 };
 
 }  // namespace plum

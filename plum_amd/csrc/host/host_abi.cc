@@ -112,6 +112,29 @@ int plum_sim_run(plum_sim* s, long long steps, long long attempted[5], long long
   return PLUM_OK;
 }
 
+int plum_sim_gc_stats(plum_sim* s, long long attempted[2], long long accepted[2], int* n_beads, int* n_mols) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  for (int k = 0; k < 2; k++) {
+    if (attempted) attempted[k] = s->s->gc_attempted[k];
+    if (accepted) accepted[k] = s->s->gc_accepted[k];
+  }
+  int n = 0;
+  for (auto& m : s->s->mols) n += m.Size();
+  if (n_beads) *n_beads = n;
+  if (n_mols) *n_mols = (int)s->s->mols.size();
+  return PLUM_OK;
+}
+
+int plum_sim_widom(plum_sim* s, double* mean_weight) {
+  if (!s || !s->s || !mean_weight) return fail(PLUM_E_ARG, "null argument");
+  try {
+    *mean_weight = s->s->ff.CalcChemicalPotentialF(s->s->mols, s->s->rng);
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_HIP, e.what());
+  }
+  return PLUM_OK;
+}
+
 int plum_sim_energy(plum_sim* s, plum_terms* running, plum_terms* full) {
   if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
   try {

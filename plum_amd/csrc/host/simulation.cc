@@ -158,6 +158,7 @@ Simulation::Simulation(std::istream& in, unsigned seed, const std::string& base_
 
 // Simulation::UpdateMolCounts (simulation.cc:1066-1094)
 void Simulation::update_counts() {
+  n_mol = (int)mols.size();
   n_chain = n_cion = n_aion = n_nion = 0;
   for (int i = r.phantom; i < (int)mols.size(); i++) {
     if (mols[i].Size() > 1) n_chain++;
@@ -168,13 +169,39 @@ void Simulation::update_counts() {
   n_chain -= r.grafted;
 }
 
-// Simulation::Run (simulation.cc:216-239) without GC and without output.
+// Simulation::Run (simulation.cc:216-239) without output.
 void Simulation::Run(long long steps) {
+  update_counts();
   for (long long s = 0; s < steps; s++) {
-    (void)rng();  // the per-step draw of the GC coin (simulation.cc:221)
-    TranslationalMove();
+    const int rn = (int)rng();  // the reference keeps the draw in an int
+    if (ff.UseGC() && rn % r.gc_freq == 0) GCMove();
+    else TranslationalMove();
     steps_done++;
   }
+}
+
+// Simulation::GCMove (simulation.cc:357-404)
+void Simulation::GCMove() {
+  const bool insert = rng() % 2;
+  if (insert) {
+    gc_attempted[0]++;
+    if (ff.CBMCFChainInsertion(mols, rng)) {
+      gc_accepted[0]++;
+      ff.EnergyInitForAddedMolecule(mols);
+    }
+  } else {
+    gc_attempted[1]++;
+    int deleted = -1;
+    if (n_mol - r.phantom - r.coion > 0) deleted = ff.CBMCFChainDeletion(mols, rng);
+    if (deleted >= r.phantom + r.coion) {
+      gc_accepted[1]++;
+      int ions = 0;
+      for (const Bead& b : mols[deleted].bds) ions += (int)std::round(std::fabs(b.Charge()));
+      mols.erase(mols.begin() + deleted, mols.begin() + deleted + ions + 1);
+    }
+  }
+  update_counts();
+  ff.UpdateMolCounts(mols);
 }
 
 // Simulation::TranslationalMove (simulation.cc:241-355)

@@ -1,5 +1,5 @@
 // Host Monte Carlo driver with the reference's move semantics
-// (Simulation::Run / TranslationalMove, simulation.cc:216-355, and the move
+// (Simulation::Run / TranslationalMove / GCMove, simulation.cc:216-404, and the move
 // generators of molecule.cc:103-312) on std::mt19937, calling the GPU
 // engine through the drop-in ForceField.  Unlike the reference, the seed is
 // a parameter (the reference seeds from the clock, simulation.cc:133-136).
@@ -29,6 +29,8 @@ class Simulation {
   Simulation(std::istream& in, unsigned seed, const std::string& base_dir = "");
   void Run(long long steps);
   void TranslationalMove();
+  // simulation.cc:357-404: insertion or deletion of one exchanged chain
+  void GCMove();
   double RadiusOfGyration() const;
   double EndToEndDistance() const;
 
@@ -40,10 +42,12 @@ class Simulation {
   long long attempted[5] = {0, 0, 0, 0, 0};
   long long accepted[5] = {0, 0, 0, 0, 0};
   long long steps_done = 0;
+  long long gc_attempted[2] = {0, 0};  // insertions, deletions
+  long long gc_accepted[2] = {0, 0};
 
  private:
   void update_counts();
-  int n_chain = 0, n_cion = 0, n_aion = 0, n_nion = 0;
+  int n_mol = 0, n_chain = 0, n_cion = 0, n_aion = 0, n_nion = 0;
 };
 
 }  // namespace plum

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libplum_host.so")
 HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim_energy",
                 "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
-                "plum_host_last_error", "plum_synth_count", "plum_synth_system")
+                "plum_host_last_error", "plum_synth_count", "plum_synth_system",
+                "plum_sim_gc_stats", "plum_sim_widom")
 
 
 class RunInSummary(C.Structure):
@@ -59,6 +60,9 @@ def lib():
         L.plum_sim_positions.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
         L.plum_sim_observables.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.plum_sim_destroy.argtypes = [vp]
+        L.plum_sim_gc_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.plum_sim_widom.argtypes = [vp, C.POINTER(C.c_double)]
         L.plum_host_last_error.restype = C.c_char_p
         L.plum_synth_count.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plum_synth_system.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_double),
@@ -100,6 +104,19 @@ class HostSimulation:
         out = np.zeros(3 * n)
         _check(lib().plum_sim_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n))
         return out.reshape(-1, 3)
+
+    def gc_stats(self):
+        """(attempted [ins, del], accepted [ins, del], n_beads, n_mols)"""
+        att = (C.c_longlong * 2)()
+        acc = (C.c_longlong * 2)()
+        nb, nm = C.c_int(), C.c_int()
+        _check(lib().plum_sim_gc_stats(self.h, att, acc, C.byref(nb), C.byref(nm)))
+        return list(att), list(acc), nb.value, nm.value
+
+    def widom(self):
+        w = C.c_double()
+        _check(lib().plum_sim_widom(self.h, C.byref(w)))
+        return w.value
 
     def observables(self):
         a, b = C.c_double(), C.c_double()

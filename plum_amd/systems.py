@@ -22,7 +22,7 @@ import numpy as np
 CONFIG_SEEDS = {"c0": 1000, "c1": 1001, "c2": 1002, "c3": 1003, "c4": 1004}
 # chemical potential (kT) of the exchanged chain + counterions of the
 # grand-canonical workload
-GC_CHEM_POT = 0.0
+GC_CHEM_POT = -400.0
 
 
 class _Grid:
@@ -382,8 +382,17 @@ def write_reference_inputs(sysdef, outdir, steps=1000, prefix="output"):
         ("s2_use_angle_potential", 0),
         ("s2_use_dihedral_potential", 0),
         ("s2_use_external_potential", sysdef["use_ext"]),
-        ("s2_use_grand_canonical_MC_move", 0),
+        ("s2_use_grand_canonical_MC_move", 1 if "gc" in sysdef else 0),
     ]
+    if "gc" in sysdef:
+        g = sysdef["gc"]
+        lines += [("s3_chemical_potential_in_kT", repr(float(g["chem_pot"]))),
+                  ("s3_deBroglie_prefactor", repr(float(g["debroglie"]))),
+                  ("s3_GCMC_move_frequency", g["freq"]),
+                  ("s3_chain_length", g["chain_len"]),
+                  ("s3_bead_charge_in_e", int(g["bead_charge"])),
+                  ("s3_bead_type", names[g.get("type", 0)]),
+                  ("s3_no_of_CBMC_trials", g["trials"])]
     if sysdef["use_pair"]:
         lines.append(("s3_short_range_potential_type", "TruncatedLJ"))
         lines.append(("s3_LJ_cutoff", sysdef["lj_cutoff"]))

@@ -102,3 +102,38 @@ def test_host_driver_replays_reference_sequence(tmp_path, name, steps):
     ref = orc.energy_full()
     for k in ref:
         assert abs(full[k] - ref[k]) <= 1e-10 * max(abs(ref[k]), 1e-2), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mu,steps", [(-45.0, 400), (-60.0, 240)])
+def test_host_grand_canonical_run_replays_reference_sequence(tmp_path, mu, steps):
+    """Grand-canonical chain exchange with configurational bias: the host
+    driver (trial energies of each growth step in one GPU pass) and the
+    oracle take the same insertions, deletions and trial moves on the same
+    seed, end with the same molecules at the same positions, and their Widom
+    test insertions agree.  At the lower chemical potential the system
+    empties and refills."""
+    from oracle import oracle as O
+    sd = systems.config("gc_small")
+    sd["gc"] = dict(sd["gc"], chem_pot=mu)
+    path = systems.write_reference_inputs(sd, str(tmp_path))
+    sim = H.HostSimulation(path, 31)
+    att, acc = sim.run(steps)
+    gatt, gacc, nb, nm = sim.gc_stats()
+    orc = O.GCSim(sd, 31)
+    orc.run(steps)
+    print("mu %.0f host gc %s %s trial %s %s | oracle gc %s %s trial %s %s | beads %d" % (
+        mu, gatt, gacc, att, acc, orc.gc_attempted, orc.gc_accepted, orc.attempted,
+        orc.accepted, nb), flush=True)
+    assert gatt == orc.gc_attempted and gacc == orc.gc_accepted
+    assert att == orc.attempted and acc == orc.accepted
+    assert nb == len(orc.sd["q"]) and nm == len(orc.sd["mol_start"]) - 1
+    if nb:
+        assert np.max(np.abs(sim.positions(nb) - orc.orc.positions())) <= 1e-9
+    w_host, w_orc = sim.widom(), orc.widom()
+    assert abs(w_host - w_orc) <= 1e-9 * max(abs(w_orc), 1e-300), (w_host, w_orc)
+    run, full = sim.energies()
+    for k in full:
+        assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), k
+    orc.close()
+    sim.close()
