@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-replicas", action="store_true")
     ap.add_argument("--brush-moves", type=int, default=20000)
     ap.add_argument("--replica-sweeps", type=int, default=512)
+    ap.add_argument("--no-gc", action="store_true")
+    ap.add_argument("--gc-steps", type=int, default=3000)
     return ap.parse_args()
 
 
@@ -237,6 +239,61 @@ def replica_leg(sweeps):
     return res
 
 
+def gc_leg(steps):
+    """Grand-canonical bulk: 32-bead chains (charge -1 per bead) exchanged
+    together with 32 counterions by configurational-bias insertion and
+    deletion (16 trials per grown bead, each growth step's trial energies in
+    one GPU pass), one exchange every 10 steps on average and translational
+    moves on the device loop in between; starting from 50 chains in a box of
+    60."""
+    import tempfile
+    from plum_amd import host as H
+    sd = systems.config("c4")
+    g = sd["gc"]
+    path = systems.write_reference_inputs(sd, tempfile.mkdtemp(prefix="plum_gc_"))
+    sim = H.HostSimulation(path, 2024)
+    per = 1 + g["chain_len"]
+    t = time.perf_counter()
+    w = [sim.widom() for _ in range(2)]
+    widom_s = (time.perf_counter() - t) / 2
+    growth_steps = 50 * (g["chain_len"] - 1)
+    sim.run_device(max(steps // 10, 100))           # warm up
+    a0, c0, n0 = sim.gc_stats()[0], sim.gc_stats()[1], None
+    att0, acc0, sec0 = sim.run_device(0)
+    chains = []
+    t = time.perf_counter()
+    chunk = max(steps // 50, 1)
+    done = 0
+    while done < steps:
+        att, acc, sec = sim.run_device(chunk)
+        done += chunk
+        nb = sim.gc_stats()[2]
+        chains.append(nb / (g["chain_len"] * 2))
+    secs = time.perf_counter() - t
+    ga, gacc, nb, nm = sim.gc_stats()
+    ins = ga[0] - a0[0]
+    dele = ga[1] - a0[1]
+    moves = sum(att) - sum(att0)
+    ch = np.array(chains)
+    blocks = ch[: len(ch) // 10 * 10].reshape(10, -1).mean(1)
+    log("gc: %d steps in %.2f s: %d insertions (%d accepted), %d deletions (%d accepted), "
+        "<chains> %.2f" % (done, secs, ins, gacc[0] - c0[0], dele, gacc[1] - c0[1], ch.mean()))
+    res = {"workload": "grand-canonical bulk, 32-bead chains (-1 per bead) + 32 counterions "
+                       "exchanged by configurational bias, 16 trials per bead, mu %.1f kT, box 60, "
+                       "start 50 chains, exchange every 10 steps" % g["chem_pot"],
+           "steps": done, "steps_per_s": done / secs,
+           "exchanges_per_s": (ins + dele) / secs,
+           "insertions": ins, "insertions_accepted": gacc[0] - c0[0],
+           "deletions": dele, "deletions_accepted": gacc[1] - c0[1],
+           "translational_moves_per_s": moves / max(sec[0] - sec0[0], 1e-9),
+           "seconds_in_exchanges": sec[1] - sec0[1], "seconds_in_device_loop": sec[0] - sec0[0],
+           "chains_mean": float(ch.mean()), "chains_err": float(blocks.std(ddof=1) / np.sqrt(10)),
+           "growth_step_us": 1e6 * widom_s / growth_steps,
+           "widom_mean_weight": float(np.mean(w))}
+    sim.close()
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -317,6 +374,8 @@ def main():
         out["brush_delayed_acceptance"] = brush_leg(a.brush_moves)
     if not a.no_replicas:
         out["replicas"] = replica_leg(a.replica_sweeps)
+    if not a.no_gc:
+        out["grand_canonical"] = gc_leg(a.gc_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sd)
     print(json.dumps(out), flush=True)

@@ -301,6 +301,36 @@ typedef struct {
 int plum_cbmc_trials(plum_ctx* ctx, const plum_cbmc_request* req, const double* trials, const double* grown,
                      double* energies);
 
+typedef struct {
+  int mode;              /* 0 insertion, 1 deletion, 2 Widom test insertion */
+  int chain_len;         /* L beads (each with a counterion when charged) */
+  int n_trials;          /* K trials per grown bead */
+  int exclude_lo, exclude_hi;  /* deletion: the chain and its counterions */
+  int type_chain, type_ion;
+  int charged;
+  double q_chain, q_ion;
+} plum_growth_request;
+
+/* One whole configurational-bias growth on the GPU with no host round trip
+   between steps: the growth loops of CBMCFChainInsertion (cbmc.cc:222-266),
+   CBMCFChainDeletion (cbmc.cc:361-400) and CalcChemicalPotentialF
+   (cbmc.cc:472-524), with their trial energies (BeadsEnergy, cbmc.cc:5-151),
+   weights, Rosenbluth product W and, for insertions, the choice of each bead.
+   The caller supplies the draws in the reference's meaning: first[6] the
+   first chain bead and its counterion (insertion, test insertion);
+   disp/ions[(s K + t) 3] the bond vector and counterion position of trial t
+   at step s = 1..L-1 (step 0's rows unused); sel[s] the uniform of the choice
+   at step s; actual[2 L 3] the existing chain then its counterions
+   (deletion: trial 0 of each step is the existing bead).  On return weight
+   is W (the first bead's Boltzmann factor times prod_s W_s / K); stop_step is
+   the step at which W reached 0 and the growth stopped (the reference returns
+   there; -1 when it grew to the end); chain[2 L 3] (may be NULL) the grown
+   chain then its counterions.  Calls that share a context serialize on its
+   stream. */
+int plum_cbmc_grow(plum_ctx* ctx, const plum_growth_request* req, const double* first, const double* disp,
+                   const double* ions, const double* sel, const double* actual, double* weight, int* stop_step,
+                   double* chain);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,13 @@ int plum_sim_create(const char* run_in_path, unsigned seed, plum_sim** out);
    draw on average a chain insertion or deletion by configurational bias
    (cbmc.cc:213-441; each growth step's trial energies in one GPU pass). */
 int plum_sim_run(plum_sim* sim, long long steps, long long attempted[5], long long accepted[5]);
+/* The step sequence of plum_sim_run (the GC coin on the host stream), with
+   the translational moves between two exchanges run as one batch by the
+   engine's device loop, plum_mc_run, on its own counter-based streams: the same Markov
+   chain, not the same draws.  No reference counterpart.  seconds[0]: wall
+   time in the device loop, [1]: in exchanges (cumulative). */
+int plum_sim_run_device(plum_sim* sim, long long steps, long long attempted[5], long long accepted[5],
+                        double seconds[2]);
 /* Grand-canonical counters: [0] insertions, [1] deletions; and the current
    number of beads and molecules (they change with accepted exchanges). */
 int plum_sim_gc_stats(plum_sim* sim, long long attempted[2], long long accepted[2], int* n_beads, int* n_mols);

@@ -15,6 +15,7 @@
 //                         the last dE per term, status flags, move counters
 #include "kernels_batch.h"
 #include "kernels_cbmc.h"
+#include "kernels_grow.h"
 #include "kernels_full.h"
 #include "kernels_move.h"
 #include "plum_gpu.h"

@@ -1,8 +1,9 @@
 // Grand-canonical chain exchange with configurational bias: the growth,
 // insertion, deletion and Widom test insertion of the reference
 // (src/force_field/cbmc.cc:152-533) on std::mt19937 with the reference's
-// order of draws, the trial energies of each growth step evaluated together
-// on the GPU (plum_cbmc_trials).
+// order of draws.  A whole growth runs on the GPU in one call
+// (plum_cbmc_grow); CBMCFGenTrialBeads, the one-step surface, evaluates its
+// K trials in one call (plum_cbmc_trials).
 #include <cmath>
 #include <stdexcept>
 
@@ -120,47 +121,83 @@ double ForceField::CBMCFGenTrialBeads(const Bead& end_bead, std::vector<Molecule
   return Wi;
 }
 
-// Growth of a new chain (cbmc.cc:222-266, and 472-524 for the test
-// insertion, which does not return early after the first bead).  chain_
-// receives the beads then their counterions; returns the Rosenbluth weight.
-double ForceField::grow_chain(std::vector<Molecule>& mols, std::mt19937& g, bool widom) {
+// The draws of one growth in the reference's order: the first bead and its
+// counterion (insertion, test insertion), then for each step s = 1..last the
+// K trials (bond length, direction, counterion; CBMCFGenTrialBeads,
+// cbmc.cc:161-212) and, for an insertion, the uniform of the choice among
+// them (cbmc.cc:241) unless s == last and !sel_last.
+void ForceField::draw_growth(std::mt19937& g, int mode, int last, bool sel_last) {
+  const int K = gc_trials_, L = gc_len_;
+  const bool ch = q_gc_ != 0;
+  first_.assign(6, 0.0);
+  disp_.assign(3 * (size_t)L * K, 0.0);
+  ion_.assign(3 * (size_t)L * K, 0.0);
+  sel_.assign(L, 0.0);
+  if (mode != 1)
+    for (int i = 0; i < (ch ? 2 : 1); i++)
+      for (int j = 0; j < 3; j++) first_[3 * i + j] = u01(g) * box_[j];
+  for (int s = 1; s <= last; s++) {
+    for (int t = 0; t < K; t++) {
+      double bond_len = 0;
+      if (r_.use_bond) bond_len = random_bond_len(g);
+      else if (r_.use_rigid_bond) bond_len = r_.rigid_bond;
+      double v[3];
+      rand_sphere(v, g);
+      const size_t o = 3 * ((size_t)s * K + t);
+      for (int j = 0; j < 3; j++) disp_[o + j] = v[j] * bond_len;
+      if (ch)
+        for (int j = 0; j < 3; j++) ion_[o + j] = u01(g) * box_[j];
+    }
+    if (mode != 1 && (s < last || sel_last)) sel_[s] = u01(g);
+  }
+}
+
+// One whole growth on the GPU (plum_cbmc_grow); returns the Rosenbluth
+// weight.  The draws are made up front; when the weight reaches 0 at step s
+// the reference stops drawing there, so the stream is rewound and redrawn up
+// to that point.
+double ForceField::grow_device(std::vector<Molecule>& mols, std::mt19937& g, int mode, int delete_id) {
+  sync(mols);
   const int L = gc_len_, K = gc_trials_;
   const bool ch = q_gc_ != 0;
+  const std::mt19937 g0 = g;
+  const int iset0 = gauss_iset_;
+  const double gset0 = gauss_gset_;
+  draw_growth(g, mode, L - 1, true);
+  plum_growth_request rq{};
+  rq.mode = mode;
+  rq.chain_len = L;
+  rq.n_trials = K;
+  rq.exclude_lo = 0;
+  rq.exclude_hi = -1;
+  if (mode == 1) {
+    act_.assign(6 * (size_t)L, 0.0);
+    for (int i = 0; i < L; i++)
+      for (int j = 0; j < 3; j++) {
+        act_[3 * (size_t)i + j] = mols[delete_id].bds[i].GetCrd(0, j);
+        if (ch) act_[3 * (size_t)(L + i) + j] = mols[delete_id + 1 + i].bds[0].GetCrd(0, j);
+      }
+    rq.exclude_lo = offset_[delete_id];
+    rq.exclude_hi = offset_[delete_id + (ch ? L : 0) + 1] - 1;
+  }
+  rq.type_chain = gc_type_;
+  rq.type_ion = gc_type_;
+  rq.charged = ch ? 1 : 0;
+  rq.q_chain = q_gc_;
+  rq.q_ion = -q_gc_;
   chain_.assign(6 * (size_t)L, 0.0);
-  double weight = 1.0;
-  for (int i = 0; i < (ch ? 2 : 1); i++)
-    for (int j = 0; j < 3; j++) chain_[3 * (size_t)(i * L) + j] = u01(g) * box_[j];
-  double first[6];
-  for (int j = 0; j < 3; j++) {
-    first[j] = chain_[j];
-    first[3 + j] = chain_[3 * (size_t)(ch ? L : 0) + j];
+  double W = 0;
+  int stop = -1;
+  check(plum_cbmc_grow(ctx_, &rq, first_.data(), disp_.data(), ion_.data(), sel_.data(),
+                       mode == 1 ? act_.data() : nullptr, &W, &stop, chain_.data()),
+        "CBMC growth");
+  if (stop >= 0) {
+    g = g0;
+    gauss_iset_ = iset0;
+    gauss_gset_ = gset0;
+    draw_growth(g, mode, stop, false);
   }
-  double e0 = 0;
-  trial_energies(1, 0, -1, first, &e0);
-  if (widom) {
-    weight *= e0 == 1e8 ? 0.0 : std::exp(-r_.beta * e0);
-  } else {
-    weight *= std::exp(-r_.beta * e0);
-    if (weight <= 0) return weight;
-  }
-  Bead end;
-  for (int i = 1; i < L; i++) {
-    end.SetAllCrd(&chain_[3 * (size_t)(i - 1)]);
-    const double Wi = CBMCFGenTrialBeads(end, mols, i, g, -1);
-    weight *= Wi / K;
-    if (weight <= 0) break;
-    const double rn = u01(g) * Wi;
-    int cb = 0;
-    double cum = wts_[0];
-    while (cum < rn && cb < K - 1) {  // the reference has no bound here
-      cb++;
-      cum += wts_[cb];
-    }
-    for (int j = 0; j < 3; j++) chain_[3 * (size_t)i + j] = tri_[3 * cb + j];
-    if (ch)
-      for (int j = 0; j < 3; j++) chain_[3 * (size_t)(i + L) + j] = tri_[3 * (K + cb) + j];
-  }
-  return weight;
+  return W;
 }
 
 // spc1: exchanged molecules present; spc2: their counterion species
@@ -180,7 +217,7 @@ bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& 
   sync(mols);
   const int L = gc_len_;
   const bool ch = q_gc_ != 0;
-  const double weight = grow_chain(mols, g, false);
+  const double weight = grow_device(mols, g, 0, -1);
   if (weight <= 0) return false;
   // cbmc.cc:268-300
   int spc1, spc2;
@@ -219,7 +256,7 @@ bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& 
 int ForceField::CBMCFChainDeletion(std::vector<Molecule>& mols, std::mt19937& g) {
   if (!r_.use_gc) throw std::runtime_error("CBMCFChainDeletion without grand-canonical moves");
   sync(mols);
-  const int L = gc_len_, K = gc_trials_;
+  const int L = gc_len_;
   const bool ch = q_gc_ != 0;
   int spc1, spc2;
   gc_counts(mols, spc1, spc2);
@@ -230,43 +267,7 @@ int ForceField::CBMCFChainDeletion(std::vector<Molecule>& mols, std::mt19937& g)
   const int counterion = ch ? L : 0;
   if (del + counterion >= (int)mols.size() || mols[del].Size() != L)
     throw std::runtime_error("CBMCFChainDeletion: molecules are not in the exchange layout (chain, then its counterions)");
-  const int second = ch ? del + 1 : del;
-  chain_.assign(6 * (size_t)L, 0.0);
-  double weight = 1.0;
-  {
-    double first[6];
-    for (int j = 0; j < 3; j++) {
-      first[j] = mols[del].bds[0].GetCrd(0, j);
-      first[3 + j] = mols[second].bds[0].GetCrd(0, j);
-    }
-    double e0 = 0;
-    trial_energies(1, 0, del, first, &e0);
-    weight *= std::exp(-r_.beta * e0);
-    for (int j = 0; j < 3; j++) {
-      chain_[j] = first[j];
-      if (ch) chain_[3 * (size_t)L + j] = first[3 + j];
-    }
-  }
-  Bead end;
-  for (int i = 1; i < L; i++) {
-    for (int j = 0; j < 3; j++) {
-      chain_[3 * (size_t)i + j] = mols[del].bds[i].GetCrd(0, j);
-      if (ch) chain_[3 * (size_t)(i + L) + j] = mols[del + i + 1].bds[0].GetCrd(0, j);
-    }
-    end.SetAllCrd(&chain_[3 * (size_t)(i - 1)]);
-    CBMCFGenTrialBeads(end, mols, i, g, del);
-    // the existing bead stands in for trial 0 (cbmc.cc:394-397)
-    double own[6], e = 0;
-    for (int j = 0; j < 3; j++) {
-      own[j] = chain_[3 * (size_t)i + j];
-      own[3 + j] = chain_[3 * (size_t)(i + (ch ? L : 0)) + j];
-    }
-    trial_energies(1, i, del, own, &e);
-    wts_[0] = std::exp(-r_.beta * e);
-    double Wi = 0;
-    for (int j = 0; j < K; j++) Wi += wts_[j];
-    weight *= Wi / K;
-  }
+  const double weight = grow_device(mols, g, 1, del);
   // cbmc.cc:403-441
   const int spc1_del = 1, spc2_del = ch ? L : 0;
   double factorial = 1;
@@ -298,7 +299,7 @@ double ForceField::CalcChemicalPotentialF(std::vector<Molecule>& mols, std::mt19
   vol_over_lam *= std::pow(vol / 15.625 / (r_.debroglie / std::pow(1.0, 1.5)), spc2_add);
   const double C = vol_over_lam * (1.0 / factorial);
   double total = 0;
-  for (int c = 0; c < n_ins; c++) total += grow_chain(mols, g, true) * C;
+  for (int c = 0; c < n_ins; c++) total += grow_device(mols, g, 2, -1) * C;
   return total / (double)n_ins;
 }
 

@@ -193,6 +193,20 @@ double ForceField::EnergyDifference(std::vector<Molecule>& mols, int moved_mol) 
   return total;
 }
 
+plum_mc_stats ForceField::DeviceMoves(std::vector<Molecule>& mols, const plum_mc_params& mc, long long n) {
+  sync(mols);
+  plum_mc_stats st{};
+  check(plum_mc_run(ctx_, &mc, n, &st), "DeviceMoves");
+  std::vector<double> xyz(3 * (size_t)std::max(1, n_beads_));
+  check(plum_get_positions(ctx_, xyz.data()), "DeviceMoves");
+  for (size_t m = 0; m < mols.size(); m++)
+    for (int k = 0; k < mols[m].Size(); k++) {
+      const double* p = &xyz[3 * (size_t)(offset_[m] + k)];
+      mols[m].bds[k].SetAllCrd(p);
+    }
+  return st;
+}
+
 void ForceField::FinalizeEnergies(std::vector<Molecule>&, bool accept, int) {
   check(plum_finalize(ctx_, accept ? 1 : 0), "FinalizeEnergies");
 }

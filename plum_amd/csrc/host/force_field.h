@@ -65,6 +65,9 @@ class ForceField {
   // force_field.cc:1088: rebuilds the energy state with the appended molecules.
   void EnergyInitForAddedMolecule(std::vector<Molecule>& mols);
   const std::vector<double>& TrialWeights() const { return wts_; }
+  // n translational moves on the device loop (plum_mc_run); the molecules'
+  // coordinates are refreshed from the device afterwards.
+  plum_mc_stats DeviceMoves(std::vector<Molecule>& mols, const plum_mc_params& mc, long long n);
 
   bool UseGC() const { return r_.use_gc; }
   bool UsePairPot() const { return r_.use_pair; }
@@ -92,7 +95,8 @@ class ForceField {
   double random_bond_len(std::mt19937& g);
   double gasdev(double mean, double stdev, std::mt19937& g);
   void trial_energies(int K, int cur, int delete_id, const double* tri, double* e);
-  double grow_chain(std::vector<Molecule>& mols, std::mt19937& g, bool widom);
+  void draw_growth(std::mt19937& g, int mode, int last, bool sel_last);
+  double grow_device(std::vector<Molecule>& mols, std::mt19937& g, int mode, int delete_id);
   void gc_counts(std::vector<Molecule>& mols, int& spc1, int& spc2);
 
   RunIn r_;
@@ -112,6 +116,7 @@ class ForceField {
   int gauss_iset_ = 0;     // gasdev's cached deviate (misc.cc:112-113)
   double gauss_gset_ = 0;
   std::vector<double> chain_, tri_, wts_, en_;
+  std::vector<double> first_, disp_, ion_, sel_, act_;  // draws of one growth
 };
 
 }  // namespace plum

@@ -112,6 +112,25 @@ int plum_sim_run(plum_sim* s, long long steps, long long attempted[5], long long
   return PLUM_OK;
 }
 
+int plum_sim_run_device(plum_sim* s, long long steps, long long attempted[5], long long accepted[5],
+                        double seconds[2]) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  try {
+    s->s->RunDevice(steps);
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_HIP, e.what());
+  }
+  for (int k = 0; k < 5; k++) {
+    if (attempted) attempted[k] = s->s->attempted[k];
+    if (accepted) accepted[k] = s->s->accepted[k];
+  }
+  if (seconds) {
+    seconds[0] = s->s->seconds_device;
+    seconds[1] = s->s->seconds_gc;
+  }
+  return PLUM_OK;
+}
+
 int plum_sim_gc_stats(plum_sim* s, long long attempted[2], long long accepted[2], int* n_beads, int* n_mols) {
   if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
   for (int k = 0; k < 2; k++) {

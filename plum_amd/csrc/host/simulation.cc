@@ -1,5 +1,6 @@
 #include "simulation.h"
 
+#include <chrono>
 #include <cmath>
 #include <sstream>
 #include <stdexcept>
@@ -178,6 +179,50 @@ void Simulation::Run(long long steps) {
     else TranslationalMove();
     steps_done++;
   }
+}
+
+void Simulation::RunDevice(long long steps) {
+  update_counts();
+  long long pending = 0;
+  auto flush = [&]() {
+    if (pending == 0) return;
+    if ((int)mols.size() - r.phantom > 0) {
+      plum_mc_params mc{};
+      for (int k = 0; k < 5; k++) mc.move_prob[k] = r.move_prob[k];
+      mc.move_size = r.move_size;
+      mc.phantom = r.phantom;
+      mc.grafted = r.grafted;
+      mc.bond_len = 0;
+      mc.vary_bond = 0;
+      if (ff.UseBondRigid()) mc.bond_len = ff.RigidBondLen();
+      if (ff.UseBondPot()) {
+        mc.bond_len = ff.EqBondLen();
+        mc.vary_bond = 1;
+      }
+      mc.seed = ((unsigned long long)rng() << 32) | rng();
+      const auto t0 = std::chrono::steady_clock::now();
+      const plum_mc_stats st = ff.DeviceMoves(mols, mc, pending);
+      seconds_device += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      for (int k = 0; k < 5; k++) {
+        attempted[k] += st.attempted[k];
+        accepted[k] += st.accepted[k];
+      }
+    }
+    pending = 0;
+  };
+  for (long long s = 0; s < steps; s++) {
+    const int rn = (int)rng();
+    if (ff.UseGC() && rn % r.gc_freq == 0) {
+      flush();
+      const auto t0 = std::chrono::steady_clock::now();
+      GCMove();
+      seconds_gc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+      pending++;
+    }
+    steps_done++;
+  }
+  flush();
 }
 
 // Simulation::GCMove (simulation.cc:357-404)

@@ -31,6 +31,12 @@ class Simulation {
   void TranslationalMove();
   // simulation.cc:357-404: insertion or deletion of one exchanged chain
   void GCMove();
+  // The same step sequence as Run (the GC coin on the host stream), with the
+  // translational moves between two exchanges run as one batch by the
+  // engine's device loop (plum_mc_run, its own counter-based streams): the
+  // same Markov chain, not the same draws.
+  void RunDevice(long long steps);
+  double seconds_device = 0, seconds_gc = 0;
   double RadiusOfGyration() const;
   double EndToEndDistance() const;
 

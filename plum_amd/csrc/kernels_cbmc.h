@@ -18,6 +18,8 @@
 
 namespace plum {
 
+constexpr int kCbmcMaxTrials = 1024;
+
 struct CbmcIn {
   int K, cur, ex_lo, ex_hi;
   int type_c, type_i, charged, pad;

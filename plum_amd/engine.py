@@ -26,7 +26,7 @@ ABI_SYMBOLS = (
     "plum_mc_set_tuning", "plum_surrogate_enable", "plum_surrogate_disable",
     "plum_surrogate_stats_get", "plum_surrogate_get_weights", "plum_surrogate_set_weights",
     "plum_surrogate_train", "plum_surrogate_forward", "plum_surrogate_gradient",
-    "plum_surrogate_debug_slot", "plum_max_mol_len", "plum_cbmc_trials",
+    "plum_surrogate_debug_slot", "plum_max_mol_len", "plum_cbmc_trials", "plum_cbmc_grow",
 )
 SURROGATE_PARAMS = 64 * 128 + 128 + 2 * (128 * 128 + 128) + 128 + 1
 DESCR = 64
@@ -110,6 +110,13 @@ class CbmcRequest(C.Structure):
                 ("charged", C.c_int), ("q_chain", C.c_double), ("q_ion", C.c_double)]
 
 
+class GrowthRequest(C.Structure):
+    _fields_ = [("mode", C.c_int), ("chain_len", C.c_int), ("n_trials", C.c_int),
+                ("exclude_lo", C.c_int), ("exclude_hi", C.c_int), ("type_chain", C.c_int),
+                ("type_ion", C.c_int), ("charged", C.c_int), ("q_chain", C.c_double),
+                ("q_ion", C.c_double)]
+
+
 class SurrogateStats(C.Structure):
     _fields_ = [("s1_rejected", C.c_longlong), ("s1_passed", C.c_longlong),
                 ("train_evals", C.c_longlong), ("stale_cuts", C.c_longlong),
@@ -167,6 +174,8 @@ def lib():
         L.plum_surrogate_debug_slot.argtypes = [vp, C.c_int, fp, dp, ip]
         L.plum_max_mol_len.argtypes = [vp]
         L.plum_cbmc_trials.argtypes = [vp, C.POINTER(CbmcRequest), dp, dp, dp]
+        L.plum_cbmc_grow.argtypes = [vp, C.POINTER(GrowthRequest), dp, dp, dp, dp, dp, dp,
+                                     C.POINTER(C.c_int), dp]
         for f in ABI_SYMBOLS:
             if f != "plum_last_error":
                 getattr(L, f).restype = C.c_int
@@ -364,6 +373,30 @@ class Engine:
         out = np.zeros(K)
         _check(lib().plum_cbmc_trials(self.h, C.byref(rq), _dp(tri), _dp(g), _dp(out)))
         return out
+
+    def cbmc_grow(self, mode, L, K, type_chain, q_chain, first=None, disp=None, ions=None,
+                  sel=None, actual=None, type_ion=0, q_ion=0.0, charged=True, exclude=None):
+        """One whole configurational-bias growth (plum_cbmc_grow): mode 0
+        insertion, 1 deletion, 2 test insertion.  disp / ions (L, K, 3) and
+        sel (L,) hold the draws of steps 1..L-1 (row 0 unused); first (2, 3)
+        the first bead and its counterion; actual (2 L, 3) the chain being
+        deleted then its counterions.  Returns (W, stop_step, chain (2 L, 3))."""
+        def arr(a, n):
+            return np.zeros(n) if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        f = arr(first, 6)
+        dsp = arr(disp, 3 * L * K)
+        ion = arr(ions, 3 * L * K)
+        sl = arr(sel, L)
+        act = arr(actual, 6 * L)
+        lo, hi = exclude if exclude is not None else (0, -1)
+        rq = GrowthRequest(mode, L, K, lo, hi, type_chain, type_ion, int(charged), q_chain,
+                           q_ion if charged else 0.0)
+        w = C.c_double()
+        stop = C.c_int()
+        chain = np.zeros(6 * L)
+        _check(lib().plum_cbmc_grow(self.h, C.byref(rq), _dp(f), _dp(dsp), _dp(ion), _dp(sl),
+                                    _dp(act), C.byref(w), C.byref(stop), _dp(chain)))
+        return w.value, stop.value, chain.reshape(-1, 3)
 
     def surrogate_forward(self, x, bf16=True):
         fp = C.POINTER(C.c_float)

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libplum_host.so")
 HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim_energy",
                 "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
                 "plum_host_last_error", "plum_synth_count", "plum_synth_system",
-                "plum_sim_gc_stats", "plum_sim_widom")
+                "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device")
 
 
 class RunInSummary(C.Structure):
@@ -63,6 +63,8 @@ def lib():
         L.plum_sim_gc_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plum_sim_widom.argtypes = [vp, C.POINTER(C.c_double)]
+        L.plum_sim_run_device.argtypes = [vp, C.c_longlong, C.POINTER(C.c_longlong),
+                                          C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         L.plum_host_last_error.restype = C.c_char_p
         L.plum_synth_count.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plum_synth_system.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_double),
@@ -104,6 +106,15 @@ class HostSimulation:
         out = np.zeros(3 * n)
         _check(lib().plum_sim_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n))
         return out.reshape(-1, 3)
+
+    def run_device(self, steps):
+        """Steps with the translational moves on the device loop; returns
+        (attempted, accepted, [seconds in the device loop, in exchanges])."""
+        att = (C.c_longlong * 5)()
+        acc = (C.c_longlong * 5)()
+        sec = (C.c_double * 2)()
+        _check(lib().plum_sim_run_device(self.h, steps, att, acc, sec))
+        return list(att), list(acc), list(sec)
 
     def gc_stats(self):
         """(attempted [ins, del], accepted [ins, del], n_beads, n_mols)"""

@@ -137,3 +137,23 @@ def test_host_grand_canonical_run_replays_reference_sequence(tmp_path, mu, steps
         assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), k
     orc.close()
     sim.close()
+
+
+@pytest.mark.gpu
+def test_host_grand_canonical_device_run_bookkeeping(tmp_path):
+    """Exchanges between batches of device-loop moves: the running totals
+    stay the from-scratch energy, and the host molecules follow the device
+    coordinates."""
+    sd = systems.config("gc_small")
+    sd["gc"] = dict(sd["gc"], chem_pot=-45.0)
+    path = systems.write_reference_inputs(sd, str(tmp_path))
+    sim = H.HostSimulation(path, 12)
+    for _ in range(4):
+        att, acc, sec = sim.run_device(300)
+        gatt, gacc, nb, nm = sim.gc_stats()
+        run, full = sim.energies()
+        for k in full:
+            assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), k
+    print("device run: gc %s %s, moves %s %s, beads %d" % (gatt, gacc, att, acc, nb), flush=True)
+    assert sum(att) > 100 and sum(gatt) > 50
+    sim.close()
