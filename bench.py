@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--replica-sweeps", type=int, default=512)
     ap.add_argument("--no-gc", action="store_true")
     ap.add_argument("--gc-steps", type=int, default=3000)
+    ap.add_argument("--gc-only", action="store_true",
+                    help="run only the grand-canonical leg (for profiling)")
+    ap.add_argument("--growth-only", action="store_true",
+                    help="only whole-chain growths of the grand-canonical workload (for profiling)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only device copies of known size (scale of the HBM counters)")
     return ap.parse_args()
 
 
@@ -239,6 +245,33 @@ def replica_leg(sweeps):
     return res
 
 
+def growth_leg(n=40):
+    """Whole 32-bead insertion growths (16 trials per bead) into the
+    grand-canonical workload's starting configuration, with fixed draws:
+    the device time of one growth."""
+    sd = systems.config("c4")
+    eng = E.Engine(sd)
+    rng = np.random.default_rng(1)
+    L, K = sd["gc"]["chain_len"], sd["gc"]["trials"]
+    box = np.asarray(sd["box"])
+    first = rng.uniform(0.1, 0.9, (2, 3)) * box
+    v = rng.normal(size=(L, K, 3))
+    disp = 0.98 * v / np.linalg.norm(v, axis=2)[..., None]
+    ions = rng.uniform(0, 1, (L, K, 3)) * box
+    sel = rng.uniform(0, 1, L)
+    kw = dict(first=first, disp=disp, ions=ions, sel=sel, q_ion=1.0)
+    for _ in range(3):
+        eng.cbmc_grow(0, L, K, 0, -1.0, **kw)
+    t = time.perf_counter()
+    for _ in range(n):
+        w, stop, _ = eng.cbmc_grow(0, L, K, 0, -1.0, **kw)
+    dt = (time.perf_counter() - t) / n
+    eng.close()
+    log("growth: %.1f us per 32-bead growth (%.1f us per step)" % (1e6 * dt, 1e6 * dt / L))
+    return {"growth_us": 1e6 * dt, "step_us": 1e6 * dt / L, "growths": n, "weight": w,
+            "stop": stop}
+
+
 def gc_leg(steps):
     """Grand-canonical bulk: 32-bead chains (charge -1 per bead) exchanged
     together with 32 counterions by configurational-bias insertion and
@@ -294,12 +327,49 @@ def gc_leg(steps):
     return res
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_compute_spec_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by the same counter on a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["bytes_per_launch"], t
+    except (OSError, ValueError, KeyError):
+        return None, None
+
+
+def copy_probe():
+    """Device-to-device copies of 256 MiB: 256 MiB read and 256 MiB written
+    per copy kernel, the known traffic the HBM counters are scaled by."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n, "copies": 5}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build measures a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path).", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
+        return
+    if a.gc_only:
+        print(json.dumps({"grand_canonical": gc_leg(a.gc_steps)}), flush=True)
+        return
+    if a.growth_only:
+        print(json.dumps({"growth": growth_leg()}), flush=True)
+        return
     t0 = time.perf_counter()
     sd = systems.config(a.config)
     n = len(sd["q"])
@@ -337,6 +407,7 @@ def main():
     achieved = stp.moves * per_move / (avg_us * launches * 1e-6) if avg_us > 0 else 0.0
     per_launch = stp.moves * per_move / max(launches, 1)
 
+    traffic, tinfo = measured_traffic()
     out = {
         "metric": "MC trial moves/s on 1 MI355X (plain vs surrogate-screened), \u0394E err vs FP64",
         "value": value, "unit": "moves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -350,7 +421,7 @@ def main():
         "kernel_avg_us": {k: round(v[0], 3) for k, v in kt.items()},
         "roofline": {"bound": "mfma", "pipe": "FP64 VALU (dense FP64 peak)", "kernel": rl_name,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "work_per_launch": per_launch, "moves_per_launch": stp.moves / max(launches, 1),
                      "basis": "per move: k-space 16 x %.1f moved beads x %d k + 8 x %d k; real "
                               "space %.0f FLOP x 2 positions x %.1f moved beads x %.0f candidate "
@@ -358,6 +429,9 @@ def main():
                                   n_moved, info.n_k_half, info.n_k_half, FLOP_PER_PAIR, n_moved,
                                   cand)},
     }
+    if tinfo is not None:
+        out["roofline"]["traffic_source"] = tinfo.get("source")
+        out["roofline"]["traffic_per_move"] = tinfo.get("bytes_per_move")
     if not a.no_fp32:
         e32 = E.Engine(sd, precision=E.PLUM_FP32_PAIR)
         st32, s32 = timed_run(e32, sd, a.steps, a.warmup, a.moves_per_step, seed=1)
