@@ -422,7 +422,8 @@ class GCSim:
                 info = OrcStepInfo()
                 lib().orc_translational_move(self.orc.h, C.byref(self.rng.s), _dp(mp),
                                              sd["move_size"], sd.get("phantom", 0),
-                                             sd.get("grafted", 0), sd["r0"], 1, C.byref(info))
+                                             sd.get("grafted", 0), sd["r0"],
+                                             1 if sd["use_bond"] else 0, C.byref(info))
                 if info.attempted:
                     self.attempted[info.move_type] += 1
                     self.accepted[info.move_type] += info.accepted

@@ -21,8 +21,11 @@ import numpy as np
 
 CONFIG_SEEDS = {"c0": 1000, "c1": 1001, "c2": 1002, "c3": 1003, "c4": 1004}
 # chemical potential (kT) of the exchanged chain + counterions of the
-# grand-canonical workload
-GC_CHEM_POT = -400.0
+# grand-canonical workload: Widom test insertions into the starting
+# configuration give beta mu = -ln(<W> C) between about -390 and -405 (the
+# V / Lambda powers of 32 counterions dominate); at -400 the 50 starting
+# chains drain away, so the workload sits a little above that
+GC_CHEM_POT = -395.0
 
 
 class _Grid:

@@ -157,3 +157,67 @@ def test_host_grand_canonical_device_run_bookkeeping(tmp_path):
     print("device run: gc %s %s, moves %s %s, beads %d" % (gatt, gacc, att, acc, nb), flush=True)
     assert sum(att) > 100 and sum(gatt) > 50
     sim.close()
+
+
+def _deck_sysdef(path):
+    """System definition of a reference run.in whose crd/top hold no beads
+    (the grand-canonical example), read by parameter name."""
+    rows = [line.split() for line in open(path) if line.strip()]
+    d = {r[0]: r[1] for r in rows}
+    types, k = [], 1
+    while "s3_bead_type_%d" % k in d:
+        types.append({"name": d["s3_bead_type_%d" % k],
+                      "sigma": float(d["s3_sigma_for_bead_type_%d" % k]),
+                      "eps": float(d["s3_epsilon_for_bead_type_%d" % k]),
+                      "wall_sigma": 0.0, "wall_eps": 0.0})
+        k += 1
+    mp = [float(d[x]) for x in ("s1_prob_b_bead_translation", "s1_prob_p_COM_translation",
+                                "s1_prob_p_pivot", "s1_prob_p_crankshaft",
+                                "s1_prob_p_random_reptation")]
+    top = {r[0]: r[1] for r in (line.split() for line in
+                                open(os.path.join(os.path.dirname(path), "input_top.dat")))
+           if len(r) > 1}
+    box = [float(top["Box_Length_%s:" % a]) for a in "XYZ"]
+    return dict(npbc=int(d["s1_PBC_dimensions"]), beta=float(d["s1_beta_(1/kBT)"]),
+                use_pair=int(d["s2_use_short_range_potential"]),
+                use_ewald=int(d["s2_use_Ewald_potential"]), use_bond=int(d["s2_use_bond_potential"]),
+                use_ext=0, lj_cutoff=float(d["s3_LJ_cutoff"]), lB=float(d["s3_Bjerrum_length"]),
+                dielectric=float(d["s3_dielectric_constant"]), alpha=float(d["s3_Ewald_alpha"]),
+                dipole=int(d["s3_use_dipole_correction"]), k_bond=0.0,
+                r0=float(d["s3_rigid_bond_length"]), wall_cut=-1.0, wall_sigma_w=1.0,
+                wall_eps_w=1.0, phantom=0, grafted=0, move_prob=mp,
+                move_size=float(d["s1_MC_move_size_in_unit_length"]), types=types, box=box,
+                xyz=np.zeros((0, 3)), q=np.zeros(0), type=np.zeros(0, dtype=np.int32),
+                mol_start=np.zeros(1, dtype=np.int32),
+                gc=dict(chain_len=int(d["s3_chain_length"]),
+                        bead_charge=float(d["s3_bead_charge_in_e"]),
+                        trials=int(d["s3_no_of_CBMC_trials"]), freq=int(d["s3_GCMC_move_frequency"]),
+                        chem_pot=float(d["s3_chemical_potential_in_kT"]),
+                        debroglie=float(d["s3_deBroglie_prefactor"]), type=0))
+
+
+@pytest.mark.gpu
+def test_reference_grand_canonical_example_replays_on_the_oracle():
+    """The reference's own grand-canonical deck (empty box of 100, 8-bead chains of
+    charge -1 with counterions, 30 trials, mu = -90, rigid bonds, WCA, l_B 2.5,
+    alpha 0.001 so 125 real-space images): host driver and oracle fill the box
+    with the same exchanges and trial moves."""
+    from oracle import oracle as O
+    path = os.path.join(EX, "bulk_muvt", "run.in")
+    sd = _deck_sysdef(path)
+    steps = 800
+    sim = H.HostSimulation(path, 7)
+    att, acc = sim.run(steps)
+    gatt, gacc, nb, nm = sim.gc_stats()
+    orc = O.GCSim(sd, 7)
+    orc.run(steps)
+    print("example deck: host gc %s %s trial %s %s | oracle gc %s %s trial %s %s | beads %d" % (
+        gatt, gacc, att, acc, orc.gc_attempted, orc.gc_accepted, orc.attempted, orc.accepted,
+        nb), flush=True)
+    assert gatt == orc.gc_attempted and gacc == orc.gc_accepted
+    assert att == orc.attempted and acc == orc.accepted
+    assert nb == len(orc.sd["q"]) and sum(gacc) > 0
+    if nb:
+        assert np.max(np.abs(sim.positions(nb) - orc.orc.positions())) <= 1e-9
+    orc.close()
+    sim.close()
