@@ -167,7 +167,7 @@ def fp32_term_errors(sd, moves=20):
     return {k: float("%.3g" % v) for k, v in worst.items()}
 
 
-def brush_leg(moves):
+def brush_leg(moves, batch=16):
     """Confined brush (64 x 50 grafted chains + counterions, 400 charged
     surface sites, slab Ewald): the plain speculative path against delayed
     acceptance with the online-trained surrogate, in accepted moves per
@@ -198,7 +198,39 @@ def brush_leg(moves):
     acc = sum(st.accepted)
     da = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
     log("brush DA: %.0f moves/s, %.0f accepted/s" % (da[0], da[1]))
-    eng.close()
+    # the batched form: several brushes advanced together, every one screened
+    # by the network trained above (it stays with the first system)
+    R = batch
+    sds = [sd] + [systems.config("c3", seed=1003 + 101 * r) for r in range(1, R)]
+    engs = [eng] + [E.Engine(x) for x in sds[1:]]
+    mcs = [dict(kw, seed=500 + r) for r in range(R)]
+    b = E.Batch(engs)
+    b.set_tuning(pmax=16, batches_per_graph=16)
+    b.mc_run(moves // R, mcs)                  # warm-up (training continues)
+    stb = b.mc_run(moves // R * 2, mcs)
+    accb = sum(stb.accepted)
+    log("brush DA batched x%d: %.0f moves/s, %.0f accepted/s" % (R, stb.moves / stb.seconds,
+                                                                   accb / stb.seconds))
+    b.close()
+    eng.surrogate_disable()
+    b = E.Batch(engs)
+    b.set_tuning(pmax=8)
+    b.mc_run(moves // R, mcs)
+    stp = b.mc_run(moves // R * 2, mcs)
+    accp = sum(stp.accepted)
+    log("brush plain batched x%d: %.0f moves/s, %.0f accepted/s" % (R, stp.moves / stp.seconds,
+                                                                      accp / stp.seconds))
+    b.close()
+    for e in engs:
+        e.close()
+    res.update({
+        "systems_in_batch": R,
+        "surrogate_batched_moves_per_s": stb.moves / stb.seconds,
+        "surrogate_batched_accepted_per_s": accb / stb.seconds,
+        "plain_batched_accepted_per_s": accp / stp.seconds,
+        "surrogate_batched_vs_plain": (accb / stb.seconds) / out["plain"][1],
+        "surrogate_batched_vs_plain_batched": (accb / stb.seconds) / (accp / stp.seconds),
+    })
     res.update({
         "plain_fp64_moves_per_s": out["plain"][0], "plain_fp64_accepted_per_s": out["plain"][1],
         "plain_fp64_accept_ratio": out["plain"][2],

@@ -31,7 +31,11 @@ struct plum_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int g_batches = 0, g_grid = 0, g_pmax = 0;  // shape the graph was captured with
-  bool da = false;        // delayed acceptance (one system with its surrogate on)
+  bool da = false;        // delayed acceptance (the first system's surrogate, shared)
+  // descriptor rows and network outputs of every system's proposals, one
+  // slice of 2 kMaxScreen maxm rows per system (R == 1: the surrogate's own)
+  float *sx = nullptr, *sf = nullptr;
+  int sx_rows = 0, sx_cap = 0, maxm_b = 0;
   bool profiling = false;
   std::vector<void*> allocs;
 };
@@ -89,6 +93,46 @@ int balloc(plum_batch* b, T** ptr, size_t count) {
 
 int batch_grid(const plum_batch* b) { return std::max(1, b->n_cu * b->tune.blocks_per_cu); }
 
+// Proposal slots for batches of up to kMaxScreen screened proposals.
+int ensure_screen_slots(plum_ctx* c) {
+  if (c->screen_cap >= kMaxScreen) return PLUM_OK;
+  Dev& d = c->d;
+  const size_t slots = (size_t)kMaxScreen * c->maxm;
+  CHK(dalloc(c, &d.mv_old, slots));
+  CHK(dalloc(c, &d.mv_new, slots));
+  CHK(dalloc(c, &d.mv_type, slots));
+  CHK(dalloc(c, &d.smv_idx, slots));
+  CHK(dalloc(c, &d.tab, slots * 2 * std::max(1, c->g.tab_w)));
+  c->screen_cap = kMaxScreen;
+  return PLUM_OK;
+}
+
+// Delayed acceptance over a batch: every system screens with the first
+// system's network; only the first feeds its training ring.
+int batch_setup_da(plum_batch* b) {
+  plum_ctx* c0 = b->reps[0];
+  b->maxm_b = 0;
+  for (int r = 0; r < b->R; r++) {
+    CHK(ensure_screen_slots(b->reps[r]));
+    b->maxm_b = std::max(b->maxm_b, b->reps[r]->maxm);
+  }
+  b->sx_rows = 2 * kMaxScreen * b->maxm_b;
+  if (b->R == 1) {
+    b->sx = c0->sur->sur_x;
+    b->sf = c0->sur->sur_f;
+    return PLUM_OK;
+  }
+  const int need = b->R * b->sx_rows;
+  if (need > b->sx_cap) {
+    CHK(balloc(b, &b->sx, (size_t)need * kDescr));
+    CHK(balloc(b, &b->sf, (size_t)need));
+    HIPCHK(hipMemsetAsync(b->sx, 0, sizeof(float) * (size_t)need * kDescr, b->st));
+    HIPCHK(hipMemsetAsync(b->sf, 0, sizeof(float) * (size_t)need, b->st));
+    b->sx_cap = need;
+  }
+  return PLUM_OK;
+}
+
 // Uploads every system's Geo / Dev (a cell rebuild changes both), launch
 // shape and Monte Carlo parameters.
 int batch_upload_state(plum_batch* b) {
@@ -96,7 +140,8 @@ int batch_upload_state(plum_batch* b) {
   std::vector<Dev> D(b->R);
   std::vector<RepCfg> RCv(b->R);
   b->max_nkb_full = 1;
-  b->da = b->R == 1 && b->reps[0]->sur && b->reps[0]->sur->on;
+  b->da = b->reps[0]->sur && b->reps[0]->sur->on;
+  if (b->da) CHK(batch_setup_da(b));
   for (int r = 0; r < b->R; r++) {
     plum_ctx* c = b->reps[r];
     c->nsplit = split_for(c->cand, b->tune.split_div);
@@ -111,10 +156,14 @@ int batch_upload_state(plum_batch* b) {
     q.nmmb = c->nmmb;
     q.rstride = 2 * c->maxm * kMaxSplit;
     q.xcap = c->xcap;
-    q.da = (b->R == 1 && c->sur && c->sur->on) ? 1 : 0;
-    q.fill = q.da ? c->sur->p.fill : 0;
-    q.r_d = q.da ? (float)c->sur->p.r_d : 0.f;
-    q.pad = 0;
+    q.da = b->da ? 1 : 0;
+    q.train = (b->da && r == 0) ? 1 : 0;
+    q.fill = q.train ? b->reps[0]->sur->p.fill : 0;
+    q.r_d = q.da ? (float)b->reps[0]->sur->p.r_d : 0.f;
+    if (b->da) {
+      D[r].sur_x = b->sx + (size_t)r * b->sx_rows * kDescr;
+      D[r].sur_f = b->sf + (size_t)r * b->sx_rows;
+    }
     b->max_nkb_full = std::max(b->max_nkb_full, c->nkb_full);
   }
   HIPCHK(hipMemcpyAsync(b->dG, G.data(), sizeof(Geo) * b->R, hipMemcpyHostToDevice, b->st));
@@ -135,13 +184,16 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
                      R);
   if (ev) (void)hipEventRecord(ev[1], b->st);
   if (b->da) {
-    // surrogate: descriptors of every moved bead (old, new), network outputs
+    // surrogate: descriptors of every moved bead (old, new) of every system,
+    // then the network on all of them in one launch
     plum_ctx* c = b->reps[0];
-    hipLaunchKernelGGL(k_descr_spec, dim3(2 * c->maxm, b->tune.pmax), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC);
+    hipLaunchKernelGGL(k_descr_spec, dim3(2 * b->maxm_b, b->tune.pmax, R), dim3(kBlock), 0, b->st, b->dG, b->dD,
+                       b->dRC);
     if (ev) (void)hipEventRecord(ev[2], b->st);
-    const int rows = b->tune.pmax * 2 * c->maxm;
+    // R == 1: the rows of the proposals in use; R > 1: every slice
+    const int rows = R == 1 ? b->tune.pmax * 2 * c->maxm : R * b->sx_rows;
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(b->n_cu, (rows + 127) / 128))), dim3(kBlock), 0, b->st,
-                       c->sur->net, c->sur->sur_x, rows, c->sur->sur_f);
+                       c->sur->net, b->sx, rows, b->sf);
   } else if (ev) {
     (void)hipEventRecord(ev[2], b->st);
   }
@@ -444,7 +496,7 @@ int check_tuning(const plum_batch_tuning* t) {
 // most the allocated proposal slots with it.
 int check_pmax(const plum_batch* b, int pmax) {
   for (plum_ctx* c : b->reps) {
-    const bool da = b->R == 1 && c->sur && c->sur->on;
+    const bool da = b->reps[0]->sur && b->reps[0]->sur->on;
     if (pmax > (da ? c->screen_cap : kMaxProp))
       return fail(PLUM_E_ARG, da ? "pmax exceeds the screened proposals of the surrogate" : "pmax must be 1..8");
   }

@@ -60,7 +60,7 @@ struct RepCfg {
   int da;       // delayed acceptance on
   int fill;     // exact slots given to first-stage rejections (training data)
   float r_d;    // descriptor radius of the surrogate
-  int pad;
+  int train;    // this system feeds the surrogate's training ring
 };
 
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
@@ -681,8 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     long long skipped = 0, valid = 0, s1r = 0, s1p = 0, tev = 0, cuts = 0;
     double last_total = C->total;
     int done = ndec;
-    int head = rc.da ? d.tr_ctl[0] : 0;
-    const int ring = rc.da ? d.tr_ctl[1] : 0;
+    int head = rc.train ? d.tr_ctl[0] : 0;
+    const int ring = rc.train ? d.tr_ctl[1] : 0;
     for (int s = 0; s < ndec; s++) {
       const PropInfo& P = sm.s_prop[s];
       if (P.prev >= 0 && ((acc_screen >> P.prev) & 1ull)) {
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         tt[T_BOND] = P.bond_trial - sm.s_bcur[s];
         dE += tt[T_BOND];
       }
-      if (rc.da && !early && P.n > 0 && ring > 0) {
+      if (rc.train && !early && P.n > 0 && ring > 0) {
         // training sample: the pair + k-space part the network estimates
         const int slot = head % ring;
         d.tr_target[slot] = lj + re + kk;
@@ -826,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     C->train_evals += tev;
     C->stale_cuts += cuts;
     C->move_counter = ctr0 + done;
-    if (rc.da) d.tr_ctl[0] = head;
+    if (rc.train) d.tr_ctl[0] = head;
     plan->acc_mask = mask;
     sm.s_mask = mask;
     sm.s_ndone = done;
@@ -837,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     if (!(mask & (1u << e))) continue;
     commit_cells(g, prop_view(g, d, rc, sm.s_screen[e], e), sm.s_nc, sm.s_flag);
   }
-  if (rc.da) {
+  if (rc.train) {
     // descriptor rows of the training samples (old rows, then new rows)
     for (int e = 0; e < kMaxProp; e++) {
       const int slot = sm.s_train[e];

@@ -147,13 +147,15 @@ __device__ void descr_row(const Geo& g, const Dev& v, float r_d, int m, int on, 
 }
 
 // Descriptors of every moved bead of every proposal, old and new: grid
-// (2 maxm, pmax); block (m2, s) is bead m2 % maxm at old (m2 < maxm) or new.
+// (2 maxm, pmax, systems); block (m2, s, r) is bead m2 % maxm of system r at
+// old (m2 < maxm) or new.
 __global__ __launch_bounds__(kBlock) void k_descr_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                        const RepCfg* __restrict__ RC) {
   __shared__ DescrSmem sm;
-  const Geo& g = G[0];
-  const Dev& d = D[0];
-  const RepCfg& rc = RC[0];
+  const Geo& g = G[blockIdx.z];
+  const Dev& d = D[blockIdx.z];
+  const RepCfg& rc = RC[blockIdx.z];
+  if ((int)blockIdx.x >= 2 * rc.maxm) return;
   const int s = blockIdx.y;
   if (s >= d.plan->n_prop) return;
   const int on = blockIdx.x >= rc.maxm ? 1 : 0, m = blockIdx.x - on * rc.maxm;

@@ -124,16 +124,7 @@ int plum_surrogate_enable(plum_ctx* c, const plum_surrogate_params* p) {
     c->sur = nullptr;
   }
   // proposal slots for the screened batch
-  if (c->screen_cap < kMaxScreen) {
-    Dev& d = c->d;
-    const size_t slots = (size_t)kMaxScreen * c->maxm;
-    CHK(dalloc(c, &d.mv_old, slots));
-    CHK(dalloc(c, &d.mv_new, slots));
-    CHK(dalloc(c, &d.mv_type, slots));
-    CHK(dalloc(c, &d.smv_idx, slots));
-    CHK(dalloc(c, &d.tab, slots * 2 * std::max(1, c->g.tab_w)));
-    c->screen_cap = kMaxScreen;
-  }
+  CHK(ensure_screen_slots(c));
   SurHost* s = new SurHost();
   s->p = *p;
   c->sur = s;

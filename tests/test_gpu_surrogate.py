@@ -177,3 +177,33 @@ def test_training_reduces_the_loss_and_the_trained_network_gains():
           "(%d accepted)" % (e0, a0, e1, a1), flush=True)
     assert a1 > 0 and e1 < e0
     eng.close()
+
+
+def test_delayed_acceptance_over_a_batch_of_systems():
+    """Four systems advanced together, all screened by the first system's
+    network (one descriptor and one inference launch per batch for all of
+    them), only the first feeding its training ring: every system keeps its
+    running totals equal to its from-scratch energy."""
+    sds = [systems.config("small", seed=50 + r) for r in range(4)]
+    engs = [E.Engine(sd) for sd in sds]
+    engs[0].surrogate_enable(screen=16, fill=2, train_every=8, train_samples=32, seed=3)
+    b = E.Batch(engs)
+    b.set_tuning(pmax=16)
+    mc = [_mc(sd, 300 + r) for r, sd in enumerate(sds)]
+    before = [e.positions() for e in engs]
+    st = b.mc_run(2000, mc)
+    s = engs[0].surrogate_stats()
+    print("batched DA: %d moves, accepted %s, stats %s" % (st.moves, list(st.accepted), s),
+          flush=True)
+    assert st.moves == 4 * 2000
+    assert s["s1_rejected"] > 0 and s["train_steps"] > 0
+    for e, x0 in zip(engs, before):
+        run, full = e.totals(), e.energy_full()
+        for k in E.TERMS:
+            if k == "dipole":
+                continue
+            assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, run[k], full[k])
+        assert np.abs(e.positions() - x0).max() > 0
+    b.close()
+    for e in engs:
+        e.close()
