@@ -57,6 +57,15 @@ int plum_sim_energy(plum_sim* sim, plum_terms* running, plum_terms* from_scratch
 int plum_sim_positions(plum_sim* sim, double* xyz, int n_beads);
 /* Simulation::RadiusOfGyration / EndToEndDistance (simulation.cc:406-501). */
 int plum_sim_observables(plum_sim* sim, double* rg2, double* ree2);
+/* Simulation::CalcRhoZ (simulation.cc:1029-1045) of the current
+   configuration: out[3 b + s] for bins of width res along z, s = chain
+   beads, cations, anions (beads per unit volume).  *n_bins receives the bin
+   count; out may be NULL to ask for it. */
+int plum_sim_density_profile(plum_sim* sim, double res, double* out, int max_bins, int* n_bins);
+/* Simulation::PrintLastCrd / PrintLastTop (simulation.cc:935-1027): writes
+   <prefix>_lastcrd.dat and <prefix>_lasttop.dat, from which a run resumes
+   (a run.in naming them reads the configuration back). */
+int plum_sim_write_last(plum_sim* sim, const char* prefix);
 int plum_sim_destroy(plum_sim* sim);
 const char* plum_host_last_error(void);
 

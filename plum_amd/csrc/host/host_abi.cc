@@ -1,4 +1,5 @@
 // extern "C" surface of the host layer (include/plum_host.h).
+#include <algorithm>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -181,6 +182,33 @@ int plum_sim_observables(plum_sim* s, double* rg2, double* ree2) {
   if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
   if (rg2) *rg2 = s->s->RadiusOfGyration();
   if (ree2) *ree2 = s->s->EndToEndDistance();
+  return PLUM_OK;
+}
+
+int plum_sim_density_profile(plum_sim* s, double res, double* out, int max_bins, int* n_bins) {
+  if (!s || !s->s || !n_bins) return fail(PLUM_E_ARG, "null argument");
+  try {
+    std::vector<double> v;
+    const int nb = s->s->DensityProfile(res, v);
+    *n_bins = nb;
+    if (out) {
+      if (nb > max_bins) return fail(PLUM_E_ARG, "buffer too small");
+      std::copy(v.begin(), v.end(), out);
+    }
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_ARG, e.what());
+  }
+  return PLUM_OK;
+}
+
+int plum_sim_write_last(plum_sim* s, const char* prefix) {
+  if (!s || !s->s || !prefix) return fail(PLUM_E_ARG, "null argument");
+  try {
+    s->s->PrintLastCrd(prefix);
+    s->s->PrintLastTop(prefix);
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_ARG, e.what());
+  }
   return PLUM_OK;
 }
 

@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <fstream>
 #include <sstream>
 #include <stdexcept>
 
@@ -306,6 +307,57 @@ void Simulation::TranslationalMove() {
     }
     if (accept) accepted[mt]++;
   }
+}
+
+int Simulation::DensityProfile(double res, std::vector<double>& out) const {
+  if (!(res > 0)) throw std::runtime_error("density profile bin width must be > 0");
+  const int nb = (int)std::floor(box[2] / res);
+  out.assign(3 * (size_t)std::max(nb, 0), 0.0);
+  for (int i = r.phantom; i < (int)mols.size(); i++) {
+    const int sp = mols[i].Size() > 1 ? 0 : (mols[i].bds[0].Charge() >= 0 ? 1 : 2);
+    for (const Bead& b : mols[i].bds) {
+      double z = b.GetCrd(0, 2);
+      z -= box[2] * std::floor(z / box[2]);
+      const int bin = (int)std::floor(z / res);
+      if (bin >= 0 && bin < nb) out[3 * (size_t)bin + sp] += 1.0 / (box[0] * box[1] * res);
+    }
+  }
+  return nb;
+}
+
+void Simulation::PrintLastCrd(const std::string& prefix) const {
+  std::ofstream f(prefix + "_lastcrd.dat");
+  if (!f) throw std::runtime_error("cannot write " + prefix + "_lastcrd.dat");
+  int n = 0;
+  for (const Molecule& m : mols) n += m.Size();
+  f << n << "\n \n";
+  for (int i = 0; i < (int)mols.size(); i++) {
+    const Molecule& m = mols[i];
+    double c[3] = {0, 0, 0}, shift[3] = {0, 0, 0};
+    for (const Bead& b : m.bds)
+      for (int k = 0; k < 3; k++) c[k] += b.GetCrd(0, k);
+    if (i >= r.phantom)
+      for (int k = 0; k < 3; k++) shift[k] = std::floor(c[k] / m.Size() / box[k]) * box[k];
+    for (const Bead& b : m.bds) {
+      f << i << ' ' << b.Symbol();
+      f.precision(18);
+      for (int k = 0; k < 3; k++) f << ' ' << b.GetCrd(0, k) - shift[k];
+      f.precision(4);
+      f << ' ' << b.Charge() << '\n';
+    }
+  }
+}
+
+void Simulation::PrintLastTop(const std::string& prefix) const {
+  std::ofstream f(prefix + "_lasttop.dat");
+  if (!f) throw std::runtime_error("cannot write " + prefix + "_lasttop.dat");
+  int n = 0;
+  for (const Molecule& m : mols) n += m.Size();
+  f << "TotNoOfBeads: " << n << "\nTotNoOfMolec: " << mols.size() << '\n';
+  f.precision(18);
+  f << "Box_Length_X: " << box[0] << "\nBox_Length_Y: " << box[1] << "\nBox_Length_Z: " << box[2] << '\n';
+  // the reference writes the bond, angle and dihedral lists empty
+  f << "bonds:\n-1\nangles:\n-1\ndiheds\n-1\n";
 }
 
 // Simulation::RadiusOfGyration (simulation.cc:406-434)

@@ -39,6 +39,15 @@ class Simulation {
   double seconds_device = 0, seconds_gc = 0;
   double RadiusOfGyration() const;
   double EndToEndDistance() const;
+  // Simulation::CalcRhoZ (simulation.cc:1029-1045) of the current
+  // configuration: out[3 b + s], s = chain beads, cations, anions, in beads
+  // per unit volume of the bins of width res; returns the bin count.
+  int DensityProfile(double res, std::vector<double>& out) const;
+  // Simulation::PrintLastCrd / PrintLastTop (simulation.cc:935-1027): the
+  // configuration as <prefix>_lastcrd.dat and <prefix>_lasttop.dat, each
+  // molecule shifted by whole boxes so that its centre lies in the box.
+  void PrintLastCrd(const std::string& prefix) const;
+  void PrintLastTop(const std::string& prefix) const;
 
   RunIn r;
   double box[3] = {0, 0, 0};

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libplum_host.so")
 HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim_energy",
                 "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
                 "plum_host_last_error", "plum_synth_count", "plum_synth_system",
-                "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device")
+                "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device",
+                "plum_sim_density_profile", "plum_sim_write_last")
 
 
 class RunInSummary(C.Structure):
@@ -63,6 +64,9 @@ def lib():
         L.plum_sim_gc_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plum_sim_widom.argtypes = [vp, C.POINTER(C.c_double)]
+        L.plum_sim_density_profile.argtypes = [vp, C.c_double, C.POINTER(C.c_double), C.c_int,
+                                               C.POINTER(C.c_int)]
+        L.plum_sim_write_last.argtypes = [vp, C.c_char_p]
         L.plum_sim_run_device.argtypes = [vp, C.c_longlong, C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         L.plum_host_last_error.restype = C.c_char_p
@@ -115,6 +119,19 @@ class HostSimulation:
         sec = (C.c_double * 2)()
         _check(lib().plum_sim_run_device(self.h, steps, att, acc, sec))
         return list(att), list(acc), list(sec)
+
+    def density_profile(self, res=1.0):
+        """rho(z) of chain beads, cations, anions: array (bins, 3)."""
+        nb = C.c_int()
+        _check(lib().plum_sim_density_profile(self.h, res, None, 0, C.byref(nb)))
+        out = np.zeros(3 * nb.value)
+        _check(lib().plum_sim_density_profile(self.h, res,
+                                              out.ctypes.data_as(C.POINTER(C.c_double)),
+                                              nb.value, C.byref(nb)))
+        return out.reshape(-1, 3)
+
+    def write_last(self, prefix):
+        _check(lib().plum_sim_write_last(self.h, prefix.encode()))
 
     def gc_stats(self):
         """(attempted [ins, del], accepted [ins, del], n_beads, n_mols)"""

@@ -221,3 +221,42 @@ def test_reference_grand_canonical_example_replays_on_the_oracle():
         assert np.max(np.abs(sim.positions(nb) - orc.orc.positions())) <= 1e-9
     orc.close()
     sim.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["small", "slab_small"])
+def test_last_state_files_resume_the_run(tmp_path, name):
+    """PrintLastCrd / PrintLastTop: the written configuration, read back by a
+    run.in that names it, is the same system up to whole-box shifts of
+    molecules, with the same energy; the density profile matches its
+    definition (simulation.cc:1029-1045) on the same coordinates."""
+    sd = systems.config(name)
+    path = systems.write_reference_inputs(sd, str(tmp_path))
+    sim = H.HostSimulation(path, 9)
+    sim.run(400)
+    n = len(sd["q"])
+    x0 = sim.positions(n)
+    _, e0 = sim.energies()
+    rho = sim.density_profile(1.0)
+    sim.write_last(str(tmp_path / "out"))
+    lines = open(path).read().splitlines()
+    lines[0] = "s1_input_coordinate_file out_lastcrd.dat"
+    lines[1] = "s1_input_topology_file out_lasttop.dat"
+    resume = tmp_path / "resume.in"
+    resume.write_text("\n".join(lines) + "\n")
+    sim2 = H.HostSimulation(str(resume), 9)
+    x1 = sim2.positions(n)
+    _, e1 = sim2.energies()
+    box = np.asarray(sd["box"])
+    ms = sd["mol_start"]
+    for m in range(len(ms) - 1):
+        d = (x1[ms[m]:ms[m + 1]] - x0[ms[m]:ms[m + 1]]) / box
+        k = np.round(d)
+        assert np.abs(d - k).max() < 1e-12 and (k == k[0]).all(), m
+    for t in e0:
+        assert abs(e1[t] - e0[t]) <= 1e-9 * max(1.0, abs(e0[t])), (t, e0[t], e1[t])
+    from test_gpu_ensemble import _rho_z
+    ref = _rho_z(x0, sd["q"], ms, sd["box"], 1.0, sd.get("phantom", 0))
+    assert np.abs(rho - ref).max() <= 1e-12
+    sim.close()
+    sim2.close()
