@@ -43,7 +43,7 @@ namespace plum {
 
 constexpr int kMaxReplicas = 1024;
 constexpr int kMaxSplit = 8;           // most blocks per real-space query
-constexpr int kDefaultSplitDiv = 384;  // candidate slots per query block
+constexpr int kDefaultSplitDiv = 1536;  // candidate slots per query block
 constexpr int kKSpec = 16;             // k vectors per speculative k-space block
 constexpr int kKSpecGroups = kBlock / kKSpec;
 
