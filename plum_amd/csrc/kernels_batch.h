@@ -265,10 +265,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
   __shared__ PlanSmem sm;
   const int r = blockIdx.x;
   if (r >= R) return;
-  const Geo& g = G[r];
-  const Dev& d = D[r];
-  const McDev& mc = MC[r];
-  const RepCfg& rc = RC[r];
+  // by-value copies through the constant address space: the per-system
+  // structs stay in scalar registers instead of being re-read after stores
+  const Geo g = load_const(G, r);
+  const Dev d = load_const(D, r);
+  const McDev mc = load_const(MC, r);
+  const RepCfg rc = load_const(RC, r);
   SpecPlan* plan = d.plan;
   const int np = plan->n_prop;
   const int t = threadIdx.x;
@@ -681,8 +683,13 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     long long skipped = 0, valid = 0, s1r = 0, s1p = 0, tev = 0, cuts = 0;
     double last_total = C->total;
     int done = ndec;
-    int head = rc.train ? d.tr_ctl[0] : 0;
-    const int ring = rc.train ? d.tr_ctl[1] : 0;
+    // the per-system constants the decisions read, held in registers (the
+    // loop stores to global memory, which would otherwise force reloads)
+    const int use_pair = g.use_pair, use_ext = g.use_ext, use_ewald = g.use_ewald, use_bond = g.use_bond;
+    const int dipole = g.dipole, dipole_exact = g.dipole_exact, da_on = rc.da, train = rc.train;
+    const double dip_coef = g.dip_coef, beta = g.beta;
+    int head = train ? d.tr_ctl[0] : 0;
+    const int ring = train ? d.tr_ctl[1] : 0;
     for (int s = 0; s < ndec; s++) {
       const PropInfo& P = sm.s_prop[s];
       if (P.prev >= 0 && ((acc_screen >> P.prev) & 1ull)) {
@@ -691,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         done = s;
         break;
       }
-      if (rc.da && acc_screen) {
+      if (da_on && acc_screen) {
         // the surrogate value of this proposal was taken before an earlier
         // acceptance; if its descriptors could see that molecule, stop here
         bool stale = false;
@@ -737,29 +744,29 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
       double tt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       double dE = 0;
       bool early = false;
-      if (g.use_pair) {
+      if (use_pair) {
         tt[T_LJ] = lj;
         dE += lj;
         if (dE >= kVeryLargeEnergy) early = true;
       }
-      if (!early && g.use_ext) {
+      if (!early && use_ext) {
         tt[T_WALL] = P.forbid ? kVeryLargeEnergy : P.ew;
         dE += tt[T_WALL];
         if (dE >= kVeryLargeEnergy) early = true;
       }
       double trial_dipl = cur_dipl;
-      if (!early && g.use_ewald) {
+      if (!early && use_ewald) {
         tt[T_REAL] = re;
         tt[T_KSPACE] = kk;
         double dd = 0;
-        if (g.dipole) {
-          if (g.dipole_exact) {
+        if (dipole) {
+          if (dipole_exact) {
             const double mz1 = Mz + P.dmz;
-            trial_dipl = g.dip_coef * mz1 * mz1;
-            dd = trial_dipl - g.dip_coef * Mz * Mz;
+            trial_dipl = dip_coef * mz1 * mz1;
+            dd = trial_dipl - dip_coef * Mz * Mz;
           } else {
             // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
-            trial_dipl = g.dip_coef * Mz * Mz;
+            trial_dipl = dip_coef * Mz * Mz;
             dd = trial_dipl - cur_dipl;
           }
         }
@@ -767,11 +774,11 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         dE += re + kk + dd;
       }
       const int len = sm.s_len[s];
-      if (!early && g.use_bond && len > 1) {
+      if (!early && use_bond && len > 1) {
         tt[T_BOND] = P.bond_trial - sm.s_bcur[s];
         dE += tt[T_BOND];
       }
-      if (rc.train && !early && P.n > 0 && ring > 0) {
+      if (train && !early && P.n > 0 && ring > 0) {
         // training sample: the pair + k-space part the network estimates
         const int slot = head % ring;
         d.tr_target[slot] = lj + re + kk;
@@ -787,10 +794,10 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         tev++;
         continue;
       }
-      if (rc.da) s1p++;
+      if (da_on) s1p++;
       // Metropolis (simulation.cc:327-332), no draw for a forbidden move; with
       // the surrogate, the second stage min(1, exp(-beta (dE - dE_s)))
-      const int acc = (dE < kVeryLargeEnergy) && (sm.s_u[s] < exp(-g.beta * (dE - sm.s_des[s])));
+      const int acc = (dE < kVeryLargeEnergy) && (sm.s_u[s] < exp(-beta * (dE - sm.s_des[s])));
       for (int k = 0; k < 8; k++) last[k] = tt[k];
       last_total = dE;
       if (P.type >= 0 && P.type < 5) {
@@ -802,9 +809,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
         mask |= 1u << e;
         acc_screen |= 1ull << s;
         Mz += P.dmz;
-        if (g.dipole) cur_dipl = trial_dipl;
+        if (dipole) cur_dipl = trial_dipl;
         for (int k = 0; k < 8; k++) run[k] += tt[k];
-        if (g.use_bond && len > 1) d.bond_cur[P.mol] = P.bond_trial;
+        if (use_bond && len > 1) d.bond_cur[P.mol] = P.bond_trial;
       }
     }
     C->Mz = Mz;
