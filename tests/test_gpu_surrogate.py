@@ -110,43 +110,53 @@ def _block_stats(v, blocks=10):
 
 
 def test_delayed_acceptance_samples_the_plain_ensemble():
-    """Energy and radius of gyration averages of the delayed-acceptance chain
-    agree with the plain chain's within their statistical errors, both chains
-    starting from a configuration equilibrated by a long plain run; the
-    delayed-acceptance run keeps its running totals equal to the from-scratch
-    energy."""
+    """Energy and radius of gyration of delayed-acceptance chains agree with
+    plain chains within 2 sigma, sigma from the spread of independent chains
+    on each side, all starting from one configuration equilibrated by a long
+    plain run.  The network is trained online first and then held fixed for
+    the measured chains; every delayed-acceptance chain keeps its running
+    totals equal to its from-scratch energy."""
     name = "small"
     sd0 = systems.config(name)
     eq = E.Engine(sd0)
     eq.mc_run(30000, **_mc(sd0, 41))
     sd = dict(sd0, xyz=eq.positions())
     eq.close()
-    res = {}
+    tr = E.Engine(sd)
+    tr.surrogate_enable(screen=16, fill=2, train_every=8, train_samples=32, seed=3)
+    tr.mc_run(6000, **_mc(sd, 90))
+    weights = tr.surrogate_weights()
+    tr.close()
+    n_chains, samples, chunk, skip = 6, 200, 150, 40
+    res = {"plain": [], "da": []}
+    stats = None
     for mode in ("plain", "da"):
-        eng = E.Engine(sd)
-        if mode == "da":
-            eng.surrogate_enable(screen=16, fill=2, train_every=8, train_samples=32, seed=3)
-            eng.mc_run(3000, **_mc(sd, 90))      # first training
-        obs = []
-        for i in range(300):
-            eng.mc_run(150, **_mc(sd, 1000 + i))
-            obs.append(_observables(eng, sd))
-        obs = np.array(obs)
-        res[mode] = [_block_stats(obs[:, j]) for j in range(2)]
-        if mode == "da":
-            print("da stats", eng.surrogate_stats(), flush=True)
-            run, full = eng.totals(), eng.energy_full()
-            for k in E.TERMS:
-                assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, run[k], full[k])
-        eng.close()
-    zs = []
+        for c in range(n_chains):
+            eng = E.Engine(sd)
+            if mode == "da":
+                eng.surrogate_enable(screen=16, fill=0, train_every=0, seed=3)
+                eng.surrogate_weights(weights)
+            obs = []
+            for i in range(samples):
+                eng.mc_run(chunk, **_mc(sd, 7919 * (c + 1) + i))
+                if i >= skip:
+                    obs.append(_observables(eng, sd))
+            res[mode].append(np.mean(obs, 0))
+            if mode == "da":
+                stats = eng.surrogate_stats()
+                run, full = eng.totals(), eng.energy_full()
+                for k in E.TERMS:
+                    assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, run[k], full[k])
+            eng.close()
+    print("da stats (last chain)", stats, flush=True)
+    p, d = np.array(res["plain"]), np.array(res["da"])
+    err = lambda a: a.std(0, ddof=1) / np.sqrt(len(a))
+    z = np.abs(p.mean(0) - d.mean(0)) / np.hypot(err(p), err(d))
     for j, label in enumerate(("energy", "Rg^2")):
-        (m1, s1), (m2, s2) = res["plain"][j], res["da"][j]
-        z = abs(m1 - m2) / np.hypot(s1, s2)
-        zs.append(z)
-        print("%s: plain %.4f +- %.4f, DA %.4f +- %.4f, |z| = %.2f" % (label, m1, s1, m2, s2, z),
-              flush=True)
-    assert max(zs) < 3.0
+        print("%s: plain %.4f +- %.4f, DA %.4f +- %.4f, |z| = %.2f" % (
+            label, p.mean(0)[j], err(p)[j], d.mean(0)[j], err(d)[j], z[j]), flush=True)
+    assert stats["s1_rejected"] > 0
+    assert z.max() < 2.0
 
 
 def test_training_reduces_the_loss_and_the_trained_network_gains():
