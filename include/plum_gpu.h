@@ -194,7 +194,7 @@ typedef struct {
   int split_div;         /* candidate slots per real-space query block */
   int batches_per_graph; /* speculative batches captured in one graph (also
                             the cell compaction period) */
-  int pmax;              /* proposals per speculative batch, 1..8 */
+  int pmax;              /* proposals per speculative batch, 1..16 (1..32 screened with the surrogate) */
   double sweep_us;       /* measured time per move of every system (plum_batch_tune) */
 } plum_batch_tuning;
 

@@ -498,7 +498,7 @@ int check_pmax(const plum_batch* b, int pmax) {
   for (plum_ctx* c : b->reps) {
     const bool da = b->reps[0]->sur && b->reps[0]->sur->on;
     if (pmax > (da ? c->screen_cap : kMaxProp))
-      return fail(PLUM_E_ARG, da ? "pmax exceeds the screened proposals of the surrogate" : "pmax must be 1..8");
+      return fail(PLUM_E_ARG, da ? "pmax exceeds the screened proposals of the surrogate" : "pmax must be 1..16");
   }
   return PLUM_OK;
 }
@@ -609,8 +609,8 @@ int plum_batch_tune(plum_batch* b, const plum_mc_params* mcp, int moves_per_tria
   if (!b || !mcp) return fail(PLUM_E_ARG, "null argument");
   if (moves_per_trial < 1) return fail(PLUM_E_ARG, "moves_per_trial must be >= 1");
   const int bpc[] = {4, 8, 16};
-  const int div[] = {192, 384, 768};
-  const int pm[] = {1, 2, 4, 8};
+  const int div[] = {384, 768, 1536};
+  const int pm[] = {1, 2, 4, 8, 16};
   const plum_batch_tuning keep = b->tune;
   plum_batch_tuning win = keep;
   double best_s = 1e300;

@@ -17,7 +17,7 @@ struct MoveRec {
   int pad;
 };
 
-constexpr int kMaxProp = 8;                                   // exactly evaluated proposals per batch
+constexpr int kMaxProp = 16;                                  // exactly evaluated proposals per batch
 constexpr int kMaxScreen = 32;                                // screened proposals per batch
 constexpr int kMaxPairs = kMaxProp * (kMaxProp - 1) / 2;       // exact-slot pairs
 constexpr int kKVals = kMaxProp + kMaxPairs;                   // k-space sums per batch

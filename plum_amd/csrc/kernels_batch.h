@@ -257,6 +257,9 @@ struct PlanSmem {
   double s_bcur[kMaxScreen];
   double s_u1[kMaxScreen];
   int s_len[kMaxScreen];
+  int s_screen[kMaxProp];
+  int s_xblk[kMaxPairs];   // cross blocks of a slot pair within range, -1 if not
+  int s_ne, s_off;
 };
 
 __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
@@ -302,64 +305,86 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     }
     __syncthreads();
   }
-  if (t != 0) return;
-  int ne = 0, ndec = np;
-  for (int s = 0; s < np; s++) {
-    const PropInfo& P = sm.s_prop[s];
-    plan->exact_of[s] = -1;
-    int pass = 1;
-    double des = 0;
-    if (rc.da && P.valid) {
-      // surrogate dE: exact spring and wall parts plus the network's estimate
-      // of the pair and k-space parts; first stage min(1, exp(-beta dE_s))
-      if (g.use_bond && sm.s_len[s] > 1) des += P.bond_trial - sm.s_bcur[s];
-      if (g.use_ext) des += P.forbid ? kVeryLargeEnergy : P.ew;
-      des += sm.s_nn[s];
-      pass = des < kVeryLargeEnergy && sm.s_u1[s] < exp(-g.beta * des);
-    }
-    plan->pass1[s] = pass;
-    plan->de_s[s] = des;
-    if (pass && P.valid) {
-      if (ne == kMaxProp) {
-        ndec = s;   // no exact slot left: this proposal is decided next batch
-        break;
+  if (t == 0) {
+    int ne = 0, ndec = np;
+    for (int s = 0; s < np; s++) {
+      const PropInfo& P = sm.s_prop[s];
+      plan->exact_of[s] = -1;
+      int pass = 1;
+      double des = 0;
+      if (rc.da && P.valid) {
+        // surrogate dE: exact spring and wall parts plus the network's estimate
+        // of the pair and k-space parts; first stage min(1, exp(-beta dE_s))
+        if (g.use_bond && sm.s_len[s] > 1) des += P.bond_trial - sm.s_bcur[s];
+        if (g.use_ext) des += P.forbid ? kVeryLargeEnergy : P.ew;
+        des += sm.s_nn[s];
+        pass = des < kVeryLargeEnergy && sm.s_u1[s] < exp(-g.beta * des);
       }
-      plan->exact_of[s] = ne;
-      plan->screen_of[ne] = s;
-      ne++;
+      plan->pass1[s] = pass;
+      plan->de_s[s] = des;
+      if (pass && P.valid) {
+        if (ne == kMaxProp) {
+          ndec = s;   // no exact slot left: this proposal is decided next batch
+          break;
+        }
+        plan->exact_of[s] = ne;
+        sm.s_screen[ne] = s;
+        ne++;
+      }
     }
-  }
-  // spare exact slots: first-stage rejections, evaluated for training only
-  for (int s = 0; s < ndec && ne < rc.fill; s++)
-    if (plan->exact_of[s] < 0 && !plan->pass1[s] && sm.s_prop[s].valid && sm.s_prop[s].n > 0) {
-      plan->exact_of[s] = ne;
-      plan->screen_of[ne] = s;
-      ne++;
+    // spare exact slots: first-stage rejections, evaluated for training only
+    for (int s = 0; s < ndec && ne < rc.fill; s++)
+      if (plan->exact_of[s] < 0 && !plan->pass1[s] && sm.s_prop[s].valid && sm.s_prop[s].n > 0) {
+        plan->exact_of[s] = ne;
+        sm.s_screen[ne] = s;
+        ne++;
+      }
+    plan->n_decide = ndec;
+    plan->n_exact = ne;
+    int off = 0;
+    plan->prop_off[0] = 0;
+    for (int e = 0; e < ne; e++) {
+      plan->screen_of[e] = sm.s_screen[e];
+      const int n = sm.s_prop[sm.s_screen[e]].n;
+      off += n > 0 ? 2 * n * rc.nsplit + nmm_blocks(n) : 0;
+      plan->prop_off[e + 1] = off;
     }
-  plan->n_decide = ndec;
-  plan->n_exact = ne;
-  int off = 0;
-  plan->prop_off[0] = 0;
-  for (int e = 0; e < ne; e++) {
-    const int n = sm.s_prop[plan->screen_of[e]].n;
-    off += n > 0 ? 2 * n * rc.nsplit + nmm_blocks(n) : 0;
-    plan->prop_off[e + 1] = off;
+    sm.s_ne = ne;
+    sm.s_off = off;
   }
+  __syncthreads();
+  // slot pairs whose moved beads can interact (bounding spheres within the
+  // search radius), one thread per pair
+  const int ne = sm.s_ne, npair = ne * (ne - 1) / 2;
+  for (int j = t; j < npair; j += kPlanThreads) {
+    int e = (int)((1.0 + sqrt(1.0 + 8.0 * j)) * 0.5);
+    while (e * (e - 1) / 2 > j) e--;
+    while ((e + 1) * e / 2 <= j) e++;
+    const int e0 = j - e * (e - 1) / 2;
+    const PropInfo& pq = sm.s_prop[sm.s_screen[e]];
+    const PropInfo& pp = sm.s_prop[sm.s_screen[e0]];
+    int xb = -1;
+    if (pq.n > 0 && pp.n > 0 && pp.mol != pq.mol &&
+        center_dist(pp.c, pq.c, g) <= pp.rad + pq.rad + g.rs + 1e-9)
+      xb = cross_blocks(pp.n, pq.n);
+    sm.s_xblk[j] = xb;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  // near pairs in (e, e0) order, e0 < e
   int nn = 0, xo = 0;
+  const int off = sm.s_off;
   plan->pair_off[0] = 0;
-  for (int e = 1; e < ne; e++) {
-    const PropInfo& pq = sm.s_prop[plan->screen_of[e]];
-    if (pq.n == 0) continue;
-    for (int e0 = 0; e0 < e; e0++) {
-      const PropInfo& pp = sm.s_prop[plan->screen_of[e0]];
-      if (pp.n == 0 || pp.mol == pq.mol) continue;
-      if (center_dist(pp.c, pq.c, g) > pp.rad + pq.rad + g.rs + 1e-9) continue;
-      plan->pair_p[nn] = e0;
-      plan->pair_q[nn] = e;
-      xo += cross_blocks(pp.n, pq.n);
-      nn++;
-      plan->pair_off[nn] = xo;
-    }
+  for (int j = 0; j < npair; j++) {
+    if (sm.s_xblk[j] < 0) continue;
+    int e = (int)((1.0 + sqrt(1.0 + 8.0 * j)) * 0.5);
+    while (e * (e - 1) / 2 > j) e--;
+    while ((e + 1) * e / 2 <= j) e++;
+    plan->pair_p[nn] = j - e * (e - 1) / 2;
+    plan->pair_q[nn] = e;
+    xo += sm.s_xblk[j];
+    nn++;
+    plan->pair_off[nn] = xo;
   }
   plan->n_near = nn;
   // k-space blocks also fold in the pending S(k) of the last batch
@@ -367,19 +392,22 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
 }
 
 // ------------------------------------------------------------ k space
-// 16 k vectors x 16 bead groups per block; every exact slot's dS_e(k)
-// accumulates in registers, the groups meet in LDS, and the 16 threads of
-// group 0 form dE_e = c_k (2 Re(S* dS_e) + |dS_e|^2) and the cross sums.  The
-// S(k) of the slots accepted by the previous batch is folded in first.
-struct KSpecSmem {
-  double2 s_ds[kMaxProp][kKSpecGroups][kKSpec];
-};
+// 16 k vectors x 16 bead groups per block.  Exact slots go in chunks of
+// kSlotChunk: each slot's dS_e(k) accumulates in registers per group, the
+// groups meet in LDS, and the per-k sums land in s_t.  Then one thread per
+// value forms, over the block's 16 k vectors, dE_e = sum c_k (2 Re(S* dS_e) +
+// |dS_e|^2) or a cross sum 2 sum c_k Re(dS_p* dS_q).  The S(k) of the slots
+// accepted by the previous batch is folded in first.
+constexpr int kSlotChunk = 8;
+static_assert(kKVals <= kBlock, "one thread per k-space value");
+static_assert(kMaxProp <= 32, "exact slots are bits of one mask");
 
-__device__ __forceinline__ double sum16(double v) {
-#pragma unroll
-  for (int off = kKSpec / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kKSpec);
-  return v;  // valid in lane 0 of each 16-lane group
-}
+struct KSpecSmem {
+  double2 s_ds[kSlotChunk][kKSpecGroups][kKSpec];
+  double2 s_t[kMaxProp][kKSpec];
+  double2 s_S[kKSpec];
+  double s_ck[kKSpec];
+};
 
 __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, const SpecPlan* plan, int ne,
                                   unsigned acc_mask, int blk, KSpecSmem& sm) {
@@ -392,78 +420,97 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
     oy = g.nkx + 1 + l.y + g.nky;
     oz = g.nkx + 1 + 2 * g.nky + 1 + l.z + g.nkz;
   }
-#pragma unroll
-  for (int e = 0; e < kMaxProp; e++) {
-    if (e < ne) {
-      double2 acc = make_double2(0, 0);
-      const int s = uload(&plan->screen_of[e]);
-      const int n = uload(&d.recs[s].n_moved);
-      if (k < g.n_kh) {
-        const double4* mo = d.mv_old + s * rc.maxm;
-        const double2* base = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
-        for (int m = grp; m < n; m += kKSpecGroups) {
-          const double qm = mo[m].w;
-          const double2* to = base + (size_t)(2 * m) * g.tab_w;
-          const double2* tn = to + g.tab_w;
-          const double2 eo = cmul(cmul(to[ox], to[oy]), to[oz]);
-          const double2 en = cmul(cmul(tn[ox], tn[oy]), tn[oz]);
-          acc.x += qm * (en.x - eo.x);
-          acc.y += qm * (en.y - eo.y);
-        }
+  // the previous batch's accepted dS rows join S(k) before this batch's
+  // rows overwrite them (the chunk loop's barrier orders the two)
+  if (grp == 0) {
+    double2 S = make_double2(0, 0);
+    double ck = 0;
+    if (k < g.n_kh) {
+      S = d.Sk[k];
+      if (acc_mask) {
+        for (int e = 0; e < kMaxProp; e++)
+          if (acc_mask & (1u << e)) {
+            const double2 prev = d.dS[(size_t)e * g.n_kh + k];
+            S.x += prev.x;
+            S.y += prev.y;
+          }
+        d.Sk[k] = S;
       }
-      sm.s_ds[e][grp][kk] = acc;
+      ck = d.ck[k];
     }
+    sm.s_S[kk] = S;
+    sm.s_ck[kk] = ck;
   }
-  __syncthreads();
-  if (grp != 0) return;
-  double2 t[kMaxProp];
+  for (int e0 = 0; e0 < ne; e0 += kSlotChunk) {
 #pragma unroll
-  for (int e = 0; e < kMaxProp; e++) t[e] = make_double2(0, 0);
-  double2 S = make_double2(0, 0);
-  double ck = 0;
-  if (k < g.n_kh) {
-    S = d.Sk[k];
-    if (acc_mask) {
-#pragma unroll
-      for (int e = 0; e < kMaxProp; e++)
-        if (acc_mask & (1u << e)) {
-          const double2 prev = d.dS[(size_t)e * g.n_kh + k];
-          S.x += prev.x;
-          S.y += prev.y;
-        }
-      d.Sk[k] = S;
-    }
-    ck = d.ck[k];
-#pragma unroll
-    for (int e = 0; e < kMaxProp; e++) {
+    for (int c = 0; c < kSlotChunk; c++) {
+      const int e = e0 + c;
       if (e < ne) {
-#pragma unroll
-        for (int q = 0; q < kKSpecGroups; q++) {
-          t[e].x += sm.s_ds[e][q][kk].x;
-          t[e].y += sm.s_ds[e][q][kk].y;
+        double2 acc = make_double2(0, 0);
+        const int s = uload(&plan->screen_of[e]);
+        const int n = uload(&d.recs[s].n_moved);
+        if (k < g.n_kh) {
+          const double4* mo = d.mv_old + s * rc.maxm;
+          const double2* base = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
+          for (int m = grp; m < n; m += kKSpecGroups) {
+            const double qm = mo[m].w;
+            const double2* to = base + (size_t)(2 * m) * g.tab_w;
+            const double2* tn = to + g.tab_w;
+            const double2 eo = cmul(cmul(to[ox], to[oy]), to[oz]);
+            const double2 en = cmul(cmul(tn[ox], tn[oy]), tn[oz]);
+            acc.x += qm * (en.x - eo.x);
+            acc.y += qm * (en.y - eo.y);
+          }
         }
-        d.dS[(size_t)e * g.n_kh + k] = t[e];
+        sm.s_ds[c][grp][kk] = acc;
       }
     }
-  }
-  // sums over the block's 16 k vectors (t = 0 past n_kh)
-  const int nkb = rc.nkb_s;
+    __syncthreads();
+    // slot c of the chunk, k vector kk: the sum over the 16 groups
+    if (grp < kSlotChunk && e0 + grp < ne) {
+      const int e = e0 + grp;
+      double2 t = make_double2(0, 0);
 #pragma unroll
-  for (int e = 0; e < kMaxProp; e++) {
-    if (e >= ne) break;
-    const double v = ck * (2.0 * (S.x * t[e].x + S.y * t[e].y) + t[e].x * t[e].x + t[e].y * t[e].y);
-    const double sv = sum16(v);
-    if (threadIdx.x == 0) d.part_ks[(size_t)e * nkb + blk] = sv;
-  }
-#pragma unroll
-  for (int q = 1; q < kMaxProp; q++) {
-    if (q >= ne) break;
-#pragma unroll
-    for (int p = 0; p < q; p++) {
-      const double x = 2.0 * ck * (t[p].x * t[q].x + t[p].y * t[q].y);
-      const double sv = sum16(x);
-      if (threadIdx.x == 0) d.part_ks[(size_t)(kMaxProp + pair_index(p, q)) * nkb + blk] = sv;
+      for (int q = 0; q < kKSpecGroups; q++) {
+        t.x += sm.s_ds[grp][q][kk].x;
+        t.y += sm.s_ds[grp][q][kk].y;
+      }
+      if (k >= g.n_kh) t = make_double2(0, 0);
+      sm.s_t[e][kk] = t;
+      if (k < g.n_kh) d.dS[(size_t)e * g.n_kh + k] = t;
     }
+    __syncthreads();
+  }
+  if (ne == 0) return;
+  // one thread per value: dE of slot v (v < ne), then the cross sums of the
+  // slot pairs (v - kMaxProp = pair_index(p, q), q < ne), fixed k order
+  const int nkb = rc.nkb_s;
+  const int v = threadIdx.x;
+  if (v < kKVals) {
+    int p = -1, q = -1;
+    if (v < kMaxProp) {
+      if (v >= ne) return;
+    } else {
+      const int j = v - kMaxProp;
+      q = (int)((1.0 + sqrt(1.0 + 8.0 * j)) * 0.5);
+      while (q * (q - 1) / 2 > j) q--;
+      while ((q + 1) * q / 2 <= j) q++;
+      p = j - q * (q - 1) / 2;
+      if (q >= ne) return;
+    }
+    double sv = 0;
+#pragma unroll 4
+    for (int i = 0; i < kKSpec; i++) {
+      const double ck = sm.s_ck[i];
+      if (v < kMaxProp) {
+        const double2 t = sm.s_t[v][i], S = sm.s_S[i];
+        sv += ck * (2.0 * (S.x * t.x + S.y * t.y) + t.x * t.x + t.y * t.y);
+      } else {
+        const double2 tp = sm.s_t[p][i], tq = sm.s_t[q][i];
+        sv += 2.0 * ck * (tp.x * tq.x + tp.y * tq.y);
+      }
+    }
+    d.part_ks[(size_t)v * nkb + blk] = sv;
   }
 }
 

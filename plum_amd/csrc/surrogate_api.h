@@ -112,7 +112,7 @@ int plum_max_mol_len(const plum_ctx* c) { return c ? c->maxm : PLUM_E_ARG; }
 int plum_surrogate_enable(plum_ctx* c, const plum_surrogate_params* p) {
   if (!c || !p) return fail(PLUM_E_ARG, "null argument");
   if (p->screen < 1 || p->screen > kMaxScreen) return fail(PLUM_E_ARG, "screen must be 1..32");
-  if (p->fill < 0 || p->fill > kMaxProp) return fail(PLUM_E_ARG, "fill must be 0..8");
+  if (p->fill < 0 || p->fill > kMaxProp) return fail(PLUM_E_ARG, "fill must be 0..16 (the exact slots)");
   if (p->train_every < 0 || p->train_samples < 1 || p->ring < p->train_samples)
     return fail(PLUM_E_ARG, "need train_samples >= 1 and ring >= train_samples");
   if (!(p->r_d > kRbfMin + 0.5) || !(p->lr > 0) || !(p->target_clip > 0))

@@ -52,31 +52,35 @@ def test_device_loop_bookkeeping(name, moves):
 
 @pytest.mark.parametrize("name,moves", [("small", 1500), ("c0", 1500), ("slab_small", 1500)])
 def test_speculative_batches_follow_the_sequential_chain(name, moves):
-    """Batches of up to 8 proposals evaluated together, resolved in order with
-    the cross terms of earlier acceptances, make the same Markov chain as one
-    proposal at a time (same decisions; positions equal to round-off)."""
+    """Batches of up to 8 and 16 proposals evaluated together, resolved in
+    order with the cross terms of earlier acceptances, make the same Markov
+    chain as one proposal at a time (same decisions; positions equal to
+    round-off)."""
     sd = systems.config(name)
     kw = dict(move_size=sd["move_size"], seed=23, phantom=sd.get("phantom", 0),
               grafted=sd.get("grafted", 0), bond_len=1.0)
     runs = {}
-    for pmax in (1, 8):
+    for pmax in (1, 8, 16):
         eng = E.Engine(sd)
         eng.set_tuning(pmax=pmax)
         st = eng.mc_run(moves, sd["move_prob"], **kw)
         runs[pmax] = (eng, st)
-    (e1, s1), (e8, s8) = runs[1], runs[8]
-    print("%s: accepted p1=%s p8=%s" % (name, list(s1.accepted), list(s8.accepted)), flush=True)
-    assert list(s1.attempted) == list(s8.attempted)
-    assert list(s1.accepted) == list(s8.accepted)
-    assert np.allclose(e1.positions(), e8.positions(), rtol=0, atol=1e-9)
-    t1, t8 = e1.totals(), e8.totals()
-    for k in TERMS:
-        assert abs(t1[k] - t8[k]) <= 1e-9 * max(1.0, abs(t1[k])), (k, t1[k], t8[k])
-    full = e8.energy_full()
-    for k in TERMS:
-        if k == "dipole":
-            continue
-        assert abs(t8[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, t8[k], full[k])
+    e1, s1 = runs[1]
+    for pmax in (8, 16):
+        ep, sp = runs[pmax]
+        print("%s: accepted p1=%s p%d=%s" % (name, list(s1.accepted), pmax, list(sp.accepted)),
+              flush=True)
+        assert list(s1.attempted) == list(sp.attempted)
+        assert list(s1.accepted) == list(sp.accepted)
+        assert np.allclose(e1.positions(), ep.positions(), rtol=0, atol=1e-9)
+        t1, tp = e1.totals(), ep.totals()
+        for k in TERMS:
+            assert abs(t1[k] - tp[k]) <= 1e-9 * max(1.0, abs(t1[k])), (k, t1[k], tp[k])
+        full = ep.energy_full()
+        for k in TERMS:
+            if k == "dipole":
+                continue
+            assert abs(tp[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, tp[k], full[k])
 
 
 def test_device_loop_is_deterministic():
