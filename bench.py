@@ -167,7 +167,7 @@ def fp32_term_errors(sd, moves=20):
     return {k: float("%.3g" % v) for k, v in worst.items()}
 
 
-def brush_leg(moves, batch=16):
+def brush_leg(moves, batch=32):
     """Confined brush (64 x 50 grafted chains + counterions, 400 charged
     surface sites, slab Ewald): the plain speculative path against delayed
     acceptance with the online-trained surrogate, in accepted moves per
