@@ -654,8 +654,26 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
 }
 
 // ------------------------------------------------------------ resolve
+// The items of consecutive accepted exact slots committed together:
+// group slot j holds items [goff[j], goff[j + 1]) of proposal gscr[j].
+struct SlotItems {
+  const double4* mv_new;
+  const int* mv_idx;
+  int maxm;
+  const int* goff;
+  const int* gscr;
+  int ns, n;
+  __device__ __forceinline__ void get(int k, double4& p, int& i) const {
+    int j = 0;
+    while (j + 1 < ns && goff[j + 1] <= k) j++;
+    const size_t o = (size_t)gscr[j] * maxm + (k - goff[j]);
+    p = mv_new[o];
+    i = mv_idx[o];
+  }
+};
+
 struct ResSmem {
-  int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
+  CommitSmem cm;
   PropInfo s_prop[kMaxScreen];
   double s_bcur[kMaxScreen];
   int s_len[kMaxScreen];
@@ -669,8 +687,217 @@ struct ResSmem {
   double2 s_xsum[kMaxPairs];
   int s_near[kMaxPairs];
   int s_train[kMaxProp];         // ring slot of a training sample per exact slot, -1 none
+  int s_goff[kMaxProp + 1], s_gscr[kMaxProp];
   int s_mask, s_ndone;
 };
+
+static_assert(kMaxScreen <= kWave, "one lane per proposal in the decision wave");
+
+__device__ __forceinline__ double lane_double(double v, int lane) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, lane);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// The reference's sequential Metropolis chain over the batch
+// (simulation.cc:241-355), one wave, one lane per proposal.  A pass
+// evaluates every undecided proposal against the acceptances so far; the
+// proposals before the first acceptance (or stop) of the pass are decided
+// as they would be one after the other, since nothing they read changes
+// until then.  The accepted one updates the cross terms, the dipole moment
+// and the stop conditions of the proposals after it, and the next pass
+// starts behind it: one pass per acceptance instead of one step per
+// proposal.  Each dE adds the cross terms in acceptance order, which is the
+// order of increasing exact slot, as the one-at-a-time chain does.
+__device__ void resolve_decide(const Geo& g, const Dev& d, const RepCfg& rc, Ctl* C, SpecPlan* plan, int np,
+                               int ndec, unsigned long long ctr0, ResSmem& sm) {
+  const int lane = threadIdx.x;
+  const bool mine = lane < np;
+  PropInfo P = {};
+  P.prev = -1;
+  P.type = -1;
+  if (mine) P = sm.s_prop[lane];
+  const int e = mine ? sm.s_exact[lane] : -1;
+  const int pass = mine ? sm.s_pass[lane] : 0;
+  const double des = mine ? sm.s_des[lane] : 0.0, u = mine ? sm.s_u[lane] : 1.0;
+  const int len = mine ? sm.s_len[lane] : 0;
+  const double bcur = mine ? sm.s_bcur[lane] : 0.0;
+  const int use_pair = g.use_pair, use_ext = g.use_ext, use_ewald = g.use_ewald, use_bond = g.use_bond;
+  const int dipole = g.dipole, dipole_exact = g.dipole_exact, da_on = rc.da, train = rc.train;
+  const double dip_coef = g.dip_coef, beta = g.beta;
+  // pair and k-space dE against the state before the batch (a proposal that
+  // moves no bead has no real-space blocks: its pair terms are zero), plus
+  // the cross terms of the acceptances so far
+  double lj = 0, re = 0, kk = 0;
+  if (e >= 0) {
+    lj = P.n > 0 ? sm.s_rsum[e].x : 0.0;
+    re = P.n > 0 ? sm.s_rsum[e].y : 0.0;
+    kk = sm.s_ksum[e];
+  }
+  const bool evaluated = P.valid && (pass || e >= 0);
+  bool stop_prev = false, stale = false;
+  unsigned mask = 0;
+  double Mz = C->Mz, cur_dipl = C->cur_dipl;
+  double run[8];
+  for (int k = 0; k < 8; k++) run[k] = C->running[k];
+  // what each proposal was decided with
+  double f_tt[8] = {0, 0, 0, 0, 0, 0, 0, 0}, f_dE = 0, f_target = 0;
+  bool f_acc = false, f_early = false;
+  int done = ndec, cut = 0, s = 0;
+  while (s < ndec) {
+    // ForceField::EnergyDifference with its early exits (force_field.cc:407-434)
+    double tt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double dE = 0;
+    bool early = false;
+    if (use_pair) {
+      tt[T_LJ] = lj;
+      dE += lj;
+      if (dE >= kVeryLargeEnergy) early = true;
+    }
+    if (!early && use_ext) {
+      tt[T_WALL] = P.forbid ? kVeryLargeEnergy : P.ew;
+      dE += tt[T_WALL];
+      if (dE >= kVeryLargeEnergy) early = true;
+    }
+    double trial_dipl = cur_dipl;
+    if (!early && use_ewald) {
+      tt[T_REAL] = re;
+      tt[T_KSPACE] = kk;
+      double dd = 0;
+      if (dipole) {
+        if (dipole_exact) {
+          const double mz1 = Mz + P.dmz;
+          trial_dipl = dip_coef * mz1 * mz1;
+          dd = trial_dipl - dip_coef * Mz * Mz;
+        } else {
+          // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
+          trial_dipl = dip_coef * Mz * Mz;
+          dd = trial_dipl - cur_dipl;
+        }
+      }
+      tt[T_DIPOLE] = dd;
+      dE += re + kk + dd;
+    }
+    if (!early && use_bond && len > 1) {
+      tt[T_BOND] = P.bond_trial - bcur;
+      dE += tt[T_BOND];
+    }
+    // Metropolis (simulation.cc:327-332), no draw for a forbidden move; with
+    // the surrogate, the second stage min(1, exp(-beta (dE - dE_s)))
+    const bool acc = evaluated && pass && (dE < kVeryLargeEnergy) && (u < exp(-beta * (dE - des)));
+    const bool stop = stop_prev || stale;
+    const bool open = lane >= s && lane < ndec;
+    const unsigned long long ev = __ballot(open && (stop || acc));
+    const int f = ev ? __builtin_ctzll(ev) : ndec;
+    if (open && (lane < f || (lane == f && !stop))) {
+      for (int k = 0; k < 8; k++) f_tt[k] = tt[k];
+      f_dE = dE;
+      f_early = early;
+      f_acc = lane == f;
+      f_target = lj + re + kk;
+    }
+    if (f >= ndec) break;
+    if (__builtin_amdgcn_readlane((int)stop, f)) {
+      // a proposal made on the assumption that its molecule's earlier
+      // proposal in the batch is rejected, or whose surrogate value could see
+      // an acceptance of this batch: it is made again next batch
+      done = f;
+      cut = __builtin_amdgcn_readlane((int)(stale && !stop_prev), f);
+      break;
+    }
+    // proposal f is accepted
+    const int ef = __builtin_amdgcn_readlane(e, f);
+    mask |= 1u << ef;
+    Mz += lane_double(P.dmz, f);
+    if (dipole) cur_dipl = lane_double(trial_dipl, f);
+    for (int k = 0; k < 8; k++) run[k] += lane_double(tt[k], f);
+    if (lane == f && use_bond && len > 1) d.bond_cur[P.mol] = P.bond_trial;
+    const double fc0 = lane_double(P.c[0], f), fc1 = lane_double(P.c[1], f), fc2 = lane_double(P.c[2], f);
+    const double frad = lane_double(P.rad, f);
+    if (lane > f && lane < ndec) {
+      if (e >= 0) {
+        const int pi = pair_index(ef, e);
+        kk += sm.s_ksum[kMaxProp + pi];
+        const int j = sm.s_near[pi];
+        if (j >= 0) {
+          lj += sm.s_xsum[j].x;
+          re += sm.s_xsum[j].y;
+        }
+      }
+      if (P.prev == f) stop_prev = true;
+      if (da_on) {
+        const double fc[3] = {fc0, fc1, fc2};
+        stale = stale || center_dist(fc, P.c, g) <= frad + P.rad + (double)rc.r_d + 1e-9;
+      }
+    }
+    s = f + 1;
+  }
+  // counters of the decided proposals
+  const bool decided = lane < done;
+  const bool valid = decided && P.valid;
+  long long att[5], acc5[5];
+  for (int k = 0; k < 5; k++) {
+    att[k] = __popcll(__ballot(valid && P.type == k));
+    acc5[k] = __popcll(__ballot(valid && f_acc && P.type == k));
+  }
+  const long long n_valid = __popcll(__ballot(valid));
+  const long long skipped = __popcll(__ballot(decided && !P.valid && P.skipped));
+  const long long s1r = __popcll(__ballot(valid && !pass));
+  const long long tev = __popcll(__ballot(valid && !pass && e >= 0));
+  const long long s1p = da_on ? __popcll(__ballot(valid && pass)) : 0;
+  // training samples in proposal order: the pair + k-space part the network
+  // estimates
+  int head = train ? d.tr_ctl[0] : 0;
+  const int ring = train ? d.tr_ctl[1] : 0;
+  const bool sample = valid && e >= 0 && train && !f_early && P.n > 0 && ring > 0;
+  const unsigned long long smask = __ballot(sample);
+  if (sample) {
+    const int slot = (head + lanes_below(smask)) % ring;
+    d.tr_target[slot] = f_target;
+    d.tr_n[slot] = P.n;
+    sm.s_train[e] = slot;
+  }
+  head += __popcll(smask);
+  // the terms of the last proposal that reached the Metropolis test
+  const unsigned long long lm = __ballot(valid && pass);
+  double last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double last_total = C->total;
+  if (lm) {
+    const int l = 63 - __builtin_clzll(lm);
+    for (int k = 0; k < 8; k++) last[k] = lane_double(f_tt[k], l);
+    last_total = lane_double(f_dE, l);
+  }
+  if (lane == 0) {
+    C->Mz = Mz;
+    C->cur_dipl = cur_dipl;
+    C->trial_dipl = cur_dipl;
+    for (int k = 0; k < 8; k++) {
+      C->running[k] = run[k];
+      C->terms[k] = last[k];
+    }
+    C->total = last_total;
+    for (int k = 0; k < 5; k++) {
+      C->attempted[k] += att[k];
+      C->accepted[k] += acc5[k];
+    }
+    C->skipped += skipped;
+    C->valid_moves += n_valid;
+    C->s1_rejected += s1r;
+    C->s1_passed += s1p;
+    C->train_evals += tev;
+    C->stale_cuts += cut;
+    C->move_counter = ctr0 + done;
+    if (train) d.tr_ctl[0] = head;
+    plan->acc_mask = mask;
+    sm.s_mask = mask;
+    sm.s_ndone = done;
+  }
+}
 
 __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                          const McDev* __restrict__ MC,
@@ -688,7 +915,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
   if (np == 0) return;
   const int ndec = plan->n_decide;
   const unsigned long long ctr0 = C->move_counter;
-  // everything the sequential decisions read, fetched in parallel
+  // everything the decisions read, fetched in parallel
   const int t = threadIdx.x;
   if (t < np) {
     const PropInfo P = d.prop[t];
@@ -715,181 +942,35 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     sm.s_train[t - 192] = -1;
   }
   __syncthreads();
-  if (t == 0) {
+  {
     const int nn = plan->n_near;
-    for (int j = 0; j < nn; j++) sm.s_near[pair_index(plan->pair_p[j], plan->pair_q[j])] = j;
+    for (int j = t; j < nn; j += kBlock) sm.s_near[pair_index(plan->pair_p[j], plan->pair_q[j])] = j;
   }
   __syncthreads();
-  if (t == 0) {
-    unsigned mask = 0;            // accepted exact slots
-    unsigned long long acc_screen = 0;  // accepted proposals
-    double Mz = C->Mz, cur_dipl = C->cur_dipl;
-    double run[8], last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < 8; k++) run[k] = C->running[k];
-    long long att[5] = {0, 0, 0, 0, 0}, acc5[5] = {0, 0, 0, 0, 0};
-    long long skipped = 0, valid = 0, s1r = 0, s1p = 0, tev = 0, cuts = 0;
-    double last_total = C->total;
-    int done = ndec;
-    // the per-system constants the decisions read, held in registers (the
-    // loop stores to global memory, which would otherwise force reloads)
-    const int use_pair = g.use_pair, use_ext = g.use_ext, use_ewald = g.use_ewald, use_bond = g.use_bond;
-    const int dipole = g.dipole, dipole_exact = g.dipole_exact, da_on = rc.da, train = rc.train;
-    const double dip_coef = g.dip_coef, beta = g.beta;
-    int head = train ? d.tr_ctl[0] : 0;
-    const int ring = train ? d.tr_ctl[1] : 0;
-    for (int s = 0; s < ndec; s++) {
-      const PropInfo& P = sm.s_prop[s];
-      if (P.prev >= 0 && ((acc_screen >> P.prev) & 1ull)) {
-        // proposed on the assumption that the molecule's earlier proposal in
-        // the batch is rejected; it was accepted: propose again next batch
-        done = s;
-        break;
-      }
-      if (da_on && acc_screen) {
-        // the surrogate value of this proposal was taken before an earlier
-        // acceptance; if its descriptors could see that molecule, stop here
-        bool stale = false;
-        for (int q = 0; q < s && !stale; q++)
-          if ((acc_screen >> q) & 1ull) {
-            const PropInfo& A = sm.s_prop[q];
-            stale = center_dist(A.c, P.c, g) <= A.rad + P.rad + (double)rc.r_d + 1e-9;
-          }
-        if (stale) {
-          done = s;
-          cuts++;
-          break;
-        }
-      }
-      if (!P.valid) {
-        if (P.skipped) skipped++;
-        continue;
-      }
-      const int e = sm.s_exact[s];
-      if (!sm.s_pass[s] && e < 0) {
-        // first-stage rejection: no exact evaluation
-        if (P.type >= 0 && P.type < 5) att[P.type] += 1;
-        valid++;
-        s1r++;
-        continue;
-      }
-      // exact dE against the state before the batch, plus the cross terms of
-      // the exact slots accepted before this proposal (a proposal that moves
-      // no bead has no real-space blocks: its pair terms are zero)
-      double lj = P.n > 0 ? sm.s_rsum[e].x : 0.0, re = P.n > 0 ? sm.s_rsum[e].y : 0.0;
-      double kk = sm.s_ksum[e];
-      for (int p = 0; p < kMaxProp; p++) {
-        if (!(mask & (1u << p))) continue;
-        const int pi = pair_index(p, e);
-        kk += sm.s_ksum[kMaxProp + pi];
-        const int j = sm.s_near[pi];
-        if (j >= 0) {
-          lj += sm.s_xsum[j].x;
-          re += sm.s_xsum[j].y;
-        }
-      }
-      // ForceField::EnergyDifference with its early exits (force_field.cc:407-434)
-      double tt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      double dE = 0;
-      bool early = false;
-      if (use_pair) {
-        tt[T_LJ] = lj;
-        dE += lj;
-        if (dE >= kVeryLargeEnergy) early = true;
-      }
-      if (!early && use_ext) {
-        tt[T_WALL] = P.forbid ? kVeryLargeEnergy : P.ew;
-        dE += tt[T_WALL];
-        if (dE >= kVeryLargeEnergy) early = true;
-      }
-      double trial_dipl = cur_dipl;
-      if (!early && use_ewald) {
-        tt[T_REAL] = re;
-        tt[T_KSPACE] = kk;
-        double dd = 0;
-        if (dipole) {
-          if (dipole_exact) {
-            const double mz1 = Mz + P.dmz;
-            trial_dipl = dip_coef * mz1 * mz1;
-            dd = trial_dipl - dip_coef * Mz * Mz;
-          } else {
-            // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
-            trial_dipl = dip_coef * Mz * Mz;
-            dd = trial_dipl - cur_dipl;
-          }
-        }
-        tt[T_DIPOLE] = dd;
-        dE += re + kk + dd;
-      }
-      const int len = sm.s_len[s];
-      if (!early && use_bond && len > 1) {
-        tt[T_BOND] = P.bond_trial - sm.s_bcur[s];
-        dE += tt[T_BOND];
-      }
-      if (train && !early && P.n > 0 && ring > 0) {
-        // training sample: the pair + k-space part the network estimates
-        const int slot = head % ring;
-        d.tr_target[slot] = lj + re + kk;
-        d.tr_n[slot] = P.n;
-        sm.s_train[e] = slot;
-        head++;
-      }
-      if (!sm.s_pass[s]) {
-        // evaluated for training only; the first stage rejected it
-        if (P.type >= 0 && P.type < 5) att[P.type] += 1;
-        valid++;
-        s1r++;
-        tev++;
-        continue;
-      }
-      if (da_on) s1p++;
-      // Metropolis (simulation.cc:327-332), no draw for a forbidden move; with
-      // the surrogate, the second stage min(1, exp(-beta (dE - dE_s)))
-      const int acc = (dE < kVeryLargeEnergy) && (sm.s_u[s] < exp(-beta * (dE - sm.s_des[s])));
-      for (int k = 0; k < 8; k++) last[k] = tt[k];
-      last_total = dE;
-      if (P.type >= 0 && P.type < 5) {
-        att[P.type] += 1;
-        if (acc) acc5[P.type] += 1;
-      }
-      valid++;
-      if (acc) {
-        mask |= 1u << e;
-        acc_screen |= 1ull << s;
-        Mz += P.dmz;
-        if (dipole) cur_dipl = trial_dipl;
-        for (int k = 0; k < 8; k++) run[k] += tt[k];
-        if (use_bond && len > 1) d.bond_cur[P.mol] = P.bond_trial;
-      }
-    }
-    C->Mz = Mz;
-    C->cur_dipl = cur_dipl;
-    C->trial_dipl = cur_dipl;
-    for (int k = 0; k < 8; k++) {
-      C->running[k] = run[k];
-      C->terms[k] = last[k];
-    }
-    C->total = last_total;
-    for (int k = 0; k < 5; k++) {
-      C->attempted[k] += att[k];
-      C->accepted[k] += acc5[k];
-    }
-    C->skipped += skipped;
-    C->valid_moves += valid;
-    C->s1_rejected += s1r;
-    C->s1_passed += s1p;
-    C->train_evals += tev;
-    C->stale_cuts += cuts;
-    C->move_counter = ctr0 + done;
-    if (rc.train) d.tr_ctl[0] = head;
-    plan->acc_mask = mask;
-    sm.s_mask = mask;
-    sm.s_ndone = done;
-  }
+  if (t < kWave) resolve_decide(g, d, rc, C, plan, np, ndec, ctr0, sm);
   __syncthreads();
+  // cell-list commits of the accepted slots, as many slots at a time as the
+  // item arrays hold
   const unsigned mask = sm.s_mask;
-  for (int e = 0; e < kMaxProp; e++) {
-    if (!(mask & (1u << e))) continue;
-    commit_cells(g, prop_view(g, d, rc, sm.s_screen[e], e), sm.s_nc, sm.s_flag);
+  int e0 = 0;
+  while (e0 < kMaxProp) {
+    int ns = 0, n = 0, e = e0;
+    for (; e < kMaxProp; e++) {
+      if (!(mask & (1u << e))) continue;
+      const int nm = sm.s_prop[sm.s_screen[e]].n;
+      if (ns > 0 && n + nm > kMaxChainDev) break;
+      if (t == 0) {
+        sm.s_goff[ns] = n;
+        sm.s_gscr[ns] = sm.s_screen[e];
+      }
+      ns++;
+      n += nm;
+    }
+    e0 = e;
+    if (ns == 0) break;
+    if (t == 0) sm.s_goff[ns] = n;
+    __syncthreads();
+    commit_cells(g, d, SlotItems{d.mv_new, d.smv_idx, rc.maxm, sm.s_goff, sm.s_gscr, ns, n}, sm.cm);
   }
   if (rc.train) {
     // descriptor rows of the training samples (old rows, then new rows)

@@ -345,13 +345,15 @@ __device__ void real_block(const Geo& g, const Dev& d, int b, int nreal, int nmm
   const int q2 = b / nsplit, part = b - q2 * nsplit;
   const int m = q2 >> 1, on = q2 & 1;
   const int nm = d.rec->n_moved;
-  if (m >= nm) {
-    if (threadIdx.x == 0) d.part_r[b] = make_double2(0, 0);
-    return;
+  // one store for both cases (a separate zero store costs every wave a
+  // register spill of the hoisted zero vector)
+  double2 e = make_double2(0, 0);
+  double sgn = 0;
+  if (m < nm) {
+    const double4 p = on ? d.mv_new[m] : d.mv_old[m];
+    e = real_query<P>(g, d, p, d.mv_type[m], d.rec->contig, d.rec->lo, d.rec->hi, part, nsplit, sm);
+    sgn = on ? 1.0 : -1.0;
   }
-  const double4 p = on ? d.mv_new[m] : d.mv_old[m];
-  const double2 e = real_query<P>(g, d, p, d.mv_type[m], d.rec->contig, d.rec->lo, d.rec->hi, part, nsplit, sm);
-  const double sgn = on ? 1.0 : -1.0;
   if (threadIdx.x == 0) d.part_r[b] = make_double2(sgn * e.x, sgn * e.y);
 }
 
@@ -441,98 +443,124 @@ __device__ void compact_cell(const Geo& g, const Dev& d, int cell) {
   d.ccount[cell] = w;
 }
 
-// Commit (accept) or roll back (reject) the pending move: positions, the
-// cell list (parallel, deterministic: a bead leaving its cell leaves a
-// tombstone; arrivals in one cell take consecutive slots in moved-list
-// order), dipole moment, spring cache, dipole lag state, running totals.
-// Runs inside one block.  S(k) is left to k_commit_sk / the next k-space
-// pass (sk_pending).
-__device__ void commit_cells(const Geo& g, const Dev& d, int* s_nc, int* s_flag) {
-  Ctl* C = d.ctl;
-  const int nm = d.rec->n_moved;
-  {
-    // new cell of every moved bead
-    for (int k = threadIdx.x; k < nm; k += kBlock) {
-      const double4 pn = d.mv_new[k];
-      const double xw = wrap_coord(pn.x, g.L[0], g.inv_L[0]);
-      const double yw = wrap_coord(pn.y, g.L[1], g.inv_L[1]);
-      const double zw = wrap_coord(pn.z, g.L[2], g.inv_L[2]);
-      const int nc = (cell_coord(xw, g.inv_h[0], g.nc[0]) * g.nc[1] + cell_coord(yw, g.inv_h[1], g.nc[1])) *
-                         g.nc[2] + cell_coord(zw, g.inv_h[2], g.nc[2]);
-      const int i = d.mv_idx[k];
-      s_nc[k] = nc;
-      s_flag[k] = (d.bcell[i] != nc) ? 1 : 0;
-    }
-    __syncthreads();
-    // a destination cell whose high-water mark would pass the capacity is
-    // compacted first (by the thread of its first arrival)
-    for (int k = threadIdx.x; k < nm; k += kBlock) {
-      if (!s_flag[k]) continue;
-      const int nc = s_nc[k];
-      int arrivals = 0;
-      bool first = true;
-      for (int k2 = 0; k2 < nm; k2++) {
-        if (!s_flag[k2] || s_nc[k2] != nc) continue;
-        if (k2 < k) first = false;
-        arrivals++;
-      }
-      if (first && d.ccount[nc] + arrivals > g.cap) compact_cell(g, d, nc);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < nm; k += kBlock) {
-      const int i = d.mv_idx[k];
-      const double4 pn = d.mv_new[k];
-      d.x[i] = pn.x; d.y[i] = pn.y; d.z[i] = pn.z;
-      const double xw = wrap_coord(pn.x, g.L[0], g.inv_L[0]);
-      const double yw = wrap_coord(pn.y, g.L[1], g.inv_L[1]);
-      const double zw = wrap_coord(pn.z, g.L[2], g.inv_L[2]);
-      const int nc = s_nc[k];
-      const int cx = nc / (g.nc[1] * g.nc[2]), cy = (nc / g.nc[2]) % g.nc[1], cz = nc % g.nc[2];
-      const float4 rel = make_float4((float)(xw - cx * g.h[0]), (float)(yw - cy * g.h[1]),
-                                     (float)(zw - cz * g.h[2]), (float)pn.w);
-      if (!s_flag[k]) {
-        const size_t si = (size_t)nc * g.cap + d.bslot[i];
-        d.cpos[si] = make_double4(xw, yw, zw, pn.w);
-        d.cpos32[si] = rel;
-        continue;
-      }
-      const int oc = d.bcell[i];
-      if (oc >= 0) d.cbead[(size_t)oc * g.cap + d.bslot[i]] = -1;
-      int rank = 0, last = 1;
-      for (int k2 = 0; k2 < nm; k2++) {
-        if (!s_flag[k2] || s_nc[k2] != nc) continue;
-        if (k2 < k) rank++;
-        if (k2 > k) last = 0;
-      }
-      const int base = d.ccount[nc];
-      const int slot = base + rank;
-      if (slot < g.cap) {
-        const size_t sn = (size_t)nc * g.cap + slot;
-        d.cbead[sn] = i;
-        d.cpos[sn] = make_double4(xw, yw, zw, pn.w);
-        d.cpos32[sn] = rel;
-        d.ctype[sn] = d.type[i];
-        d.bcell[i] = nc;
-        d.bslot[i] = slot;
-      } else {
-        C->overflow = 1;
-        d.bcell[i] = -1;
-        d.bslot[i] = -1;
-      }
-      s_flag[k] = last ? -(min(slot + 1, g.cap)) - 1 : 1;  // marks the count writer
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < nm; k += kBlock)
-      if (s_flag[k] < -1) d.ccount[s_nc[k]] = -s_flag[k] - 1;
+// The moved beads of one commit: item k's trial position and bead index.
+struct MoveItems {
+  const double4* mv_new;
+  const int* mv_idx;
+  int n;
+  __device__ __forceinline__ void get(int k, double4& p, int& i) const {
+    p = mv_new[k];
+    i = mv_idx[k];
   }
+};
+
+// Per-item scratch of a commit: new cell, key (the new cell of a bead that
+// changes cell, -1 otherwise; read four at a time) and arrival rank/count.
+struct CommitSmem {
+  alignas(16) int key[kMaxChainDev];
+  int nc[kMaxChainDev];
+  int rank[kMaxChainDev];
+};
+
+// Positions and cell list of accepted moved beads (parallel, deterministic:
+// a bead leaving its cell leaves a tombstone; arrivals in one cell take
+// consecutive slots in item order).  Runs inside one block.  The items may
+// span several accepted moves of distinct molecules: the beads left live in
+// every cell, and their order, are those of committing the moves one after
+// the other.
+template <class Items>
+__device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, CommitSmem& cs) {
+  Ctl* C = d.ctl;
+  const int nm = items.n;
+  const int n4 = (nm + 3) >> 2;
+  // new cell of every moved bead
+  for (int k = threadIdx.x; k < 4 * n4; k += kBlock) {
+    if (k >= nm) {
+      cs.key[k] = -1;
+      continue;
+    }
+    double4 pn;
+    int i;
+    items.get(k, pn, i);
+    const double xw = wrap_coord(pn.x, g.L[0], g.inv_L[0]);
+    const double yw = wrap_coord(pn.y, g.L[1], g.inv_L[1]);
+    const double zw = wrap_coord(pn.z, g.L[2], g.inv_L[2]);
+    const int nc = (cell_coord(xw, g.inv_h[0], g.nc[0]) * g.nc[1] + cell_coord(yw, g.inv_h[1], g.nc[1])) *
+                       g.nc[2] + cell_coord(zw, g.inv_h[2], g.nc[2]);
+    cs.nc[k] = nc;
+    cs.key[k] = (d.bcell[i] != nc) ? nc : -1;
+  }
+  __syncthreads();
+  // rank of each arrival among the arrivals in its cell (item order) and
+  // their count, in one pass over the keys; a destination cell whose
+  // high-water mark would pass the capacity is compacted first (by the
+  // thread of its first arrival)
+  const int4* k4 = reinterpret_cast<const int4*>(cs.key);
+  for (int k = threadIdx.x; k < nm; k += kBlock) {
+    const int key = cs.key[k];
+    if (key < 0) continue;
+    int before = 0, total = 0;
+#pragma unroll 4
+    for (int q = 0; q < n4; q++) {
+      const int4 v = k4[q];
+      const int b = 4 * q;
+      const int m0 = v.x == key, m1 = v.y == key, m2 = v.z == key, m3 = v.w == key;
+      total += m0 + m1 + m2 + m3;
+      before += (m0 & (b < k)) + (m1 & (b + 1 < k)) + (m2 & (b + 2 < k)) + (m3 & (b + 3 < k));
+    }
+    cs.rank[k] = before | (total << 16);
+    if (before == 0 && d.ccount[key] + total > g.cap) compact_cell(g, d, key);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nm; k += kBlock) {
+    double4 pn;
+    int i;
+    items.get(k, pn, i);
+    d.x[i] = pn.x; d.y[i] = pn.y; d.z[i] = pn.z;
+    const double xw = wrap_coord(pn.x, g.L[0], g.inv_L[0]);
+    const double yw = wrap_coord(pn.y, g.L[1], g.inv_L[1]);
+    const double zw = wrap_coord(pn.z, g.L[2], g.inv_L[2]);
+    const int nc = cs.nc[k];
+    const int cx = nc / (g.nc[1] * g.nc[2]), cy = (nc / g.nc[2]) % g.nc[1], cz = nc % g.nc[2];
+    const float4 rel = make_float4((float)(xw - cx * g.h[0]), (float)(yw - cy * g.h[1]),
+                                   (float)(zw - cz * g.h[2]), (float)pn.w);
+    if (cs.key[k] < 0) {
+      const size_t si = (size_t)nc * g.cap + d.bslot[i];
+      d.cpos[si] = make_double4(xw, yw, zw, pn.w);
+      d.cpos32[si] = rel;
+      continue;
+    }
+    const int oc = d.bcell[i];
+    if (oc >= 0) d.cbead[(size_t)oc * g.cap + d.bslot[i]] = -1;
+    const int rank = cs.rank[k] & 0xffff, total = cs.rank[k] >> 16;
+    const int slot = d.ccount[nc] + rank;
+    if (slot < g.cap) {
+      const size_t sn = (size_t)nc * g.cap + slot;
+      d.cbead[sn] = i;
+      d.cpos[sn] = make_double4(xw, yw, zw, pn.w);
+      d.cpos32[sn] = rel;
+      d.ctype[sn] = d.type[i];
+      d.bcell[i] = nc;
+      d.bslot[i] = slot;
+    } else {
+      C->overflow = 1;
+      d.bcell[i] = -1;
+      d.bslot[i] = -1;
+    }
+    // the last arrival writes the cell's new count
+    cs.rank[k] = rank == total - 1 ? min(slot + 1, g.cap) : -1;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nm; k += kBlock)
+    if (cs.key[k] >= 0 && cs.rank[k] >= 0) d.ccount[cs.nc[k]] = cs.rank[k];
   __syncthreads();
 }
 
-__device__ void commit_block(const Geo& g, const Dev& d, int acc, int* s_nc, int* s_flag) {
+__device__ void commit_block(const Geo& g, const Dev& d, int acc, CommitSmem& cs) {
   Ctl* C = d.ctl;
   const int nm = d.rec->n_moved;
   const int mol = d.rec->mol;
-  if (acc) commit_cells(g, d, s_nc, s_flag);
+  if (acc) commit_cells(g, d, MoveItems{d.mv_new, d.mv_idx, nm}, cs);
   if (threadIdx.x == 0) {
     if (acc) {
       C->Mz += C->dMz;
@@ -659,8 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_epilogue(Geo g, Dev d, int nkb, int 
 
 // Host path: commit / roll back after the caller's decision.
 __global__ __launch_bounds__(kBlock) void k_commit(Geo g, Dev d, int accept) {
-  __shared__ int s_nc[kMaxChainDev], s_flag[kMaxChainDev];
-  commit_block(g, d, accept && d.rec->valid, s_nc, s_flag);
+  __shared__ CommitSmem cs;
+  commit_block(g, d, accept && d.rec->valid, cs);
 }
 
 // Removes tombstones cell by cell, keeping slot order.
