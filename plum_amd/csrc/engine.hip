@@ -424,7 +424,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
           kl.push_back(make_int4(lx, ly, lz, 0));
           ck.push_back(pref0 * (exp(-k2 / (4 * p.alpha)) / k2));
         }
-    g.tab_w = (g.nkx + 1) + (2 * g.nky + 1) + (2 * g.nkz + 1);
+    g.tab_w = (g.nkx + 1) + (g.nky + 1) + (g.nkz + 1);
   }
   g.n_kh = (int)kl.size();
   info.n_k_half = g.n_kh;

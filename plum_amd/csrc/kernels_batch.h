@@ -173,7 +173,9 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   const Dev v = prop_view(g, d, rc, p, 0);
   propose_block(g, v, mc, ctr0 + p, sel, sm.pr);
   __syncthreads();
-  if (g.use_ewald && g.n_kh > 0) tables_block(g, v);
+  // the tables reuse the generator's bond array as their LDS stage
+  if (g.use_ewald && g.n_kh > 0)
+    tables_block(g, v, reinterpret_cast<double2*>(&sm.pr.s_b[0][0]), (int)(sizeof(sm.pr.s_b) / sizeof(double2)));
   // spring energy of the trial molecule, wall and dipole parts, bounding
   // sphere: everything of the dE that involves this proposal alone
   const int nm = v.rec->n_moved;
@@ -413,13 +415,8 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
                                   unsigned acc_mask, int blk, KSpecSmem& sm) {
   const int kk = threadIdx.x % kKSpec, grp = threadIdx.x / kKSpec;
   const int k = blk * kKSpec + kk;
-  int ox = 0, oy = 0, oz = 0;
-  if (k < g.n_kh) {
-    const int4 l = d.kl[k];
-    ox = l.x;
-    oy = g.nkx + 1 + l.y + g.nky;
-    oz = g.nkx + 1 + 2 * g.nky + 1 + l.z + g.nkz;
-  }
+  TabIdx ti = {0, 0, 0, false, false};
+  if (k < g.n_kh) ti = tab_index(g, d.kl[k]);
   // the previous batch's accepted dS rows join S(k) before this batch's
   // rows overwrite them (the chunk loop's barrier orders the two)
   if (grp == 0) {
@@ -456,8 +453,8 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
             const double qm = mo[m].w;
             const double2* to = base + (size_t)(2 * m) * g.tab_w;
             const double2* tn = to + g.tab_w;
-            const double2 eo = cmul(cmul(to[ox], to[oy]), to[oz]);
-            const double2 en = cmul(cmul(tn[ox], tn[oy]), tn[oz]);
+            const double2 eo = tab_phase(to, ti);
+            const double2 en = tab_phase(tn, ti);
             acc.x += qm * (en.x - eo.x);
             acc.y += qm * (en.y - eo.y);
           }

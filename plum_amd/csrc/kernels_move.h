@@ -25,9 +25,10 @@ __global__ void k_host_moves(Geo g, Dev d) {
   }
 }
 
-// Phase tables e^{i k_l x} for l = 0..nx (x), -ny..ny (y), -nz..nz (z) of every
-// moved bead at its old and new position, with k_l = (l*2) kPi / L exactly as
-// potential_ewald_coul.cc:89-96 forms it (FP64 sincos).
+// Phase tables e^{i k_l x} for l = 0..n_d of each dimension d of every moved
+// bead at its old and new position, with k_l = (l*2) kPi / L exactly as
+// potential_ewald_coul.cc:89-96 forms it (FP64 sincos).  Negative l is read
+// as the conjugate (tab_phase).
 __global__ void k_tables(Geo g, Dev d) {
   const int nm = d.rec->n_moved;
   const int total = nm * 2 * g.tab_w;
@@ -36,8 +37,8 @@ __global__ void k_tables(Geo g, Dev d) {
     const int m = row >> 1, on = row & 1;
     int dim, l;
     if (off <= g.nkx) { dim = 0; l = off; }
-    else if (off <= g.nkx + 2 * g.nky + 1) { dim = 1; l = off - (g.nkx + 1) - g.nky; }
-    else { dim = 2; l = off - (g.nkx + 1) - (2 * g.nky + 1) - g.nkz; }
+    else if (off <= g.nkx + 1 + g.nky) { dim = 1; l = off - (g.nkx + 1); }
+    else { dim = 2; l = off - (g.nkx + 1) - (g.nky + 1); }
     const double4 p = on ? d.mv_new[m] : d.mv_old[m];
     const double xv = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
     const double xw = wrap_coord(xv, g.L[dim], g.inv_L[dim]);
@@ -46,6 +47,31 @@ __global__ void k_tables(Geo g, Dev d) {
     sincos(kd * xw, &s, &c);
     d.tab[(size_t)row * g.tab_w + off] = make_double2(c, s);
   }
+}
+
+// Table offsets of k vector l (x, y, z segments of l >= 0) and whether its y
+// and z phases are conjugated (negative l).
+struct TabIdx {
+  int ox, oy, oz;
+  bool ny, nz;
+};
+
+__device__ __forceinline__ TabIdx tab_index(const Geo& g, int4 l) {
+  TabIdx t;
+  t.ox = l.x;
+  t.oy = g.nkx + 1 + abs(l.y);
+  t.oz = g.nkx + 1 + g.nky + 1 + abs(l.z);
+  t.ny = l.y < 0;
+  t.nz = l.z < 0;
+  return t;
+}
+
+// e^{i k.r} of one table row: e^{-i l theta} = conj(e^{i l theta}).
+__device__ __forceinline__ double2 tab_phase(const double2* t, const TabIdx& i) {
+  double2 y = t[i.oy], z = t[i.oz];
+  if (i.ny) y.y = -y.y;
+  if (i.nz) z.y = -z.y;
+  return cmul(cmul(t[i.ox], y), z);
 }
 
 // ------------------------------------------------------------ k space
@@ -67,15 +93,14 @@ __device__ void kspace_block(const Geo& g, const Dev& d, int blk, KSmem& sm) {
   const int nm = d.rec->n_moved;
   double2 dS = make_double2(0, 0);
   if (k < g.n_kh) {
-    const int4 l = d.kl[k];
-    const int ox = l.x, oy = g.nkx + 1 + l.y + g.nky, oz = g.nkx + 1 + 2 * g.nky + 1 + l.z + g.nkz;
+    const TabIdx ti = tab_index(g, d.kl[k]);
 #pragma unroll 2
     for (int m = grp; m < nm; m += kKGroups) {
       const double qm = d.mv_old[m].w;
       const double2* to = d.tab + (size_t)(2 * m) * g.tab_w;
       const double2* tn = to + g.tab_w;
-      const double2 eo = cmul(cmul(to[ox], to[oy]), to[oz]);
-      const double2 en = cmul(cmul(tn[ox], tn[oy]), tn[oz]);
+      const double2 eo = tab_phase(to, ti);
+      const double2 en = tab_phase(tn, ti);
       dS.x += qm * (en.x - eo.x);
       dS.y += qm * (en.y - eo.y);
     }
@@ -760,7 +785,7 @@ __device__ MoveSel select_move(const Dev& d, const McDev& mc, unsigned long long
 // the view of one proposal slot (its rec / mv_* arrays); nothing shared per
 // bead is written, so two proposals of one molecule can coexist in a batch.
 struct PrSmem {
-  double s_b[kMaxChainDev][3];
+  alignas(16) double s_b[kMaxChainDev][3];
   int s_mol, s_type, s_valid, s_a, s_len;
   double s_par[4];
   int s_ipar[3];
@@ -850,19 +875,51 @@ __device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, unsig
       s_b[i][2] = nrm * bz;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // the two halves outward from the pivot, one lane each, in LDS (each
+    // bond vector is replaced by the position it leads to), then written
+    // out in parallel
+    if (threadIdx.x < 2) {
+      const int dir = threadIdx.x == 0 ? 1 : -1;
       double px = d.x[a + pivot], py = d.y[a + pivot], pz = d.z[a + pivot];
-      d.mv_new[pivot] = make_double4(px, py, pz, d.q[a + pivot]);
-      for (int i = pivot + 1; i < len; i++) {
-        px += s_b[i][0]; py += s_b[i][1]; pz += s_b[i][2];
-        d.mv_new[i] = make_double4(px, py, pz, d.q[a + i]);
+      if (threadIdx.x == 0) {
+        s_b[pivot][0] = px;
+        s_b[pivot][1] = py;
+        s_b[pivot][2] = pz;
       }
-      px = d.x[a + pivot]; py = d.y[a + pivot]; pz = d.z[a + pivot];
-      for (int i = pivot - 1; i >= 0; i--) {
+      // eight bonds are read together, then summed in order: one LDS
+      // round trip per eight beads instead of one per bead, and no branch
+      // inside a group
+      const int cnt = dir > 0 ? len - 1 - pivot : pivot;
+      int j = 0;
+      for (; j + 8 <= cnt; j += 8) {
+        double b[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = pivot + dir * (j + u + 1);
+          b[u][0] = s_b[i][0];
+          b[u][1] = s_b[i][1];
+          b[u][2] = s_b[i][2];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = pivot + dir * (j + u + 1);
+          px += b[u][0]; py += b[u][1]; pz += b[u][2];
+          s_b[i][0] = px;
+          s_b[i][1] = py;
+          s_b[i][2] = pz;
+        }
+      }
+      for (; j < cnt; j++) {
+        const int i = pivot + dir * (j + 1);
         px += s_b[i][0]; py += s_b[i][1]; pz += s_b[i][2];
-        d.mv_new[i] = make_double4(px, py, pz, d.q[a + i]);
+        s_b[i][0] = px;
+        s_b[i][1] = py;
+        s_b[i][2] = pz;
       }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += kBlock)
+      d.mv_new[i] = make_double4(s_b[i][0], s_b[i][1], s_b[i][2], d.q[a + i]);
     nmv = len;
   } else if (mt == 3) {
     // Molecule::Crankshaft: rotate beads first+1..last-1 about first->last
@@ -941,32 +998,81 @@ __device__ void propose_block(const Geo& g, const Dev& d, const McDev& mc, unsig
 
 namespace plum {
 
-// Phase tables of the pending move by recurrence, inside one block:
-// e^{i l theta} = e^{i (l-1) theta} e^{i theta}, l = 0..n_d, negative l by
-// conjugation; theta_d = (2 kPi / L_d) x_d on the wrapped coordinate.
-__device__ void tables_block(const Geo& g, const Dev& d) {
+// One dimension's segment of a phase-table row (t = row start) by recurrence:
+// e^{i l theta} = e^{i (l-1) theta} e^{i theta}, l = 0..n_d, theta_d =
+// (2 kPi / L_d) x_d on the wrapped coordinate.
+__device__ __forceinline__ void tables_row(const Geo& g, int dim, double xv, double2* t) {
+  const double xw = wrap_coord(xv, g.L[dim], g.inv_L[dim]);
+  const double k1 = (double)2 * kPi / g.L[dim];
+  double sn, cs;
+  sincos(k1 * xw, &sn, &cs);
+  const double2 base = make_double2(cs, sn);
+  const int n = dim == 0 ? g.nkx : (dim == 1 ? g.nky : g.nkz);
+  t += dim == 0 ? 0 : (dim == 1 ? g.nkx + 1 : g.nkx + 1 + g.nky + 1);
+  double2 e = make_double2(1.0, 0.0);
+  t[0] = e;
+  for (int l = 1; l <= n; l++) {
+    e = cmul(e, base);
+    t[l] = e;
+  }
+}
+
+// Phase tables of the pending move inside one block.  One thread per (row,
+// dimension) runs tables_row into an LDS stage (rows padded to an odd number
+// of entries), a chunk of rows at a time, and the block copies the chunk out
+// with consecutive threads on consecutive entries (a thread writing its own
+// row straight to memory scatters every store over 64 rows).
+__device__ void tables_block(const Geo& g, const Dev& d, double2* stage, int cap) {
   const int nm = d.rec->n_moved;
-  const int nseq = nm * 2 * 3;
-  for (int sidx = threadIdx.x; sidx < nseq; sidx += kBlock) {
-    const int row = sidx / 3, dim = sidx - row * 3;
-    const int m = row >> 1, on = row & 1;
-    const double4 p = on ? d.mv_new[m] : d.mv_old[m];
-    const double xv = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
-    const double xw = wrap_coord(xv, g.L[dim], g.inv_L[dim]);
-    const double k1 = (double)2 * kPi / g.L[dim];
-    double sn, cs;
-    sincos(k1 * xw, &sn, &cs);
-    const double2 base = make_double2(cs, sn);
-    double2* t = d.tab + (size_t)row * g.tab_w;
-    const int n = dim == 0 ? g.nkx : (dim == 1 ? g.nky : g.nkz);
-    const int off = dim == 0 ? 0 : (dim == 1 ? g.nkx + 1 + g.nky : g.nkx + 1 + 2 * g.nky + 1 + g.nkz);
-    double2 e = make_double2(1.0, 0.0);
-    t[off] = e;
-    for (int l = 1; l <= n; l++) {
-      e = cmul(e, base);
-      t[off + l] = e;
-      if (dim != 0) t[off - l] = conj2(e);
+  const int nrow = 2 * nm, w = g.tab_w, ws = w | 1;
+  if (ws > cap || cap / ws < 1) {
+    // rows too wide for the stage: straight to memory
+    for (int sidx = threadIdx.x; sidx < nrow * 3; sidx += kBlock) {
+      const int row = sidx / 3, dim = sidx - row * 3;
+      const int m = row >> 1, on = row & 1;
+      const double4 p = on ? d.mv_new[m] : d.mv_old[m];
+      tables_row(g, dim, dim == 0 ? p.x : (dim == 1 ? p.y : p.z), d.tab + (size_t)row * w);
     }
+    return;
+  }
+  // a chunk is at most one (row, dimension) per thread; the coordinate of
+  // the next chunk is loaded while this one runs
+  const int cr = min(cap / ws, kBlock / 3);
+  auto coord = [&](int r0, int nr) {
+    double x = 0;
+    if (threadIdx.x < nr * 3) {
+      const int lr = threadIdx.x / 3, dim = threadIdx.x - lr * 3;
+      const int row = r0 + lr;
+      const double4 p = (row & 1) ? d.mv_new[row >> 1] : d.mv_old[row >> 1];
+      x = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
+    }
+    return x;
+  };
+  double xn = coord(0, min(cr, nrow));
+  for (int r0 = 0; r0 < nrow; r0 += cr) {
+    const int nr = min(cr, nrow - r0);
+    const double x = xn;
+    if (r0 + cr < nrow) xn = coord(r0 + cr, min(cr, nrow - r0 - cr));
+    if (threadIdx.x < nr * 3) {
+      const int lr = threadIdx.x / 3, dim = threadIdx.x - lr * 3;
+      tables_row(g, dim, x, stage + (size_t)lr * ws);
+    }
+    __syncthreads();
+    // row and column of entry i advance by kBlock = q w + c per step
+    double2* dst = d.tab + (size_t)r0 * w;
+    const int q = kBlock / w, c = kBlock - q * w;
+    int r = threadIdx.x / w, col = threadIdx.x - r * w;
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nr * w; i += kBlock) {
+      dst[i] = stage[r * ws + col];
+      r += q;
+      col += c;
+      if (col >= w) {
+        col -= w;
+        r++;
+      }
+    }
+    __syncthreads();
   }
 }
 
