@@ -36,6 +36,7 @@ struct plum_batch {
   // slice of 2 kMaxScreen maxm rows per system (R == 1: the surrogate's own)
   float *sx = nullptr, *sf = nullptr;
   int sx_rows = 0, sx_cap = 0, maxm_b = 0;
+  int tab_chunks = 1;     // phase-table chunks per proposal (plan kernel rows y > 0)
   bool profiling = false;
   std::vector<void*> allocs;
 };
@@ -198,7 +199,9 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     (void)hipEventRecord(ev[2], b->st);
   }
   if (ev) (void)hipEventRecord(ev[3], b->st);
-  hipLaunchKernelGGL(k_plan_spec, dim3(R), dim3(kPlanThreads), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, b->d_vb, R);
+  // y = 0: the plan of system x; y > 0: chunk y - 1 of its phase tables
+  hipLaunchKernelGGL(k_plan_spec, dim3(R, 1 + b->tune.pmax * b->tab_chunks), dim3(kPlanThreads), 0, b->st, b->dG,
+                     b->dD, b->dMC, b->dRC, b->d_vb, R, b->tab_chunks);
   if (ev) (void)hipEventRecord(ev[4], b->st);
   if (b->precision == PLUM_FP32_PAIR)
     hipLaunchKernelGGL(k_compute_spec<1>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
@@ -360,6 +363,10 @@ int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   std::vector<int> off(n + 1, 0);
   for (int r = 0; r < n; r++) off[r + 1] = off[r] + replicas[r]->g.ncell;
   b->total_cells = off[n];
+  for (int r = 0; r < n; r++) {
+    const int cr = tab_chunk_rows(replicas[r]->g.tab_w);
+    b->tab_chunks = std::max(b->tab_chunks, (2 * replicas[r]->maxm + cr - 1) / cr);
+  }
   if (hipMemcpy(b->d_cell_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(PLUM_E_HIP, "upload of the cell offsets failed"));
   for (plum_ctx* c : b->reps)

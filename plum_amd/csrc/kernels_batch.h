@@ -173,9 +173,6 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   const Dev v = prop_view(g, d, rc, p, 0);
   propose_block(g, v, mc, ctr0 + p, sel, sm.pr);
   __syncthreads();
-  // the tables reuse the generator's bond array as their LDS stage
-  if (g.use_ewald && g.n_kh > 0)
-    tables_block(g, v, reinterpret_cast<double2*>(&sm.pr.s_b[0][0]), (int)(sizeof(sm.pr.s_b) / sizeof(double2)));
   // spring energy of the trial molecule, wall and dipole parts, bounding
   // sphere: everything of the dE that involves this proposal alone
   const int nm = v.rec->n_moved;
@@ -264,12 +261,45 @@ struct PlanSmem {
   int s_ne, s_off;
 };
 
+// Phase tables of the proposals, in chunks of rows beside the plan (the plan
+// is one block per system; the chunks fill the rest of the chip meanwhile).
+constexpr int kTabStage = 3072;   // LDS stage of a table chunk, entries
+
+__host__ __device__ __forceinline__ int tab_chunk_rows(int tab_w) {
+  const int ws = tab_w | 1;
+  return ws <= kTabStage ? min(kTabStage / ws, kPlanThreads / 3) : kPlanThreads / 3;
+}
+
+__device__ void plan_tables(const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
+                            int r, int c, int nch, double2* stage) {
+  const Geo& g = G[r];
+  const Dev& d = D[r];
+  const RepCfg& rc = RC[r];
+  if (!g.use_ewald || g.n_kh == 0) return;
+  const int p = c / nch, ch = c - p * nch;
+  if (p >= d.plan->n_prop) return;
+  const Dev v = prop_view(g, d, rc, p, 0);
+  const int nrow = 2 * v.rec->n_moved, cr = tab_chunk_rows(g.tab_w);
+  const int r0 = ch * cr;
+  if (r0 >= nrow) return;
+  const int ws = (g.tab_w | 1) <= kTabStage ? (g.tab_w | 1) : 0;
+  tables_chunk(g, v, r0, min(cr, nrow - r0), stage, ws);
+}
+
 __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                             const McDev* __restrict__ MC,
-                            const RepCfg* __restrict__ RC, int* vb_count, int R) {
-  __shared__ PlanSmem sm;
+                            const RepCfg* __restrict__ RC, int* vb_count, int R, int nch) {
+  __shared__ union {
+    PlanSmem p;
+    double2 stage[kTabStage];
+  } smu;
+  PlanSmem& sm = smu.p;
   const int r = blockIdx.x;
   if (r >= R) return;
+  if (blockIdx.y > 0) {
+    plan_tables(G, D, RC, r, blockIdx.y - 1, nch, smu.stage);
+    return;
+  }
   // by-value copies through the constant address space: the per-system
   // structs stay in scalar registers instead of being re-read after stores
   const Geo g = load_const(G, r);

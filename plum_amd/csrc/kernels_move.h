@@ -1017,62 +1017,38 @@ __device__ __forceinline__ void tables_row(const Geo& g, int dim, double xv, dou
   }
 }
 
-// Phase tables of the pending move inside one block.  One thread per (row,
-// dimension) runs tables_row into an LDS stage (rows padded to an odd number
-// of entries), a chunk of rows at a time, and the block copies the chunk out
-// with consecutive threads on consecutive entries (a thread writing its own
-// row straight to memory scatters every store over 64 rows).
-__device__ void tables_block(const Geo& g, const Dev& d, double2* stage, int cap) {
-  const int nm = d.rec->n_moved;
-  const int nrow = 2 * nm, w = g.tab_w, ws = w | 1;
-  if (ws > cap || cap / ws < 1) {
-    // rows too wide for the stage: straight to memory
-    for (int sidx = threadIdx.x; sidx < nrow * 3; sidx += kBlock) {
-      const int row = sidx / 3, dim = sidx - row * 3;
-      const int m = row >> 1, on = row & 1;
-      const double4 p = on ? d.mv_new[m] : d.mv_old[m];
-      tables_row(g, dim, dim == 0 ? p.x : (dim == 1 ? p.y : p.z), d.tab + (size_t)row * w);
-    }
-    return;
+// Rows [r0, r0 + nr) of the phase tables of the pending move (row 2m + on:
+// bead m, old/new), one block.  One thread per (row, dimension) runs
+// tables_row into an LDS stage of rows padded to an odd number of entries
+// (ws), and the block copies the rows out with consecutive threads on
+// consecutive entries (a thread writing its own row straight to memory
+// scatters every store over 64 rows); with no stage (ws = 0) the rows go
+// straight to memory.  nr * 3 <= blockDim.x.
+__device__ void tables_chunk(const Geo& g, const Dev& d, int r0, int nr, double2* stage, int ws) {
+  const int w = g.tab_w;
+  const int t = threadIdx.x;
+  if (t < nr * 3) {
+    const int lr = t / 3, dim = t - lr * 3;
+    const int row = r0 + lr;
+    const double4 p = (row & 1) ? d.mv_new[row >> 1] : d.mv_old[row >> 1];
+    tables_row(g, dim, dim == 0 ? p.x : (dim == 1 ? p.y : p.z),
+               ws ? stage + (size_t)lr * ws : d.tab + (size_t)row * w);
   }
-  // a chunk is at most one (row, dimension) per thread; the coordinate of
-  // the next chunk is loaded while this one runs
-  const int cr = min(cap / ws, kBlock / 3);
-  auto coord = [&](int r0, int nr) {
-    double x = 0;
-    if (threadIdx.x < nr * 3) {
-      const int lr = threadIdx.x / 3, dim = threadIdx.x - lr * 3;
-      const int row = r0 + lr;
-      const double4 p = (row & 1) ? d.mv_new[row >> 1] : d.mv_old[row >> 1];
-      x = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
-    }
-    return x;
-  };
-  double xn = coord(0, min(cr, nrow));
-  for (int r0 = 0; r0 < nrow; r0 += cr) {
-    const int nr = min(cr, nrow - r0);
-    const double x = xn;
-    if (r0 + cr < nrow) xn = coord(r0 + cr, min(cr, nrow - r0 - cr));
-    if (threadIdx.x < nr * 3) {
-      const int lr = threadIdx.x / 3, dim = threadIdx.x - lr * 3;
-      tables_row(g, dim, x, stage + (size_t)lr * ws);
-    }
-    __syncthreads();
-    // row and column of entry i advance by kBlock = q w + c per step
-    double2* dst = d.tab + (size_t)r0 * w;
-    const int q = kBlock / w, c = kBlock - q * w;
-    int r = threadIdx.x / w, col = threadIdx.x - r * w;
+  if (!ws) return;
+  __syncthreads();
+  // row and column of entry i advance by blockDim.x = q w + c per step
+  double2* dst = d.tab + (size_t)r0 * w;
+  const int q = blockDim.x / w, c = blockDim.x - q * w;
+  int r = t / w, col = t - r * w;
 #pragma unroll 4
-    for (int i = threadIdx.x; i < nr * w; i += kBlock) {
-      dst[i] = stage[r * ws + col];
-      r += q;
-      col += c;
-      if (col >= w) {
-        col -= w;
-        r++;
-      }
+  for (int i = t; i < nr * w; i += blockDim.x) {
+    dst[i] = stage[r * ws + col];
+    r += q;
+    col += c;
+    if (col >= w) {
+      col -= w;
+      r++;
     }
-    __syncthreads();
   }
 }
 
