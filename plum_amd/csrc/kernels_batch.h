@@ -258,8 +258,10 @@ struct PlanSmem {
   int s_len[kMaxScreen];
   int s_screen[kMaxProp];
   int s_xblk[kMaxPairs];   // cross blocks of a slot pair within range, -1 if not
+  int s_pq[kMaxPairs];     // slots (p, q) of pair j: p | q << 8
   int s_ne, s_off;
 };
+static_assert(kMaxPairs <= 2 * kWave, "near-pair compaction: two pairs per lane of one wave");
 
 // Phase tables of the proposals, in chunks of rows beside the plan (the plan
 // is one block per system; the chunks fill the rest of the chip meanwhile).
@@ -400,27 +402,46 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
         center_dist(pp.c, pq.c, g) <= pp.rad + pq.rad + g.rs + 1e-9)
       xb = cross_blocks(pp.n, pq.n);
     sm.s_xblk[j] = xb;
+    sm.s_pq[j] = e0 | (e << 8);
   }
   __syncthreads();
-  if (t != 0) return;
-  // near pairs in (e, e0) order, e0 < e
-  int nn = 0, xo = 0;
-  const int off = sm.s_off;
-  plan->pair_off[0] = 0;
-  for (int j = 0; j < npair; j++) {
-    if (sm.s_xblk[j] < 0) continue;
-    int e = (int)((1.0 + sqrt(1.0 + 8.0 * j)) * 0.5);
-    while (e * (e - 1) / 2 > j) e--;
-    while ((e + 1) * e / 2 <= j) e++;
-    plan->pair_p[nn] = j - e * (e - 1) / 2;
-    plan->pair_q[nn] = e;
-    xo += sm.s_xblk[j];
+  if (t >= kWave) return;
+  // near pairs in (e, e0) order, e0 < e, compacted by one wave: lane l holds
+  // pairs 2l and 2l + 1; a prefix scan gives each its place and block offset
+  const int j0 = 2 * t, j1 = j0 + 1;
+  const int x0 = j0 < npair ? sm.s_xblk[j0] : -1, x1 = j1 < npair ? sm.s_xblk[j1] : -1;
+  const int c0 = x0 >= 0, c1 = x1 >= 0;
+  const int own_c = c0 + c1, own_b = (c0 ? x0 : 0) + (c1 ? x1 : 0);
+  int cs = own_c, bs = own_b;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int a = __shfl_up(cs, o, kWave), b = __shfl_up(bs, o, kWave);
+    if (t >= o) {
+      cs += a;
+      bs += b;
+    }
+  }
+  int nn = cs - own_c, xo = bs - own_b;
+  if (t == 0) plan->pair_off[0] = 0;
+  if (c0) {
+    plan->pair_p[nn] = sm.s_pq[j0] & 0xff;
+    plan->pair_q[nn] = sm.s_pq[j0] >> 8;
+    xo += x0;
     nn++;
     plan->pair_off[nn] = xo;
   }
-  plan->n_near = nn;
-  // k-space blocks also fold in the pending S(k) of the last batch
-  vb_count[r] = (np > 0 && (ne > 0 || plan->acc_mask)) ? rc.nkb_s + off + xo : 0;
+  if (c1) {
+    plan->pair_p[nn] = sm.s_pq[j1] & 0xff;
+    plan->pair_q[nn] = sm.s_pq[j1] >> 8;
+    xo += x1;
+    nn++;
+    plan->pair_off[nn] = xo;
+  }
+  if (t == kWave - 1) {
+    plan->n_near = cs;
+    // k-space blocks also fold in the pending S(k) of the last batch
+    vb_count[r] = (np > 0 && (ne > 0 || plan->acc_mask)) ? rc.nkb_s + sm.s_off + bs : 0;
+  }
 }
 
 // ------------------------------------------------------------ k space
