@@ -188,13 +188,16 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     // surrogate: descriptors of every moved bead (old, new) of every system,
     // then the network on all of them in one launch
     plum_ctx* c = b->reps[0];
-    hipLaunchKernelGGL(k_descr_spec, dim3(2 * b->maxm_b, b->tune.pmax, R), dim3(kBlock), 0, b->st, b->dG, b->dD,
-                       b->dRC);
+    hipLaunchKernelGGL(k_descr_spec, dim3((2 * b->maxm_b + kDescrWaves - 1) / kDescrWaves, b->tune.pmax, R),
+                       dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC);
     if (ev) (void)hipEventRecord(ev[2], b->st);
-    // R == 1: the rows of the proposals in use; R > 1: every slice
-    const int rows = R == 1 ? b->tune.pmax * 2 * c->maxm : R * b->sx_rows;
+    // the rows of the proposals in use: the first 2 pmax maxm rows of each
+    // system's slice (R == 1: the surrogate's own rows, maxm of that system)
+    const int per = 2 * b->tune.pmax * (R == 1 ? c->maxm : b->maxm_b);
+    const int stride = R == 1 ? per : b->sx_rows;
+    const int rows = R * per;
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(b->n_cu, (rows + 127) / 128))), dim3(kBlock), 0, b->st,
-                       c->sur->net, b->sx, rows, b->sf);
+                       c->sur->net, b->sx, rows, b->sf, per, stride);
   } else if (ev) {
     (void)hipEventRecord(ev[2], b->st);
   }

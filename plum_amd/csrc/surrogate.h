@@ -35,18 +35,29 @@ __host__ __device__ __forceinline__ float rbf_gamma(float r_d) {
 }
 
 // ------------------------------------------------------------ descriptors
-struct DescrSmem {
-  int s_cell[kMaxQueryCells];
-  int s_img[kMaxQueryCells];
-  int s_end[kMaxQueryCells];
-  float s_rel[kMaxQueryCells][3];
-  float red[kBlock / kWave][kRbf * kChan];
+// Per-wave scratch of descr_wave: one chunk of up to 64 query cells.
+struct DescrWaveSmem {
+  int s_cell[kWave];
+  int s_img[kWave];
+  int s_end[kWave];
+  float s_rel[kWave][3];
 };
+constexpr int kDescrWaves = kBlock / kWave;
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Descriptor of moved bead m of proposal view v at its old (on = 0) or new
 // (on = 1) position, against every unmoved bead image within r_d (images as
-// the LJ term takes them: z unshifted with two periodic dimensions).
-__device__ void descr_row(const Geo& g, const Dev& v, float r_d, int m, int on, float* out, DescrSmem& sm) {
+// the LJ term takes them: z unshifted with two periodic dimensions).  One
+// wave per row, no block barriers: the wave scans the query cells 64 at a
+// time, each lane sums its candidates, and a butterfly over the 64 lanes
+// (32 + 16 + ... + 1 exchanges, fixed order) leaves feature f in lane f.
+__device__ void descr_wave(const Geo& g, const Dev& v, float r_d, int m, int on, float* out, DescrWaveSmem& sm) {
+  const int lane = threadIdx.x % kWave;
   const double4 p = on ? v.mv_new[m] : v.mv_old[m];
   const int contig = v.rec->contig, mlo = v.rec->lo, mhi = v.rec->hi;
   const double pw[3] = {wrap_coord(p.x, g.L[0], g.inv_L[0]), wrap_coord(p.y, g.L[1], g.inv_L[1]),
@@ -61,15 +72,14 @@ __device__ void descr_row(const Geo& g, const Dev& v, float r_d, int m, int on, 
   const float qi = (float)p.w;
   const float rd2 = r_d * r_d, gam = rbf_gamma(r_d), dmu = (r_d - kRbfMin) / (kRbf - 1);
   const float pi_rd = 3.14159265358979f / r_d;
-  float acc[kChan][kRbf];
+  float acc[kWave];
 #pragma unroll
-  for (int c = 0; c < kChan; c++)
-#pragma unroll
-    for (int k = 0; k < kRbf; k++) acc[c][k] = 0.f;
-  for (int base = 0; base < ncq; base += kMaxQueryCells) {
-    const int nloc = min(kMaxQueryCells, ncq - base);
-    for (int t = threadIdx.x; t < nloc; t += kBlock) {
-      const int c = base + t;
+  for (int f = 0; f < kWave; f++) acc[f] = 0.f;
+  for (int base = 0; base < ncq; base += kWave) {
+    const int nloc = min(kWave, ncq - base);
+    int end = 0;
+    if (lane < nloc) {
+      const int c = base + lane;
       const int cz = c % nd[2], cy = (c / nd[2]) % nd[1], cx = c / (nd[2] * nd[1]);
       const int u[3] = {lo[0] + cx, lo[1] + cy, lo[2] + cz};
       int cell = 0, img = 0;
@@ -79,16 +89,21 @@ __device__ void descr_row(const Geo& g, const Dev& v, float r_d, int m, int on, 
         const int w = u[k] - q * g.nc[k];
         cell = cell * g.nc[k] + w;
         img |= ((q + 128) & 0xff) << (8 * k);
-        sm.s_rel[t][k] = (float)(pw[k] - (double)u[k] * g.h[k]);
+        sm.s_rel[lane][k] = (float)(pw[k] - (double)u[k] * g.h[k]);
       }
-      sm.s_cell[t] = cell;
-      sm.s_img[t] = img;
-      sm.s_end[t] = (g.npbc == 3 || ((img >> 16) & 0xff) == 128) ? v.ccount[cell] : 0;
+      sm.s_cell[lane] = cell;
+      sm.s_img[lane] = img;
+      end = (g.npbc == 3 || ((img >> 16) & 0xff) == 128) ? v.ccount[cell] : 0;
     }
-    __syncthreads();
-    lds_scan(sm.s_end, nloc);
-    const int total = sm.s_end[nloc - 1];
-    for (int it = threadIdx.x; it < total; it += kBlock) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int u = __shfl_up(end, o, kWave);
+      if (lane >= o) end += u;
+    }
+    sm.s_end[lane] = end;
+    const int total = __shfl(end, nloc - 1, kWave);
+    wave_sync_lds();
+    for (int it = lane; it < total; it += kWave) {
       int lo_i = 0, hi_i = nloc - 1;
       while (lo_i < hi_i) {
         const int mid = (lo_i + hi_i) >> 1;
@@ -112,56 +127,54 @@ __device__ void descr_row(const Geo& g, const Dev& v, float r_d, int m, int on, 
       for (int k = 0; k < kRbf; k++) {
         const float dr = r - (kRbfMin + k * dmu);
         const float ph = __expf(-gam * dr * dr) * fc;
-        acc[0][k] += ch == 0 ? ph : 0.f;
-        acc[1][k] += ch == 1 ? ph : 0.f;
-        acc[2][k] += ch == 2 ? ph : 0.f;
+        acc[k] += ch == 0 ? ph : 0.f;
+        acc[kRbf + k] += ch == 1 ? ph : 0.f;
+        acc[2 * kRbf + k] += ch == 2 ? ph : 0.f;
       }
     }
-    __syncthreads();
+    wave_sync_lds();   // the next chunk rewrites the cell arrays
   }
-  // fixed-order reduction: lanes of a wave, then the waves
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  // butterfly: at offset o each lane keeps the half of its 2o values that its
+  // bit o selects and adds its partner's copy of that half
 #pragma unroll
-  for (int c = 0; c < kChan; c++)
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const bool up = (lane & o) != 0;
 #pragma unroll
-    for (int k = 0; k < kRbf; k++) {
-      float x = acc[c][k];
-#pragma unroll
-      for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_down(x, off, kWave);
-      if (lane == 0) sm.red[w][c * kRbf + k] = x;
+    for (int i = 0; i < o; i++) {
+      const float keep = up ? acc[i + o] : acc[i];
+      const float give = up ? acc[i] : acc[i + o];
+      acc[i] = keep + __shfl_xor(give, o, kWave);
     }
-  __syncthreads();
-  const int f = threadIdx.x;
-  if (f < kRbf * kChan) {
-    float s = 0;
-    for (int q = 0; q < kBlock / kWave; q++) s += sm.red[q][f];
-    out[f] = kFeatScale * s;
-  } else if (f < kDescr) {
-    float val = 0;
-    if (f == kRbf * kChan) val = qi;
-    else if (g.npbc == 2 && f == kRbf * kChan + 1) val = __expf(-0.5f * (float)p.z);
-    else if (g.npbc == 2 && f == kRbf * kChan + 2) val = __expf(-0.5f * (float)(g.Lsim[2] - p.z));
-    out[f] = val;
   }
-  __syncthreads();
+  float val;
+  if (lane < kRbf * kChan) val = kFeatScale * acc[0];
+  else if (lane == kRbf * kChan) val = qi;
+  else if (g.npbc == 2 && lane == kRbf * kChan + 1) val = __expf(-0.5f * (float)p.z);
+  else if (g.npbc == 2 && lane == kRbf * kChan + 2) val = __expf(-0.5f * (float)(g.Lsim[2] - p.z));
+  else val = 0.f;
+  if (lane < kDescr) out[lane] = val;
 }
 
+static_assert(kRbf * kChan + 3 <= kDescr && kDescr == kWave, "descriptor row = one value per lane");
+
 // Descriptors of every moved bead of every proposal, old and new: grid
-// (2 maxm, pmax, systems); block (m2, s, r) is bead m2 % maxm of system r at
-// old (m2 < maxm) or new.
+// (2 maxm / 4, pmax, systems), one wave per row; row m2 of system r is bead
+// m2 % maxm at old (m2 < maxm) or new.
 __global__ __launch_bounds__(kBlock) void k_descr_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                        const RepCfg* __restrict__ RC) {
-  __shared__ DescrSmem sm;
+  __shared__ DescrWaveSmem sm[kDescrWaves];
   const Geo& g = G[blockIdx.z];
   const Dev& d = D[blockIdx.z];
   const RepCfg& rc = RC[blockIdx.z];
-  if ((int)blockIdx.x >= 2 * rc.maxm) return;
+  const int w = threadIdx.x / kWave;
+  const int m2 = blockIdx.x * kDescrWaves + w;
+  if (m2 >= 2 * rc.maxm) return;
   const int s = blockIdx.y;
   if (s >= d.plan->n_prop) return;
-  const int on = blockIdx.x >= rc.maxm ? 1 : 0, m = blockIdx.x - on * rc.maxm;
+  const int on = m2 >= rc.maxm ? 1 : 0, m = m2 - on * rc.maxm;
   const Dev v = prop_view(g, d, rc, s, 0);
   if (m >= v.rec->n_moved) return;
-  descr_row(g, v, rc.r_d, m, on, d.sur_x + ((size_t)(2 * s + on) * rc.maxm + m) * kDescr, sm);
+  descr_wave(g, v, rc.r_d, m, on, d.sur_x + ((size_t)(2 * s + on) * rc.maxm + m) * kDescr, sm[w]);
 }
 
 // ------------------------------------------------------------ inference
@@ -232,8 +245,10 @@ __device__ __forceinline__ void mlp_layer(const unsigned short* in, const unsign
 }
 
 // f(row) for `rows` descriptor rows; each wave takes 32 rows at a time.
+// Logical row i of `rows` is physical row (i / per) * stride + i % per of x
+// and f (per = stride: contiguous rows).
 __global__ __launch_bounds__(kBlock) void k_mlp_infer(const NetBF16* __restrict__ net, const float* __restrict__ x,
-                                                      int rows, float* __restrict__ f) {
+                                                      int rows, float* __restrict__ f, int per, int stride) {
   __shared__ MlpSmem sm;
   for (int i = threadIdx.x; i < kHid * kDescr; i += kBlock) sm.w1t[i] = net->w1t[i];
   for (int i = threadIdx.x; i < kHid * kHid; i += kBlock) {
@@ -255,7 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(const NetBF16* __restrict_
     // input rows (zero past the end) in BF16
     for (int i = lane; i < 32 * kDescr; i += kWave) {
       const int rr = base + i / kDescr;
-      a0[i] = to_bf16(rr < rows ? x[(size_t)rr * kDescr + (i % kDescr)] : 0.f);
+      const size_t pr = (size_t)(rr / per) * stride + rr % per;
+      a0[i] = to_bf16(rr < rows ? x[pr * kDescr + (i % kDescr)] : 0.f);
     }
     __builtin_amdgcn_wave_barrier();
     mlp_layer<kDescr>(a0, sm.w1t, sm.b1, a1);
@@ -269,7 +285,8 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(const NetBF16* __restrict_
     float s = 0;
     for (int j = half * (kHid / 2); j < (half + 1) * (kHid / 2); j++) s += from_bf16(a1[rrow * kHid + j]) * sm.w4[j];
     s += __shfl_down(s, 32, kWave);
-    if (half == 0 && base + rrow < rows) f[base + rrow] = s + b4;
+    const int ro = base + rrow;
+    if (half == 0 && ro < rows) f[(size_t)(ro / per) * stride + ro % per] = s + b4;
     __builtin_amdgcn_wave_barrier();
   }
 }

@@ -274,7 +274,7 @@ int plum_surrogate_forward(plum_ctx* c, const float* x, int rows, int bf16, floa
   HIPCHK(hipMemcpyAsync(s->X, x, sizeof(float) * rows * kDescr, hipMemcpyHostToDevice, st));
   if (bf16)
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(256, (rows + 127) / 128))), dim3(kBlock), 0, st, s->net,
-                       s->X, rows, s->F);
+                       s->X, rows, s->F, rows, rows);
   else
     sur_forward32(c, st, rows);
   HIPCHK(hipGetLastError());
