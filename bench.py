@@ -179,7 +179,7 @@ def brush_leg(moves, batch=32):
                        "walls z=0,40, slab Ewald (Lz 190), moves 20/50/30 COM/pivot/crankshaft",
            "particles": int(len(sd["q"])), "moves": moves}
     out = {}
-    for label, pmax in (("plain_p1", 1), ("plain", 8)):
+    for label, pmax in (("plain_p1", 1), ("plain_p8", 8), ("plain_p16", 16)):
         eng = E.Engine(sd)
         eng.set_tuning(pmax=pmax)
         eng.mc_run(2000, seed=1, **kw)
@@ -188,6 +188,9 @@ def brush_leg(moves, batch=32):
         out[label] = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
         log("brush %s: %.0f moves/s, %.0f accepted/s" % (label, out[label][0], out[label][1]))
         eng.close()
+    # the plain path at its better batch size is the reference point
+    out["plain"] = max(out["plain_p8"], out["plain_p16"], key=lambda v: v[1])
+    plain_pmax = 8 if out["plain"] is out["plain_p8"] else 16
     eng = E.Engine(sd)
     eng.surrogate_enable(screen=32, fill=4, train_every=16, train_samples=128, ring=2048, lr=1e-3,
                          r_d=4.0, seed=1)
@@ -214,7 +217,7 @@ def brush_leg(moves, batch=32):
     b.close()
     eng.surrogate_disable()
     b = E.Batch(engs)
-    b.set_tuning(pmax=8)
+    b.set_tuning(pmax=16)
     b.mc_run(moves // R, mcs)
     stp = b.mc_run(moves // R * 2, mcs)
     accp = sum(stp.accepted)
@@ -233,7 +236,8 @@ def brush_leg(moves, batch=32):
     })
     res.update({
         "plain_fp64_moves_per_s": out["plain"][0], "plain_fp64_accepted_per_s": out["plain"][1],
-        "plain_fp64_accept_ratio": out["plain"][2],
+        "plain_fp64_accept_ratio": out["plain"][2], "plain_fp64_proposals_per_batch": plain_pmax,
+        "plain_p8_accepted_per_s": out["plain_p8"][1], "plain_p16_accepted_per_s": out["plain_p16"][1],
         "plain_one_proposal_moves_per_s": out["plain_p1"][0],
         "plain_one_proposal_accepted_per_s": out["plain_p1"][1],
         "surrogate_moves_per_s": da[0], "surrogate_accepted_per_s": da[1],
