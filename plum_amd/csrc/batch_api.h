@@ -150,6 +150,7 @@ int batch_upload_state(plum_batch* b) {
     D[r] = c->d;
     RepCfg& q = RCv[r];
     q.nkb_s = c->nkb_s;
+    q.nrun = c->nrun;
     q.nsplit = c->nsplit;
     q.ncell = c->g.ncell;
     q.pmax = b->tune.pmax;

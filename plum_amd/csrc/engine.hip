@@ -54,6 +54,7 @@ struct plum_ctx {
   int nsplit = 1;        // blocks per real-space query
   double cand = 0;       // expected candidate pair-images per query
   int nkb_s = 0;         // k-space blocks of the speculative loop
+  int nrun = 0;          // k-vector runs of the speculative k-space (Dev::krun)
   int xcap = 0;          // cross-term partial capacity
   plum_batch* self = nullptr;  // this system as a batch of one (device loop)
   SurHost* sur = nullptr;      // delayed-acceptance surrogate (null when off)
@@ -428,6 +429,18 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   }
   g.n_kh = (int)kl.size();
   info.n_k_half = g.n_kh;
+  // runs of up to kKRun k vectors with one (l_x, l_y) and consecutive l_z
+  // (the list has l_z fastest), one lane each in the speculative k-space
+  std::vector<int2> krun;
+  for (int k = 0; k < g.n_kh;) {
+    int len = 1;
+    while (len < kKRun && k + len < g.n_kh && kl[k + len].x == kl[k].x && kl[k + len].y == kl[k].y &&
+           kl[k + len].z == kl[k].z + len)
+      len++;
+    krun.push_back(make_int2(k, len));
+    k += len;
+  }
+  c->nrun = (int)krun.size();
   c->nkb = std::max(1, (g.n_kh + kKPerBlock - 1) / kKPerBlock);
   c->nkb_full = std::max(1, (g.n_kh + kBlock - 1) / kBlock);
 
@@ -470,7 +483,8 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   ALLOC(d.mv_type, (size_t)kMaxProp * maxm);
   ALLOC(d.recs, kMaxScreen); ALLOC(d.smv_idx, (size_t)kMaxProp * maxm);
   ALLOC(d.prop, kMaxScreen); ALLOC(d.plan, 1);
-  c->nkb_s = (g.use_ewald && g.n_kh > 0) ? (g.n_kh + kKSpec - 1) / kKSpec : 0;
+  ALLOC(d.krun, std::max(1, c->nrun));
+  c->nkb_s = (g.use_ewald && g.n_kh > 0) ? (c->nrun + kKSpec - 1) / kKSpec : 0;
   ALLOC(d.part_ks, (size_t)kKVals * std::max(1, c->nkb_s));
   c->xcap = kMaxPairs * cross_blocks(maxm, maxm);
   ALLOC(d.part_x, c->xcap);
@@ -514,6 +528,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
       (rc = up(d.type, type, sizeof(int) * n)) || (rc = up(d.mol_of, mol_of.data(), sizeof(int) * n)) ||
       (rc = up(d.mol_start, ms, sizeof(int) * (n_mols + 1))) ||
       (rc = up(d.kl, kl.data(), sizeof(int4) * kl.size())) ||
+      (rc = up(d.krun, krun.data(), sizeof(int2) * krun.size())) ||
       (rc = up(d.ck, ck.data(), sizeof(double) * ck.size())) ||
       (rc = up(d.lj, lj.data(), sizeof(LJPair) * lj.size())) ||
       (rc = up(d.lj32, lj32.data(), sizeof(LJPair32) * lj32.size())) ||

@@ -106,6 +106,7 @@ struct Dev {
   int* ctype;
   // k space
   int4* kl;
+  int2* krun;             // runs of k vectors: (first, count), one (l_x, l_y), consecutive l_z
   double* ck;
   double2 *Sk, *dS;
   double2* tab;           // phase tables

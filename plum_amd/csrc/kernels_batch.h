@@ -49,7 +49,7 @@ constexpr int kKSpecGroups = kBlock / kKSpec;
 
 // Per-system launch shape.
 struct RepCfg {
-  int nkb_s;    // k-space blocks of kKSpec vectors (0 without Ewald)
+  int nkb_s;    // k-space blocks of kKSpec runs (0 without Ewald)
   int nsplit;   // blocks per real-space query
   int ncell;    // cells
   int pmax;     // proposals per batch (<= kMaxProp, <= kMaxScreen with the surrogate)
@@ -61,6 +61,7 @@ struct RepCfg {
   int fill;     // exact slots given to first-stage rejections (training data)
   float r_d;    // descriptor radius of the surrogate
   int train;    // this system feeds the surrogate's training ring
+  int nrun;     // k-vector runs (Dev::krun)
 };
 
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
@@ -445,87 +446,137 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
 }
 
 // ------------------------------------------------------------ k space
-// 16 k vectors x 16 bead groups per block.  Exact slots go in chunks of
+// 16 runs x 16 bead groups per block.  A run is up to kKRun k vectors of one
+// (l_x, l_y) row with consecutive l_z, all taken by one lane: it reads the
+// x, y and first z phase and e^{i theta_z} of each moved bead from the
+// tables and advances the z phase by multiplication (e^{i (l+1) theta} =
+// e^{i l theta} e^{i theta}, conjugated rows included), four table reads for
+// up to four k vectors instead of three each.  Exact slots go in chunks of
 // kSlotChunk: each slot's dS_e(k) accumulates in registers per group, the
 // groups meet in LDS, and the per-k sums land in s_t.  Then one thread per
-// value forms, over the block's 16 k vectors, dE_e = sum c_k (2 Re(S* dS_e) +
+// value forms, over the block's k vectors, dE_e = sum c_k (2 Re(S* dS_e) +
 // |dS_e|^2) or a cross sum 2 sum c_k Re(dS_p* dS_q).  The S(k) of the slots
 // accepted by the previous batch is folded in first.
-constexpr int kSlotChunk = 8;
+constexpr int kKRun = 4;
+constexpr int kKBlk = kKSpec * kKRun;   // k vectors a block covers at most
+constexpr int kSlotChunk = 1;
 static_assert(kKVals <= kBlock, "one thread per k-space value");
+static_assert(kKBlk <= kBlock && kSlotChunk * kKBlk <= kBlock, "one thread per k vector in the block's sums");
 static_assert(kMaxProp <= 32, "exact slots are bits of one mask");
 
 struct KSpecSmem {
-  double2 s_ds[kSlotChunk][kKSpecGroups][kKSpec];
-  double2 s_t[kMaxProp][kKSpec];
-  double2 s_S[kKSpec];
-  double s_ck[kKSpec];
+  double2 s_ds[kSlotChunk][kKSpecGroups][kKBlk];
+  double2 s_t[kMaxProp][kKBlk];
+  double2 s_S[kKBlk];
+  double s_ck[kKBlk];
+  int s_k0[kKSpec], s_len[kKSpec];
 };
+
+// Phases of a run of len k vectors for one table row t (ti: the run's first
+// k vector; z1: offset of e^{i theta_z}).
+__device__ __forceinline__ void run_phase(const double2* t, const TabIdx& ti, int z1, double2 (&e)[kKRun]) {
+  double2 y = t[ti.oy], z = t[ti.oz];
+  const double2 b = t[z1];
+  if (ti.ny) y.y = -y.y;
+  if (ti.nz) z.y = -z.y;
+  const double2 xy = cmul(t[ti.ox], y);
+  e[0] = cmul(xy, z);
+#pragma unroll
+  for (int i = 1; i < kKRun; i++) {
+    z = cmul(z, b);
+    e[i] = cmul(xy, z);
+  }
+}
 
 __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, const SpecPlan* plan, int ne,
                                   unsigned acc_mask, int blk, KSpecSmem& sm) {
   const int kk = threadIdx.x % kKSpec, grp = threadIdx.x / kKSpec;
-  const int k = blk * kKSpec + kk;
+  const int t = threadIdx.x;
+  if (t < kKSpec) {
+    const int r = blk * kKSpec + t;
+    const int2 run = r < rc.nrun ? d.krun[r] : make_int2(0, 0);
+    sm.s_k0[t] = run.x;
+    sm.s_len[t] = run.y;
+  }
+  __syncthreads();
+  const int k0 = sm.s_k0[kk], len = sm.s_len[kk];
   TabIdx ti = {0, 0, 0, false, false};
-  if (k < g.n_kh) ti = tab_index(g, d.kl[k]);
+  if (len > 0) ti = tab_index(g, d.kl[k0]);
+  const int z1 = g.nkx + 1 + g.nky + 1 + 1;
+  // k vector kb of the block: run kb / kKRun, entry kb % kKRun
+  const int kb = t, kr = kb / kKRun, ki = kb - kr * kKRun;
+  const bool kvalid = kb < kKBlk && ki < sm.s_len[kr];
+  const int kt = kvalid ? sm.s_k0[kr] + ki : 0;
   // the previous batch's accepted dS rows join S(k) before this batch's
   // rows overwrite them (the chunk loop's barrier orders the two)
-  if (grp == 0) {
+  if (kb < kKBlk) {
     double2 S = make_double2(0, 0);
     double ck = 0;
-    if (k < g.n_kh) {
-      S = d.Sk[k];
+    if (kvalid) {
+      S = d.Sk[kt];
       if (acc_mask) {
         for (int e = 0; e < kMaxProp; e++)
           if (acc_mask & (1u << e)) {
-            const double2 prev = d.dS[(size_t)e * g.n_kh + k];
+            const double2 prev = d.dS[(size_t)e * g.n_kh + kt];
             S.x += prev.x;
             S.y += prev.y;
           }
-        d.Sk[k] = S;
+        d.Sk[kt] = S;
       }
-      ck = d.ck[k];
+      ck = d.ck[kt];
     }
-    sm.s_S[kk] = S;
-    sm.s_ck[kk] = ck;
+    sm.s_S[kb] = S;
+    sm.s_ck[kb] = ck;
   }
   for (int e0 = 0; e0 < ne; e0 += kSlotChunk) {
 #pragma unroll
     for (int c = 0; c < kSlotChunk; c++) {
       const int e = e0 + c;
       if (e < ne) {
-        double2 acc = make_double2(0, 0);
+        double2 acc[kKRun];
+#pragma unroll
+        for (int i = 0; i < kKRun; i++) acc[i] = make_double2(0, 0);
         const int s = uload(&plan->screen_of[e]);
         const int n = uload(&d.recs[s].n_moved);
-        if (k < g.n_kh) {
+        if (len > 0) {
           const double4* mo = d.mv_old + s * rc.maxm;
           const double2* base = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
           for (int m = grp; m < n; m += kKSpecGroups) {
             const double qm = mo[m].w;
             const double2* to = base + (size_t)(2 * m) * g.tab_w;
             const double2* tn = to + g.tab_w;
-            const double2 eo = tab_phase(to, ti);
-            const double2 en = tab_phase(tn, ti);
-            acc.x += qm * (en.x - eo.x);
-            acc.y += qm * (en.y - eo.y);
+            double2 eo[kKRun], en[kKRun];
+            run_phase(to, ti, z1, eo);
+            run_phase(tn, ti, z1, en);
+#pragma unroll
+            for (int i = 0; i < kKRun; i++) {
+              acc[i].x += qm * (en[i].x - eo[i].x);
+              acc[i].y += qm * (en[i].y - eo[i].y);
+            }
           }
         }
-        sm.s_ds[c][grp][kk] = acc;
+#pragma unroll
+        for (int i = 0; i < kKRun; i++) sm.s_ds[c][grp][kk * kKRun + i] = acc[i];
       }
     }
     __syncthreads();
-    // slot c of the chunk, k vector kk: the sum over the 16 groups
-    if (grp < kSlotChunk && e0 + grp < ne) {
-      const int e = e0 + grp;
-      double2 t = make_double2(0, 0);
+    // slot c of the chunk, k vector j of the block: the sum over the groups
+    if (t < kSlotChunk * kKBlk) {
+      const int c = t / kKBlk, j = t - c * kKBlk;
+      const int e = e0 + c;
+      if (e < ne) {
+        const int jr = j / kKRun, ji = j - jr * kKRun;
+        const bool jv = ji < sm.s_len[jr];
+        double2 v = make_double2(0, 0);
 #pragma unroll
-      for (int q = 0; q < kKSpecGroups; q++) {
-        t.x += sm.s_ds[grp][q][kk].x;
-        t.y += sm.s_ds[grp][q][kk].y;
+        for (int q = 0; q < kKSpecGroups; q++) {
+          v.x += sm.s_ds[c][q][j].x;
+          v.y += sm.s_ds[c][q][j].y;
+        }
+        if (!jv) v = make_double2(0, 0);
+        sm.s_t[e][j] = v;
+        if (jv) d.dS[(size_t)e * g.n_kh + sm.s_k0[jr] + ji] = v;
       }
-      if (k >= g.n_kh) t = make_double2(0, 0);
-      sm.s_t[e][kk] = t;
-      if (k < g.n_kh) d.dS[(size_t)e * g.n_kh + k] = t;
     }
     __syncthreads();
   }
@@ -548,11 +599,11 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
     }
     double sv = 0;
 #pragma unroll 4
-    for (int i = 0; i < kKSpec; i++) {
+    for (int i = 0; i < kKBlk; i++) {
       const double ck = sm.s_ck[i];
       if (v < kMaxProp) {
-        const double2 t = sm.s_t[v][i], S = sm.s_S[i];
-        sv += ck * (2.0 * (S.x * t.x + S.y * t.y) + t.x * t.x + t.y * t.y);
+        const double2 tt = sm.s_t[v][i], S = sm.s_S[i];
+        sv += ck * (2.0 * (S.x * tt.x + S.y * tt.y) + tt.x * tt.x + tt.y * tt.y);
       } else {
         const double2 tp = sm.s_t[p][i], tq = sm.s_t[q][i];
         sv += 2.0 * ck * (tp.x * tq.x + tp.y * tq.y);
