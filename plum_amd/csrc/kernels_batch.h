@@ -472,22 +472,6 @@ struct KSpecSmem {
   int s_k0[kKSpec], s_len[kKSpec];
 };
 
-// Phases of a run of len k vectors for one table row t (ti: the run's first
-// k vector; z1: offset of e^{i theta_z}).
-__device__ __forceinline__ void run_phase(const double2* t, const TabIdx& ti, int z1, double2 (&e)[kKRun]) {
-  double2 y = t[ti.oy], z = t[ti.oz];
-  const double2 b = t[z1];
-  if (ti.ny) y.y = -y.y;
-  if (ti.nz) z.y = -z.y;
-  const double2 xy = cmul(t[ti.ox], y);
-  e[0] = cmul(xy, z);
-#pragma unroll
-  for (int i = 1; i < kKRun; i++) {
-    z = cmul(z, b);
-    e[i] = cmul(xy, z);
-  }
-}
-
 __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, const SpecPlan* plan, int ne,
                                   unsigned acc_mask, int blk, KSpecSmem& sm) {
   const int kk = threadIdx.x % kKSpec, grp = threadIdx.x / kKSpec;
@@ -545,13 +529,34 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
             const double qm = mo[m].w;
             const double2* to = base + (size_t)(2 * m) * g.tab_w;
             const double2* tn = to + g.tab_w;
-            double2 eo[kKRun], en[kKRun];
-            run_phase(to, ti, z1, eo);
-            run_phase(tn, ti, z1, en);
+            // the eight table reads of both positions first, then new minus old
+            double2 ox_n = tn[ti.ox], oy_n = tn[ti.oy], oz_n = tn[ti.oz], b_n = tn[z1];
+            double2 ox_o = to[ti.ox], oy_o = to[ti.oy], oz_o = to[ti.oz], b_o = to[z1];
+            {
+              if (ti.ny) oy_n.y = -oy_n.y;
+              if (ti.nz) oz_n.y = -oz_n.y;
+              const double2 xy = cmul(ox_n, oy_n);
+              double2 z = oz_n;
 #pragma unroll
-            for (int i = 0; i < kKRun; i++) {
-              acc[i].x += qm * (en[i].x - eo[i].x);
-              acc[i].y += qm * (en[i].y - eo[i].y);
+              for (int i = 0; i < kKRun; i++) {
+                if (i) z = cmul(z, b_n);
+                const double2 e = cmul(xy, z);
+                acc[i].x += qm * e.x;
+                acc[i].y += qm * e.y;
+              }
+            }
+            {
+              if (ti.ny) oy_o.y = -oy_o.y;
+              if (ti.nz) oz_o.y = -oz_o.y;
+              const double2 xy = cmul(ox_o, oy_o);
+              double2 z = oz_o;
+#pragma unroll
+              for (int i = 0; i < kKRun; i++) {
+                if (i) z = cmul(z, b_o);
+                const double2 e = cmul(xy, z);
+                acc[i].x -= qm * e.x;
+                acc[i].y -= qm * e.y;
+              }
             }
           }
         }
