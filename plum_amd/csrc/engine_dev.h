@@ -111,6 +111,13 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;  // valid in thread 0
 }
 
+// Orders one wave's LDS writes before its lanes read each other's entries.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The reference's minimum-image difference vector b - a
 // (GetDistVector, misc.cc:43-56).
 __device__ __forceinline__ void min_image(const double* a, const double* b,

@@ -64,6 +64,11 @@ struct RepCfg {
   int nrun;     // k-vector runs (Dev::krun)
 };
 
+// Real-space query blocks of a move of nm beads: one query per wave, the
+// old and the new position of each bead.
+constexpr int kQWaves = kBlock / kWave;
+__host__ __device__ __forceinline__ int qwave_blocks(int nm) { return (2 * nm + kQWaves - 1) / kQWaves; }
+
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
 __host__ __device__ __forceinline__ int nmm_blocks(int nm) {
   const int npair = nm * (nm - 1) / 2;
@@ -381,7 +386,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     for (int e = 0; e < ne; e++) {
       plan->screen_of[e] = sm.s_screen[e];
       const int n = sm.s_prop[sm.s_screen[e]].n;
-      off += n > 0 ? 2 * n * rc.nsplit + nmm_blocks(n) : 0;
+      off += n > 0 ? qwave_blocks(n) + nmm_blocks(n) : 0;
       plan->prop_off[e + 1] = off;
     }
     sm.s_ne = ne;
@@ -618,6 +623,179 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
   }
 }
 
+// ------------------------------------------------------------ real space
+// One real-space query per wave (kQWaves per block, no block barriers): the
+// wave lists the query cells 64 at a time with a wave prefix over their
+// counts, each lane takes candidate slots lane, lane + 64, ..., and a wave
+// reduction sums the lanes in fixed order.  Same pair terms as real_query.
+template <int P>
+struct QWaveSmem {
+  int s_cell[kWave];
+  int s_img[kWave];
+  int s_end[kWave];
+  float s_rel[P ? kWave : 1][3];
+};
+
+struct QueryPt {
+  double4 p;
+  double pw[3];
+  const LJPair* ljrow;
+  const LJPair32* ljrow32;
+  float qpf, rs2f_wide, saf, lBf;
+  double lj_close;
+};
+
+// LJ + real-space terms of query point q against the bead in cell slot si
+// (image img; rel: the query point relative to that cell, FP32 path).
+template <int P>
+__device__ __forceinline__ void query_candidate(const Geo& g, const Dev& d, const QueryPt& q, size_t si, int img,
+                                                const float* rel, double& elj, double& ere) {
+  const int iz = ((img >> 16) & 0xff) - 128;
+  const bool lj_ok = g.use_pair && (g.npbc == 3 || iz == 0);
+  const int ix = (img & 0xff) - 128, iy = ((img >> 8) & 0xff) - 128;
+  if constexpr (P == 0) {
+    const double4 pj = d.cpos[si];
+    const double dx = pj.x + ix * g.L[0] - q.pw[0];
+    const double dy = pj.y + iy * g.L[1] - q.pw[1];
+    const double dz = pj.z + iz * g.L[2] - q.pw[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 > g.rs2) return;
+    const double r = sqrt(r2);
+    if (lj_ok) elj += lj_energy(r, q.ljrow[d.ctype[si]]);
+    if (g.use_ewald) {
+      const double qq = q.p.w * pj.w;
+      if (qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
+    }
+  } else {
+    // FP32 prefilter on cell-relative coordinates (slightly widened so no
+    // pair the FP64 test keeps is dropped), then the exact FP64 distance and
+    // cutoff test, then FP32 pair math
+    const float4 pr = d.cpos32[si];
+    const float fx = pr.x - rel[0], fy = pr.y - rel[1], fz = pr.z - rel[2];
+    if (fx * fx + fy * fy + fz * fz > q.rs2f_wide) return;
+    const double4 pj = d.cpos[si];
+    const double dx = pj.x + ix * g.L[0] - q.pw[0];
+    const double dy = pj.y + iy * g.L[1] - q.pw[1];
+    const double dz = pj.z + iz * g.L[2] - q.pw[2];
+    const double r2d = dx * dx + dy * dy + dz * dz;
+    if (r2d > g.rs2) return;
+    const double rd = sqrt(r2d);
+    if (rd < q.lj_close) {
+      // close contacts carry the large, steep terms: FP64 like P == 0
+      if (lj_ok) elj += lj_energy(rd, q.ljrow[d.ctype[si]]);
+      if (g.use_ewald) {
+        const double qq = q.p.w * pj.w;
+        if (qq != 0 && rd > 0 && rd <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * rd) / rd;
+      }
+      return;
+    }
+    const float r2 = (float)r2d;
+    if (lj_ok) {
+      const LJPair32 lp = q.ljrow32[d.ctype[si]];
+      if (rd < lp.cut) {
+        const float s2 = lp.sigma2 / r2;
+        const float r6 = s2 * s2 * s2;
+        elj += (double)(lp.eps4 * (r6 * r6 - r6) - lp.shift);
+      }
+    }
+    if (g.use_ewald) {
+      const float qq = q.qpf * pr.w;
+      if (qq != 0.f && rd <= g.rc) {
+        const float r = (float)rd;
+        ere += (double)(q.lBf * qq * erfcf(q.saf * r) / r);
+      }
+    }
+  }
+}
+
+// LJ + real-space energy of point p (charge p.w, type tm) against every bead
+// image in the cell list within the search radius, skipping the index range
+// [mlo, mhi] (contig) or the moved flags, one wave.  Result valid in lane 0.
+template <int P>
+__device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm, int contig, int mlo, int mhi,
+                                   QWaveSmem<P>& sm) {
+  const int lane = threadIdx.x % kWave;
+  QueryPt q;
+  q.p = p;
+  q.pw[0] = wrap_coord(p.x, g.L[0], g.inv_L[0]);
+  q.pw[1] = wrap_coord(p.y, g.L[1], g.inv_L[1]);
+  q.pw[2] = wrap_coord(p.z, g.L[2], g.inv_L[2]);
+  q.ljrow = d.lj + tm * g.n_types;
+  q.ljrow32 = d.lj32 + tm * g.n_types;
+  q.qpf = (float)p.w;
+  q.rs2f_wide = (float)(g.rs2 * (1.0 + 1e-4)) + 1e-4f;
+  q.saf = (float)g.sqrt_alpha;
+  q.lBf = (float)g.lB;
+  q.lj_close = g.fp64_below;
+  int lo[3], nd[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = (int)floor((q.pw[k] - g.rs) * g.inv_h[k]);
+    const int hi = (int)floor((q.pw[k] + g.rs) * g.inv_h[k]);
+    nd[k] = hi - lo[k] + 1;
+  }
+  const int ncq = nd[0] * nd[1] * nd[2];
+  double elj = 0, ere = 0;
+  for (int base = 0; base < ncq; base += kWave) {
+    const int nloc = min(kWave, ncq - base);
+    int end = 0;
+    if (lane < nloc) {
+      const int c = base + lane;
+      const int cz = c % nd[2], cy = (c / nd[2]) % nd[1], cx = c / (nd[2] * nd[1]);
+      const int u[3] = {lo[0] + cx, lo[1] + cy, lo[2] + cz};
+      int cell = 0, img = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int qk = floordiv(u[k], g.nc[k]);
+        const int w = u[k] - qk * g.nc[k];
+        cell = cell * g.nc[k] + w;
+        img |= ((qk + 128) & 0xff) << (8 * k);
+        if constexpr (P != 0) sm.s_rel[lane][k] = (float)(q.pw[k] - (double)u[k] * g.h[k]);
+      }
+      sm.s_cell[lane] = cell;
+      sm.s_img[lane] = img;
+      // cells whose box (widened by a rounding margin) lies beyond the search
+      // radius hold no candidate
+      double dmin2 = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double b0 = (double)u[k] * g.h[k] - 1e-9 * g.L[k], b1 = b0 + g.h[k] + 2e-9 * g.L[k];
+        const double dd = fmax(0.0, fmax(b0 - q.pw[k], q.pw[k] - b1));
+        dmin2 += dd * dd;
+      }
+      end = dmin2 > g.rs2 ? 0 : d.ccount[cell];
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int v = __shfl_up(end, o, kWave);
+      if (lane >= o) end += v;
+    }
+    sm.s_end[lane] = end;
+    const int total = __shfl(end, nloc - 1, kWave);
+    wave_sync_lds();
+    for (int it = lane; it < total; it += kWave) {
+      int lo_i = 0, hi_i = nloc - 1;
+      while (lo_i < hi_i) {
+        const int mid = (lo_i + hi_i) >> 1;
+        if (sm.s_end[mid] > it) hi_i = mid;
+        else lo_i = mid + 1;
+      }
+      const int t = lo_i;
+      const size_t si = (size_t)sm.s_cell[t] * g.cap + (it - (t ? sm.s_end[t - 1] : 0));
+      const int j = d.cbead[si];
+      if (j < 0 || (contig ? (j >= mlo && j <= mhi) : d.moved[j])) continue;
+      query_candidate<P>(g, d, q, si, sm.s_img[t], P ? sm.s_rel[t] : nullptr, elj, ere);
+    }
+    wave_sync_lds();   // the next chunk rewrites the cell arrays
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    elj += __shfl_down(elj, off, kWave);
+    ere += __shfl_down(ere, off, kWave);
+  }
+  return make_double2(elj, ere);
+}
+
 // Cross block: LJ + real pairs between slot q's and slot p's moved beads,
 // u(i',j') - u(i,j') - u(i',j) + u(i,j): the change of either one's dE when
 // the other has been accepted (same pair functions as the moved-moved pairs).
@@ -656,7 +834,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   __shared__ int s_pre[kMaxReplicas];
   __shared__ union U {
     KSpecSmem k;
-    RSmem<P> r;
+    QWaveSmem<P> q[kQWaves];
   } sm;
   __shared__ CSpecSmem cs;
   for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];
@@ -682,7 +860,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const int local = b - uload(&plan->prop_off[e]);
         const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
         const int nm = uload(&v.rec->n_moved);
-        real_block<P>(g, v, local, 2 * nm * rc.nsplit, nmm_blocks(nm), rc.nsplit, sm.r);
+        const int nqb = qwave_blocks(nm);
+        if (local < nqb) {
+          // query q2 = 2 m + on: moved bead m at its old (on = 0) or new position
+          const int w = threadIdx.x / kWave, q2 = local * kQWaves + w;
+          if (q2 < 2 * nm) {
+            const int m = q2 >> 1, on = q2 & 1;
+            const double4 p = on ? v.mv_new[m] : v.mv_old[m];
+            const double2 eq = real_query_wave<P>(g, v, p, v.mv_type[m], v.rec->contig, v.rec->lo, v.rec->hi, sm.q[w]);
+            const double sgn = on ? 1.0 : -1.0;
+            if (threadIdx.x % kWave == 0) v.part_r[q2] = make_double2(sgn * eq.x, sgn * eq.y);
+          }
+        } else {
+          mm_block(g, v, local - nqb, nmm_blocks(nm), cs.sh);
+        }
       } else {
         b -= nprop_blocks;
         const int nn = uload(&plan->n_near);
@@ -727,7 +918,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     if (e >= ne) return;
     const int nm = d.recs[plan->screen_of[e]].n_moved;
     if (nm == 0) return;
-    const int nreal = 2 * nm * rc.nsplit, nmm = nmm_blocks(nm);
+    const int nreal = 2 * nm, nmm = nmm_blocks(nm);
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;

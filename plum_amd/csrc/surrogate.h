@@ -44,12 +44,6 @@ struct DescrWaveSmem {
 };
 constexpr int kDescrWaves = kBlock / kWave;
 
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Descriptor of moved bead m of proposal view v at its old (on = 0) or new
 // (on = 1) position, against every unmoved bead image within r_d (images as
 // the LJ term takes them: z unshifted with two periodic dimensions).  One
