@@ -152,6 +152,7 @@ int batch_upload_state(plum_batch* b) {
     q.nkb_s = c->nkb_s;
     q.nrun = c->nrun;
     q.nsplit = c->nsplit;
+    q.qwave = c->nsplit == 1 ? 1 : 0;
     q.ncell = c->g.ncell;
     q.pmax = b->tune.pmax;
     q.maxm = c->maxm;

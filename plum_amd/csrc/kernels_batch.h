@@ -62,12 +62,17 @@ struct RepCfg {
   float r_d;    // descriptor radius of the surrogate
   int train;    // this system feeds the surrogate's training ring
   int nrun;     // k-vector runs (Dev::krun)
+  int qwave;    // real-space queries one per wave (nsplit == 1), else nsplit blocks each
 };
 
-// Real-space query blocks of a move of nm beads: one query per wave, the
-// old and the new position of each bead.
+// Real-space query blocks of a move of nm beads, the old and the new position
+// of each bead: one query per wave when a query's candidates fit one block
+// (qwave), else nsplit blocks per query.
 constexpr int kQWaves = kBlock / kWave;
 __host__ __device__ __forceinline__ int qwave_blocks(int nm) { return (2 * nm + kQWaves - 1) / kQWaves; }
+__host__ __device__ __forceinline__ int query_blocks(int nm, int qwave, int nsplit) {
+  return qwave ? qwave_blocks(nm) : 2 * nm * nsplit;
+}
 
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
 __host__ __device__ __forceinline__ int nmm_blocks(int nm) {
@@ -386,7 +391,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     for (int e = 0; e < ne; e++) {
       plan->screen_of[e] = sm.s_screen[e];
       const int n = sm.s_prop[sm.s_screen[e]].n;
-      off += n > 0 ? qwave_blocks(n) + nmm_blocks(n) : 0;
+      off += n > 0 ? query_blocks(n, rc.qwave, rc.nsplit) + nmm_blocks(n) : 0;
       plan->prop_off[e + 1] = off;
     }
     sm.s_ne = ne;
@@ -835,6 +840,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   __shared__ union U {
     KSpecSmem k;
     QWaveSmem<P> q[kQWaves];
+    RSmem<P> r;
   } sm;
   __shared__ CSpecSmem cs;
   for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];
@@ -861,7 +867,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
         const int nm = uload(&v.rec->n_moved);
         const int nqb = qwave_blocks(nm);
-        if (local < nqb) {
+        if (!rc.qwave) {
+          real_block<P>(g, v, local, 2 * nm * rc.nsplit, nmm_blocks(nm), rc.nsplit, sm.r);
+        } else if (local < nqb) {
           // query q2 = 2 m + on: moved bead m at its old (on = 0) or new position
           const int w = threadIdx.x / kWave, q2 = local * kQWaves + w;
           if (q2 < 2 * nm) {
@@ -918,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     if (e >= ne) return;
     const int nm = d.recs[plan->screen_of[e]].n_moved;
     if (nm == 0) return;
-    const int nreal = 2 * nm, nmm = nmm_blocks(nm);
+    const int nreal = 2 * nm * (rc.qwave ? 1 : rc.nsplit), nmm = nmm_blocks(nm);
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;
