@@ -152,12 +152,12 @@ int batch_upload_state(plum_batch* b) {
     q.nkb_s = c->nkb_s;
     q.nrun = c->nrun;
     q.nsplit = c->nsplit;
-    q.qwave = c->nsplit == 1 ? 1 : 0;
+    q.qwave = query_waves(c->nsplit, b->tune.pmax * b->R);
     q.ncell = c->g.ncell;
     q.pmax = b->tune.pmax;
     q.maxm = c->maxm;
     q.nmmb = c->nmmb;
-    q.rstride = 2 * c->maxm * kMaxSplit;
+    q.rstride = 2 * c->maxm * kMaxQWave;
     q.xcap = c->xcap;
     q.da = b->da ? 1 : 0;
     q.train = (b->da && r == 0) ? 1 : 0;

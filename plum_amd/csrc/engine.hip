@@ -498,7 +498,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
     c->nsplit = split_for(cand, kDefaultSplitDiv);
   }
   // room for any split (the batch autotuner may change it)
-  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, (size_t)kMaxProp * 2 * maxm * kMaxSplit);
+  ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, (size_t)kMaxProp * 2 * maxm * kMaxQWave);
   ALLOC(d.part_mm, (size_t)kMaxProp * c->nmmb);
   ALLOC(d.sk_part, (size_t)c->nchunk * std::max(1, g.n_kh));
   ALLOC(c->epart_k, c->nkb_full);

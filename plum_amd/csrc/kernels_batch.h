@@ -62,17 +62,21 @@ struct RepCfg {
   float r_d;    // descriptor radius of the surrogate
   int train;    // this system feeds the surrogate's training ring
   int nrun;     // k-vector runs (Dev::krun)
-  int qwave;    // real-space queries one per wave (nsplit == 1), else nsplit blocks each
+  int qwave;    // waves per real-space query (query_waves)
 };
 
 // Real-space query blocks of a move of nm beads, the old and the new position
-// of each bead: one query per wave when a query's candidates fit one block
-// (qwave), else nsplit blocks per query.
+// of each bead, qw waves per query (each takes a share of the candidate
+// slots), kQWaves waves per block.
 constexpr int kQWaves = kBlock / kWave;
-__host__ __device__ __forceinline__ int qwave_blocks(int nm) { return (2 * nm + kQWaves - 1) / kQWaves; }
-__host__ __device__ __forceinline__ int query_blocks(int nm, int qwave, int nsplit) {
-  return qwave ? qwave_blocks(nm) : 2 * nm * nsplit;
+constexpr int kMaxQWave = 4 * kMaxSplit;
+// One wave per split share; a batch of few proposals (fewer than 16 over all
+// systems) spreads each query over four times as many waves, as its queries
+// alone would leave most of the chip idle.
+__host__ __device__ __forceinline__ int query_waves(int nsplit, int proposals) {
+  return nsplit == 1 ? 1 : (proposals >= 16 ? nsplit : kQWaves * nsplit);
 }
+__host__ __device__ __forceinline__ int query_blocks(int nm, int qw) { return (2 * nm * qw + kQWaves - 1) / kQWaves; }
 
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
 __host__ __device__ __forceinline__ int nmm_blocks(int nm) {
@@ -391,7 +395,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     for (int e = 0; e < ne; e++) {
       plan->screen_of[e] = sm.s_screen[e];
       const int n = sm.s_prop[sm.s_screen[e]].n;
-      off += n > 0 ? query_blocks(n, rc.qwave, rc.nsplit) + nmm_blocks(n) : 0;
+      off += n > 0 ? query_blocks(n, rc.qwave) + nmm_blocks(n) : 0;
       plan->prop_off[e + 1] = off;
     }
     sm.s_ne = ne;
@@ -715,10 +719,11 @@ __device__ __forceinline__ void query_candidate(const Geo& g, const Dev& d, cons
 
 // LJ + real-space energy of point p (charge p.w, type tm) against every bead
 // image in the cell list within the search radius, skipping the index range
-// [mlo, mhi] (contig) or the moved flags, one wave.  Result valid in lane 0.
+// [mlo, mhi] (contig) or the moved flags, one wave taking share `part` of
+// `nparts` of the candidate slots.  Result valid in lane 0.
 template <int P>
 __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm, int contig, int mlo, int mhi,
-                                   QWaveSmem<P>& sm) {
+                                   int part, int nparts, QWaveSmem<P>& sm) {
   const int lane = threadIdx.x % kWave;
   QueryPt q;
   q.p = p;
@@ -778,7 +783,7 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
     sm.s_end[lane] = end;
     const int total = __shfl(end, nloc - 1, kWave);
     wave_sync_lds();
-    for (int it = lane; it < total; it += kWave) {
+    for (int it = part * kWave + lane; it < total; it += nparts * kWave) {
       int lo_i = 0, hi_i = nloc - 1;
       while (lo_i < hi_i) {
         const int mid = (lo_i + hi_i) >> 1;
@@ -840,7 +845,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   __shared__ union U {
     KSpecSmem k;
     QWaveSmem<P> q[kQWaves];
-    RSmem<P> r;
   } sm;
   __shared__ CSpecSmem cs;
   for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];
@@ -866,18 +870,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const int local = b - uload(&plan->prop_off[e]);
         const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
         const int nm = uload(&v.rec->n_moved);
-        const int nqb = qwave_blocks(nm);
-        if (!rc.qwave) {
-          real_block<P>(g, v, local, 2 * nm * rc.nsplit, nmm_blocks(nm), rc.nsplit, sm.r);
-        } else if (local < nqb) {
-          // query q2 = 2 m + on: moved bead m at its old (on = 0) or new position
-          const int w = threadIdx.x / kWave, q2 = local * kQWaves + w;
-          if (q2 < 2 * nm) {
+        const int qw = rc.qwave, nqb = query_blocks(nm, qw);
+        if (local < nqb) {
+          // wave gw: share gw % qw of query q2 = gw / qw = 2 m + on, moved
+          // bead m at its old (on = 0) or new position
+          const int w = threadIdx.x / kWave, gw = local * kQWaves + w;
+          if (gw < 2 * nm * qw) {
+            const int q2 = gw / qw, part = gw - q2 * qw;
             const int m = q2 >> 1, on = q2 & 1;
             const double4 p = on ? v.mv_new[m] : v.mv_old[m];
-            const double2 eq = real_query_wave<P>(g, v, p, v.mv_type[m], v.rec->contig, v.rec->lo, v.rec->hi, sm.q[w]);
+            const double2 eq =
+                real_query_wave<P>(g, v, p, v.mv_type[m], v.rec->contig, v.rec->lo, v.rec->hi, part, qw, sm.q[w]);
             const double sgn = on ? 1.0 : -1.0;
-            if (threadIdx.x % kWave == 0) v.part_r[q2] = make_double2(sgn * eq.x, sgn * eq.y);
+            if (threadIdx.x % kWave == 0) v.part_r[gw] = make_double2(sgn * eq.x, sgn * eq.y);
           }
         } else {
           mm_block(g, v, local - nqb, nmm_blocks(nm), cs.sh);
@@ -926,7 +931,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     if (e >= ne) return;
     const int nm = d.recs[plan->screen_of[e]].n_moved;
     if (nm == 0) return;
-    const int nreal = 2 * nm * (rc.qwave ? 1 : rc.nsplit), nmm = nmm_blocks(nm);
+    const int nreal = 2 * nm * rc.qwave, nmm = nmm_blocks(nm);
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;
