@@ -213,20 +213,25 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
     cy += pn.y + po.y;
     cz += pn.z + po.z;
   }
-  const double ewt = block_sum<kBlock>(ew, sm.sh);
-  const double dmzt = block_sum<kBlock>(dmz, sm.sh);
-  const double cxt = block_sum<kBlock>(cx, sm.sh);
-  const double cyt = block_sum<kBlock>(cy, sm.sh);
-  const double czt = block_sum<kBlock>(cz, sm.sh);
-  if (threadIdx.x == 0) {
-    const double inv = nm > 0 ? 1.0 / (2.0 * nm) : 0.0;
-    sm.sbuf[0] = cxt * inv;
-    sm.sbuf[1] = cyt * inv;
-    sm.sbuf[2] = czt * inv;
-  }
+  // the five sums in one pass (waves, then the wave partials in order, as
+  // block_sum does), then the bounding radius (max)
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  double v5[5] = {ew, dmz, cx, cy, cz};
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v5[i] += __shfl_down(v5[i], off, kWave);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < 5; i++) sm.sbuf[5 * wv + i] = v5[i];
   __syncthreads();
-  const double c0 = sm.sbuf[0], c1 = sm.sbuf[1], c2 = sm.sbuf[2];
-  __syncthreads();
+  double t5[5] = {0, 0, 0, 0, 0};
+  for (int w = 0; w < kBlock / kWave; w++)
+#pragma unroll
+    for (int i = 0; i < 5; i++) t5[i] += sm.sbuf[5 * w + i];
+  const double ewt = t5[0], dmzt = t5[1];
+  const double inv = nm > 0 ? 1.0 / (2.0 * nm) : 0.0;
+  const double c0 = t5[2] * inv, c1 = t5[3] * inv, c2 = t5[4] * inv;
   double rad = 0;
   for (int k = threadIdx.x; k < nm; k += kBlock) {
     const double4 pn = v.mv_new[k], po = v.mv_old[k];
@@ -234,11 +239,13 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
     const double dd = (po.x - c0) * (po.x - c0) + (po.y - c1) * (po.y - c1) + (po.z - c2) * (po.z - c2);
     rad = fmax(rad, fmax(dn, dd));
   }
-  sm.sbuf[threadIdx.x] = rad;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) rad = fmax(rad, __shfl_down(rad, off, kWave));
+  if (lane == 0) sm.sbuf[5 * (kBlock / kWave) + wv] = rad;
   __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sm.sbuf[threadIdx.x] = fmax(sm.sbuf[threadIdx.x], sm.sbuf[threadIdx.x + off]);
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    rad = 0;
+    for (int w = 0; w < kBlock / kWave; w++) rad = fmax(rad, sm.sbuf[5 * (kBlock / kWave) + w]);
   }
   if (threadIdx.x == 0) {
     PropInfo& pi = d.prop[p];
@@ -248,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
     pi.c[0] = c0;
     pi.c[1] = c1;
     pi.c[2] = c2;
-    pi.rad = sqrt(sm.sbuf[0]);
+    pi.rad = sqrt(rad);
     pi.forbid = sm.s_forbid;
     pi.n = nm;
     pi.mol = mol;
