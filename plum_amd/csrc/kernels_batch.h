@@ -98,6 +98,11 @@ __device__ __forceinline__ T load_const(const T* p, int i) {
 
 __device__ __forceinline__ int uload(const int* p) { return __builtin_amdgcn_readfirstlane(*p); }
 
+// Set bits of m below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 // System r owning virtual block v: first r with pre[r] > v (inclusive prefix).
 __device__ __forceinline__ int find_replica(const int* pre, int R, int v) {
   int lo = 0, hi = R - 1;
@@ -361,52 +366,66 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     }
     __syncthreads();
   }
-  if (t == 0) {
-    int ne = 0, ndec = np;
-    for (int s = 0; s < np; s++) {
-      const PropInfo& P = sm.s_prop[s];
-      plan->exact_of[s] = -1;
-      int pass = 1;
-      double des = 0;
-      if (rc.da && P.valid) {
-        // surrogate dE: exact spring and wall parts plus the network's estimate
-        // of the pair and k-space parts; first stage min(1, exp(-beta dE_s))
-        if (g.use_bond && sm.s_len[s] > 1) des += P.bond_trial - sm.s_bcur[s];
-        if (g.use_ext) des += P.forbid ? kVeryLargeEnergy : P.ew;
-        des += sm.s_nn[s];
-        pass = des < kVeryLargeEnergy && sm.s_u1[s] < exp(-g.beta * des);
-      }
-      plan->pass1[s] = pass;
-      plan->de_s[s] = des;
-      if (pass && P.valid) {
-        if (ne == kMaxProp) {
-          ndec = s;   // no exact slot left: this proposal is decided next batch
-          break;
-        }
-        plan->exact_of[s] = ne;
-        sm.s_screen[ne] = s;
-        ne++;
-      }
+  if (t < kWave) {
+    // one lane per proposal: the first stage, then exact slots in proposal
+    // order (a proposal past the last slot is decided next batch), then spare
+    // slots for first-stage rejections (training data), by ballots
+    const bool have = t < np;
+    PropInfo P = {};
+    if (have) P = sm.s_prop[t];
+    int pass = 1;
+    double des = 0;
+    if (have && rc.da && P.valid) {
+      // surrogate dE: exact spring and wall parts plus the network's estimate
+      // of the pair and k-space parts; first stage min(1, exp(-beta dE_s))
+      if (g.use_bond && sm.s_len[t] > 1) des += P.bond_trial - sm.s_bcur[t];
+      if (g.use_ext) des += P.forbid ? kVeryLargeEnergy : P.ew;
+      des += sm.s_nn[t];
+      pass = des < kVeryLargeEnergy && sm.s_u1[t] < exp(-g.beta * des);
     }
-    // spare exact slots: first-stage rejections, evaluated for training only
-    for (int s = 0; s < ndec && ne < rc.fill; s++)
-      if (plan->exact_of[s] < 0 && !plan->pass1[s] && sm.s_prop[s].valid && sm.s_prop[s].n > 0) {
-        plan->exact_of[s] = ne;
-        sm.s_screen[ne] = s;
-        ne++;
-      }
-    plan->n_decide = ndec;
-    plan->n_exact = ne;
-    int off = 0;
-    plan->prop_off[0] = 0;
-    for (int e = 0; e < ne; e++) {
-      plan->screen_of[e] = sm.s_screen[e];
-      const int n = sm.s_prop[sm.s_screen[e]].n;
-      off += n > 0 ? query_blocks(n, rc.qwave) + nmm_blocks(n) : 0;
-      plan->prop_off[e + 1] = off;
+    const unsigned long long ex = __ballot(have && pass && P.valid);
+    const int nex = __popcll(ex);
+    int ndec = np;
+    if (nex > kMaxProp) {
+      // no exact slot left for the (kMaxProp + 1)-th: decided next batch
+      unsigned long long m = ex;
+      for (int i = 0; i < kMaxProp; i++) m &= m - 1;
+      ndec = __builtin_ctzll(m);
     }
-    sm.s_ne = ne;
-    sm.s_off = off;
+    const int ne0 = min(nex, kMaxProp);
+    int exact = (have && pass && P.valid && t < ndec) ? lanes_below(ex) : -1;
+    const unsigned long long fl = __ballot(have && t < ndec && !pass && P.valid && P.n > 0);
+    const int nfill = max(0, min(__popcll(fl), rc.fill - ne0));
+    if (((fl >> t) & 1ull) && lanes_below(fl) < nfill) exact = ne0 + lanes_below(fl);
+    const int ne = ne0 + nfill;
+    if (have) {
+      plan->exact_of[t] = exact;
+      plan->pass1[t] = pass;
+      plan->de_s[t] = des;
+      if (exact >= 0) sm.s_screen[exact] = t;
+    }
+    wave_sync_lds();
+    // blocks per exact slot, prefix over the slots
+    int nb = 0;
+    if (t < ne) {
+      const int n = sm.s_prop[sm.s_screen[t]].n;
+      nb = n > 0 ? query_blocks(n, rc.qwave) + nmm_blocks(n) : 0;
+      plan->screen_of[t] = sm.s_screen[t];
+    }
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int u = __shfl_up(nb, o, kWave);
+      if (t >= o) nb += u;
+    }
+    if (t < ne) plan->prop_off[t + 1] = nb;
+    const int off = ne > 0 ? __shfl(nb, ne - 1, kWave) : 0;
+    if (t == 0) {
+      plan->prop_off[0] = 0;
+      plan->n_decide = ndec;
+      plan->n_exact = ne;
+      sm.s_ne = ne;
+      sm.s_off = off;
+    }
   }
   __syncthreads();
   // slot pairs whose moved beads can interact (bounding spheres within the
@@ -1013,10 +1032,6 @@ __device__ __forceinline__ double lane_double(double v, int lane) {
   const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, lane);
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), lane);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 // The reference's sequential Metropolis chain over the batch
