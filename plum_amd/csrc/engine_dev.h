@@ -111,6 +111,13 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;  // valid in thread 0
 }
 
+// Fixed-order wave reduction (shuffle down).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;  // valid in lane 0
+}
+
 // Orders one wave's LDS writes before its lanes read each other's entries.
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

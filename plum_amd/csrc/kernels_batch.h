@@ -11,23 +11,24 @@
 // fences, which on a multi-XCD part would write back and invalidate L2),
 // plus the surrogate's two between proposal and plan:
 //   k_propose_spec   grid (pmax, R): proposal s of system r (move number
-//                    counter + s), its phase tables, spring / wall / dipole
-//                    parts and bounding sphere (a molecule proposed twice is
-//                    proposed from its current coordinates the second time).
-//   k_plan_spec      grid R: first-stage decisions, exact slots, block
-//                    counts; pairs of exact slots within interaction range
-//                    get cross-term blocks.
+//                    counter + s), its spring / wall / dipole parts and
+//                    bounding sphere (a molecule proposed twice is proposed
+//                    from its current coordinates the second time).
+//   k_plan_spec      grid (R, 1 + pmax chunks): first-stage decisions, exact
+//                    slots, block counts; pairs of exact slots within
+//                    interaction range get cross-term blocks; beside it, the
+//                    proposals' phase tables in chunks of rows.
 //   k_compute_spec   all systems' k-space blocks (dS_e(k) of every exact slot,
 //                    dE_k per slot and the cross sums 2 c_k Re(dS_p* dS_q)),
-//                    real-space query blocks and moved-moved blocks per
-//                    slot, and cross blocks (LJ + real pairs between two
-//                    slots' moved beads, old/new combinations), spread over
-//                    one grid.
-//   k_reduce_spec    one block per sum and system, fixed order.
+//                    real-space query blocks (queries by waves) and
+//                    moved-moved blocks per slot, and cross blocks (LJ + real
+//                    pairs between two slots' moved beads, old/new
+//                    combinations), spread over one grid.
+//   k_reduce_spec    one wave per sum and system, fixed order.
 //   k_resolve_spec   grid R: the reference's sequential Metropolis chain over
-//                    the batch (simulation.cc:241-355), each dE corrected by
-//                    the cross terms of the proposals accepted before it,
-//                    then the commits.
+//                    the batch (simulation.cc:241-355), decided by one wave,
+//                    each dE corrected by the cross terms of the proposals
+//                    accepted before it, then the commits.
 // Every proposal is evaluated against the state before the batch; accepting
 // p changes q's dE by exactly the cross terms, so the chain is the sequential
 // one (the random numbers depend only on the move number).  A second
@@ -929,12 +930,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
 }
 
-// Sums of a batch, one block per (sum, system), fixed order: blockIdx.x in
+// Sums of a batch, one wave per (sum, system), fixed order: blockIdx.x in
 // [0, kKVals) k-space sums, then kMaxProp (LJ, real) sums per exact slot
 // (real-space query blocks + moved-moved blocks), then kMaxPairs cross sums.
-__global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
-                                                        const RepCfg* __restrict__ RC, int R) {
-  __shared__ double sh[kBlock / kWave];
+__global__ __launch_bounds__(kWave) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
+                                                       const RepCfg* __restrict__ RC, int R) {
   const int j = blockIdx.x, r = blockIdx.y;
   if (r >= R) return;
   const Dev& d = D[r];
@@ -942,16 +942,17 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
   SpecPlan* plan = d.plan;
   if (plan->n_prop == 0) return;
   const int ne = plan->n_exact;
+  const int t = threadIdx.x;
   if (j < kKVals) {
     if (j >= ne && (j < kMaxProp || j >= kMaxProp + ne * (ne - 1) / 2)) return;
     if (rc.nkb_s == 0) {
-      if (threadIdx.x == 0) plan->ksum[j] = 0;
+      if (t == 0) plan->ksum[j] = 0;
       return;
     }
     double s = 0;
-    for (int i = threadIdx.x; i < rc.nkb_s; i += kBlock) s += d.part_ks[(size_t)j * rc.nkb_s + i];
-    const double t = block_sum<kBlock>(s, sh);
-    if (threadIdx.x == 0) plan->ksum[j] = t;
+    for (int i = t; i < rc.nkb_s; i += kWave) s += d.part_ks[(size_t)j * rc.nkb_s + i];
+    s = wave_sum(s);
+    if (t == 0) plan->ksum[j] = s;
   } else if (j < kKVals + kMaxProp) {
     const int e = j - kKVals;
     if (e >= ne) return;
@@ -961,29 +962,29 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;
-    for (int i = threadIdx.x; i < nreal; i += kBlock) {
+    for (int i = t; i < nreal; i += kWave) {
       lj += pr[i].x;
       re += pr[i].y;
     }
-    for (int i = threadIdx.x; i < nmm; i += kBlock) {
+    for (int i = t; i < nmm; i += kWave) {
       lj += pm[i].x;
       re += pm[i].y;
     }
-    const double a = block_sum<kBlock>(lj, sh);
-    const double c = block_sum<kBlock>(re, sh);
-    if (threadIdx.x == 0) plan->rsum[e] = make_double2(a, c);
+    lj = wave_sum(lj);
+    re = wave_sum(re);
+    if (t == 0) plan->rsum[e] = make_double2(lj, re);
   } else {
     const int jj = j - kKVals - kMaxProp;
     if (jj >= plan->n_near) return;
     const int off = plan->pair_off[jj], nblk = plan->pair_off[jj + 1] - off;
     double lj = 0, re = 0;
-    for (int i = threadIdx.x; i < nblk; i += kBlock) {
+    for (int i = t; i < nblk; i += kWave) {
       lj += d.part_x[off + i].x;
       re += d.part_x[off + i].y;
     }
-    const double a = block_sum<kBlock>(lj, sh);
-    const double c = block_sum<kBlock>(re, sh);
-    if (threadIdx.x == 0) plan->xsum[jj] = make_double2(a, c);
+    lj = wave_sum(lj);
+    re = wave_sum(re);
+    if (t == 0) plan->xsum[jj] = make_double2(lj, re);
   }
 }
 

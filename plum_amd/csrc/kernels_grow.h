@@ -246,12 +246,6 @@ __global__ __launch_bounds__(kBlock) void k_grow_trials(Geo g, Dev d, const Grow
 
 constexpr int kSelThreads = 1024;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-  return v;  // lane 0
-}
-
 // One block: each trial's energy from the partial sums (part 1 per split,
 // parts 2-4, k space), the weights, the Rosenbluth update and the choice.
 __global__ __launch_bounds__(kSelThreads) void k_grow_select(Geo g, Dev d, const GrowIn* __restrict__ inp,
