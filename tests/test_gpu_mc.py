@@ -83,6 +83,33 @@ def test_speculative_batches_follow_the_sequential_chain(name, moves):
             assert abs(tp[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, tp[k], full[k])
 
 
+@pytest.mark.parametrize("pmax", [1, 16])
+def test_queries_shared_by_several_waves_follow_the_same_chain(pmax):
+    """Real-space queries spread over several waves (a small split_div: one
+    wave per share, four times as many in a batch of fewer than 16
+    proposals) make the same chain as one wave per query: same decisions,
+    positions equal to round-off, running totals equal to a from-scratch
+    evaluation."""
+    sd = systems.config("c0")
+    kw = dict(move_size=sd["move_size"], seed=31, bond_len=1.0)
+    runs = []
+    for div in (1536, 64):
+        eng = E.Engine(sd)
+        eng.set_tuning(pmax=pmax, split_div=div)
+        st = eng.mc_run(1500, sd["move_prob"], **kw)
+        runs.append((eng, st))
+    (e1, s1), (e2, s2) = runs
+    print("pmax %d: accepted %s / %s" % (pmax, list(s1.accepted), list(s2.accepted)), flush=True)
+    assert list(s1.attempted) == list(s2.attempted)
+    assert list(s1.accepted) == list(s2.accepted)
+    assert np.allclose(e1.positions(), e2.positions(), rtol=0, atol=1e-9)
+    run, full = e2.totals(), e2.energy_full()
+    for k in TERMS:
+        if k == "dipole":
+            continue
+        assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, run[k], full[k])
+
+
 def test_device_loop_is_deterministic():
     sd = systems.config("small")
     out = []
