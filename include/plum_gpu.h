@@ -193,8 +193,9 @@ typedef struct {
   int blocks_per_cu;     /* k_compute_spec work-groups per compute unit */
   int split_div;         /* candidate slots per share of a real-space query: one
                             block per share on the one-move path, one wave per share
-                            in speculative batches (one wave when a query has fewer
-                            than 1.5 split_div slots) */
+                            in speculative batches (four when a batch holds fewer than
+                            16 proposals; one wave in all when a query has fewer than
+                            1.5 split_div slots) */
   int batches_per_graph; /* speculative batches captured in one graph (also
                             the cell compaction period) */
   int pmax;              /* proposals per speculative batch, 1..16 (1..32 screened with the surrogate) */
