@@ -95,6 +95,77 @@ def real_space_work(sd, info):
     return rho * float(np.prod(2 * rs + h))
 
 
+def real_space_union(sd, info, pos, n_sample=24, seed=3):
+    """Distinct candidate slots one trial move's real-space queries read: the
+    beads in the union of the cells the moved beads' queries visit (cells
+    whose box lies within the search radius of a moved bead), averaged over
+    sampled chains and the move mix (COM, pivot and reptation move the whole
+    chain, crankshaft about a third of it)."""
+    L = np.array(info.box_e)
+    nc = np.array(info.n_cells)
+    h = L / nc
+    rs = max(info.real_cutoff, 2.5)
+    w = pos - L * np.floor(pos / L)
+    c3 = np.minimum((w / h).astype(int), nc - 1)
+    flat = (c3[:, 0] * nc[1] + c3[:, 1]) * nc[2] + c3[:, 2]
+    counts = np.bincount(flat, minlength=int(np.prod(nc)))
+    ms = sd["mol_start"]
+    chains = [m for m in range(len(ms) - 1) if ms[m + 1] - ms[m] > 1]
+    rng = np.random.default_rng(seed)
+    span = [np.arange(-int(np.ceil(rs / h[k])) - 1, int(np.ceil(rs / h[k])) + 2) for k in range(3)]
+    off = np.stack(np.meshgrid(*span, indexing="ij"), -1).reshape(-1, 3)
+
+    def union(beads):
+        cells = set()
+        for b in beads:
+            u = np.floor(w[b] / h).astype(int) + off
+            lo = u * h
+            dd = np.maximum(0.0, np.maximum(lo - w[b], w[b] - (lo + h)))
+            keep = u[(dd * dd).sum(1) <= rs * rs]
+            cc = keep % nc
+            cells.update(((cc[:, 0] * nc[1] + cc[:, 1]) * nc[2] + cc[:, 2]).tolist())
+        return int(counts[list(cells)].sum())
+
+    mp = sd["move_prob"]
+    whole, third = [], []
+    for m in rng.choice(chains, size=min(n_sample, len(chains)), replace=False):
+        a, b = ms[m], ms[m + 1]
+        whole.append(union(range(a, b)))
+        k = max((b - a) // 3, 1)
+        s0 = a + (b - a - k) // 2
+        third.append(union(range(s0, s0 + k)))
+    return ((mp[1] + mp[2] + mp[4]) * np.mean(whole) + mp[3] * np.mean(third)) / max(sum(mp[1:]), 1e-12)
+
+
+def fp32_spec_errors(sd, moves=300):
+    """Per-term dE error of the FP32-pair speculative kernel (k_compute_spec<1>,
+    the timed loop): its recorded decisions replayed through the FP64 host path
+    on the same trial coordinates; error / max(|dE|, 1 kT) and in kT."""
+    e32 = E.Engine(sd, precision=E.PLUM_FP32_PAIR)
+    e32.mc_run(200, seed=5, **mc_kwargs(sd))
+    start = e32.positions()
+    e32.mc_record(moves + 16)
+    e32.mc_run(moves, seed=6, **mc_kwargs(sd))
+    recs, _ = e32.mc_records(moves + 16)
+    e32.close()
+    e64 = E.Engine(dict(sd, xyz=start), precision=E.PLUM_FP64)
+    worst = {k: 0.0 for k in E.TERMS}
+    worst_abs = {k: 0.0 for k in E.TERMS}
+    for r in recs:
+        if not r["evaluated"]:
+            continue
+        t64, d64 = e64.delta(r["mol"], r["idx"], r["trial"])
+        if t64 < 1e8 and r["total"] < 1e8:
+            for k in E.TERMS:
+                err = abs(r["dE"][k] - d64[k])
+                worst_abs[k] = max(worst_abs[k], err)
+                worst[k] = max(worst[k], err / max(abs(d64[k]), 1.0))
+        e64.finalize(r["accepted"])
+    e64.close()
+    return ({k: float("%.3g" % v) for k, v in worst.items()},
+            {k: float("%.3g" % v) for k, v in worst_abs.items()}, len(recs))
+
+
 def cpu_baseline(sd, target_s=15.0):
     """The reference algorithm (oracle, FP64, one core): pairwise real space
     over all images plus the pairwise full-grid k-space sum, for ONE moved
@@ -442,6 +513,10 @@ def main():
     per_move = (kflop + rflop) / 1e12
     achieved = stp.moves * per_move / (avg_us * launches * 1e-6) if avg_us > 0 else 0.0
     per_launch = stp.moves * per_move / max(launches, 1)
+    # algorithmic HBM bytes per move: k space 40 N_k (S(k) read, dS write,
+    # c_k read), real space 20 B per distinct candidate slot
+    n_union = real_space_union(sd, info, eng.positions())
+    alg_bytes = 40.0 * info.n_k_half + 20.0 * n_union
 
     traffic, tinfo = measured_traffic()
     out = {
@@ -455,10 +530,15 @@ def main():
                    "particles": n, "moves_per_step": a.moves_per_step},
         "accept_ratio": acc_ratio,
         "kernel_avg_us": {k: round(v[0], 3) for k, v in kt.items()},
-        "roofline": {"bound": "mfma", "pipe": "FP64 VALU (dense FP64 peak)", "kernel": rl_name,
+        "roofline": {"bound": "valu", "pipe": "FP64 VALU (dense FP64 vector peak)", "kernel": rl_name,
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "work_per_launch": per_launch, "moves_per_launch": stp.moves / max(launches, 1),
+                     "algorithmic_bytes_per_move": alg_bytes,
+                     "algorithmic_bytes_basis": "40 B x %d k + 20 B x %.0f distinct candidate slots" % (
+                         info.n_k_half, n_union),
+                     "hbm_achieved_gbs": alg_bytes * stp.moves / (avg_us * launches * 1e-6) / 1e9
+                     if avg_us > 0 else 0.0,
                      "basis": "per move: k-space 16 x %.1f moved beads x %d k + 8 x %d k; real "
                               "space %.0f FLOP x 2 positions x %.1f moved beads x %.0f candidate "
                               "pair-images; x moves per launch" % (
@@ -479,6 +559,10 @@ def main():
         out["fp32pair_kernel_avg_us"] = {k: round(v[0], 3) for k, v in kt32.items()}
         e32.close()
     eng.close()
+    rel, ab, nrec = fp32_spec_errors(sd)
+    out["dE_err_fp32pair_spec_kernel"] = {
+        "rel_to_max_dE_1kT": rel, "abs_kT": ab, "decisions": nrec,
+        "basis": "k_compute_spec<1> recorded decisions replayed through the FP64 host path"}
     out["dE_rel_err_fp32pair_vs_fp64"] = fp32_term_errors(sd)
     if not a.no_brush:
         out["brush_delayed_acceptance"] = brush_leg(a.brush_moves)

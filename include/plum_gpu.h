@@ -4,8 +4,11 @@
  * Drop-in boundary for the energy calls of nuwapi/Plum's ForceField
  * (src/force_field/force_field.h:207-294).  Plain C: opaque context, plain
  * pointers and sizes, status-code returns (0 = ok, < 0 = error, message via
- * plum_last_error), no exceptions across the boundary, one HIP stream per
- * context.  Inputs stay caller-owned; device buffers are context-owned.
+ * plum_last_error), no exceptions across the boundary.  A context owns its
+ * HIP streams: one for the host-path calls and, once the device loop or the
+ * surrogate is used, a second for those; work queued on the first is ordered
+ * before the device loop, and every call that returns results synchronizes.
+ * Inputs stay caller-owned; device buffers are context-owned.
  *
  * Units and conventions are the reference's: kT = 1 energy unit, lengths in
  * sigma, `alpha` is the Ewald kappa^2 (sigma^-2), the Ewald cutoffs are
@@ -171,6 +174,37 @@ typedef struct {
    moved sets), so the chain is the sequential one. */
 int plum_mc_run(plum_ctx* ctx, const plum_mc_params* mc, long long n_moves,
                 plum_mc_stats* stats);
+
+/* One decided proposal of plum_mc_run: what Simulation::TranslationalMove's
+   Metropolis test (simulation.cc:325-334) was given -- the per-term dE of
+   ForceField::EnergyDifference (force_field.cc:407-434) with the exact cross
+   terms of the proposals accepted before it in the batch, and the reference's
+   early exits -- and its decision.  For replay checks of the device loop. */
+typedef struct {
+  unsigned long long move;   /* move number (Philox counter) */
+  int mol, n_moved, move_type, accepted;
+  int evaluated;             /* the exact dE was computed (0: a first-stage
+                                rejection of delayed acceptance) */
+  int early;                 /* an early exit (LJ or wall >= 1e8) */
+  int first_stage_pass;      /* 1 without the surrogate */
+  int lo, hi;                /* moved index range */
+  plum_terms dE;
+  double total;
+  double dE_surrogate;       /* first-stage dE (0 without the surrogate) */
+} plum_decision;
+
+/* Arms a bounded record of the next `capacity` proposals plum_mc_run decides
+   on this context (0 disarms); the count restarts at 0.  No reference
+   counterpart. */
+int plum_mc_record(plum_ctx* ctx, int capacity);
+/* Copies up to `cap` recorded decisions in decision order; idx / xyz
+   receive each one's moved bead indices and trial coordinates in slots of
+   plum_max_mol_len(ctx) beads (cap * that many ints / 3 doubles; unused
+   slots -1 / 0).  n_decided (may be NULL) receives the number of proposals
+   decided since the record was armed, which may exceed what was kept.
+   Returns the number of records copied or a negative status. */
+int plum_mc_get_records(plum_ctx* ctx, int cap, plum_decision* out, int* idx, double* xyz,
+                        long long* n_decided);
 
 /* Average device duration (microseconds, HIP events on the context stream)
    of each engine kernel over the last plum_mc_run with profiling enabled.

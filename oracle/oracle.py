@@ -138,6 +138,8 @@ def lib():
         L.orc_gc_widom.restype = C.c_double
         L.orc_gasdev.argtypes = [C.c_double, C.c_double, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
         L.orc_gasdev.restype = C.c_double
+        L.orc_random_bond_len.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
+        L.orc_random_bond_len.restype = C.c_double
         L.orc_translational_move.argtypes = [C.c_void_p, C.POINTER(OrcMT), dp, C.c_double,
                                              C.c_int, C.c_int, C.c_double, C.c_int,
                                              C.POINTER(OrcStepInfo)]
@@ -289,6 +291,12 @@ class Oracle:
         lo, hi = exclude_mols if exclude_mols is not None else (0, -1)
         return lib().orc_cbmc_beads_energy(self.h, _dp(p1), int(t1), float(q1), _dp(p2), int(t2),
                                            float(q2), int(ch), _dp(gc), _dp(gi), cur, lo, hi)
+
+    def random_bond_lengths(self, rng, gauss, n):
+        """n draws of PotentialSpring::RandomBondLen on the MT19937 `rng`
+        with gasdev state `gauss` (an OrcGauss)."""
+        f = lib().orc_random_bond_len
+        return np.array([f(self.h, C.byref(rng.s), C.byref(gauss)) for _ in range(n)])
 
     def rg(self):
         return lib().orc_radius_of_gyration(self.h)

@@ -358,6 +358,9 @@ orc_sys* orc_create_ex(const orc_params* p, int n, const double* xyz,
   /* ForceField::InitializeEnergy -> running totals (force_field.cc:365-381) */
   double t[ORC_NTERMS] = {0, 0, 0, 0, 0, 0, 0};
   if (init_energy) orc_energy_full(s, t);
+  /* the dipole term is O(N): set even when the pair sums are skipped, so
+     the lag state of a replay starts where the reference's would */
+  else if (p->use_ewald && p->dipole) t[ORC_T_DIPOLE] = dipole_energy_current(s);
   s->E_lj = t[ORC_T_LJ];
   s->E_real = t[ORC_T_REAL];
   s->E_kspace = t[ORC_T_KSPACE];
@@ -568,17 +571,42 @@ static double ewald_delta(orc_sys* s, int mol, double* d_real, double* d_repl,
         dr += nr - orl;
         dk += nk - ok;
       }
-  for (int m = 0; m < s->n_mols; m++) {
-    if (m == mol) continue;
-    for (int j = s->mol_start[m]; j < s->mol_start[m + 1]; j++)
-      for (int k = a; k < b; k++)
-        if (s->moved[k]) {
-          double nr = real_ij(s, k, 1, j, 1), nk = repl_ij(s, k, 1, j, 1);
-          double orl = real_ij(s, k, 0, j, 0), ok = repl_ij(s, k, 0, j, 0);
-          dE += (nr + nk - (orl + ok));
-          dr += nr - orl;
-          dk += nk - ok;
-        }
+  /* moved beads x the other molecules: the pair values are evaluated in
+     parallel into a buffer (OpenMP over the partner beads) and then summed
+     serially in the reference's loop order, so the result is the same,
+     bit for bit, as the serial loop's */
+  int nmv = 0;
+  for (int k = a; k < b; k++) nmv += s->moved[k] ? 1 : 0;
+  const int n_out = s->n - (b - a);
+  if (nmv > 0 && n_out > 0) {
+    int* mk = (int*)malloc(sizeof(int) * nmv);
+    double* pv = (double*)malloc(sizeof(double) * 4 * (size_t)nmv * n_out);
+    int c = 0;
+    for (int k = a; k < b; k++)
+      if (s->moved[k]) mk[c++] = k;
+#pragma omp parallel for schedule(static)
+    for (int jj = 0; jj < n_out; jj++) {
+      const int j = jj < a ? jj : jj + (b - a);
+      for (int u = 0; u < nmv; u++) {
+        double* o = &pv[4 * ((size_t)jj * nmv + u)];
+        o[0] = real_ij(s, mk[u], 1, j, 1);
+        o[1] = repl_ij(s, mk[u], 1, j, 1);
+        o[2] = real_ij(s, mk[u], 0, j, 0);
+        o[3] = repl_ij(s, mk[u], 0, j, 0);
+      }
+    }
+    /* molecule order m = 0..n_mols-1 without mol is bead order without
+       [a, b), since molecules are contiguous and ordered */
+    for (int jj = 0; jj < n_out; jj++)
+      for (int u = 0; u < nmv; u++) {
+        const double* o = &pv[4 * ((size_t)jj * nmv + u)];
+        double nr = o[0], nk = o[1], orl = o[2], ok = o[3];
+        dE += (nr + nk - (orl + ok));
+        dr += nr - orl;
+        dk += nk - ok;
+      }
+    free(pv);
+    free(mk);
   }
   for (int i = a; i < b; i++)
     if (s->moved[i]) {
@@ -1105,6 +1133,10 @@ static double random_bond_len(const orc_sys* s, orc_mt19937* r, orc_gauss* gs) {
     ready = u01(r) < (len * len / a);
   }
   return len;
+}
+
+double orc_random_bond_len(orc_sys* s, orc_mt19937* r, orc_gauss* gs) {
+  return random_bond_len(s, r, gs);
 }
 
 /* ForceField::UpdateMolCounts (force_field.cc:1258-1277) */

@@ -187,6 +187,9 @@ int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc,
    insertions (mu_tot_ins, 50 in the reference). */
 double orc_gc_widom(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, int n_ins);
 double orc_gasdev(double mean, double stdev, orc_mt19937* r, orc_gauss* gs);
+/* PotentialSpring::RandomBondLen (potential_spring.cc:50-61) with the
+   system's k_bond, r0 and beta. */
+double orc_random_bond_len(orc_sys* s, orc_mt19937* r, orc_gauss* gs);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,10 @@ struct plum_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int g_batches = 0, g_grid = 0, g_pmax = 0;  // shape the graph was captured with
+  // device pointers the captured kernels received as arguments: the
+  // surrogate's BF16 network and the descriptor rows / outputs (they change
+  // when the surrogate is enabled again or the row buffers grow)
+  const void *g_net = nullptr, *g_sx = nullptr, *g_sf = nullptr;
   bool da = false;        // delayed acceptance (the first system's surrogate, shared)
   // descriptor rows and network outputs of every system's proposals, one
   // slice of 2 kMaxScreen maxm rows per system (R == 1: the surrogate's own)
@@ -235,6 +239,22 @@ void batch_drop_graph(plum_batch* b) {
   b->gexec = nullptr;
   b->graph = nullptr;
   b->g_batches = b->g_grid = b->g_pmax = 0;
+  b->g_net = b->g_sx = b->g_sf = nullptr;
+}
+
+// The surrogate network a batch's launches read (null without delayed
+// acceptance).
+const void* batch_net(const plum_batch* b) {
+  return b->da && b->reps[0]->sur ? (const void*)b->reps[0]->sur->net : nullptr;
+}
+
+// True when the captured graph no longer matches what launch_batch would
+// launch now: its shape or any device pointer baked into its arguments.
+bool graph_stale(const plum_batch* b, int nb) {
+  const int key_pmax = b->tune.pmax + (b->da ? 1000 : 0);
+  return !b->gexec || b->g_batches != nb || b->g_grid != batch_grid(b) || b->g_pmax != key_pmax ||
+         b->g_net != batch_net(b) || b->g_sx != (b->da ? (const void*)b->sx : nullptr) ||
+         b->g_sf != (b->da ? (const void*)b->sf : nullptr);
 }
 
 // `nb` batches of every system, then compaction and the status gather.  With
@@ -274,8 +294,7 @@ int batch_replay(plum_batch* b, int nb, bool use_graph, plum_ctx* prof) {
     HIPCHK(hipGetLastError());
     return PLUM_OK;
   }
-  const int key_pmax = b->tune.pmax + (b->da ? 1000 : 0);
-  if (!b->gexec || b->g_batches != nb || b->g_grid != batch_grid(b) || b->g_pmax != key_pmax) {
+  if (graph_stale(b, nb)) {
     batch_drop_graph(b);
     HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < nb; i++) launch_batch(b);
@@ -286,7 +305,10 @@ int batch_replay(plum_batch* b, int nb, bool use_graph, plum_ctx* prof) {
     HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     b->g_batches = nb;
     b->g_grid = batch_grid(b);
-    b->g_pmax = key_pmax;
+    b->g_pmax = b->tune.pmax + (b->da ? 1000 : 0);
+    b->g_net = batch_net(b);
+    b->g_sx = b->da ? b->sx : nullptr;
+    b->g_sf = b->da ? b->sf : nullptr;
   }
   HIPCHK(hipGraphLaunch(b->gexec, b->st));
   return PLUM_OK;
@@ -387,6 +409,13 @@ int batch_run(plum_batch* b, const plum_mc_params* mcp, long long n_moves, bool 
   const int R = b->R;
   for (int r = 0; r < R; r++)
     if (b->reps[r]->pending) return fail(PLUM_E_STATE, "a system has a pending trial move");
+  // work still queued on a system's own stream (a rollback's commit kernel,
+  // a host-path upload) comes before this batch's kernels on the batch stream
+  for (int r = 0; r < R; r++) {
+    plum_ctx* c = b->reps[r];
+    HIPCHK(hipEventRecord(c->ev_idle, c->st));
+    HIPCHK(hipStreamWaitEvent(b->st, c->ev_idle, 0));
+  }
   b->mc.assign(R, McDev{});
   for (int r = 0; r < R; r++) CHK(mc_setup(b->reps[r], &mcp[r], b->st, &b->mc[r]));
   std::vector<unsigned long long> m0(R);
@@ -536,6 +565,78 @@ int plum_mc_run(plum_ctx* c, const plum_mc_params* mcp, long long n_moves, plum_
   CHK(batch_run(c->self, mcp, n_moves, true, c->profiling ? c : nullptr, nullptr, &st));
   if (stats) *stats = st;
   return PLUM_OK;
+}
+
+int plum_mc_record(plum_ctx* c, int capacity) {
+  if (!c) return fail(PLUM_E_ARG, "null context");
+  if (capacity < 0 || capacity > (1 << 24)) return fail(PLUM_E_ARG, "record capacity must be 0..2^24");
+  if (capacity == 0) {
+    c->d.drec = nullptr;
+    c->d.drec_pos = nullptr;
+    c->d.drec_idx = nullptr;
+    c->d.drec_ctl = nullptr;
+    return PLUM_OK;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (c->self) HIPCHK(hipStreamSynchronize(c->self->st));
+  if (capacity > c->drec_cap || !c->d.drec_ctl) {
+    Dev& d = c->d;
+    CHK(dalloc(c, &d.drec, (size_t)capacity));
+    CHK(dalloc(c, &d.drec_pos, (size_t)capacity * c->maxm));
+    CHK(dalloc(c, &d.drec_idx, (size_t)capacity * c->maxm));
+    CHK(dalloc(c, &d.drec_ctl, 2));
+    c->drec_cap = capacity;
+  }
+  const int ctl[2] = {0, capacity};
+  HIPCHK(hipMemcpyAsync(c->d.drec_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return PLUM_OK;
+}
+
+int plum_mc_get_records(plum_ctx* c, int cap, plum_decision* out, int* idx, double* xyz, long long* n_decided) {
+  if (!c) return fail(PLUM_E_ARG, "null context");
+  if (!c->d.drec_ctl) return fail(PLUM_E_STATE, "decision record not armed (plum_mc_record)");
+  if (cap < 0 || (cap > 0 && (!out || !idx || !xyz))) return fail(PLUM_E_ARG, "null output");
+  int ctl[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(ctl, c->d.drec_ctl, sizeof(ctl), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (n_decided) *n_decided = ctl[0];
+  const int n = std::min(std::min(ctl[0], ctl[1]), cap);
+  if (n <= 0) return 0;
+  const int M = c->maxm;
+  std::vector<DecRec> r(n);
+  std::vector<double4> pos((size_t)n * M);
+  std::vector<int> id((size_t)n * M);
+  HIPCHK(hipMemcpyAsync(r.data(), c->d.drec, sizeof(DecRec) * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(pos.data(), c->d.drec_pos, sizeof(double4) * pos.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(id.data(), c->d.drec_idx, sizeof(int) * id.size(), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (int k = 0; k < n; k++) {
+    plum_decision& o = out[k];
+    const DecRec& s = r[k];
+    o.move = s.move;
+    o.mol = s.mol;
+    o.n_moved = s.n;
+    o.move_type = s.type;
+    o.accepted = s.accepted;
+    o.evaluated = s.evaluated;
+    o.early = s.early;
+    o.first_stage_pass = s.pass;
+    o.lo = s.lo;
+    o.hi = s.hi;
+    to_terms(s.terms, &o.dE);
+    o.total = s.total;
+    o.dE_surrogate = s.de_s;
+    for (int m = 0; m < M; m++) {
+      const size_t j = (size_t)k * M + m;
+      const bool in = m < s.n;
+      idx[j] = in ? id[j] : -1;
+      xyz[3 * j] = in ? pos[j].x : 0.0;
+      xyz[3 * j + 1] = in ? pos[j].y : 0.0;
+      xyz[3 * j + 2] = in ? pos[j].z : 0.0;
+    }
+  }
+  return n;
 }
 
 int plum_mc_get_tuning(plum_ctx* c, plum_batch_tuning* out) {

@@ -68,6 +68,8 @@ struct plum_ctx {
   Ctl* h_ctl = nullptr;  // pinned copy of the control block
   size_t in_bytes = 0, off_xyz = 0;
   char* d_in = nullptr;
+  int drec_cap = 0;      // decision-record capacity allocated (plum_mc_record)
+  hipEvent_t ev_idle = nullptr;  // marks the end of the work queued on st
   bool pending = false;
   bool profiling = false;
   std::vector<void*> dev_allocs;
@@ -387,6 +389,12 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   double sig_max = 0;
   for (int t = 0; t < p.n_types; t++) sig_max = std::max(sig_max, p.lj_sigma[t]);
   g.fp64_below = 1.3 * std::max(sig_max, 0.0);
+  // explicit pair sums need only the minimum image when no other image can
+  // reach rc along a periodic edge (edges longer than 2 rc); a non-periodic z
+  // is checked per pair (real_pair)
+  g.single_image = p.use_ewald ? 1 : 0;
+  for (int k = 0; k < p.npbc && p.use_ewald; k++)
+    if (!(2 * g.rc * (1 + 1e-9) < g.L[k])) g.single_image = 0;
   g.rs = std::max(g.rc, lj_max);
   if (g.rs <= 0) g.rs = 1.0;
   g.rs2 = g.rs * g.rs;
@@ -459,6 +467,11 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   if (hipStreamCreate(&c->st) != hipSuccess) {
     delete c;
     return fail(PLUM_E_HIP, "hipStreamCreate failed");
+  }
+  if (hipEventCreateWithFlags(&c->ev_idle, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return fail(PLUM_E_HIP, "hipEventCreate failed");
   }
   Dev& d = c->d;
   std::vector<double> hx(n), hy(n), hz(n);
@@ -550,13 +563,21 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
 
 int plum_destroy(plum_ctx* c) {
   if (!c) return PLUM_OK;
-  if (c->self) plum_batch_destroy(c->self);
-  if (c->sur) sur_free(c->sur);
-  if (c->cbmc) cbmc_free(c->cbmc);
+  // Teardown order: drain the context stream first (a rollback's commit
+  // kernel may still be queued on it), then the device-loop batch (it drains
+  // its own stream before destroying its graph), then the growth graph and
+  // the surrogate, and only then the buffers those graphs and kernels read.
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->self) plum_batch_destroy(c->self);
+  c->self = nullptr;
+  if (c->cbmc) cbmc_free(c->cbmc);
+  c->cbmc = nullptr;
+  if (c->sur) sur_free(c->sur);
+  c->sur = nullptr;
   for (void* p : c->dev_allocs) (void)hipFree(p);
   if (c->h_in) (void)hipHostFree(c->h_in);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+  if (c->ev_idle) (void)hipEventDestroy(c->ev_idle);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return PLUM_OK;

@@ -50,6 +50,7 @@ struct Geo {
   double wall_r3_cut;   // pow(1/wall_cut, 3), host-evaluated
   int dipole, dipole_exact;
   double fp64_below;    // FP32 path: pairs closer than this use FP64 math
+  int single_image;     // only the minimum image can lie within rc (real_pair)
 };
 
 // LJ pair table entry for (ti, tj): sigma, epsilon, cutoff distance (strict
@@ -153,6 +154,15 @@ __device__ __forceinline__ double bb_dist(const double* a, const double* b,
   return sqrt(s);
 }
 
+// Slab dipole energy 2 pi l_B M_z^2 / V (PotentialEwaldCoul::DipoleE,
+// potential_ewald_coul.cc:415-428) with coef = 2 pi l_B / V.  Each product is
+// rounded on its own (no fused multiply-add with a later subtraction), so an
+// unchanged M_z gives the same energy bit for bit, as the reference's
+// repeated evaluation does, and the lagged dE of such a move is exactly 0.
+__device__ __forceinline__ double dipole_energy(double coef, double mz) {
+  return __dmul_rn(__dmul_rn(coef, mz), mz);
+}
+
 // Truncated-shifted LJ / WCA at distance r (potential_truncated_lj.cc:49-85).
 __device__ __forceinline__ double lj_energy(double r, const LJPair& p) {
   if (r <= 0) return kVeryLargeEnergy;
@@ -181,6 +191,25 @@ __device__ inline double real_pair_ref(const double* a, double qa, const double*
         double r = sqrt(x * x + y * y + z * z);
         if (r > 0 && r <= g.rc) e += pref * erfc(g.sqrt_alpha * r) / r;
       }
+  return e;
+}
+
+// The same pair sum when only the minimum image can lie within rc: every
+// periodic edge is longer than 2 rc (g.single_image), and along a
+// non-periodic z the separation is shorter than the padded period minus rc.
+// It is then the (0, 0, 0) term of the image loop above, equal to that loop
+// bit for bit since every other image falls beyond rc.
+__device__ __forceinline__ double real_pair(const double* a, double qa, const double* b, double qb,
+                                           const Geo& g) {
+  if (qa * qb == 0) return 0.0;
+  double d[3];
+  min_image(a, b, g.L, g.npbc, d);
+  if (!g.single_image || (g.npbc < 3 && !(fabs(d[2]) + g.rc * (1 + 1e-9) < g.L[2])))
+    return real_pair_ref(a, qa, b, qb, g);
+  const double pref = g.lB * qa * qb;
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  double e = 0;
+  if (r > 0 && r <= g.rc) e += pref * erfc(g.sqrt_alpha * r) / r;
   return e;
 }
 

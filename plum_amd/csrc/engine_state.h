@@ -64,6 +64,18 @@ struct SpecPlan {
   double2 xsum[kMaxPairs];       // (LJ, real) cross term per near pair
 };
 
+// One decided proposal of the device loop (bounded record, for replay
+// checks): what the Metropolis test was given and what it decided.
+struct DecRec {
+  unsigned long long move;   // move number (Philox counter) of the proposal
+  int mol, n, type, accepted;
+  int lo, hi, contig, evaluated;   // evaluated: the exact dE was computed
+  int early, pass, pad0, pad1;     // early: an early exit (1e8); pass: first stage passed
+  double terms[8];           // per-term dE, cross terms of earlier acceptances included
+  double total;              // dE with the reference's early exits
+  double de_s;               // surrogate dE of the first stage (0 without it)
+};
+
 // Control block: scalars of the energy state and the run.
 struct Ctl {
   double Mz;            // sum q z over current coordinates
@@ -148,6 +160,13 @@ struct Dev {
   double* tr_target;
   int* tr_n;
   int* tr_ctl;            // [0] samples written (ring head), [1] ring capacity
+  // decision record (null when off): record k holds proposal metadata and
+  // its maxm trial positions / bead indices at k * maxm; rec_ctl[0] counts
+  // decided proposals since the record was armed, rec_ctl[1] is the capacity
+  DecRec* drec;
+  double4* drec_pos;
+  int* drec_idx;
+  int* drec_ctl;
 };
 
 // Parameters of the device-resident loop, by value.

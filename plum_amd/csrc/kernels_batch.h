@@ -852,8 +852,8 @@ __device__ void cross_block(const Geo& g, const Dev& d, const RepCfg& rc, int sp
             lj_energy(bb_dist(an, bo, g.Lsim, g.npbc), lp) + lj_energy(bb_dist(ao, bo, g.Lsim, g.npbc), lp);
     }
     if (g.use_ewald && a0.w * b0.w != 0)
-      ere = real_pair_ref(an, a1.w, bn, b1.w, g) - real_pair_ref(ao, a0.w, bn, b1.w, g) -
-            real_pair_ref(an, a1.w, bo, b0.w, g) + real_pair_ref(ao, a0.w, bo, b0.w, g);
+      ere = real_pair(an, a1.w, bn, b1.w, g) - real_pair(ao, a0.w, bn, b1.w, g) -
+            real_pair(an, a1.w, bo, b0.w, g) + real_pair(ao, a0.w, bo, b0.w, g);
   }
   const double a = block_sum<kBlock>(elj, sh);
   const double c = block_sum<kBlock>(ere, sh);
@@ -1022,6 +1022,7 @@ struct ResSmem {
   double2 s_xsum[kMaxPairs];
   int s_near[kMaxPairs];
   int s_train[kMaxProp];         // ring slot of a training sample per exact slot, -1 none
+  int s_drec[kMaxScreen];        // decision-record index of a proposal, -1 none
   int s_goff[kMaxProp + 1], s_gscr[kMaxProp];
   int s_mask, s_ndone;
 };
@@ -1103,11 +1104,11 @@ __device__ void resolve_decide(const Geo& g, const Dev& d, const RepCfg& rc, Ctl
       if (dipole) {
         if (dipole_exact) {
           const double mz1 = Mz + P.dmz;
-          trial_dipl = dip_coef * mz1 * mz1;
-          dd = trial_dipl - dip_coef * Mz * Mz;
+          trial_dipl = dipole_energy(dip_coef, mz1);
+          dd = trial_dipl - dipole_energy(dip_coef, Mz);
         } else {
           // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
-          trial_dipl = dip_coef * Mz * Mz;
+          trial_dipl = dipole_energy(dip_coef, Mz);
           dd = trial_dipl - cur_dipl;
         }
       }
@@ -1181,6 +1182,38 @@ __device__ void resolve_decide(const Geo& g, const Dev& d, const RepCfg& rc, Ctl
   const long long s1r = __popcll(__ballot(valid && !pass));
   const long long tev = __popcll(__ballot(valid && !pass && e >= 0));
   const long long s1p = da_on ? __popcll(__ballot(valid && pass)) : 0;
+  if (d.drec) {
+    // bounded record of the decided proposals, in decision order: the terms
+    // and total the Metropolis test was given (trial positions copied by
+    // the block afterwards)
+    const unsigned long long rm = __ballot(valid);
+    const int rhead = d.drec_ctl[0], rcap = d.drec_ctl[1];
+    int slot = -1;
+    if (valid) {
+      const int k = rhead + lanes_below(rm);
+      if (k < rcap) {
+        slot = k;
+        DecRec& o = d.drec[k];
+        o.move = ctr0 + lane;
+        o.mol = P.mol;
+        o.n = P.n;
+        o.type = P.type;
+        o.accepted = f_acc ? 1 : 0;
+        o.lo = d.recs[lane].lo;
+        o.hi = d.recs[lane].hi;
+        o.contig = d.recs[lane].contig;
+        o.evaluated = evaluated ? 1 : 0;
+        o.early = f_early ? 1 : 0;
+        o.pass = pass;
+        o.pad0 = o.pad1 = 0;
+        for (int k2 = 0; k2 < 8; k2++) o.terms[k2] = f_tt[k2];
+        o.total = f_dE;
+        o.de_s = des;
+      }
+    }
+    if (mine) sm.s_drec[lane] = slot;
+    if (lane == 0) d.drec_ctl[0] = rhead + (int)__popcll(rm);
+  }
   // training samples in proposal order: the pair + k-space part the network
   // estimates
   int head = train ? d.tr_ctl[0] : 0;
@@ -1272,6 +1305,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     sm.s_screen[t - 192] = plan->screen_of[t - 192];
     sm.s_train[t - 192] = -1;
   }
+  if (t >= 224 && t < 224 + kMaxScreen) sm.s_drec[t - 224] = -1;
   __syncthreads();
   {
     const int nn = plan->n_near;
@@ -1280,6 +1314,18 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
   __syncthreads();
   if (t < kWave) resolve_decide(g, d, rc, C, plan, np, ndec, ctr0, sm);
   __syncthreads();
+  if (d.drec) {
+    // trial positions and bead indices of the recorded proposals
+    for (int s = 0; s < np; s++) {
+      const int slot = sm.s_drec[s];
+      if (slot < 0) continue;
+      const int n = sm.s_prop[s].n;
+      for (int i = t; i < n; i += kBlock) {
+        d.drec_pos[(size_t)slot * rc.maxm + i] = d.mv_new[(size_t)s * rc.maxm + i];
+        d.drec_idx[(size_t)slot * rc.maxm + i] = d.smv_idx[(size_t)s * rc.maxm + i];
+      }
+    }
+  }
   // cell-list commits of the accepted slots, as many slots at a time as the
   // item arrays hold
   const unsigned mask = sm.s_mask;

@@ -130,12 +130,12 @@ __global__ __launch_bounds__(kBlock) void k_cbmc_final(Geo g, Dev d, const CbmcI
       }
     }
     if (g.use_ewald) {
-      ere += real_pair_ref(r1, in->q_c, gc, in->q_c, g);
+      ere += real_pair(r1, in->q_c, gc, in->q_c, g);
       if (ch) {
         const double* gi = grown + 3 * (size_t)(cur + i);
-        ere += real_pair_ref(r1, in->q_c, gi, in->q_i, g);
-        ere += real_pair_ref(r2, in->q_i, gc, in->q_c, g);
-        ere += real_pair_ref(r2, in->q_i, gi, in->q_i, g);
+        ere += real_pair(r1, in->q_c, gi, in->q_i, g);
+        ere += real_pair(r2, in->q_i, gc, in->q_c, g);
+        ere += real_pair(r2, in->q_i, gi, in->q_i, g);
       }
     }
   }
@@ -164,10 +164,10 @@ __global__ __launch_bounds__(kBlock) void k_cbmc_final(Geo g, Dev d, const CbmcI
     double ewald_e = 0;
     if (g.use_ewald && pair_e < kVeryLargeEnergy) {
       // real space (parts 1, 2), self energies, half self-images, (b1, b2)
-      ewald_e = c + kk + g.self_coef * in->q_c * in->q_c + 0.5 * real_pair_ref(r1, in->q_c, r1, in->q_c, g);
+      ewald_e = c + kk + g.self_coef * in->q_c * in->q_c + 0.5 * real_pair(r1, in->q_c, r1, in->q_c, g);
       if (ch)
-        ewald_e += g.self_coef * in->q_i * in->q_i + real_pair_ref(r1, in->q_c, r2, in->q_i, g) +
-                   0.5 * real_pair_ref(r2, in->q_i, r2, in->q_i, g);
+        ewald_e += g.self_coef * in->q_i * in->q_i + real_pair(r1, in->q_c, r2, in->q_i, g) +
+                   0.5 * real_pair(r2, in->q_i, r2, in->q_i, g);
     }
     out[t] = pair_e >= kVeryLargeEnergy ? kVeryLargeEnergy : pair_e + ewald_e;
   }

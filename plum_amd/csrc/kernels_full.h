@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_full_small(Geo g, Dev d, int nrealb,
       f[T_REAL] = rt;
       f[T_KSPACE] = kt;
       f[T_SELF] = est;
-      if (g.dipole) f[T_DIPOLE] = g.dip_coef * mzt * mzt;
+      if (g.dipole) f[T_DIPOLE] = dipole_energy(g.dip_coef, mzt);
     }
     if (g.use_bond) f[T_BOND] = eb_all;
     if (g.use_ext) f[T_WALL] = ewt;

@@ -199,7 +199,7 @@ __device__ void mm_block(const Geo& g, const Dev& d, int blk, int nblk, double* 
       const LJPair lp = d.lj[d.mv_type[a] * g.n_types + d.mv_type[b]];
       elj += lj_energy(bb_dist(na, nb, g.Lsim, g.npbc), lp) - lj_energy(bb_dist(oa, ob, g.Lsim, g.npbc), lp);
     }
-    if (g.use_ewald) ere += real_pair_ref(na, na4.w, nb, nb4.w, g) - real_pair_ref(oa, oa4.w, ob, ob4.w, g);
+    if (g.use_ewald) ere += real_pair(na, na4.w, nb, nb4.w, g) - real_pair(oa, oa4.w, ob, ob4.w, g);
   }
   const double a = block_sum<kBlock>(elj, sh);
   const double c = block_sum<kBlock>(ere, sh);
@@ -679,11 +679,11 @@ __device__ void epilogue_block(const Geo& g, const Dev& d, int nkb, int nrb, int
       if (g.dipole) {
         if (g.dipole_exact) {
           const double mz1 = C->Mz + dmzt;
-          C->trial_dipl = g.dip_coef * mz1 * mz1;
-          dd = C->trial_dipl - g.dip_coef * C->Mz * C->Mz;
+          C->trial_dipl = dipole_energy(g.dip_coef, mz1);
+          dd = C->trial_dipl - dipole_energy(g.dip_coef, C->Mz);
         } else {
           // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
-          C->trial_dipl = g.dip_coef * C->Mz * C->Mz;
+          C->trial_dipl = dipole_energy(g.dip_coef, C->Mz);
           dd = C->trial_dipl - C->cur_dipl;
         }
       }

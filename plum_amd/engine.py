@@ -27,6 +27,7 @@ ABI_SYMBOLS = (
     "plum_surrogate_stats_get", "plum_surrogate_get_weights", "plum_surrogate_set_weights",
     "plum_surrogate_train", "plum_surrogate_forward", "plum_surrogate_gradient",
     "plum_surrogate_debug_slot", "plum_max_mol_len", "plum_cbmc_trials", "plum_cbmc_grow",
+    "plum_mc_record", "plum_mc_get_records",
 )
 SURROGATE_PARAMS = 64 * 128 + 128 + 2 * (128 * 128 + 128) + 128 + 1
 DESCR = 64
@@ -117,6 +118,14 @@ class GrowthRequest(C.Structure):
                 ("q_ion", C.c_double)]
 
 
+class Decision(C.Structure):
+    _fields_ = [("move", C.c_ulonglong), ("mol", C.c_int), ("n_moved", C.c_int),
+                ("move_type", C.c_int), ("accepted", C.c_int), ("evaluated", C.c_int),
+                ("early", C.c_int), ("first_stage_pass", C.c_int), ("lo", C.c_int),
+                ("hi", C.c_int), ("dE", Terms), ("total", C.c_double),
+                ("dE_surrogate", C.c_double)]
+
+
 class SurrogateStats(C.Structure):
     _fields_ = [("s1_rejected", C.c_longlong), ("s1_passed", C.c_longlong),
                 ("train_evals", C.c_longlong), ("stale_cuts", C.c_longlong),
@@ -176,6 +185,9 @@ def lib():
         L.plum_cbmc_trials.argtypes = [vp, C.POINTER(CbmcRequest), dp, dp, dp]
         L.plum_cbmc_grow.argtypes = [vp, C.POINTER(GrowthRequest), dp, dp, dp, dp, dp, dp,
                                      C.POINTER(C.c_int), dp]
+        L.plum_mc_record.argtypes = [vp, C.c_int]
+        L.plum_mc_get_records.argtypes = [vp, C.c_int, C.POINTER(Decision), ip, dp,
+                                          C.POINTER(C.c_longlong)]
         for f in ABI_SYMBOLS:
             if f != "plum_last_error":
                 getattr(L, f).restype = C.c_int
@@ -317,6 +329,38 @@ class Engine:
         st = MCStats()
         _check(lib().plum_mc_run(self.h, C.byref(mp), n_moves, C.byref(st)))
         return st
+
+    def mc_record(self, capacity):
+        """Arms a record of the next `capacity` decided proposals of mc_run
+        (0 disarms)."""
+        _check(lib().plum_mc_record(self.h, int(capacity)))
+
+    def mc_records(self, cap):
+        """Recorded decisions in order: a list of dicts (move, mol, type,
+        accepted, evaluated, early, pass, dE per term, total, dE_s, idx,
+        trial (n, 3)) and the number of proposals decided since arming."""
+        m = self.max_mol_len()
+        out = (Decision * max(cap, 1))()
+        idx = np.zeros(max(cap, 1) * m, dtype=np.int32)
+        xyz = np.zeros(max(cap, 1) * m * 3)
+        nd = C.c_longlong()
+        n = lib().plum_mc_get_records(self.h, cap, out, _ip(idx), _dp(xyz), C.byref(nd))
+        if n < 0:
+            _check(n)
+        recs = []
+        for k in range(n):
+            o = out[k]
+            nm = o.n_moved
+            recs.append({
+                "move": o.move, "mol": o.mol, "n": nm, "type": o.move_type,
+                "accepted": bool(o.accepted), "evaluated": bool(o.evaluated),
+                "early": bool(o.early), "pass": bool(o.first_stage_pass), "lo": o.lo,
+                "hi": o.hi, "dE": o.dE.as_dict(), "total": o.total,
+                "dE_s": o.dE_surrogate,
+                "idx": idx[k * m:k * m + nm].copy(),
+                "trial": xyz[3 * k * m:3 * (k * m + nm)].reshape(-1, 3).copy(),
+            })
+        return recs, nd.value
 
     def set_profiling(self, on):
         _check(lib().plum_set_profiling(self.h, int(on)))
