@@ -317,6 +317,29 @@ int plum_surrogate_gradient(plum_ctx* ctx, const float* x, const double* target,
 int plum_surrogate_debug_slot(plum_ctx* ctx, int slot, float* rows, double* pos, int* n);
 int plum_max_mol_len(const plum_ctx* ctx);
 
+/* ---- pressure --------------------------------------------------------- */
+
+/* One volume perturbation of ForceField::CalcPressureVolScalingHSELSlit
+   (pressure.cc:187-387) on the current configuration: the box is stretched
+   along z by kDz = 1e-5 (every z scaled by 1 + kDz/Lz with the spring
+   potential on, else each molecule shifted by kDz com_z/Lz; grafted "L"
+   beads stay, "R" beads move by kDz) and dU is the energy change, over all
+   pairs once (self pairs halved in the Ewald part; the first `phantom`
+   molecules are single-bead surface sites, paired only from the first half
+   of their list as the reference does; LJ between mobile beads), walls,
+   springs and the slab dipole.  The caller accumulates the estimators
+   (Yethiraj: <N/V> - <dU>/(A dz); de Miguel: ln <(1 + dz/Lz)^N e^{-beta dU}>
+   / (beta A dz)). */
+typedef struct {
+  double dU;             /* total energy change of the stretch */
+  double el, lj, wall, bond, dipole;   /* its parts (el: Ewald real + k space) */
+  int n_mol;             /* molecules besides the surface sites */
+  double vol;            /* Lx Ly Lz of the simulation box */
+  double area_dz;        /* Lx Ly kDz */
+} plum_vp;
+
+int plum_vp_sample(plum_ctx* ctx, int phantom, plum_vp* out);
+
 /* ---- configurational-bias chain growth (grand-canonical moves) --------- */
 
 typedef struct {

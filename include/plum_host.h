@@ -66,6 +66,16 @@ int plum_sim_density_profile(plum_sim* sim, double res, double* out, int max_bin
    <prefix>_lastcrd.dat and <prefix>_lasttop.dat, from which a run resumes
    (a run.in naming them reads the configuration back). */
 int plum_sim_write_last(plum_sim* sim, const char* prefix);
+/* ForceField::CalcPressureVolScalingHSELSlit (pressure.cc:187-387): one
+   volume-perturbation sample of the current configuration (plum_vp_sample,
+   the run.in's surface sites as phantom), added to the running estimators;
+   sample (may be NULL) receives it, p_yethiraj / p_demiguel the averages so
+   far in the reference's form (beta <N/V> - <dU>/(A dz), and
+   beta/(A dz) ln <(1 + dz/Lz)^N e^{-beta dU}>), samples their count.  The
+   reference computes these for boxes without walls and prints "nan"
+   (pressure.cc:490-500); here they are returned. */
+int plum_sim_pressure_sample(plum_sim* sim, plum_vp* sample, double* p_yethiraj, double* p_demiguel,
+                             long long* samples);
 int plum_sim_destroy(plum_sim* sim);
 const char* plum_host_last_error(void);
 

@@ -138,6 +138,7 @@ def lib():
         L.orc_gc_widom.restype = C.c_double
         L.orc_gasdev.argtypes = [C.c_double, C.c_double, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
         L.orc_gasdev.restype = C.c_double
+        L.orc_vp_sample.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_random_bond_len.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
         L.orc_random_bond_len.restype = C.c_double
         L.orc_translational_move.argtypes = [C.c_void_p, C.POINTER(OrcMT), dp, C.c_double,
@@ -297,6 +298,15 @@ class Oracle:
         with gasdev state `gauss` (an OrcGauss)."""
         f = lib().orc_random_bond_len
         return np.array([f(self.h, C.byref(rng.s), C.byref(gauss)) for _ in range(n)])
+
+    def vp_sample(self, phantom=0):
+        """One volume perturbation of the pressure estimator
+        (pressure.cc:187-387): dict of dU and its parts, the molecule count,
+        volume and area x dz."""
+        out = np.zeros(9)
+        lib().orc_vp_sample(self.h, int(phantom), _dp(out))
+        keys = ("dU", "el", "lj", "wall", "bond", "dipole", "n_mol", "vol", "a_dz")
+        return dict(zip(keys, out))
 
     def rg(self):
         return lib().orc_radius_of_gyration(self.h)

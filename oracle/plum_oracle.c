@@ -1329,3 +1329,165 @@ double orc_gc_widom(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc,
   free(chain);
   return total / (double)n_ins;
 }
+
+/* ------------------------------------------------------------------ */
+/* volume-perturbation pressure (pressure.cc:187-387)                   */
+/* ------------------------------------------------------------------ */
+static const double kDz = 0.00001; /* utilities/constants.h:55 */
+
+/* PotentialEwaldCoul::PairEnergyRealForP (potential_ewald_coul.cc:167-196):
+   the real-space sum in the box stretched along z by kDz (no early return
+   for an uncharged pair: the prefactor is then 0). */
+static double real_pair_forp(const orc_sys* s, const double* a, double qa, const double* b,
+                             double qb) {
+  double energy = 0, dist[3];
+  const double* L = s->e.box_e;
+  const double Ls[3] = {L[0], L[1], L[2] + kDz};
+  get_dist_vector(a, b, Ls, s->p.npbc, dist);
+  const double prefactor = s->p.lB * qa * qb;
+  const int* c = s->e.real_cell;
+  const double sa = sqrt(s->p.alpha);
+  for (int i = -c[0]; i <= c[0]; i++)
+    for (int j = -c[1]; j <= c[1]; j++)
+      for (int k = -c[2]; k <= c[2]; k++) {
+        double rv0 = dist[0] + i * L[0];
+        double rv1 = dist[1] + j * L[1];
+        double rv2 = dist[2] + k * (L[2] + kDz);
+        double r = sqrt(rv0 * rv0 + rv1 * rv1 + rv2 * rv2);
+        if (r > 0 && r <= s->e.real_cutoff) energy += prefactor * erfc(sa * r) / r;
+      }
+  return energy;
+}
+
+/* PotentialEwaldCoul::PairEnergyReplForP (potential_ewald_coul.cc:228-250)
+   with the kz_forP / k2_forP / ek2_forP / box_vol_forP tables of
+   :52-54, :86-113 (the same l grid, z wavenumbers of the stretched box). */
+static double repl_pair_forp(const orc_sys* s, const double* a, double qa, const double* b,
+                             double qb) {
+  double energy = 0, r[3];
+  const double* L = s->e.box_e;
+  const double Ls[3] = {L[0], L[1], L[2] + kDz};
+  get_dist_vector(a, b, Ls, s->p.npbc, r);
+  const double vol_forp = L[0] * L[1] * (L[2] + kDz);
+  const double prefactor = s->p.lB * qa * qb / (kPi * vol_forp) * (4 * kPi * kPi);
+  const int* rc = s->e.repl_cell;
+  for (int lx = 0; lx < s->ceto[0]; lx++)
+    for (int ly = 0; ly < s->ceto[1]; ly++)
+      for (int lz = 0; lz < s->ceto[2]; lz++) {
+        const double kzp = (double)((lz - rc[2]) * 2) * kPi / (L[2] + kDz);
+        const double k2p = s->kx[lx] * s->kx[lx] + s->ky[ly] * s->ky[ly] + kzp * kzp;
+        if (k2p > 0 && k2p <= s->e.repl_cutoff) {
+          const double ek2p = exp(-k2p / (4 * s->p.alpha)) / k2p;
+          energy += prefactor * ek2p * cos(s->kx[lx] * r[0] + s->ky[ly] * r[1] + kzp * r[2]);
+        }
+      }
+  return energy;
+}
+
+/* PotentialTruncatedLJ::PairEnergy with an explicit box (the pressure
+   estimator passes the stretched box). */
+static double lj_pair_box(const orc_sys* s, const double* a, int ta, const double* b, int tb,
+                          const double box[3]) {
+  orc_sys t = *s;
+  for (int k = 0; k < 3; k++) t.p.box[k] = box[k];
+  return lj_pair_xyz(&t, a, ta, b, tb);
+}
+
+/* PotentialTruncatedLJWall::BeadEnergy with the wall at z = Lz. */
+static double wall_bead_zL(const orc_sys* s, double z, int t, double Lz) {
+  orc_sys u = *s;
+  u.p.box[2] = Lz;
+  return wall_bead_z(&u, z, t);
+}
+
+/* ForceField::CalcPressureVolScalingHSELSlit (pressure.cc:187-387), the
+   energy change of one stretch of the box along z by kDz: every bead's z
+   scaled by 1 + kDz/Lz with the bond potential on (grafted "L" beads stay,
+   "R" beads move by kDz), each molecule's z shifted by kDz com_z/Lz without
+   it; all pairs once (self pairs halved in the Ewald part, phantom surface
+   sites only across the two halves of the site list, LJ between non-phantom
+   beads only), walls, springs and the slab dipole.
+   out: [0] dU, [1] Ewald pairs, [2] LJ pairs, [3] wall, [4] springs,
+   [5] dipole, [6] molecules n_mol - phantom, [7] volume Lx Ly Lz,
+   [8] area x dz. */
+void orc_vp_sample(orc_sys* s, int phantom, double out[9]) {
+  const double* box = s->p.box;
+  const int n_mol = s->n_mols;
+  double* trial = s->trial;
+  double* dcz = malloc(sizeof(double) * (n_mol ? n_mol : 1));
+  for (int i = 0; i < n_mol; i++) {
+    const int a = s->mol_start[i], b = s->mol_start[i + 1];
+    double com_z = 0;
+    for (int j = a; j < b; j++) com_z += s->cur[3 * j + 2];
+    com_z /= (b - a);
+    dcz[i] = kDz * (com_z / box[2]);
+    if (s->p.type_kind[s->type[a]] == 2) dcz[i] = kDz;
+    else if (s->p.type_kind[s->type[a]] == 1) dcz[i] = 0;
+  }
+  for (int i = 0; i < n_mol; i++)
+    for (int j = s->mol_start[i]; j < s->mol_start[i + 1]; j++) {
+      trial[3 * j] = s->cur[3 * j];
+      trial[3 * j + 1] = s->cur[3 * j + 1];
+      const int kind = s->p.type_kind[s->type[j]];
+      if (s->p.use_bond && kind == 0) trial[3 * j + 2] = s->cur[3 * j + 2] * (1.0 + kDz / box[2]);
+      else trial[3 * j + 2] = s->cur[3 * j + 2] + dcz[i];
+    }
+  const double box_sc[3] = {box[0], box[1], box[2] + kDz};
+  double dU = 0, d_el = 0, d_hs = 0, d_ext = 0, d_bond = 0, d_dip = 0;
+  for (int i = 0; i < n_mol; i++) {
+    for (int j = s->mol_start[i]; j < s->mol_start[i + 1]; j++) {
+      for (int k = i; k < n_mol; k++) {
+        for (int l = s->mol_start[k]; l < s->mol_start[k + 1]; l++) {
+          if (!((k > i || (k == i && l >= j)) &&
+                (i >= phantom || (i < phantom / 2 && k >= phantom / 2))))
+            continue;
+          if (s->p.use_ewald) {
+            double oldE = real_pair_xyz(s, &s->cur[3 * j], s->q[j], &s->cur[3 * l], s->q[l]) +
+                          repl_pair_xyz(s, &s->cur[3 * j], s->q[j], &s->cur[3 * l], s->q[l]);
+            if (j == l) oldE *= 0.5;
+            double newE = real_pair_forp(s, &trial[3 * j], s->q[j], &trial[3 * l], s->q[l]);
+            newE += repl_pair_forp(s, &trial[3 * j], s->q[j], &trial[3 * l], s->q[l]);
+            if (j == l) newE *= 0.5;
+            dU += newE - oldE;
+            d_el += newE - oldE;
+          }
+          if (s->p.use_pair && i >= phantom && k >= phantom && j != l) {
+            double oldE = lj_pair_xyz(s, &s->cur[3 * j], s->type[j], &s->cur[3 * l], s->type[l]);
+            double newE = lj_pair_box(s, &trial[3 * j], s->type[j], &trial[3 * l], s->type[l], box_sc);
+            dU += newE - oldE;
+            d_hs += newE - oldE;
+          }
+        }
+      }
+      if (s->p.use_ext && i >= phantom) {
+        double oldE = wall_bead_z(s, s->cur[3 * j + 2], s->type[j]);
+        double newE = wall_bead_zL(s, trial[3 * j + 2], s->type[j], box_sc[2]);
+        dU += newE - oldE;
+        d_ext += newE - oldE;
+      }
+    }
+    if (s->p.use_bond) {
+      const double oldE = s->bond_cur[i];
+      const double newE = spring_molecule(s, i);   /* on the trial coordinates */
+      dU += newE - oldE;
+      d_bond += newE - oldE;
+    }
+  }
+  if (s->p.use_ewald && s->p.dipole) {
+    double mo = 0, mn = 0;
+    for (int j = 0; j < s->n; j++) {
+      mo += s->q[j] * s->cur[3 * j + 2];
+      mn += s->q[j] * trial[3 * j + 2];
+    }
+    const double vol = box[0] * box[1] * box[2];
+    d_dip = (s->p.lB * 2 * kPi) * (mn * mn / (box[0] * box[1] * (box[2] + kDz)) - mo * mo / vol);
+    dU += d_dip;
+  }
+  /* unscale: trial back to current */
+  memcpy(trial, s->cur, sizeof(double) * 3 * s->n);
+  free(dcz);
+  out[0] = dU; out[1] = d_el; out[2] = d_hs; out[3] = d_ext; out[4] = d_bond; out[5] = d_dip;
+  out[6] = n_mol - phantom;
+  out[7] = box[0] * box[1] * box[2];
+  out[8] = box[0] * box[1] * kDz;
+}

@@ -190,6 +190,10 @@ double orc_gasdev(double mean, double stdev, orc_mt19937* r, orc_gauss* gs);
 /* PotentialSpring::RandomBondLen (potential_spring.cc:50-61) with the
    system's k_bond, r0 and beta. */
 double orc_random_bond_len(orc_sys* s, orc_mt19937* r, orc_gauss* gs);
+/* One volume perturbation of ForceField::CalcPressureVolScalingHSELSlit
+   (pressure.cc:187-387): out[0] dU, [1..5] its Ewald-pair, LJ-pair, wall,
+   spring and dipole parts, [6] n_mol - phantom, [7] Lx Ly Lz, [8] Lx Ly kDz. */
+void orc_vp_sample(orc_sys* s, int phantom, double out[9]);
 
 #ifdef __cplusplus
 }

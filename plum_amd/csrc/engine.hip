@@ -18,6 +18,7 @@
 #include "kernels_grow.h"
 #include "kernels_full.h"
 #include "kernels_move.h"
+#include "kernels_pressure.h"
 #include "plum_gpu.h"
 #include "surrogate.h"
 
@@ -25,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -34,7 +36,9 @@ using namespace plum;
 struct plum_batch;
 struct SurHost;
 struct CbmcHost;
+struct VpHost;
 void cbmc_free(CbmcHost* h);
+void vp_free(VpHost* h);
 extern "C" int plum_batch_destroy(plum_batch* batch);
 void sur_free(SurHost* s);
 
@@ -44,6 +48,7 @@ struct plum_ctx {
   plum_info info{};
   Dev d{};
   hipStream_t st = nullptr;
+  int device = 0;        // HIP device the context lives on
   int maxm = 1;          // capacity of the moved list (largest molecule)
   int nkb = 0;           // blocks of the k-space delta kernel (64 k each)
   int nkb_full = 0;      // blocks of the full S(k) kernels (256 k each)
@@ -59,6 +64,7 @@ struct plum_ctx {
   plum_batch* self = nullptr;  // this system as a batch of one (device loop)
   SurHost* sur = nullptr;      // delayed-acceptance surrogate (null when off)
   CbmcHost* cbmc = nullptr;    // configurational-bias trial buffers
+  VpHost* vp = nullptr;        // volume-perturbation pressure buffers
   int screen_cap = kMaxProp;   // proposal slots allocated
   double* epart_k = nullptr;
   std::vector<int> mol_start;
@@ -384,11 +390,21 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
     wall[t].kind = p.type_kind[t];
     wall[t].pad = 0;
   }
-  // FP32 path: pairs inside 1.3 x the largest LJ sigma (the steep, large
-  // close-contact terms) are evaluated in FP64
+  // FP32 path: pairs closer than 1.3 x the largest LJ sigma (the steep,
+  // large close-contact terms), and pairs whose real-space term for unit
+  // charges exceeds 3e-3 kT, are evaluated in FP64.  FP32 pair math carries
+  // ~1e-7 of relative error per pair; summed over the ~10^5 pair images of a
+  // long chain's move (old and new) the large near terms would leave ~1e-5 kT
+  // of error in dE, the small far ones leave < 1e-6 kT.
   double sig_max = 0;
   for (int t = 0; t < p.n_types; t++) sig_max = std::max(sig_max, p.lj_sigma[t]);
-  g.fp64_below = 1.3 * std::max(sig_max, 0.0);
+  double r_u = 0;
+  if (p.use_ewald && p.lB > 0) {
+    const double sa = sqrt(p.alpha);
+    r_u = 0.05;
+    while (r_u < info.real_cutoff && p.lB * erfc(sa * r_u) / r_u > 3e-3) r_u += 0.05;
+  }
+  g.fp64_below = std::max(1.3 * std::max(sig_max, 0.0), r_u);
   // explicit pair sums need only the minimum image when no other image can
   // reach rc along a periodic edge (edges longer than 2 rc); a non-periodic z
   // is checked per pair (real_pair)
@@ -464,7 +480,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   c->nmmb = nmm_blocks(maxm);
   c->nchunk = (n + c->chunk - 1) / c->chunk;
 
-  if (hipStreamCreate(&c->st) != hipSuccess) {
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
     delete c;
     return fail(PLUM_E_HIP, "hipStreamCreate failed");
   }
@@ -574,6 +590,8 @@ int plum_destroy(plum_ctx* c) {
   c->cbmc = nullptr;
   if (c->sur) sur_free(c->sur);
   c->sur = nullptr;
+  if (c->vp) vp_free(c->vp);
+  c->vp = nullptr;
   for (void* p : c->dev_allocs) (void)hipFree(p);
   if (c->h_in) (void)hipHostFree(c->h_in);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -741,3 +759,4 @@ int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, 
 #include "batch_api.h"
 #include "surrogate_api.h"
 #include "cbmc_api.h"
+#include "pressure_api.h"

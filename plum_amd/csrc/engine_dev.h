@@ -160,7 +160,14 @@ __device__ __forceinline__ double bb_dist(const double* a, const double* b,
 // unchanged M_z gives the same energy bit for bit, as the reference's
 // repeated evaluation does, and the lagged dE of such a move is exactly 0.
 __device__ __forceinline__ double dipole_energy(double coef, double mz) {
-  return __dmul_rn(__dmul_rn(coef, mz), mz);
+#pragma clang fp contract(off)
+  return coef * mz * mz;
+}
+
+// e_new - e_old of two dipole energies, not fused with their products.
+__device__ __forceinline__ double dipole_change(double e_new, double e_old) {
+#pragma clang fp contract(off)
+  return e_new - e_old;
 }
 
 // Truncated-shifted LJ / WCA at distance r (potential_truncated_lj.cc:49-85).

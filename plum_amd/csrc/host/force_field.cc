@@ -1,5 +1,6 @@
 #include "force_field.h"
 
+#include <cmath>
 #include <sstream>
 #include <stdexcept>
 
@@ -221,6 +222,29 @@ plum_terms ForceField::FullEnergy() {
   plum_terms t;
   check(plum_energy_full(ctx_, &t), "FullEnergy");
   return t;
+}
+
+plum_vp ForceField::CalcPressureVolScalingHSELSlit(std::vector<Molecule>& mols) {
+  sync(mols);
+  plum_vp v{};
+  check(plum_vp_sample(ctx_, r_.phantom, &v), "CalcPressureVolScalingHSELSlit");
+  vp_z_++;
+  vp_du_ += v.dU;
+  vp_ideal_ += v.n_mol / v.vol;
+  const double dz = v.area_dz / (box_[0] * box_[1]);
+  vp_dm_ += std::pow(1.0 + dz / box_[2], v.n_mol) * std::exp(-r_.beta * v.dU);
+  vp_adz_ = v.area_dz;
+  return v;
+}
+
+double ForceField::PressureYethiraj() const {
+  if (vp_z_ == 0) return 0;
+  return (r_.beta * vp_ideal_ - vp_du_ / vp_adz_) / vp_z_;
+}
+
+double ForceField::PressureDeMiguel() const {
+  if (vp_z_ == 0) return 0;
+  return r_.beta / vp_adz_ * std::log(vp_dm_ / vp_z_);
 }
 
 double ForceField::TotPairEnergy() { return Totals().lj; }

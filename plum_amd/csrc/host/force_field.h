@@ -69,6 +69,16 @@ class ForceField {
   // coordinates are refreshed from the device afterwards.
   plum_mc_stats DeviceMoves(std::vector<Molecule>& mols, const plum_mc_params& mc, long long n);
 
+  // pressure.cc:187-387: one volume perturbation of the current
+  // configuration (plum_vp_sample on the GPU), accumulated as the reference
+  // accumulates it; returns the sample.  The estimators in the reference's
+  // form: Yethiraj (beta <N/V> - <dU> / (A dz)) and de Miguel
+  // (beta / (A dz) ln <(1 + dz/Lz)^N e^{-beta dU}>).
+  plum_vp CalcPressureVolScalingHSELSlit(std::vector<Molecule>& mols);
+  double PressureYethiraj() const;
+  double PressureDeMiguel() const;
+  long long PressureSamples() const { return vp_z_; }
+
   bool UseGC() const { return r_.use_gc; }
   bool UsePairPot() const { return r_.use_pair; }
   bool UseEwaldPot() const { return r_.use_ewald; }
@@ -117,6 +127,10 @@ class ForceField {
   double gauss_gset_ = 0;
   std::vector<double> chain_, tri_, wts_, en_;
   std::vector<double> first_, disp_, ion_, sel_, act_;  // draws of one growth
+  // pressure accumulators (pressure.cc:187-387): samples, sum of N / V, sum
+  // of dU, sum of (1 + dz/Lz)^N e^{-beta dU}, and the last A dz
+  long long vp_z_ = 0;
+  double vp_ideal_ = 0, vp_du_ = 0, vp_dm_ = 0, vp_adz_ = 0;
 };
 
 }  // namespace plum

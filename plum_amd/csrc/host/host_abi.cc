@@ -212,6 +212,21 @@ int plum_sim_write_last(plum_sim* s, const char* prefix) {
   return PLUM_OK;
 }
 
+int plum_sim_pressure_sample(plum_sim* s, plum_vp* sample, double* p_yethiraj, double* p_demiguel,
+                             long long* samples) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null argument");
+  try {
+    const plum_vp v = s->s->ff.CalcPressureVolScalingHSELSlit(s->s->mols);
+    if (sample) *sample = v;
+    if (p_yethiraj) *p_yethiraj = s->s->ff.PressureYethiraj();
+    if (p_demiguel) *p_demiguel = s->s->ff.PressureDeMiguel();
+    if (samples) *samples = s->s->ff.PressureSamples();
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_HIP, e.what());
+  }
+  return PLUM_OK;
+}
+
 int plum_sim_destroy(plum_sim* s) {
   if (!s) return PLUM_OK;
   delete s->s;

@@ -664,12 +664,13 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
 // wave lists the query cells 64 at a time with a wave prefix over their
 // counts, each lane takes candidate slots lane, lane + 64, ..., and a wave
 // reduction sums the lanes in fixed order.  Same pair terms as real_query.
-template <int P>
+// A candidate's bead index, position and type are loaded together, one
+// candidate ahead of the arithmetic, so a lane waits on one round trip per
+// candidate at most and usually on none.
 struct QWaveSmem {
   int s_cell[kWave];
   int s_img[kWave];
   int s_end[kWave];
-  float s_rel[P ? kWave : 1][3];
 };
 
 struct QueryPt {
@@ -677,69 +678,48 @@ struct QueryPt {
   double pw[3];
   const LJPair* ljrow;
   const LJPair32* ljrow32;
-  float qpf, rs2f_wide, saf, lBf;
+  float qpf, saf, lBf;
   double lj_close;
 };
 
-// LJ + real-space terms of query point q against the bead in cell slot si
-// (image img; rel: the query point relative to that cell, FP32 path).
+// LJ + real-space terms of query point q against a bead at pj (wrapped
+// position and charge) of type tj in image img.  P = 1: FP32 pair math
+// beyond lj_close (the FP64 distance and cutoff tests are kept).
 template <int P>
-__device__ __forceinline__ void query_candidate(const Geo& g, const Dev& d, const QueryPt& q, size_t si, int img,
-                                                const float* rel, double& elj, double& ere) {
+__device__ __forceinline__ void pair_terms(const Geo& g, const QueryPt& q, double4 pj, int tj, int img, double& elj,
+                                           double& ere) {
   const int iz = ((img >> 16) & 0xff) - 128;
   const bool lj_ok = g.use_pair && (g.npbc == 3 || iz == 0);
   const int ix = (img & 0xff) - 128, iy = ((img >> 8) & 0xff) - 128;
-  if constexpr (P == 0) {
-    const double4 pj = d.cpos[si];
-    const double dx = pj.x + ix * g.L[0] - q.pw[0];
-    const double dy = pj.y + iy * g.L[1] - q.pw[1];
-    const double dz = pj.z + iz * g.L[2] - q.pw[2];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 > g.rs2) return;
-    const double r = sqrt(r2);
-    if (lj_ok) elj += lj_energy(r, q.ljrow[d.ctype[si]]);
+  const double dx = pj.x + ix * g.L[0] - q.pw[0];
+  const double dy = pj.y + iy * g.L[1] - q.pw[1];
+  const double dz = pj.z + iz * g.L[2] - q.pw[2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 > g.rs2) return;
+  const double r = sqrt(r2);
+  if (P == 0 || r < q.lj_close) {
+    // FP64 (the FP32 path keeps the steep, large close-contact terms here)
+    if (lj_ok) elj += lj_energy(r, q.ljrow[tj]);
     if (g.use_ewald) {
       const double qq = q.p.w * pj.w;
       if (qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
     }
-  } else {
-    // FP32 prefilter on cell-relative coordinates (slightly widened so no
-    // pair the FP64 test keeps is dropped), then the exact FP64 distance and
-    // cutoff test, then FP32 pair math
-    const float4 pr = d.cpos32[si];
-    const float fx = pr.x - rel[0], fy = pr.y - rel[1], fz = pr.z - rel[2];
-    if (fx * fx + fy * fy + fz * fz > q.rs2f_wide) return;
-    const double4 pj = d.cpos[si];
-    const double dx = pj.x + ix * g.L[0] - q.pw[0];
-    const double dy = pj.y + iy * g.L[1] - q.pw[1];
-    const double dz = pj.z + iz * g.L[2] - q.pw[2];
-    const double r2d = dx * dx + dy * dy + dz * dz;
-    if (r2d > g.rs2) return;
-    const double rd = sqrt(r2d);
-    if (rd < q.lj_close) {
-      // close contacts carry the large, steep terms: FP64 like P == 0
-      if (lj_ok) elj += lj_energy(rd, q.ljrow[d.ctype[si]]);
-      if (g.use_ewald) {
-        const double qq = q.p.w * pj.w;
-        if (qq != 0 && rd > 0 && rd <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * rd) / rd;
-      }
-      return;
+    return;
+  }
+  const float r2f = (float)r2;
+  if (lj_ok) {
+    const LJPair32 lp = q.ljrow32[tj];
+    if (r < lp.cut) {
+      const float s2 = lp.sigma2 / r2f;
+      const float r6 = s2 * s2 * s2;
+      elj += (double)(lp.eps4 * (r6 * r6 - r6) - lp.shift);
     }
-    const float r2 = (float)r2d;
-    if (lj_ok) {
-      const LJPair32 lp = q.ljrow32[d.ctype[si]];
-      if (rd < lp.cut) {
-        const float s2 = lp.sigma2 / r2;
-        const float r6 = s2 * s2 * s2;
-        elj += (double)(lp.eps4 * (r6 * r6 - r6) - lp.shift);
-      }
-    }
-    if (g.use_ewald) {
-      const float qq = q.qpf * pr.w;
-      if (qq != 0.f && rd <= g.rc) {
-        const float r = (float)rd;
-        ere += (double)(q.lBf * qq * erfcf(q.saf * r) / r);
-      }
+  }
+  if (g.use_ewald) {
+    const float qq = q.qpf * (float)pj.w;
+    if (qq != 0.f && r <= g.rc) {
+      const float rf = (float)r;
+      ere += (double)(q.lBf * qq * erfcf(q.saf * rf) / rf);
     }
   }
 }
@@ -750,7 +730,7 @@ __device__ __forceinline__ void query_candidate(const Geo& g, const Dev& d, cons
 // `nparts` of the candidate slots.  Result valid in lane 0.
 template <int P>
 __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm, int contig, int mlo, int mhi,
-                                   int part, int nparts, QWaveSmem<P>& sm) {
+                                   int part, int nparts, QWaveSmem& sm) {
   const int lane = threadIdx.x % kWave;
   QueryPt q;
   q.p = p;
@@ -760,7 +740,6 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
   q.ljrow = d.lj + tm * g.n_types;
   q.ljrow32 = d.lj32 + tm * g.n_types;
   q.qpf = (float)p.w;
-  q.rs2f_wide = (float)(g.rs2 * (1.0 + 1e-4)) + 1e-4f;
   q.saf = (float)g.sqrt_alpha;
   q.lBf = (float)g.lB;
   q.lj_close = g.fp64_below;
@@ -787,7 +766,6 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
         const int w = u[k] - qk * g.nc[k];
         cell = cell * g.nc[k] + w;
         img |= ((qk + 128) & 0xff) << (8 * k);
-        if constexpr (P != 0) sm.s_rel[lane][k] = (float)(q.pw[k] - (double)u[k] * g.h[k]);
       }
       sm.s_cell[lane] = cell;
       sm.s_img[lane] = img;
@@ -810,18 +788,46 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
     sm.s_end[lane] = end;
     const int total = __shfl(end, nloc - 1, kWave);
     wave_sync_lds();
-    for (int it = part * kWave + lane; it < total; it += nparts * kWave) {
+    const int stride = nparts * kWave;
+    // candidate it: its cell slot and image (binary search over the wave's
+    // cell prefix in LDS)
+    auto locate = [&](int it, size_t& si, int& img) {
       int lo_i = 0, hi_i = nloc - 1;
       while (lo_i < hi_i) {
         const int mid = (lo_i + hi_i) >> 1;
         if (sm.s_end[mid] > it) hi_i = mid;
         else lo_i = mid + 1;
       }
-      const int t = lo_i;
-      const size_t si = (size_t)sm.s_cell[t] * g.cap + (it - (t ? sm.s_end[t - 1] : 0));
-      const int j = d.cbead[si];
-      if (j < 0 || (contig ? (j >= mlo && j <= mhi) : d.moved[j])) continue;
-      query_candidate<P>(g, d, q, si, sm.s_img[t], P ? sm.s_rel[t] : nullptr, elj, ere);
+      si = (size_t)sm.s_cell[lo_i] * g.cap + (it - (lo_i ? sm.s_end[lo_i - 1] : 0));
+      img = sm.s_img[lo_i];
+    };
+    int it = part * kWave + lane;
+    int j = -1, tj = 0, img = 0;
+    double4 pj = make_double4(0, 0, 0, 0);
+    if (it < total) {
+      size_t si;
+      locate(it, si, img);
+      j = d.cbead[si];
+      pj = d.cpos[si];
+      tj = d.ctype[si];
+    }
+    while (it < total) {
+      const int it2 = it + stride;
+      int j2 = -1, tj2 = 0, img2 = 0;
+      double4 pj2 = make_double4(0, 0, 0, 0);
+      if (it2 < total) {
+        size_t si2;
+        locate(it2, si2, img2);
+        j2 = d.cbead[si2];
+        pj2 = d.cpos[si2];
+        tj2 = d.ctype[si2];
+      }
+      if (j >= 0 && !(contig ? (j >= mlo && j <= mhi) : d.moved[j])) pair_terms<P>(g, q, pj, tj, img, elj, ere);
+      it = it2;
+      j = j2;
+      tj = tj2;
+      img = img2;
+      pj = pj2;
     }
     wave_sync_lds();   // the next chunk rewrites the cell arrays
   }
@@ -871,7 +877,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   __shared__ int s_pre[kMaxReplicas];
   __shared__ union U {
     KSpecSmem k;
-    QWaveSmem<P> q[kQWaves];
+    QWaveSmem q[kQWaves];
   } sm;
   __shared__ CSpecSmem cs;
   for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];
@@ -1105,11 +1111,11 @@ __device__ void resolve_decide(const Geo& g, const Dev& d, const RepCfg& rc, Ctl
         if (dipole_exact) {
           const double mz1 = Mz + P.dmz;
           trial_dipl = dipole_energy(dip_coef, mz1);
-          dd = trial_dipl - dipole_energy(dip_coef, Mz);
+          dd = dipole_change(trial_dipl, dipole_energy(dip_coef, Mz));
         } else {
           // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
           trial_dipl = dipole_energy(dip_coef, Mz);
-          dd = trial_dipl - cur_dipl;
+          dd = dipole_change(trial_dipl, cur_dipl);
         }
       }
       tt[T_DIPOLE] = dd;

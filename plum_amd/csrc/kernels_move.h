@@ -680,11 +680,11 @@ __device__ void epilogue_block(const Geo& g, const Dev& d, int nkb, int nrb, int
         if (g.dipole_exact) {
           const double mz1 = C->Mz + dmzt;
           C->trial_dipl = dipole_energy(g.dip_coef, mz1);
-          dd = C->trial_dipl - dipole_energy(g.dip_coef, C->Mz);
+          dd = dipole_change(C->trial_dipl, dipole_energy(g.dip_coef, C->Mz));
         } else {
           // potential_ewald.cc:528-531: dipole of the CURRENT coordinates
           C->trial_dipl = dipole_energy(g.dip_coef, C->Mz);
-          dd = C->trial_dipl - C->cur_dipl;
+          dd = dipole_change(C->trial_dipl, C->cur_dipl);
         }
       }
       t[T_DIPOLE] = dd;

@@ -27,7 +27,7 @@ ABI_SYMBOLS = (
     "plum_surrogate_stats_get", "plum_surrogate_get_weights", "plum_surrogate_set_weights",
     "plum_surrogate_train", "plum_surrogate_forward", "plum_surrogate_gradient",
     "plum_surrogate_debug_slot", "plum_max_mol_len", "plum_cbmc_trials", "plum_cbmc_grow",
-    "plum_mc_record", "plum_mc_get_records",
+    "plum_mc_record", "plum_mc_get_records", "plum_vp_sample",
 )
 SURROGATE_PARAMS = 64 * 128 + 128 + 2 * (128 * 128 + 128) + 128 + 1
 DESCR = 64
@@ -126,6 +126,15 @@ class Decision(C.Structure):
                 ("dE_surrogate", C.c_double)]
 
 
+class VpSample(C.Structure):
+    _fields_ = [("dU", C.c_double), ("el", C.c_double), ("lj", C.c_double), ("wall", C.c_double),
+                ("bond", C.c_double), ("dipole", C.c_double), ("n_mol", C.c_int),
+                ("vol", C.c_double), ("area_dz", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SurrogateStats(C.Structure):
     _fields_ = [("s1_rejected", C.c_longlong), ("s1_passed", C.c_longlong),
                 ("train_evals", C.c_longlong), ("stale_cuts", C.c_longlong),
@@ -185,6 +194,7 @@ def lib():
         L.plum_cbmc_trials.argtypes = [vp, C.POINTER(CbmcRequest), dp, dp, dp]
         L.plum_cbmc_grow.argtypes = [vp, C.POINTER(GrowthRequest), dp, dp, dp, dp, dp, dp,
                                      C.POINTER(C.c_int), dp]
+        L.plum_vp_sample.argtypes = [vp, C.c_int, C.POINTER(VpSample)]
         L.plum_mc_record.argtypes = [vp, C.c_int]
         L.plum_mc_get_records.argtypes = [vp, C.c_int, C.POINTER(Decision), ip, dp,
                                           C.POINTER(C.c_longlong)]
@@ -361,6 +371,12 @@ class Engine:
                 "trial": xyz[3 * k * m:3 * (k * m + nm)].reshape(-1, 3).copy(),
             })
         return recs, nd.value
+
+    def vp_sample(self, phantom=0):
+        """One volume perturbation of the pressure estimator (plum_vp_sample)."""
+        o = VpSample()
+        _check(lib().plum_vp_sample(self.h, int(phantom), C.byref(o)))
+        return o.as_dict()
 
     def set_profiling(self, on):
         _check(lib().plum_set_profiling(self.h, int(on)))

@@ -13,7 +13,7 @@ HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim
                 "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
                 "plum_host_last_error", "plum_synth_count", "plum_synth_system",
                 "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device",
-                "plum_sim_density_profile", "plum_sim_write_last")
+                "plum_sim_density_profile", "plum_sim_write_last", "plum_sim_pressure_sample")
 
 
 class RunInSummary(C.Structure):
@@ -69,6 +69,8 @@ def lib():
         L.plum_sim_write_last.argtypes = [vp, C.c_char_p]
         L.plum_sim_run_device.argtypes = [vp, C.c_longlong, C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+        L.plum_sim_pressure_sample.argtypes = [vp, C.POINTER(E.VpSample), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         L.plum_host_last_error.restype = C.c_char_p
         L.plum_synth_count.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plum_synth_system.argtypes = [C.POINTER(SynthSpec), C.POINTER(C.c_double),
@@ -132,6 +134,15 @@ class HostSimulation:
 
     def write_last(self, prefix):
         _check(lib().plum_sim_write_last(self.h, prefix.encode()))
+
+    def pressure_sample(self):
+        """One volume-perturbation pressure sample: (sample dict, Yethiraj
+        and de Miguel averages so far, number of samples)."""
+        v = E.VpSample()
+        py, pd, n = C.c_double(), C.c_double(), C.c_longlong()
+        _check(lib().plum_sim_pressure_sample(self.h, C.byref(v), C.byref(py), C.byref(pd),
+                                              C.byref(n)))
+        return v.as_dict(), py.value, pd.value, n.value
 
     def gc_stats(self):
         """(attempted [ins, del], accepted [ins, del], n_beads, n_mols)"""

@@ -159,9 +159,9 @@ def test_delayed_acceptance_decisions_match_the_oracle():
                          r_d=3.0, seed=3)
     eng.mc_run(400, sd["move_prob"], **_kw(sd, 51))     # some online training first
     sd2 = dict(sd, xyz=eng.positions())
-    eng.mc_record(2000)
-    eng.mc_run(1500, sd["move_prob"], **_kw(sd, 52))
-    recs, n_dec = eng.mc_records(2000)
+    eng.mc_record(4000)
+    eng.mc_run(3000, sd["move_prob"], **_kw(sd, 52))
+    recs, n_dec = eng.mc_records(4000)
     assert n_dec == len(recs)
     ev = [r for r in recs if r["evaluated"]]
     assert any(not r["pass"] for r in recs) and len(ev) >= 300
