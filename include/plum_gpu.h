@@ -362,6 +362,17 @@ typedef struct {
 int plum_cbmc_trials(plum_ctx* ctx, const plum_cbmc_request* req, const double* trials, const double* grown,
                      double* energies);
 
+/* plum_cbmc_trials without any allocation.  No counterpart in the reference:
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops).  The request, the coordinates, the partial
+   sums and the energies are staged in one 128 KiB file-scope device array per
+   device instead of context buffers allocated on first use; limits: n_trials
+   <= 64, grown <= 512, at most 32768 half-space k vectors (PLUM_E_CAPACITY
+   beyond them).  Every call on a device shares that device's array: calls
+   that overlap on one device race. */
+int plum_cbmc_trials_noalloc(plum_ctx* ctx, const plum_cbmc_request* req, const double* trials,
+                             const double* grown, double* energies);
+
 typedef struct {
   int mode;              /* 0 insertion, 1 deletion, 2 Widom test insertion */
   int chain_len;         /* L beads (each with a counterion when charged) */

@@ -759,4 +759,5 @@ int plum_kernel_times(plum_ctx* c, int cap, const char** names, double* avg_us, 
 #include "batch_api.h"
 #include "surrogate_api.h"
 #include "cbmc_api.h"
+#include "noalloc_api.h"
 #include "pressure_api.h"

@@ -27,7 +27,7 @@ ABI_SYMBOLS = (
     "plum_surrogate_stats_get", "plum_surrogate_get_weights", "plum_surrogate_set_weights",
     "plum_surrogate_train", "plum_surrogate_forward", "plum_surrogate_gradient",
     "plum_surrogate_debug_slot", "plum_max_mol_len", "plum_cbmc_trials", "plum_cbmc_grow",
-    "plum_mc_record", "plum_mc_get_records", "plum_vp_sample",
+    "plum_mc_record", "plum_mc_get_records", "plum_vp_sample", "plum_cbmc_trials_noalloc",
 )
 SURROGATE_PARAMS = 64 * 128 + 128 + 2 * (128 * 128 + 128) + 128 + 1
 DESCR = 64
@@ -195,6 +195,7 @@ def lib():
         L.plum_cbmc_grow.argtypes = [vp, C.POINTER(GrowthRequest), dp, dp, dp, dp, dp, dp,
                                      C.POINTER(C.c_int), dp]
         L.plum_vp_sample.argtypes = [vp, C.c_int, C.POINTER(VpSample)]
+        L.plum_cbmc_trials_noalloc.argtypes = [vp, C.POINTER(CbmcRequest), dp, dp, dp]
         L.plum_mc_record.argtypes = [vp, C.c_int]
         L.plum_mc_get_records.argtypes = [vp, C.c_int, C.POINTER(Decision), ip, dp,
                                           C.POINTER(C.c_longlong)]
