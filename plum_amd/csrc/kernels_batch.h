@@ -664,13 +664,23 @@ __device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, 
 // wave lists the query cells 64 at a time with a wave prefix over their
 // counts, each lane takes candidate slots lane, lane + 64, ..., and a wave
 // reduction sums the lanes in fixed order.  Same pair terms as real_query.
-// A candidate's bead index, position and type are loaded together, one
-// candidate ahead of the arithmetic, so a lane waits on one round trip per
-// candidate at most and usually on none.
+// Two passes per candidate, so that the lanes never diverge on the pair
+// arithmetic: the distance filter (a candidate's bead index, position and
+// type loaded together, one candidate ahead) pushes the pairs within the
+// search radius into a per-wave queue in LDS (ballot + lane prefix); each
+// time the queue holds 64 pairs, every lane takes one and evaluates its LJ and
+// erfc terms.  About a quarter of the candidates lie within the radius, so
+// the wave runs the pair arithmetic a quarter as often as a lane-per-candidate
+// loop would.
+constexpr int kQueue = 2 * kWave;   // pair queue of one wave (holds < 2 fills)
+
 struct QWaveSmem {
   int s_cell[kWave];
   int s_img[kWave];
   int s_end[kWave];
+  double s_r2[kQueue];     // squared distance of a queued pair
+  double s_q[kQueue];      // charge of its partner
+  int s_t[kQueue];         // partner type | 1 << 16 when LJ applies (no z image in a slab)
 };
 
 struct QueryPt {
@@ -682,26 +692,18 @@ struct QueryPt {
   double lj_close;
 };
 
-// LJ + real-space terms of query point q against a bead at pj (wrapped
-// position and charge) of type tj in image img.  P = 1: FP32 pair math
+// LJ + real-space terms of one queued pair (squared distance r2 <= rs^2,
+// partner charge qj, type tj, whether LJ applies).  P = 1: FP32 pair math
 // beyond lj_close (the FP64 distance and cutoff tests are kept).
 template <int P>
-__device__ __forceinline__ void pair_terms(const Geo& g, const QueryPt& q, double4 pj, int tj, int img, double& elj,
-                                           double& ere) {
-  const int iz = ((img >> 16) & 0xff) - 128;
-  const bool lj_ok = g.use_pair && (g.npbc == 3 || iz == 0);
-  const int ix = (img & 0xff) - 128, iy = ((img >> 8) & 0xff) - 128;
-  const double dx = pj.x + ix * g.L[0] - q.pw[0];
-  const double dy = pj.y + iy * g.L[1] - q.pw[1];
-  const double dz = pj.z + iz * g.L[2] - q.pw[2];
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 > g.rs2) return;
+__device__ __forceinline__ void pair_terms(const Geo& g, const QueryPt& q, double r2, double qj, int tj, bool lj_ok,
+                                           double& elj, double& ere) {
   const double r = sqrt(r2);
   if (P == 0 || r < q.lj_close) {
     // FP64 (the FP32 path keeps the steep, large close-contact terms here)
     if (lj_ok) elj += lj_energy(r, q.ljrow[tj]);
     if (g.use_ewald) {
-      const double qq = q.p.w * pj.w;
+      const double qq = q.p.w * qj;
       if (qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
     }
     return;
@@ -716,7 +718,7 @@ __device__ __forceinline__ void pair_terms(const Geo& g, const QueryPt& q, doubl
     }
   }
   if (g.use_ewald) {
-    const float qq = q.qpf * (float)pj.w;
+    const float qq = q.qpf * (float)qj;
     if (qq != 0.f && r <= g.rc) {
       const float rf = (float)r;
       ere += (double)(q.lBf * qq * erfcf(q.saf * rf) / rf);
@@ -752,6 +754,19 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
   }
   const int ncq = nd[0] * nd[1] * nd[2];
   double elj = 0, ere = 0;
+  int qhead = 0, qn = 0;   // queue start and fill (wave-uniform)
+  // the pairs queued at qhead .. qhead + 63, one per lane
+  auto drain = [&](int count) {
+    wave_sync_lds();
+    if (lane < count) {
+      const int k = (qhead + lane) % kQueue;
+      const int t = sm.s_t[k];
+      pair_terms<P>(g, q, sm.s_r2[k], sm.s_q[k], t & 0xffff, (t >> 16) != 0, elj, ere);
+    }
+    qhead = (qhead + count) % kQueue;
+    qn -= count;
+    wave_sync_lds();
+  };
   for (int base = 0; base < ncq; base += kWave) {
     const int nloc = min(kWave, ncq - base);
     int end = 0;
@@ -811,7 +826,9 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
       pj = d.cpos[si];
       tj = d.ctype[si];
     }
-    while (it < total) {
+    // wave-uniform trip count: the lanes past `total` ride along
+    const int rounds = total > part * kWave ? (total - part * kWave + stride - 1) / stride : 0;
+    for (int rd = 0; rd < rounds; rd++) {
       const int it2 = it + stride;
       int j2 = -1, tj2 = 0, img2 = 0;
       double4 pj2 = make_double4(0, 0, 0, 0);
@@ -822,7 +839,29 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
         pj2 = d.cpos[si2];
         tj2 = d.ctype[si2];
       }
-      if (j >= 0 && !(contig ? (j >= mlo && j <= mhi) : d.moved[j])) pair_terms<P>(g, q, pj, tj, img, elj, ere);
+      // distance filter of candidate it
+      bool in = false;
+      double r2 = 0;
+      bool lj_ok = false;
+      if (it < total && j >= 0 && !(contig ? (j >= mlo && j <= mhi) : d.moved[j])) {
+        const int iz = ((img >> 16) & 0xff) - 128;
+        const int ix = (img & 0xff) - 128, iy = ((img >> 8) & 0xff) - 128;
+        const double dx = pj.x + ix * g.L[0] - q.pw[0];
+        const double dy = pj.y + iy * g.L[1] - q.pw[1];
+        const double dz = pj.z + iz * g.L[2] - q.pw[2];
+        r2 = dx * dx + dy * dy + dz * dz;
+        in = r2 <= g.rs2;
+        lj_ok = g.use_pair && (g.npbc == 3 || iz == 0);
+      }
+      const unsigned long long m = __ballot(in);
+      if (in) {
+        const int k = (qhead + qn + lanes_below(m)) % kQueue;
+        sm.s_r2[k] = r2;
+        sm.s_q[k] = pj.w;
+        sm.s_t[k] = tj | (lj_ok ? 1 << 16 : 0);
+      }
+      qn += __popcll(m);
+      if (qn >= kWave) drain(kWave);
       it = it2;
       j = j2;
       tj = tj2;
@@ -831,6 +870,7 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
     }
     wave_sync_lds();   // the next chunk rewrites the cell arrays
   }
+  if (qn > 0) drain(qn);
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     elj += __shfl_down(elj, off, kWave);
