@@ -41,6 +41,7 @@ struct plum_batch {
   float *sx = nullptr, *sf = nullptr;
   int sx_rows = 0, sx_cap = 0, maxm_b = 0;
   int tab_chunks = 1;     // phase-table chunks per proposal (plan kernel rows y > 0)
+  int fold_chunks = 0;    // S(k) fold chunks of kPlanThreads k vectors (plan kernel)
   bool profiling = false;
   std::vector<void*> allocs;
 };
@@ -208,9 +209,11 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     (void)hipEventRecord(ev[2], b->st);
   }
   if (ev) (void)hipEventRecord(ev[3], b->st);
-  // y = 0: the plan of system x; y > 0: chunk y - 1 of its phase tables
-  hipLaunchKernelGGL(k_plan_spec, dim3(R, 1 + b->tune.pmax * b->tab_chunks), dim3(kPlanThreads), 0, b->st, b->dG,
-                     b->dD, b->dMC, b->dRC, b->d_vb, R, b->tab_chunks);
+  // y = 0: the plan of system x; then chunks of its phase tables; then chunks
+  // of its S(k) fold
+  const int ntab = b->tune.pmax * b->tab_chunks;
+  hipLaunchKernelGGL(k_plan_spec, dim3(R, 1 + ntab + b->fold_chunks), dim3(kPlanThreads), 0, b->st, b->dG, b->dD,
+                     b->dMC, b->dRC, b->d_vb, R, b->tab_chunks, ntab);
   if (ev) (void)hipEventRecord(ev[4], b->st);
   if (b->precision == PLUM_FP32_PAIR)
     hipLaunchKernelGGL(k_compute_spec<1>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
@@ -219,7 +222,7 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     hipLaunchKernelGGL(k_compute_spec<0>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
                        b->d_vb, R);
   if (ev) (void)hipEventRecord(ev[5], b->st);
-  hipLaunchKernelGGL(k_reduce_spec, dim3(kKVals + kMaxProp + kMaxPairs, R), dim3(kWave), 0, b->st, b->dG, b->dD,
+  hipLaunchKernelGGL(k_reduce_spec, dim3(kKVals + kMaxProp + kMaxPairs, R), dim3(kBlock), 0, b->st, b->dG, b->dD,
                      b->dRC, R);
   if (ev) (void)hipEventRecord(ev[6], b->st);
   hipLaunchKernelGGL(k_resolve_spec, dim3(R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, R);
@@ -393,6 +396,7 @@ int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   for (int r = 0; r < n; r++) {
     const int cr = tab_chunk_rows(replicas[r]->g.tab_w);
     b->tab_chunks = std::max(b->tab_chunks, (2 * replicas[r]->maxm + cr - 1) / cr);
+    b->fold_chunks = std::max(b->fold_chunks, (replicas[r]->g.n_kh + kPlanThreads - 1) / kPlanThreads);
   }
   if (hipMemcpy(b->d_cell_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(PLUM_E_HIP, "upload of the cell offsets failed"));
@@ -722,7 +726,7 @@ int plum_batch_tune(plum_batch* b, const plum_mc_params* mcp, int moves_per_tria
   if (!b || !mcp) return fail(PLUM_E_ARG, "null argument");
   if (moves_per_trial < 1) return fail(PLUM_E_ARG, "moves_per_trial must be >= 1");
   const int bpc[] = {4, 8, 16};
-  const int div[] = {384, 768, 1536};
+  const int div[] = {192, 384, 768};
   const int pm[] = {1, 2, 4, 8, 16};
   const plum_batch_tuning keep = b->tune;
   plum_batch_tuning win = keep;

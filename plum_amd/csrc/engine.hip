@@ -433,7 +433,9 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   info.n_k_full = 0;
   if (p.use_ewald) {
     const int* n3 = info.repl_cell;
-    g.nkx = n3[0]; g.nky = n3[1]; g.nkz = n3[2];
+    // (the z tables always hold l_z = 1, the step of the speculative
+    // k-space's z recurrence)
+    g.nkx = n3[0]; g.nky = n3[1]; g.nkz = std::max(n3[2], 1);
     const double pref0 = p.lB / (kPi * info.vol_e) * (4 * kPi * kPi);
     for (int lx = -n3[0]; lx <= n3[0]; lx++)
       for (int ly = -n3[1]; ly <= n3[1]; ly++)

@@ -55,7 +55,8 @@ struct SpecPlan {
   int pass1[kMaxScreen];         // first stage of delayed acceptance passed (1 without it)
   int pair_p[kMaxPairs], pair_q[kMaxPairs];
   int pair_off[kMaxPairs + 1];   // cross-term blocks per near pair (prefix)
-  int prop_off[kMaxProp + 1];    // real-space + moved-moved blocks per exact slot (prefix)
+  int q_off[kMaxProp + 1];       // real-space query blocks per exact slot (prefix)
+  int mm_off[kMaxProp + 1];      // moved-moved blocks per exact slot (prefix)
   unsigned acc_mask;             // exact slots accepted by the last resolve: S(k) += dS pending
   int pad;
   double de_s[kMaxScreen];       // surrogate dE of each proposal (0 without it)

@@ -14,17 +14,20 @@
 //                    counter + s), its spring / wall / dipole parts and
 //                    bounding sphere (a molecule proposed twice is proposed
 //                    from its current coordinates the second time).
-//   k_plan_spec      grid (R, 1 + pmax chunks): first-stage decisions, exact
-//                    slots, block counts; pairs of exact slots within
-//                    interaction range get cross-term blocks; beside it, the
-//                    proposals' phase tables in chunks of rows.
-//   k_compute_spec   all systems' k-space blocks (dS_e(k) of every exact slot,
-//                    dE_k per slot and the cross sums 2 c_k Re(dS_p* dS_q)),
-//                    real-space query blocks (queries by waves) and
-//                    moved-moved blocks per slot, and cross blocks (LJ + real
-//                    pairs between two slots' moved beads, old/new
-//                    combinations), spread over one grid.
-//   k_reduce_spec    one wave per sum and system, fixed order.
+//   k_plan_spec      grid (R, 1 + pmax chunks + fold chunks): first-stage
+//                    decisions, exact slots, block counts; pairs of exact
+//                    slots within interaction range get cross-term blocks;
+//                    beside it, the proposals' phase tables in chunks of rows
+//                    and the fold of the previous batch's accepted dS(k) into
+//                    S(k).
+//   k_compute_spec   all systems' k-space waves (dS_e(k) of every exact slot
+//                    and its dE_k per k block), real-space query blocks
+//                    (cluster queries by waves) and moved-moved blocks per
+//                    slot, and cross blocks (LJ + real pairs between two
+//                    slots' moved beads, old/new combinations), spread over
+//                    one grid.
+//   k_reduce_spec    one block per sum and system, fixed order; the k-space
+//                    cross sums 2 c_k Re(dS_p* dS_q) formed from the dS rows.
 //   k_resolve_spec   grid R: the reference's sequential Metropolis chain over
 //                    the batch (simulation.cc:241-355), decided by one wave,
 //                    each dE corrected by the cross terms of the proposals
@@ -44,9 +47,8 @@ namespace plum {
 
 constexpr int kMaxReplicas = 1024;
 constexpr int kMaxSplit = 8;           // most blocks per real-space query
-constexpr int kDefaultSplitDiv = 1536;  // candidate slots per query block
+constexpr int kDefaultSplitDiv = 384;   // candidate slots per query share
 constexpr int kKSpec = 16;             // k vectors per speculative k-space block
-constexpr int kKSpecGroups = kBlock / kKSpec;
 
 // Per-system launch shape.
 struct RepCfg {
@@ -66,9 +68,10 @@ struct RepCfg {
   int qwave;    // waves per real-space query (query_waves)
 };
 
-// Real-space query blocks of a move of nm beads, the old and the new position
-// of each bead, qw waves per query (each takes a share of the candidate
-// slots), kQWaves waves per block.
+// Real-space query blocks of a move of nm beads: one query per cluster of up
+// to kCluster consecutive moved beads at their old and one at their new
+// positions, qw waves per query (each takes a share of the candidate slots),
+// kQWaves waves per block.
 constexpr int kQWaves = kBlock / kWave;
 constexpr int kMaxQWave = 4 * kMaxSplit;
 // One wave per split share; a batch of few proposals (fewer than 16 over all
@@ -77,7 +80,14 @@ constexpr int kMaxQWave = 4 * kMaxSplit;
 __host__ __device__ __forceinline__ int query_waves(int nsplit, int proposals) {
   return nsplit == 1 ? 1 : (proposals >= 16 ? nsplit : kQWaves * nsplit);
 }
-__host__ __device__ __forceinline__ int query_blocks(int nm, int qw) { return (2 * nm * qw + kQWaves - 1) / kQWaves; }
+constexpr int kCluster = 8;   // moved beads per real-space query (cluster)
+__host__ __device__ __forceinline__ int query_clusters(int nm) { return (nm + kCluster - 1) / kCluster; }
+__host__ __device__ __forceinline__ int query_blocks(int nm, int qw) {
+  return (2 * query_clusters(nm) * qw + kQWaves - 1) / kQWaves;
+}
+
+// k-space blocks per k block: one wave per exact slot.
+__host__ __device__ __forceinline__ int kspace_groups(int ne) { return (ne + kQWaves - 1) / kQWaves; }
 
 // Moved-moved pair blocks: 256 pairs each, independent of the launch size.
 __host__ __device__ __forceinline__ int nmm_blocks(int nm) {
@@ -316,9 +326,30 @@ __device__ void plan_tables(const Geo* __restrict__ G, const Dev* __restrict__ D
   tables_chunk(g, v, r0, min(cr, nrow - r0), stage, ws);
 }
 
+// The S(k) of the slots the previous batch accepted, folded in by chunks of
+// k vectors beside the plan (before this batch's k-space overwrites their dS
+// rows), in slot order.  A batch with no proposal leaves them pending (its
+// resolve keeps the mask).
+__device__ void plan_fold(const Geo* __restrict__ G, const Dev* __restrict__ D, int r, int f) {
+  const Geo& g = G[r];
+  const Dev& d = D[r];
+  if (d.plan->n_prop == 0) return;
+  const unsigned mask = d.plan->acc_mask;
+  const int k = f * kPlanThreads + threadIdx.x;
+  if (!mask || k >= g.n_kh) return;
+  double2 S = d.Sk[k];
+  for (int e = 0; e < kMaxProp; e++)
+    if (mask & (1u << e)) {
+      const double2 ds = d.dS[(size_t)e * g.n_kh + k];
+      S.x += ds.x;
+      S.y += ds.y;
+    }
+  d.Sk[k] = S;
+}
+
 __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                             const McDev* __restrict__ MC,
-                            const RepCfg* __restrict__ RC, int* vb_count, int R, int nch) {
+                            const RepCfg* __restrict__ RC, int* vb_count, int R, int nch, int ntab) {
   __shared__ union {
     PlanSmem p;
     double2 stage[kTabStage];
@@ -327,7 +358,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
   const int r = blockIdx.x;
   if (r >= R) return;
   if (blockIdx.y > 0) {
-    plan_tables(G, D, RC, r, blockIdx.y - 1, nch, smu.stage);
+    const int c = blockIdx.y - 1;
+    if (c < ntab) plan_tables(G, D, RC, r, c, nch, smu.stage);
+    else plan_fold(G, D, r, c - ntab);
     return;
   }
   // by-value copies through the constant address space: the per-system
@@ -406,22 +439,30 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
       if (exact >= 0) sm.s_screen[exact] = t;
     }
     wave_sync_lds();
-    // blocks per exact slot, prefix over the slots
-    int nb = 0;
+    // query and moved-moved blocks per exact slot, prefixes over the slots
+    int nq = 0, nmb = 0;
     if (t < ne) {
       const int n = sm.s_prop[sm.s_screen[t]].n;
-      nb = n > 0 ? query_blocks(n, rc.qwave) + nmm_blocks(n) : 0;
+      nq = n > 0 ? query_blocks(n, rc.qwave) : 0;
+      nmb = n > 0 ? nmm_blocks(n) : 0;
       plan->screen_of[t] = sm.s_screen[t];
     }
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
-      const int u = __shfl_up(nb, o, kWave);
-      if (t >= o) nb += u;
+      const int u = __shfl_up(nq, o, kWave), v = __shfl_up(nmb, o, kWave);
+      if (t >= o) {
+        nq += u;
+        nmb += v;
+      }
     }
-    if (t < ne) plan->prop_off[t + 1] = nb;
-    const int off = ne > 0 ? __shfl(nb, ne - 1, kWave) : 0;
+    if (t < ne) {
+      plan->q_off[t + 1] = nq;
+      plan->mm_off[t + 1] = nmb;
+    }
+    const int off = ne > 0 ? __shfl(nq, ne - 1, kWave) + __shfl(nmb, ne - 1, kWave) : 0;
     if (t == 0) {
-      plan->prop_off[0] = 0;
+      plan->q_off[0] = 0;
+      plan->mm_off[0] = 0;
       plan->n_decide = ndec;
       plan->n_exact = ne;
       sm.s_ne = ne;
@@ -481,236 +522,157 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
   }
   if (t == kWave - 1) {
     plan->n_near = cs;
-    // k-space blocks also fold in the pending S(k) of the last batch
-    vb_count[r] = (np > 0 && (ne > 0 || plan->acc_mask)) ? rc.nkb_s + sm.s_off + bs : 0;
+    // k-space blocks: one wave per exact slot and k block
+    vb_count[r] = (np > 0 && ne > 0) ? rc.nkb_s * kspace_groups(ne) + sm.s_off + bs : 0;
   }
 }
 
 // ------------------------------------------------------------ k space
-// 16 runs x 16 bead groups per block.  A run is up to kKRun k vectors of one
-// (l_x, l_y) row with consecutive l_z, all taken by one lane: it reads the
-// x, y and first z phase and e^{i theta_z} of each moved bead from the
-// tables and advances the z phase by multiplication (e^{i (l+1) theta} =
-// e^{i l theta} e^{i theta}, conjugated rows included), four table reads for
-// up to four k vectors instead of three each.  Exact slots go in chunks of
-// kSlotChunk: each slot's dS_e(k) accumulates in registers per group, the
-// groups meet in LDS, and the per-k sums land in s_t.  Then one thread per
-// value forms, over the block's k vectors, dE_e = sum c_k (2 Re(S* dS_e) +
-// |dS_e|^2) or a cross sum 2 sum c_k Re(dS_p* dS_q).  The S(k) of the slots
-// accepted by the previous batch is folded in first.
+// One wave per (k block, exact slot), kQWaves slots per block, no block
+// barriers.  A k block is kKSpec runs; a run is up to kKRun k vectors of one
+// (l_x, l_y) row with consecutive l_z.  Lane (run kk, bead group grp) reads
+// the x, y and first z phase and e^{i theta_z} of each of its moved beads at
+// both positions from the tables and advances the z phase by multiplication
+// (e^{i (l+1) theta} = e^{i l theta} e^{i theta}, conjugated rows included):
+// four table reads for up to four k vectors instead of three each.  The bead
+// groups of a run meet by wave shuffles; the run's lane writes dS_e(k) and the
+// wave sums the slot's dE_e = sum_k c_k (2 Re(S* dS_e) + |dS_e|^2) over the
+// block's k vectors (S(k) with the previous batch's acceptances folded in by
+// the plan).  The cross sums 2 sum_k c_k Re(dS_p* dS_q) are formed from the
+// dS rows by k_reduce_spec.
 constexpr int kKRun = 4;
-constexpr int kKBlk = kKSpec * kKRun;   // k vectors a block covers at most
-constexpr int kSlotChunk = 1;
-static_assert(kKVals <= kBlock, "one thread per k-space value");
-static_assert(kKBlk <= kBlock && kSlotChunk * kKBlk <= kBlock, "one thread per k vector in the block's sums");
+constexpr int kKGrp = kWave / kKSpec;   // bead groups per wave
+static_assert(kKSpec * kKGrp == kWave, "a wave is the runs x bead groups of one slot");
 static_assert(kMaxProp <= 32, "exact slots are bits of one mask");
 
-struct KSpecSmem {
-  double2 s_ds[kSlotChunk][kKSpecGroups][kKBlk];
-  double2 s_t[kMaxProp][kKBlk];
-  double2 s_S[kKBlk];
-  double s_ck[kKBlk];
-  int s_k0[kKSpec], s_len[kKSpec];
-};
-
-__device__ void kspace_spec_block(const Geo& g, const Dev& d, const RepCfg& rc, const SpecPlan* plan, int ne,
-                                  unsigned acc_mask, int blk, KSpecSmem& sm) {
-  const int kk = threadIdx.x % kKSpec, grp = threadIdx.x / kKSpec;
-  const int t = threadIdx.x;
-  if (t < kKSpec) {
-    const int r = blk * kKSpec + t;
-    const int2 run = r < rc.nrun ? d.krun[r] : make_int2(0, 0);
-    sm.s_k0[t] = run.x;
-    sm.s_len[t] = run.y;
-  }
-  __syncthreads();
-  const int k0 = sm.s_k0[kk], len = sm.s_len[kk];
+__device__ void kspace_spec_wave(const Geo& g, const Dev& d, const RepCfg& rc, const SpecPlan* plan, int kb, int e) {
+  const int lane = threadIdx.x % kWave;
+  const int kk = lane % kKSpec, grp = lane / kKSpec;
+  const int s = uload(&plan->screen_of[e]);
+  const int n = uload(&d.recs[s].n_moved);
+  const int r = kb * kKSpec + kk;
+  const int2 run = r < rc.nrun ? d.krun[r] : make_int2(0, 0);
+  const int k0 = run.x, len = run.y;
   TabIdx ti = {0, 0, 0, false, false};
   if (len > 0) ti = tab_index(g, d.kl[k0]);
   const int z1 = g.nkx + 1 + g.nky + 1 + 1;
-  // k vector kb of the block: run kb / kKRun, entry kb % kKRun
-  const int kb = t, kr = kb / kKRun, ki = kb - kr * kKRun;
-  const bool kvalid = kb < kKBlk && ki < sm.s_len[kr];
-  const int kt = kvalid ? sm.s_k0[kr] + ki : 0;
-  // the previous batch's accepted dS rows join S(k) before this batch's
-  // rows overwrite them (the chunk loop's barrier orders the two)
-  if (kb < kKBlk) {
-    double2 S = make_double2(0, 0);
-    double ck = 0;
-    if (kvalid) {
-      S = d.Sk[kt];
-      if (acc_mask) {
-        for (int e = 0; e < kMaxProp; e++)
-          if (acc_mask & (1u << e)) {
-            const double2 prev = d.dS[(size_t)e * g.n_kh + kt];
-            S.x += prev.x;
-            S.y += prev.y;
-          }
-        d.Sk[kt] = S;
-      }
-      ck = d.ck[kt];
-    }
-    sm.s_S[kb] = S;
-    sm.s_ck[kb] = ck;
+  // S(k) and c_k of the run, for the run's lane (read while the beads load)
+  double2 S[kKRun];
+  double ck[kKRun];
+#pragma unroll
+  for (int i = 0; i < kKRun; i++) {
+    const bool own = grp == 0 && i < len;
+    S[i] = own ? d.Sk[k0 + i] : make_double2(0, 0);
+    ck[i] = own ? d.ck[k0 + i] : 0.0;
   }
-  for (int e0 = 0; e0 < ne; e0 += kSlotChunk) {
+  double2 acc[kKRun];
 #pragma unroll
-    for (int c = 0; c < kSlotChunk; c++) {
-      const int e = e0 + c;
-      if (e < ne) {
-        double2 acc[kKRun];
+  for (int i = 0; i < kKRun; i++) acc[i] = make_double2(0, 0);
+  if (len > 0) {
+    const double4* mo = d.mv_old + s * rc.maxm;
+    const double2* base = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
+    for (int m = grp; m < n; m += kKGrp) {
+      const double qm = mo[m].w;
+      const double2* to = base + (size_t)(2 * m) * g.tab_w;
+      const double2* tn = to + g.tab_w;
+      // the eight table reads of both positions first, then new minus old
+      double2 ox_n = tn[ti.ox], oy_n = tn[ti.oy], oz_n = tn[ti.oz], b_n = tn[z1];
+      double2 ox_o = to[ti.ox], oy_o = to[ti.oy], oz_o = to[ti.oz], b_o = to[z1];
+      {
+        if (ti.ny) oy_n.y = -oy_n.y;
+        if (ti.nz) oz_n.y = -oz_n.y;
+        const double2 xy = cmul(ox_n, oy_n);
+        double2 z = oz_n;
 #pragma unroll
-        for (int i = 0; i < kKRun; i++) acc[i] = make_double2(0, 0);
-        const int s = uload(&plan->screen_of[e]);
-        const int n = uload(&d.recs[s].n_moved);
-        if (len > 0) {
-          const double4* mo = d.mv_old + s * rc.maxm;
-          const double2* base = d.tab + (size_t)s * rc.maxm * 2 * g.tab_w;
-          for (int m = grp; m < n; m += kKSpecGroups) {
-            const double qm = mo[m].w;
-            const double2* to = base + (size_t)(2 * m) * g.tab_w;
-            const double2* tn = to + g.tab_w;
-            // the eight table reads of both positions first, then new minus old
-            double2 ox_n = tn[ti.ox], oy_n = tn[ti.oy], oz_n = tn[ti.oz], b_n = tn[z1];
-            double2 ox_o = to[ti.ox], oy_o = to[ti.oy], oz_o = to[ti.oz], b_o = to[z1];
-            {
-              if (ti.ny) oy_n.y = -oy_n.y;
-              if (ti.nz) oz_n.y = -oz_n.y;
-              const double2 xy = cmul(ox_n, oy_n);
-              double2 z = oz_n;
-#pragma unroll
-              for (int i = 0; i < kKRun; i++) {
-                if (i) z = cmul(z, b_n);
-                const double2 e = cmul(xy, z);
-                acc[i].x += qm * e.x;
-                acc[i].y += qm * e.y;
-              }
-            }
-            {
-              if (ti.ny) oy_o.y = -oy_o.y;
-              if (ti.nz) oz_o.y = -oz_o.y;
-              const double2 xy = cmul(ox_o, oy_o);
-              double2 z = oz_o;
-#pragma unroll
-              for (int i = 0; i < kKRun; i++) {
-                if (i) z = cmul(z, b_o);
-                const double2 e = cmul(xy, z);
-                acc[i].x -= qm * e.x;
-                acc[i].y -= qm * e.y;
-              }
-            }
-          }
+        for (int i = 0; i < kKRun; i++) {
+          if (i) z = cmul(z, b_n);
+          const double2 ph = cmul(xy, z);
+          acc[i].x += qm * ph.x;
+          acc[i].y += qm * ph.y;
         }
-#pragma unroll
-        for (int i = 0; i < kKRun; i++) sm.s_ds[c][grp][kk * kKRun + i] = acc[i];
       }
-    }
-    __syncthreads();
-    // slot c of the chunk, k vector j of the block: the sum over the groups
-    if (t < kSlotChunk * kKBlk) {
-      const int c = t / kKBlk, j = t - c * kKBlk;
-      const int e = e0 + c;
-      if (e < ne) {
-        const int jr = j / kKRun, ji = j - jr * kKRun;
-        const bool jv = ji < sm.s_len[jr];
-        double2 v = make_double2(0, 0);
+      {
+        if (ti.ny) oy_o.y = -oy_o.y;
+        if (ti.nz) oz_o.y = -oz_o.y;
+        const double2 xy = cmul(ox_o, oy_o);
+        double2 z = oz_o;
 #pragma unroll
-        for (int q = 0; q < kKSpecGroups; q++) {
-          v.x += sm.s_ds[c][q][j].x;
-          v.y += sm.s_ds[c][q][j].y;
+        for (int i = 0; i < kKRun; i++) {
+          if (i) z = cmul(z, b_o);
+          const double2 ph = cmul(xy, z);
+          acc[i].x -= qm * ph.x;
+          acc[i].y -= qm * ph.y;
         }
-        if (!jv) v = make_double2(0, 0);
-        sm.s_t[e][j] = v;
-        if (jv) d.dS[(size_t)e * g.n_kh + sm.s_k0[jr] + ji] = v;
       }
     }
-    __syncthreads();
   }
-  if (ne == 0) return;
-  // one thread per value: dE of slot v (v < ne), then the cross sums of the
-  // slot pairs (v - kMaxProp = pair_index(p, q), q < ne), fixed k order
-  const int nkb = rc.nkb_s;
-  const int v = threadIdx.x;
-  if (v < kKVals) {
-    int p = -1, q = -1;
-    if (v < kMaxProp) {
-      if (v >= ne) return;
-    } else {
-      const int j = v - kMaxProp;
-      q = (int)((1.0 + sqrt(1.0 + 8.0 * j)) * 0.5);
-      while (q * (q - 1) / 2 > j) q--;
-      while ((q + 1) * q / 2 <= j) q++;
-      p = j - q * (q - 1) / 2;
-      if (q >= ne) return;
+  // the bead groups of run kk sit in lanes kk, kk + kKSpec, ...; a butterfly
+  // leaves their sum in each of them
+#pragma unroll
+  for (int o = kKSpec; o < kWave; o <<= 1)
+#pragma unroll
+    for (int i = 0; i < kKRun; i++) {
+      acc[i].x += __shfl_xor(acc[i].x, o, kWave);
+      acc[i].y += __shfl_xor(acc[i].y, o, kWave);
     }
-    double sv = 0;
-#pragma unroll 4
-    for (int i = 0; i < kKBlk; i++) {
-      const double ck = sm.s_ck[i];
-      if (v < kMaxProp) {
-        const double2 tt = sm.s_t[v][i], S = sm.s_S[i];
-        sv += ck * (2.0 * (S.x * tt.x + S.y * tt.y) + tt.x * tt.x + tt.y * tt.y);
-      } else {
-        const double2 tp = sm.s_t[p][i], tq = sm.s_t[q][i];
-        sv += 2.0 * ck * (tp.x * tq.x + tp.y * tq.y);
+  double part = 0;
+  if (grp == 0) {
+#pragma unroll
+    for (int i = 0; i < kKRun; i++)
+      if (i < len) {
+        const double2 v = acc[i];
+        d.dS[(size_t)e * g.n_kh + k0 + i] = v;
+        part += ck[i] * (2.0 * (S[i].x * v.x + S[i].y * v.y) + v.x * v.x + v.y * v.y);
       }
-    }
-    d.part_ks[(size_t)v * nkb + blk] = sv;
   }
+  part = wave_sum(part);
+  if (lane == 0) d.part_ks[(size_t)e * rc.nkb_s + kb] = part;
 }
 
 // ------------------------------------------------------------ real space
-// One real-space query per wave (kQWaves per block, no block barriers): the
-// wave lists the query cells 64 at a time with a wave prefix over their
-// counts, each lane takes candidate slots lane, lane + 64, ..., and a wave
-// reduction sums the lanes in fixed order.  Same pair terms as real_query.
-// Two passes per candidate, so that the lanes never diverge on the pair
-// arithmetic: the distance filter (a candidate's bead index, position and
-// type loaded together, one candidate ahead) pushes the pairs within the
-// search radius into a per-wave queue in LDS (ballot + lane prefix); each
-// time the queue holds 64 pairs, every lane takes one and evaluates its LJ and
-// erfc terms.  About a quarter of the candidates lie within the radius, so
-// the wave runs the pair arithmetic a quarter as often as a lane-per-candidate
-// loop would.
+// Cluster queries, one per wave (kQWaves per block, no block barriers): a wave
+// takes up to kCluster consecutive moved beads of one proposal, all at their
+// old or all at their new positions (the query points), and one share of the
+// candidate slots of the cells within the search radius of the points'
+// bounding box.  The wave lists those cells 64 at a time with a wave prefix
+// over their counts; each lane takes candidate slots lane, lane + 64, ...
+// (bead index, position and type loaded together, one candidate ahead) and
+// tests its candidate against every point, so a candidate's load, cell
+// lookup and image decode serve all the points.  The pairs within the radius
+// go into a per-wave queue in LDS (ballot + lane prefix); each time the queue
+// holds 64 pairs every lane takes one and evaluates its LJ and erfc terms, so
+// the lanes never diverge on the pair arithmetic.  A wave reduction sums the
+// lanes in fixed order.  Same pair terms as real_query.
 constexpr int kQueue = 2 * kWave;   // pair queue of one wave (holds < 2 fills)
+static_assert(kCluster <= kWave, "one lane per query point");
 
 struct QWaveSmem {
   int s_cell[kWave];
   int s_img[kWave];
   int s_end[kWave];
+  double4 s_pt[kCluster];  // query points (the image nearest the first point) and charges
+  int s_ptt[kCluster];     // their types
   double s_r2[kQueue];     // squared distance of a queued pair
-  double s_q[kQueue];      // charge of its partner
-  int s_t[kQueue];         // partner type | 1 << 16 when LJ applies (no z image in a slab)
-};
-
-struct QueryPt {
-  double4 p;
-  double pw[3];
-  const LJPair* ljrow;
-  const LJPair32* ljrow32;
-  float qpf, saf, lBf;
-  double lj_close;
+  double s_qq[kQueue];     // its charge product
+  int s_t[kQueue];         // its LJ pair index | 1 << 24 when LJ applies (no z image in a slab)
 };
 
 // LJ + real-space terms of one queued pair (squared distance r2 <= rs^2,
-// partner charge qj, type tj, whether LJ applies).  P = 1: FP32 pair math
-// beyond lj_close (the FP64 distance and cutoff tests are kept).
+// charge product qq, LJ pair index lji, whether LJ applies).  P = 1: FP32 pair
+// math beyond fp64_below (the FP64 distance and cutoff tests are kept).
 template <int P>
-__device__ __forceinline__ void pair_terms(const Geo& g, const QueryPt& q, double r2, double qj, int tj, bool lj_ok,
+__device__ __forceinline__ void pair_terms(const Geo& g, const Dev& d, double r2, double qq, int lji, bool lj_ok,
                                            double& elj, double& ere) {
   const double r = sqrt(r2);
-  if (P == 0 || r < q.lj_close) {
+  if (P == 0 || r < g.fp64_below) {
     // FP64 (the FP32 path keeps the steep, large close-contact terms here)
-    if (lj_ok) elj += lj_energy(r, q.ljrow[tj]);
-    if (g.use_ewald) {
-      const double qq = q.p.w * qj;
-      if (qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
-    }
+    if (lj_ok) elj += lj_energy(r, d.lj[lji]);
+    if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
     return;
   }
   const float r2f = (float)r2;
   if (lj_ok) {
-    const LJPair32 lp = q.ljrow32[tj];
+    const LJPair32 lp = d.lj32[lji];
     if (r < lp.cut) {
       const float s2 = lp.sigma2 / r2f;
       const float r6 = s2 * s2 * s2;
@@ -718,55 +680,83 @@ __device__ __forceinline__ void pair_terms(const Geo& g, const QueryPt& q, doubl
     }
   }
   if (g.use_ewald) {
-    const float qq = q.qpf * (float)qj;
-    if (qq != 0.f && r <= g.rc) {
+    const float qqf = (float)qq;
+    if (qqf != 0.f && r <= g.rc) {
       const float rf = (float)r;
-      ere += (double)(q.lBf * qq * erfcf(q.saf * rf) / rf);
+      ere += (double)((float)g.lB * qqf * erfcf((float)g.sqrt_alpha * rf) / rf);
     }
   }
 }
 
-// LJ + real-space energy of point p (charge p.w, type tm) against every bead
-// image in the cell list within the search radius, skipping the index range
-// [mlo, mhi] (contig) or the moved flags, one wave taking share `part` of
-// `nparts` of the candidate slots.  Result valid in lane 0.
+// LJ + real-space energy of the np points pts[0 .. np) (charges in .w, types
+// ptype[]) against every bead image in the cell list within the search
+// radius, skipping the index range [mlo, mhi] (contig) or the moved flags,
+// one wave taking share `part` of `nparts` of the candidate slots.  Result
+// valid in lane 0.
 template <int P>
-__device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm, int contig, int mlo, int mhi,
-                                   int part, int nparts, QWaveSmem& sm) {
+__device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4* __restrict__ pts,
+                                      const int* __restrict__ ptype, int np, int contig, int mlo, int mhi, int part,
+                                      int nparts, QWaveSmem& sm) {
   const int lane = threadIdx.x % kWave;
-  QueryPt q;
-  q.p = p;
-  q.pw[0] = wrap_coord(p.x, g.L[0], g.inv_L[0]);
-  q.pw[1] = wrap_coord(p.y, g.L[1], g.inv_L[1]);
-  q.pw[2] = wrap_coord(p.z, g.L[2], g.inv_L[2]);
-  q.ljrow = d.lj + tm * g.n_types;
-  q.ljrow32 = d.lj32 + tm * g.n_types;
-  q.qpf = (float)p.w;
-  q.saf = (float)g.sqrt_alpha;
-  q.lBf = (float)g.lB;
-  q.lj_close = g.fp64_below;
+  // each point as the image nearest the first point's wrapped position (an
+  // image of a point has the same pair images within the radius), and the
+  // points' bounding box
+  double bl[3], bh[3];
+  {
+    const double4 p0 = pts[0];
+    const double p0v[3] = {p0.x, p0.y, p0.z};
+    double qv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) qv[k] = wrap_coord(p0v[k], g.L[k], g.inv_L[k]);
+    if (lane < np) {
+      const double4 p = pts[lane];
+      const double pv[3] = {p.x, p.y, p.z};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double dk = pv[k] - p0v[k];
+        dk -= g.L[k] * rint(dk * g.inv_L[k]);
+        qv[k] += dk;
+      }
+      sm.s_pt[lane] = make_double4(qv[0], qv[1], qv[2], p.w);
+      sm.s_ptt[lane] = ptype[lane];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bl[k] = qv[k];
+      bh[k] = qv[k];
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        bl[k] = fmin(bl[k], __shfl_xor(bl[k], o, kWave));
+        bh[k] = fmax(bh[k], __shfl_xor(bh[k], o, kWave));
+      }
+  }
   int lo[3], nd[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    lo[k] = (int)floor((q.pw[k] - g.rs) * g.inv_h[k]);
-    const int hi = (int)floor((q.pw[k] + g.rs) * g.inv_h[k]);
+    lo[k] = (int)floor((bl[k] - g.rs) * g.inv_h[k]);
+    const int hi = (int)floor((bh[k] + g.rs) * g.inv_h[k]);
     nd[k] = hi - lo[k] + 1;
   }
   const int ncq = nd[0] * nd[1] * nd[2];
+  const int nt = g.n_types;
   double elj = 0, ere = 0;
   int qhead = 0, qn = 0;   // queue start and fill (wave-uniform)
-  // the pairs queued at qhead .. qhead + 63, one per lane
+  // the pairs queued at qhead .. qhead + count - 1, one per lane
   auto drain = [&](int count) {
     wave_sync_lds();
     if (lane < count) {
       const int k = (qhead + lane) % kQueue;
       const int t = sm.s_t[k];
-      pair_terms<P>(g, q, sm.s_r2[k], sm.s_q[k], t & 0xffff, (t >> 16) != 0, elj, ere);
+      pair_terms<P>(g, d, sm.s_r2[k], sm.s_qq[k], t & 0xffffff, (t >> 24) != 0, elj, ere);
     }
     qhead = (qhead + count) % kQueue;
     qn -= count;
     wave_sync_lds();
   };
+  wave_sync_lds();   // the points are in LDS
   for (int base = 0; base < ncq; base += kWave) {
     const int nloc = min(kWave, ncq - base);
     int end = 0;
@@ -785,12 +775,12 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
       sm.s_cell[lane] = cell;
       sm.s_img[lane] = img;
       // cells whose box (widened by a rounding margin) lies beyond the search
-      // radius hold no candidate
+      // radius of the bounding box hold no candidate
       double dmin2 = 0;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const double b0 = (double)u[k] * g.h[k] - 1e-9 * g.L[k], b1 = b0 + g.h[k] + 2e-9 * g.L[k];
-        const double dd = fmax(0.0, fmax(b0 - q.pw[k], q.pw[k] - b1));
+        const double dd = fmax(0.0, fmax(b0 - bh[k], bl[k] - b1));
         dmin2 += dd * dd;
       }
       end = dmin2 > g.rs2 ? 0 : d.ccount[cell];
@@ -839,29 +829,36 @@ __device__ double2 real_query_wave(const Geo& g, const Dev& d, double4 p, int tm
         pj2 = d.cpos[si2];
         tj2 = d.ctype[si2];
       }
-      // distance filter of candidate it
-      bool in = false;
-      double r2 = 0;
-      bool lj_ok = false;
-      if (it < total && j >= 0 && !(contig ? (j >= mlo && j <= mhi) : d.moved[j])) {
+      // candidate it (its image position) against every point
+      const bool ok = it < total && j >= 0 && !(contig ? (j >= mlo && j <= mhi) : d.moved[j]);
+      double cx = 0, cy = 0, cz = 0;
+      int tflag = 0;
+      if (ok) {
         const int iz = ((img >> 16) & 0xff) - 128;
         const int ix = (img & 0xff) - 128, iy = ((img >> 8) & 0xff) - 128;
-        const double dx = pj.x + ix * g.L[0] - q.pw[0];
-        const double dy = pj.y + iy * g.L[1] - q.pw[1];
-        const double dz = pj.z + iz * g.L[2] - q.pw[2];
-        r2 = dx * dx + dy * dy + dz * dz;
-        in = r2 <= g.rs2;
-        lj_ok = g.use_pair && (g.npbc == 3 || iz == 0);
+        cx = pj.x + ix * g.L[0];
+        cy = pj.y + iy * g.L[1];
+        cz = pj.z + iz * g.L[2];
+        tflag = (g.use_pair && (g.npbc == 3 || iz == 0)) ? 1 << 24 : 0;
       }
-      const unsigned long long m = __ballot(in);
-      if (in) {
-        const int k = (qhead + qn + lanes_below(m)) % kQueue;
-        sm.s_r2[k] = r2;
-        sm.s_q[k] = pj.w;
-        sm.s_t[k] = tj | (lj_ok ? 1 << 16 : 0);
+#pragma unroll 1
+      for (int i = 0; i < np; i++) {
+        const double4 qp = sm.s_pt[i];
+        const double dx = cx - qp.x, dy = cy - qp.y, dz = cz - qp.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const bool in = ok && r2 <= g.rs2;
+        const unsigned long long m = __ballot(in);
+        if (m) {
+          if (in) {
+            const int k = (qhead + qn + lanes_below(m)) % kQueue;
+            sm.s_r2[k] = r2;
+            sm.s_qq[k] = qp.w * pj.w;
+            sm.s_t[k] = (sm.s_ptt[i] * nt + tj) | tflag;
+          }
+          qn += __popcll(m);
+          if (qn >= kWave) drain(kWave);
+        }
       }
-      qn += __popcll(m);
-      if (qn >= kWave) drain(kWave);
       it = it2;
       j = j2;
       tj = tj2;
@@ -915,8 +912,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
     const int* __restrict__ vb_count, int R) {
   __shared__ int s_pre[kMaxReplicas];
-  __shared__ union U {
-    KSpecSmem k;
+  __shared__ struct U {
     QWaveSmem q[kQWaves];
   } sm;
   __shared__ CSpecSmem cs;
@@ -932,36 +928,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const RepCfg rc = load_const(RC, r);
     const SpecPlan* plan = d.plan;
     const int ne = uload(&plan->n_exact);
-    if (b < rc.nkb_s) {
-      kspace_spec_block(g, d, rc, plan, ne, (unsigned)uload((const int*)&plan->acc_mask), b, sm.k);
+    const int ngrp = kspace_groups(ne);
+    // longest work first: queries, k space, moved-moved pairs, cross terms
+    const int nqt = uload(&plan->q_off[ne]);
+    if (b < nqt) {
+      int e = 0;
+      while (e + 1 < ne && uload(&plan->q_off[e + 1]) <= b) e++;
+      const int local = b - uload(&plan->q_off[e]);
+      const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
+      const int nm = uload(&v.rec->n_moved);
+      const int qw = rc.qwave;
+      // wave gw: share gw % qw of query q2 = gw / qw = 2 c + on, cluster c
+      // (moved beads c kCluster ..) at their old (on = 0) or new positions
+      const int w = threadIdx.x / kWave, gw = local * kQWaves + w;
+      if (gw < 2 * query_clusters(nm) * qw) {
+        const int q2 = gw / qw, part = gw - q2 * qw;
+        const int m0 = (q2 >> 1) * kCluster, on = q2 & 1;
+        const double2 eq = real_query_cluster<P>(g, v, (on ? v.mv_new : v.mv_old) + m0, v.mv_type + m0,
+                                                 min(kCluster, nm - m0), v.rec->contig, v.rec->lo, v.rec->hi, part,
+                                                 qw, sm.q[w]);
+        const double sgn = on ? 1.0 : -1.0;
+        if (threadIdx.x % kWave == 0) v.part_r[gw] = make_double2(sgn * eq.x, sgn * eq.y);
+      }
+    } else if ((b -= nqt) < rc.nkb_s * ngrp) {
+      // k block b / ngrp, exact slots kQWaves (b % ngrp) + wave
+      const int kb = b / ngrp, e = (b - kb * ngrp) * kQWaves + threadIdx.x / kWave;
+      if (e < ne) kspace_spec_wave(g, d, rc, plan, kb, e);
     } else {
-      b -= rc.nkb_s;
-      const int nprop_blocks = uload(&plan->prop_off[ne]);
-      if (b < nprop_blocks) {
+      b -= rc.nkb_s * ngrp;
+      const int nmt = uload(&plan->mm_off[ne]);
+      if (b < nmt) {
         int e = 0;
-        while (e + 1 < ne && uload(&plan->prop_off[e + 1]) <= b) e++;
-        const int local = b - uload(&plan->prop_off[e]);
+        while (e + 1 < ne && uload(&plan->mm_off[e + 1]) <= b) e++;
         const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
         const int nm = uload(&v.rec->n_moved);
-        const int qw = rc.qwave, nqb = query_blocks(nm, qw);
-        if (local < nqb) {
-          // wave gw: share gw % qw of query q2 = gw / qw = 2 m + on, moved
-          // bead m at its old (on = 0) or new position
-          const int w = threadIdx.x / kWave, gw = local * kQWaves + w;
-          if (gw < 2 * nm * qw) {
-            const int q2 = gw / qw, part = gw - q2 * qw;
-            const int m = q2 >> 1, on = q2 & 1;
-            const double4 p = on ? v.mv_new[m] : v.mv_old[m];
-            const double2 eq =
-                real_query_wave<P>(g, v, p, v.mv_type[m], v.rec->contig, v.rec->lo, v.rec->hi, part, qw, sm.q[w]);
-            const double sgn = on ? 1.0 : -1.0;
-            if (threadIdx.x % kWave == 0) v.part_r[gw] = make_double2(sgn * eq.x, sgn * eq.y);
-          }
-        } else {
-          mm_block(g, v, local - nqb, nmm_blocks(nm), cs.sh);
-        }
+        mm_block(g, v, b - uload(&plan->mm_off[e]), nmm_blocks(nm), cs.sh);
       } else {
-        b -= nprop_blocks;
+        b -= nmt;
         const int nn = uload(&plan->n_near);
         int j = 0;
         while (j + 1 < nn && uload(&plan->pair_off[j + 1]) <= b) j++;
@@ -972,64 +975,84 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         cross_block(g, d, rc, sp, sq, np_, nq, b - off, cs.sh, d.part_x + b);
       }
     }
-    __syncthreads();  // the LDS union is reused by the next virtual block
+    __syncthreads();  // the LDS is reused by the next virtual block
   }
 }
 
-// Sums of a batch, one wave per (sum, system), fixed order: blockIdx.x in
-// [0, kKVals) k-space sums, then kMaxProp (LJ, real) sums per exact slot
-// (real-space query blocks + moved-moved blocks), then kMaxPairs cross sums.
-__global__ __launch_bounds__(kWave) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
-                                                       const RepCfg* __restrict__ RC, int R) {
+// Sums of a batch, one block per (sum, system), fixed order: blockIdx.x in
+// [0, kMaxProp) the k-space dE per exact slot (the k blocks' partials), then
+// kMaxPairs k-space cross sums 2 sum_k c_k Re(dS_p* dS_q) of the slot pairs
+// p < q (pair_index order) from the dS rows, then kMaxProp (LJ, real) sums per
+// exact slot (real-space query blocks + moved-moved blocks), then kMaxPairs
+// real-space cross sums of the near pairs.
+__global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
+                                                        const RepCfg* __restrict__ RC, int R) {
+  __shared__ double sh[kBlock / kWave];
   const int j = blockIdx.x, r = blockIdx.y;
   if (r >= R) return;
+  const Geo& g = G[r];
   const Dev& d = D[r];
   const RepCfg& rc = RC[r];
   SpecPlan* plan = d.plan;
   if (plan->n_prop == 0) return;
   const int ne = plan->n_exact;
   const int t = threadIdx.x;
-  if (j < kKVals) {
-    if (j >= ne && (j < kMaxProp || j >= kMaxProp + ne * (ne - 1) / 2)) return;
-    if (rc.nkb_s == 0) {
-      if (t == 0) plan->ksum[j] = 0;
-      return;
-    }
+  if (j < kMaxProp) {
+    if (j >= ne) return;
     double s = 0;
-    for (int i = t; i < rc.nkb_s; i += kWave) s += d.part_ks[(size_t)j * rc.nkb_s + i];
-    s = wave_sum(s);
+    for (int i = t; i < rc.nkb_s; i += kBlock) s += d.part_ks[(size_t)j * rc.nkb_s + i];
+    s = block_sum<kBlock>(s, sh);
     if (t == 0) plan->ksum[j] = s;
+  } else if (j < kKVals) {
+    const int pj = j - kMaxProp;
+    int q = (int)((1.0 + sqrt(1.0 + 8.0 * pj)) * 0.5);
+    while (q * (q - 1) / 2 > pj) q--;
+    while ((q + 1) * q / 2 <= pj) q++;
+    const int p = pj - q * (q - 1) / 2;
+    if (q >= ne) return;
+    double s = 0;
+    if (rc.nkb_s > 0) {
+      const double2* a = d.dS + (size_t)p * g.n_kh;
+      const double2* b = d.dS + (size_t)q * g.n_kh;
+#pragma unroll 8
+      for (int k = t; k < g.n_kh; k += kBlock) {
+        const double2 u = a[k], v = b[k];
+        s += d.ck[k] * (u.x * v.x + u.y * v.y);
+      }
+    }
+    s = block_sum<kBlock>(s, sh);
+    if (t == 0) plan->ksum[j] = 2.0 * s;
   } else if (j < kKVals + kMaxProp) {
     const int e = j - kKVals;
     if (e >= ne) return;
     const int nm = d.recs[plan->screen_of[e]].n_moved;
     if (nm == 0) return;
-    const int nreal = 2 * nm * rc.qwave, nmm = nmm_blocks(nm);
+    const int nreal = 2 * query_clusters(nm) * rc.qwave, nmm = nmm_blocks(nm);
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;
-    for (int i = t; i < nreal; i += kWave) {
+    for (int i = t; i < nreal; i += kBlock) {
       lj += pr[i].x;
       re += pr[i].y;
     }
-    for (int i = t; i < nmm; i += kWave) {
+    for (int i = t; i < nmm; i += kBlock) {
       lj += pm[i].x;
       re += pm[i].y;
     }
-    lj = wave_sum(lj);
-    re = wave_sum(re);
+    lj = block_sum<kBlock>(lj, sh);
+    re = block_sum<kBlock>(re, sh);
     if (t == 0) plan->rsum[e] = make_double2(lj, re);
   } else {
     const int jj = j - kKVals - kMaxProp;
     if (jj >= plan->n_near) return;
     const int off = plan->pair_off[jj], nblk = plan->pair_off[jj + 1] - off;
     double lj = 0, re = 0;
-    for (int i = t; i < nblk; i += kWave) {
+    for (int i = t; i < nblk; i += kBlock) {
       lj += d.part_x[off + i].x;
       re += d.part_x[off + i].y;
     }
-    lj = wave_sum(lj);
-    re = wave_sum(re);
+    lj = block_sum<kBlock>(lj, sh);
+    re = block_sum<kBlock>(re, sh);
     if (t == 0) plan->xsum[jj] = make_double2(lj, re);
   }
 }
