@@ -222,8 +222,8 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     hipLaunchKernelGGL(k_compute_spec<0>, dim3(batch_grid(b)), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC,
                        b->d_vb, R);
   if (ev) (void)hipEventRecord(ev[5], b->st);
-  hipLaunchKernelGGL(k_reduce_spec, dim3(kKVals + kMaxProp + kMaxPairs, R), dim3(kBlock), 0, b->st, b->dG, b->dD,
-                     b->dRC, R);
+  hipLaunchKernelGGL(k_reduce_spec, dim3(kKVals + kMaxProp + kMaxPairs, R), dim3(kReduceThreads), 0, b->st, b->dG,
+                     b->dD, b->dRC, R);
   if (ev) (void)hipEventRecord(ev[6], b->st);
   hipLaunchKernelGGL(k_resolve_spec, dim3(R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dMC, b->dRC, R);
   if (ev) (void)hipEventRecord(ev[7], b->st);

@@ -39,7 +39,8 @@ struct PropInfo {
   int n, mol, type;    // moved beads, molecule, move type
   int valid, skipped;  // attempted move / dropped after a cell overflow
   int prev;            // earlier proposal of the same molecule in the batch, -1 none
-  int pad;
+  int len;             // beads of the molecule
+  double bcur;         // spring energy of the molecule at its current coordinates
 };
 
 // Plan and reduced sums of a speculative batch.  Proposals live in screen

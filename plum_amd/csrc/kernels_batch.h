@@ -48,7 +48,7 @@ namespace plum {
 constexpr int kMaxReplicas = 1024;
 constexpr int kMaxSplit = 8;           // most blocks per real-space query
 constexpr int kDefaultSplitDiv = 384;   // candidate slots per query share
-constexpr int kKSpec = 16;             // k vectors per speculative k-space block
+constexpr int kKSpec = 32;             // k-vector runs per speculative k-space wave
 
 // Per-system launch shape.
 struct RepCfg {
@@ -209,6 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   const int nm = v.rec->n_moved;
   const int mol = v.rec->mol;
   const int a = d.mol_start[mol], bend = d.mol_start[mol + 1];
+  const double bcur = d.bond_cur[mol];
   double ebt = 0;
   // (a valid move that moves no bead keeps the molecule's current energy)
   if (v.rec->valid && g.use_bond && bend - a > 1)
@@ -279,6 +280,8 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
     pi.valid = v.rec->valid;
     pi.skipped = (overflow && was_valid) ? 1 : 0;
     pi.prev = sm.s_prev;
+    pi.len = bend - a;
+    pi.bcur = bcur;
   }
 }
 
@@ -376,8 +379,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
   if (t < np) {
     const PropInfo P = d.prop[t];
     sm.s_prop[t] = P;
-    sm.s_bcur[t] = d.bond_cur[P.mol];
-    sm.s_len[t] = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
+    sm.s_bcur[t] = P.bcur;
+    sm.s_len[t] = P.len;
     const unsigned long long ctr = ctr0 + t;
     const uint4 u = Philox::gen(make_uint4((uint32_t)ctr, (uint32_t)(ctr >> 32), 0xDA1u, 0u),
                                 make_uint2((uint32_t)mc.seed, (uint32_t)(mc.seed >> 32)));
@@ -556,15 +559,6 @@ __device__ void kspace_spec_wave(const Geo& g, const Dev& d, const RepCfg& rc, c
   TabIdx ti = {0, 0, 0, false, false};
   if (len > 0) ti = tab_index(g, d.kl[k0]);
   const int z1 = g.nkx + 1 + g.nky + 1 + 1;
-  // S(k) and c_k of the run, for the run's lane (read while the beads load)
-  double2 S[kKRun];
-  double ck[kKRun];
-#pragma unroll
-  for (int i = 0; i < kKRun; i++) {
-    const bool own = grp == 0 && i < len;
-    S[i] = own ? d.Sk[k0 + i] : make_double2(0, 0);
-    ck[i] = own ? d.ck[k0 + i] : 0.0;
-  }
   double2 acc[kKRun];
 #pragma unroll
   for (int i = 0; i < kKRun; i++) acc[i] = make_double2(0, 0);
@@ -620,9 +614,9 @@ __device__ void kspace_spec_wave(const Geo& g, const Dev& d, const RepCfg& rc, c
 #pragma unroll
     for (int i = 0; i < kKRun; i++)
       if (i < len) {
-        const double2 v = acc[i];
+        const double2 v = acc[i], S = d.Sk[k0 + i];
         d.dS[(size_t)e * g.n_kh + k0 + i] = v;
-        part += ck[i] * (2.0 * (S[i].x * v.x + S[i].y * v.y) + v.x * v.x + v.y * v.y);
+        part += d.ck[k0 + i] * (2.0 * (S.x * v.x + S.y * v.y) + v.x * v.x + v.y * v.y);
       }
   }
   part = wave_sum(part);
@@ -979,15 +973,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
 }
 
+constexpr int kReduceThreads = 1024;
+
 // Sums of a batch, one block per (sum, system), fixed order: blockIdx.x in
 // [0, kMaxProp) the k-space dE per exact slot (the k blocks' partials), then
 // kMaxPairs k-space cross sums 2 sum_k c_k Re(dS_p* dS_q) of the slot pairs
 // p < q (pair_index order) from the dS rows, then kMaxProp (LJ, real) sums per
 // exact slot (real-space query blocks + moved-moved blocks), then kMaxPairs
 // real-space cross sums of the near pairs.
-__global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                         const RepCfg* __restrict__ RC, int R) {
-  __shared__ double sh[kBlock / kWave];
+  __shared__ double sh[kReduceThreads / kWave];
   const int j = blockIdx.x, r = blockIdx.y;
   if (r >= R) return;
   const Geo& g = G[r];
@@ -1000,8 +996,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
   if (j < kMaxProp) {
     if (j >= ne) return;
     double s = 0;
-    for (int i = t; i < rc.nkb_s; i += kBlock) s += d.part_ks[(size_t)j * rc.nkb_s + i];
-    s = block_sum<kBlock>(s, sh);
+    for (int i = t; i < rc.nkb_s; i += kReduceThreads) s += d.part_ks[(size_t)j * rc.nkb_s + i];
+    s = block_sum<kReduceThreads>(s, sh);
     if (t == 0) plan->ksum[j] = s;
   } else if (j < kKVals) {
     const int pj = j - kMaxProp;
@@ -1014,13 +1010,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     if (rc.nkb_s > 0) {
       const double2* a = d.dS + (size_t)p * g.n_kh;
       const double2* b = d.dS + (size_t)q * g.n_kh;
-#pragma unroll 8
-      for (int k = t; k < g.n_kh; k += kBlock) {
+#pragma unroll 4
+      for (int k = t; k < g.n_kh; k += kReduceThreads) {
         const double2 u = a[k], v = b[k];
         s += d.ck[k] * (u.x * v.x + u.y * v.y);
       }
     }
-    s = block_sum<kBlock>(s, sh);
+    s = block_sum<kReduceThreads>(s, sh);
     if (t == 0) plan->ksum[j] = 2.0 * s;
   } else if (j < kKVals + kMaxProp) {
     const int e = j - kKVals;
@@ -1031,28 +1027,28 @@ __global__ __launch_bounds__(kBlock) void k_reduce_spec(const Geo* __restrict__ 
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;
-    for (int i = t; i < nreal; i += kBlock) {
+    for (int i = t; i < nreal; i += kReduceThreads) {
       lj += pr[i].x;
       re += pr[i].y;
     }
-    for (int i = t; i < nmm; i += kBlock) {
+    for (int i = t; i < nmm; i += kReduceThreads) {
       lj += pm[i].x;
       re += pm[i].y;
     }
-    lj = block_sum<kBlock>(lj, sh);
-    re = block_sum<kBlock>(re, sh);
+    lj = block_sum<kReduceThreads>(lj, sh);
+    re = block_sum<kReduceThreads>(re, sh);
     if (t == 0) plan->rsum[e] = make_double2(lj, re);
   } else {
     const int jj = j - kKVals - kMaxProp;
     if (jj >= plan->n_near) return;
     const int off = plan->pair_off[jj], nblk = plan->pair_off[jj + 1] - off;
     double lj = 0, re = 0;
-    for (int i = t; i < nblk; i += kBlock) {
+    for (int i = t; i < nblk; i += kReduceThreads) {
       lj += d.part_x[off + i].x;
       re += d.part_x[off + i].y;
     }
-    lj = block_sum<kBlock>(lj, sh);
-    re = block_sum<kBlock>(re, sh);
+    lj = block_sum<kReduceThreads>(lj, sh);
+    re = block_sum<kReduceThreads>(re, sh);
     if (t == 0) plan->xsum[jj] = make_double2(lj, re);
   }
 }
@@ -1353,8 +1349,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
   if (t < np) {
     const PropInfo P = d.prop[t];
     sm.s_prop[t] = P;
-    sm.s_bcur[t] = d.bond_cur[P.mol];
-    sm.s_len[t] = d.mol_start[P.mol + 1] - d.mol_start[P.mol];
+    sm.s_bcur[t] = P.bcur;
+    sm.s_len[t] = P.len;
     sm.s_des[t] = plan->de_s[t];
     sm.s_pass[t] = plan->pass1[t];
     sm.s_exact[t] = plan->exact_of[t];

@@ -480,11 +480,18 @@ struct MoveItems {
 };
 
 // Per-item scratch of a commit: new cell, key (the new cell of a bead that
-// changes cell, -1 otherwise; read four at a time) and arrival rank/count.
+// changes cell, -1 otherwise; read four at a time), arrival rank/count, and
+// what the commit reads of the bead (cell, slot, type) and of its new cell
+// (count), all read in one pass.
 struct CommitSmem {
   alignas(16) int key[kMaxChainDev];
   int nc[kMaxChainDev];
   int rank[kMaxChainDev];
+  int oc[kMaxChainDev];    // cell of the bead before the commit
+  int os[kMaxChainDev];    // its slot there
+  int ty[kMaxChainDev];    // its type
+  int cnt[kMaxChainDev];   // count of its new cell
+  int compacted;           // a destination cell was compacted (slots and counts moved)
 };
 
 // Positions and cell list of accepted moved beads (parallel, deterministic:
@@ -492,12 +499,17 @@ struct CommitSmem {
 // consecutive slots in item order).  Runs inside one block.  The items may
 // span several accepted moves of distinct molecules: the beads left live in
 // every cell, and their order, are those of committing the moves one after
-// the other.
+// the other.  Everything the commit reads is read in one pass first; only a
+// compaction (rare) moves slots and counts, and then they are read again.
 template <class Items>
 __device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, CommitSmem& cs) {
   Ctl* C = d.ctl;
   const int nm = items.n;
   const int n4 = (nm + 3) >> 2;
+  if (threadIdx.x == 0) cs.compacted = 0;
+  // item threadIdx.x kept in registers when every item has its own thread
+  double4 pn0 = make_double4(0, 0, 0, 0);
+  int i0 = -1;
   // new cell of every moved bead
   for (int k = threadIdx.x; k < 4 * n4; k += kBlock) {
     if (k >= nm) {
@@ -507,13 +519,22 @@ __device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, Com
     double4 pn;
     int i;
     items.get(k, pn, i);
+    if (k == threadIdx.x) {
+      pn0 = pn;
+      i0 = i;
+    }
     const double xw = wrap_coord(pn.x, g.L[0], g.inv_L[0]);
     const double yw = wrap_coord(pn.y, g.L[1], g.inv_L[1]);
     const double zw = wrap_coord(pn.z, g.L[2], g.inv_L[2]);
     const int nc = (cell_coord(xw, g.inv_h[0], g.nc[0]) * g.nc[1] + cell_coord(yw, g.inv_h[1], g.nc[1])) *
                        g.nc[2] + cell_coord(zw, g.inv_h[2], g.nc[2]);
+    const int oc = d.bcell[i], os = d.bslot[i], ty = d.type[i], cnt = d.ccount[nc];
     cs.nc[k] = nc;
-    cs.key[k] = (d.bcell[i] != nc) ? nc : -1;
+    cs.key[k] = (oc != nc) ? nc : -1;
+    cs.oc[k] = oc;
+    cs.os[k] = os;
+    cs.ty[k] = ty;
+    cs.cnt[k] = cnt;
   }
   __syncthreads();
   // rank of each arrival among the arrivals in its cell (item order) and
@@ -534,13 +555,22 @@ __device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, Com
       before += (m0 & (b < k)) + (m1 & (b + 1 < k)) + (m2 & (b + 2 < k)) + (m3 & (b + 3 < k));
     }
     cs.rank[k] = before | (total << 16);
-    if (before == 0 && d.ccount[key] + total > g.cap) compact_cell(g, d, key);
+    if (before == 0 && cs.cnt[k] + total > g.cap) {
+      compact_cell(g, d, key);
+      cs.compacted = 1;
+    }
   }
   __syncthreads();
+  const bool reread = cs.compacted != 0;
   for (int k = threadIdx.x; k < nm; k += kBlock) {
     double4 pn;
     int i;
-    items.get(k, pn, i);
+    if (nm <= kBlock) {
+      pn = pn0;
+      i = i0;
+    } else {
+      items.get(k, pn, i);
+    }
     d.x[i] = pn.x; d.y[i] = pn.y; d.z[i] = pn.z;
     const double xw = wrap_coord(pn.x, g.L[0], g.inv_L[0]);
     const double yw = wrap_coord(pn.y, g.L[1], g.inv_L[1]);
@@ -549,22 +579,23 @@ __device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, Com
     const int cx = nc / (g.nc[1] * g.nc[2]), cy = (nc / g.nc[2]) % g.nc[1], cz = nc % g.nc[2];
     const float4 rel = make_float4((float)(xw - cx * g.h[0]), (float)(yw - cy * g.h[1]),
                                    (float)(zw - cz * g.h[2]), (float)pn.w);
+    const int os = reread ? d.bslot[i] : cs.os[k];
     if (cs.key[k] < 0) {
-      const size_t si = (size_t)nc * g.cap + d.bslot[i];
+      const size_t si = (size_t)nc * g.cap + os;
       d.cpos[si] = make_double4(xw, yw, zw, pn.w);
       d.cpos32[si] = rel;
       continue;
     }
-    const int oc = d.bcell[i];
-    if (oc >= 0) d.cbead[(size_t)oc * g.cap + d.bslot[i]] = -1;
+    const int oc = cs.oc[k];
+    if (oc >= 0) d.cbead[(size_t)oc * g.cap + os] = -1;
     const int rank = cs.rank[k] & 0xffff, total = cs.rank[k] >> 16;
-    const int slot = d.ccount[nc] + rank;
+    const int slot = (reread ? d.ccount[nc] : cs.cnt[k]) + rank;
     if (slot < g.cap) {
       const size_t sn = (size_t)nc * g.cap + slot;
       d.cbead[sn] = i;
       d.cpos[sn] = make_double4(xw, yw, zw, pn.w);
       d.cpos32[sn] = rel;
-      d.ctype[sn] = d.type[i];
+      d.ctype[sn] = cs.ty[k];
       d.bcell[i] = nc;
       d.bslot[i] = slot;
     } else {
