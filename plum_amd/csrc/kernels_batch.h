@@ -217,8 +217,15 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   if (threadIdx.x == 0) sm.s_forbid = 0;
   __syncthreads();
   double ew = 0, dmz = 0, cx = 0, cy = 0, cz = 0;
+  // bead threadIdx.x kept in registers for the radius pass when every bead
+  // has its own thread
+  double4 pn0 = make_double4(0, 0, 0, 0), po0 = pn0;
   for (int k = threadIdx.x; k < nm; k += kBlock) {
     const double4 pn = v.mv_new[k], po = v.mv_old[k];
+    if (k == threadIdx.x) {
+      pn0 = pn;
+      po0 = po;
+    }
     if (g.use_ext) {
       const WallType w = d.wall[v.mv_type[k]];
       const double en = wall_energy(pn.z, w, g);
@@ -251,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_propose_spec(const Geo* __restrict__
   const double c0 = t5[2] * inv, c1 = t5[3] * inv, c2 = t5[4] * inv;
   double rad = 0;
   for (int k = threadIdx.x; k < nm; k += kBlock) {
-    const double4 pn = v.mv_new[k], po = v.mv_old[k];
+    const double4 pn = nm <= kBlock ? pn0 : v.mv_new[k], po = nm <= kBlock ? po0 : v.mv_old[k];
     const double dn = (pn.x - c0) * (pn.x - c0) + (pn.y - c1) * (pn.y - c1) + (pn.z - c2) * (pn.z - c2);
     const double dd = (po.x - c0) * (po.x - c0) + (po.y - c1) * (po.y - c1) + (po.z - c2) * (po.z - c2);
     rad = fmax(rad, fmax(dn, dd));
@@ -1090,6 +1097,7 @@ struct ResSmem {
   int s_drec[kMaxScreen];        // decision-record index of a proposal, -1 none
   int s_goff[kMaxProp + 1], s_gscr[kMaxProp];
   int s_mask, s_ndone;
+  double s_Mz, s_dipl, s_run[8];   // control block: dipole moment, dipole lag, running totals
 };
 
 static_assert(kMaxScreen <= kWave, "one lane per proposal in the decision wave");
@@ -1139,9 +1147,9 @@ __device__ void resolve_decide(const Geo& g, const Dev& d, const RepCfg& rc, Ctl
   const bool evaluated = P.valid && (pass || e >= 0);
   bool stop_prev = false, stale = false;
   unsigned mask = 0;
-  double Mz = C->Mz, cur_dipl = C->cur_dipl;
+  double Mz = sm.s_Mz, cur_dipl = sm.s_dipl;
   double run[8];
-  for (int k = 0; k < 8; k++) run[k] = C->running[k];
+  for (int k = 0; k < 8; k++) run[k] = sm.s_run[k];
   // what each proposal was decided with
   double f_tt[8] = {0, 0, 0, 0, 0, 0, 0, 0}, f_dE = 0, f_target = 0;
   bool f_acc = false, f_early = false;
@@ -1371,6 +1379,11 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
     sm.s_train[t - 192] = -1;
   }
   if (t >= 224 && t < 224 + kMaxScreen) sm.s_drec[t - 224] = -1;
+  if (t == kBlock - 1) {
+    sm.s_Mz = C->Mz;
+    sm.s_dipl = C->cur_dipl;
+  }
+  if (t >= kBlock - 9 && t < kBlock - 1) sm.s_run[t - (kBlock - 9)] = C->running[t - (kBlock - 9)];
   __syncthreads();
   {
     const int nn = plan->n_near;
@@ -1396,6 +1409,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_spec(const Geo* __restrict__
   const unsigned mask = sm.s_mask;
   int e0 = 0;
   while (e0 < kMaxProp) {
+    if (e0 > 0) __syncthreads();   // the last group's commit is done with the group arrays
     int ns = 0, n = 0, e = e0;
     for (; e < kMaxProp; e++) {
       if (!(mask & (1u << e))) continue;

@@ -501,6 +501,8 @@ struct CommitSmem {
 // every cell, and their order, are those of committing the moves one after
 // the other.  Everything the commit reads is read in one pass first; only a
 // compaction (rare) moves slots and counts, and then they are read again.
+// No barrier at the end: a caller that commits again, or reads what the
+// commit wrote, synchronizes the block first.
 template <class Items>
 __device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, CommitSmem& cs) {
   Ctl* C = d.ctl;
@@ -603,13 +605,16 @@ __device__ void commit_cells(const Geo& g, const Dev& d, const Items& items, Com
       d.bcell[i] = -1;
       d.bslot[i] = -1;
     }
-    // the last arrival writes the cell's new count
-    cs.rank[k] = rank == total - 1 ? min(slot + 1, g.cap) : -1;
+    // the last arrival writes the cell's new count (after every thread has
+    // read the counts when they are read again)
+    const int cnt_new = rank == total - 1 ? min(slot + 1, g.cap) : -1;
+    if (!reread && cnt_new >= 0) d.ccount[nc] = cnt_new;
+    cs.rank[k] = cnt_new;
   }
+  if (!reread) return;
   __syncthreads();
   for (int k = threadIdx.x; k < nm; k += kBlock)
     if (cs.key[k] >= 0 && cs.rank[k] >= 0) d.ccount[cs.nc[k]] = cs.rank[k];
-  __syncthreads();
 }
 
 __device__ void commit_block(const Geo& g, const Dev& d, int acc, CommitSmem& cs) {
