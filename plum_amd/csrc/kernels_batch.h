@@ -109,6 +109,14 @@ __device__ __forceinline__ T load_const(const T* p, int i) {
 
 __device__ __forceinline__ int uload(const int* p) { return __builtin_amdgcn_readfirstlane(*p); }
 
+// A value every lane of the wave holds, kept in scalar registers.
+__device__ __forceinline__ double uniform_double(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Set bits of m below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -733,6 +741,12 @@ __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4*
         bl[k] = fmin(bl[k], __shfl_xor(bl[k], o, kWave));
         bh[k] = fmax(bh[k], __shfl_xor(bh[k], o, kWave));
       }
+    // the butterfly leaves the box in every lane: scalar registers from here
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bl[k] = uniform_double(bl[k]);
+      bh[k] = uniform_double(bh[k]);
+    }
   }
   int lo[3], nd[3];
 #pragma unroll
@@ -792,7 +806,7 @@ __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4*
       if (lane >= o) end += v;
     }
     sm.s_end[lane] = end;
-    const int total = __shfl(end, nloc - 1, kWave);
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(end, nloc - 1, kWave));
     wave_sync_lds();
     const int stride = nparts * kWave;
     // candidate it: its cell slot and image (binary search over the wave's
