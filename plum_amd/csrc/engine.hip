@@ -515,6 +515,17 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   ALLOC(d.recs, kMaxScreen); ALLOC(d.smv_idx, (size_t)kMaxProp * maxm);
   ALLOC(d.prop, kMaxScreen); ALLOC(d.plan, 1);
   ALLOC(d.krun, std::max(1, c->nrun));
+  // erfc nodes {erfc(x_i), (2 / sqrt(pi)) exp(-x_i^2)}, x_i = i / kErfcPerUnit,
+  // each rounded once from long double (erfc_nodes)
+  std::vector<double2> erfc_h(kErfcNodes);
+  for (int i = 0; i < kErfcNodes; i++) {
+    const long double x = (long double)i / kErfcPerUnit;
+    erfc_h[i] = make_double2((double)erfcl(x), (double)(2.0L / sqrtl(3.141592653589793238462643383279502884L) *
+                                                        expl(-x * x)));
+  }
+  double2* erfc_d = nullptr;
+  ALLOC(erfc_d, kErfcNodes);
+  g.erfc_tab = erfc_d;
   c->nkb_s = (g.use_ewald && g.n_kh > 0) ? (c->nrun + kKSpec - 1) / kKSpec : 0;
   ALLOC(d.part_ks, (size_t)kKVals * std::max(1, c->nkb_s));
   c->xcap = kMaxPairs * cross_blocks(maxm, maxm);
@@ -563,7 +574,8 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
       (rc = up(d.ck, ck.data(), sizeof(double) * ck.size())) ||
       (rc = up(d.lj, lj.data(), sizeof(LJPair) * lj.size())) ||
       (rc = up(d.lj32, lj32.data(), sizeof(LJPair32) * lj32.size())) ||
-      (rc = up(d.wall, wall.data(), sizeof(WallType) * wall.size())))
+      (rc = up(d.wall, wall.data(), sizeof(WallType) * wall.size())) ||
+      (rc = up(erfc_d, erfc_h.data(), sizeof(double2) * kErfcNodes)))
     return bail(rc);
   if (hipMemsetAsync(d.moved, 0, n, c->st) != hipSuccess ||
       hipMemsetAsync(d.mv_slot, 0xff, sizeof(int) * n, c->st) != hipSuccess ||

@@ -51,7 +51,50 @@ struct Geo {
   int dipole, dipole_exact;
   double fp64_below;    // FP32 path: pairs closer than this use FP64 math
   int single_image;     // only the minimum image can lie within rc (real_pair)
+  const double2* erfc_tab;  // erfc nodes (erfc_nodes), kErfcNodes entries
 };
+
+// FP64 erfc(x), x >= 0, by the Taylor series about the nearest tabulated
+// node x_i = i / kErfcPerUnit, with the derivatives of erfc in terms of
+// Hermite polynomials (Abramowitz & Stegun 7.1.19: the (n+1)-th derivative of
+// erfc(x) is -(2 / sqrt(pi)) (-1)^n H_n(x) e^{-x^2}):
+//   erfc(x_i + t) = erfc(x_i) - E_i sum_{n >= 0} (-1)^n H_n(x_i) t^{n+1} / (n+1)!,
+//   E_i = (2 / sqrt(pi)) e^{-x_i^2},   |t| <= 1 / (2 kErfcPerUnit) = 1/256.
+// Node i holds {erfc(x_i), E_i}, each rounded once from extended precision
+// on the host.  Term n is about (2 x_i |t|)^n / n! of the first, and 2 x_i |t|
+// <= 1/16 below x = 8, so the eight terms kept leave a relative remainder
+// below (1/16)^9 / 9! < 5e-17; with the rounding of the nodes and of the sum
+// the result is within a few ulp of erfc(x).  Beyond kErfcMax erfc(x) <
+// 1.2e-29 and is taken as 0.  It keeps few registers live (no branch over
+// argument ranges, no exp), which the pair kernels need.
+constexpr int kErfcPerUnit = 128;
+constexpr double kErfcMax = 8.0;
+constexpr int kErfcNodes = 8 * kErfcPerUnit + 1;
+
+__device__ __forceinline__ double erfc_nodes(const Geo& g, double x) {
+  if (!(x < kErfcMax)) return 0.0;
+  const int i = (int)(x * kErfcPerUnit + 0.5);
+  const double xi = (double)i * (1.0 / kErfcPerUnit);
+  const double t = x - xi;   // exact: x and x_i are within a factor of 2 (or x_i = 0)
+  const double2 node = g.erfc_tab[i];
+  // q_n = (-1)^n H_n(x_i): q_0 = 1, q_1 = -2 x_i, q_{n+1} = -2 x_i q_n - 2 n q_{n-1};
+  // w_n = t^{n+1} / (n+1)!
+  const double m2x = -2.0 * xi;
+  double qm = 1.0, q = m2x;
+  double w = t;
+  double s = w;
+  w *= t * 0.5;
+  s += q * w;
+#pragma unroll
+  for (int n = 1; n < 7; n++) {
+    const double qn = m2x * q - (2.0 * n) * qm;
+    qm = q;
+    q = qn;
+    w *= t * (1.0 / (n + 2));
+    s += q * w;
+  }
+  return node.x - node.y * s;
+}
 
 // LJ pair table entry for (ti, tj): sigma, epsilon, cutoff distance (strict
 // r < cut), energy shift; computed on the host with the reference's own
@@ -196,7 +239,7 @@ __device__ inline double real_pair_ref(const double* a, double qa, const double*
       for (int k = -g.real_cell[2]; k <= g.real_cell[2]; k++) {
         double x = d[0] + i * g.L[0], y = d[1] + j * g.L[1], z = d[2] + k * g.L[2];
         double r = sqrt(x * x + y * y + z * z);
-        if (r > 0 && r <= g.rc) e += pref * erfc(g.sqrt_alpha * r) / r;
+        if (r > 0 && r <= g.rc) e += pref * erfc_nodes(g, g.sqrt_alpha * r) / r;
       }
   return e;
 }
@@ -216,7 +259,7 @@ __device__ __forceinline__ double real_pair(const double* a, double qa, const do
   const double pref = g.lB * qa * qb;
   const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   double e = 0;
-  if (r > 0 && r <= g.rc) e += pref * erfc(g.sqrt_alpha * r) / r;
+  if (r > 0 && r <= g.rc) e += pref * erfc_nodes(g, g.sqrt_alpha * r) / r;
   return e;
 }
 

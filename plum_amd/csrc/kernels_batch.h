@@ -676,7 +676,7 @@ __device__ __forceinline__ void pair_terms(const Geo& g, const Dev& d, double r2
   if (P == 0 || r < g.fp64_below) {
     // FP64 (the FP32 path keeps the steep, large close-contact terms here)
     if (lj_ok) elj += lj_energy(r, d.lj[lji]);
-    if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
+    if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc_nodes(g, g.sqrt_alpha * r) / r;
     return;
   }
   const float r2f = (float)r2;
@@ -923,7 +923,7 @@ struct CSpecSmem {
 };
 
 template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_compute_spec(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_compute_spec(
     const Geo* __restrict__ G, const Dev* __restrict__ D, const RepCfg* __restrict__ RC,
     const int* __restrict__ vb_count, int R) {
   __shared__ int s_pre[kMaxReplicas];

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_full_real(Geo g, Dev d) {
               if (g.use_ewald) {
                 const double qq = p.w * pj.w;
                 if (qq != 0 && r > 0 && r <= g.rc) {
-                  const double e = g.lB * qq * erfc(g.sqrt_alpha * r) / r;
+                  const double e = g.lB * qq * erfc_nodes(g, g.sqrt_alpha * r) / r;
                   ere += (j == i) ? 0.5 * e : e;
                 }
               }

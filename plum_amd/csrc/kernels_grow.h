@@ -116,7 +116,7 @@ __device__ __forceinline__ double real_pair_image(const double* a, double qa, co
   min_image(a, b, g.L, g.npbc, dv);
   const double x = dv[0] + i * g.L[0], y = dv[1] + j * g.L[1], z = dv[2] + k * g.L[2];
   const double r = sqrt(x * x + y * y + z * z);
-  return (r > 0 && r <= g.rc) ? g.lB * qa * qb * erfc(g.sqrt_alpha * r) / r : 0.0;
+  return (r > 0 && r <= g.rc) ? g.lB * qa * qb * erfc_nodes(g, g.sqrt_alpha * r) / r : 0.0;
 }
 
 // Parts 2-4 of trial t (cbmc.cc:52-151): LJ with the grown chain (not the

@@ -309,7 +309,7 @@ __device__ double2 real_query(const Geo& g, const Dev& d, double4 p, int tm, int
         if (lj_ok) elj += lj_energy(r, ljrow[d.ctype[si]]);
         if (g.use_ewald) {
           const double qq = p.w * pj.w;
-          if (qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * r) / r;
+          if (qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc_nodes(g, g.sqrt_alpha * r) / r;
         }
       } else {
         // FP32 prefilter on cell-relative coordinates (slightly widened so
@@ -332,7 +332,7 @@ __device__ double2 real_query(const Geo& g, const Dev& d, double4 p, int tm, int
           if (lj_ok) elj += lj_energy(rd, ljrow[d.ctype[si]]);
           if (g.use_ewald) {
             const double qq = p.w * pj.w;
-            if (qq != 0 && rd > 0 && rd <= g.rc) ere += g.lB * qq * erfc(g.sqrt_alpha * rd) / rd;
+            if (qq != 0 && rd > 0 && rd <= g.rc) ere += g.lB * qq * erfc_nodes(g, g.sqrt_alpha * rd) / rd;
           }
           continue;
         }

@@ -47,7 +47,7 @@ __device__ inline double real_pair_stretch(const double* a, double qa, const dou
   double e = 0;
   if (g.single_image && (g.npbc == 3 || fabs(d[2]) + g.rc * (1 + 1e-9) < Ls[2])) {
     const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    if (r > 0 && r <= g.rc) e += pref * erfc(g.sqrt_alpha * r) / r;
+    if (r > 0 && r <= g.rc) e += pref * erfc_nodes(g, g.sqrt_alpha * r) / r;
     return e;
   }
   for (int i = -g.real_cell[0]; i <= g.real_cell[0]; i++)
@@ -55,7 +55,7 @@ __device__ inline double real_pair_stretch(const double* a, double qa, const dou
       for (int k = -g.real_cell[2]; k <= g.real_cell[2]; k++) {
         const double x = d[0] + i * Ls[0], y = d[1] + j * Ls[1], z = d[2] + k * Ls[2];
         const double r = sqrt(x * x + y * y + z * z);
-        if (r > 0 && r <= g.rc) e += pref * erfc(g.sqrt_alpha * r) / r;
+        if (r > 0 && r <= g.rc) e += pref * erfc_nodes(g, g.sqrt_alpha * r) / r;
       }
   return e;
 }
