@@ -66,6 +66,17 @@ int plum_sim_density_profile(plum_sim* sim, double res, double* out, int max_bin
    <prefix>_lastcrd.dat and <prefix>_lasttop.dat, from which a run resumes
    (a run.in naming them reads the configuration back). */
 int plum_sim_write_last(plum_sim* sim, const char* prefix);
+/* Simulation::Run with its sampling and output (simulation.cc:216-239;
+   Sample :723-798, PrintStatHeader/PrintStat :685-867, PrintTraj :869-933,
+   PrintLastCrd/PrintLastTop :935-1027, PrintLastRhoZ :1047-1064): steps
+   steps writing <prefix>_stat.dat, <prefix>_traj_{p,c,a,n}.xyz,
+   <prefix>_lastcrd.dat(2), <prefix>_lasttop.dat and <prefix>_lastrhoZ.dat(2)
+   on the run.in's sample, stat and trajectory frequencies.  device != 0
+   runs the translational moves between two exchanges or output steps as
+   device batches.  The pressure column is nan, as the reference prints it
+   without walls (pressure.cc:490-500); with walls its wall-force pressure is
+   not built and the six columns are nan. */
+int plum_sim_run_output(plum_sim* sim, long long steps, const char* prefix, int device);
 /* ForceField::CalcPressureVolScalingHSELSlit (pressure.cc:187-387): one
    volume-perturbation sample of the current configuration (plum_vp_sample,
    the run.in's surface sites as phantom), added to the running estimators;

@@ -212,6 +212,17 @@ int plum_sim_write_last(plum_sim* s, const char* prefix) {
   return PLUM_OK;
 }
 
+int plum_sim_run_output(plum_sim* s, long long steps, const char* prefix, int device) {
+  if (!s || !s->s || !prefix) return fail(PLUM_E_ARG, "null argument");
+  if (steps < 0) return fail(PLUM_E_ARG, "steps must be >= 0");
+  try {
+    s->s->RunWithOutput(steps, prefix, device != 0);
+  } catch (const std::exception& e) {
+    return fail(PLUM_E_HIP, e.what());
+  }
+  return PLUM_OK;
+}
+
 int plum_sim_pressure_sample(plum_sim* s, plum_vp* sample, double* p_yethiraj, double* p_demiguel,
                              long long* samples) {
   if (!s || !s->s) return fail(PLUM_E_ARG, "null argument");

@@ -182,39 +182,39 @@ void Simulation::Run(long long steps) {
   }
 }
 
+// n translational moves as one batch of the engine's device loop.
+void Simulation::device_moves(long long n) {
+  if (n == 0 || (int)mols.size() - r.phantom <= 0) return;
+  plum_mc_params mc{};
+  for (int k = 0; k < 5; k++) mc.move_prob[k] = r.move_prob[k];
+  mc.move_size = r.move_size;
+  mc.phantom = r.phantom;
+  mc.grafted = r.grafted;
+  mc.bond_len = 0;
+  mc.vary_bond = 0;
+  if (ff.UseBondRigid()) mc.bond_len = ff.RigidBondLen();
+  if (ff.UseBondPot()) {
+    mc.bond_len = ff.EqBondLen();
+    mc.vary_bond = 1;
+  }
+  mc.seed = ((unsigned long long)rng() << 32) | rng();
+  const auto t0 = std::chrono::steady_clock::now();
+  const plum_mc_stats st = ff.DeviceMoves(mols, mc, n);
+  seconds_device += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int k = 0; k < 5; k++) {
+    attempted[k] += st.attempted[k];
+    accepted[k] += st.accepted[k];
+  }
+}
+
 void Simulation::RunDevice(long long steps) {
   update_counts();
   long long pending = 0;
-  auto flush = [&]() {
-    if (pending == 0) return;
-    if ((int)mols.size() - r.phantom > 0) {
-      plum_mc_params mc{};
-      for (int k = 0; k < 5; k++) mc.move_prob[k] = r.move_prob[k];
-      mc.move_size = r.move_size;
-      mc.phantom = r.phantom;
-      mc.grafted = r.grafted;
-      mc.bond_len = 0;
-      mc.vary_bond = 0;
-      if (ff.UseBondRigid()) mc.bond_len = ff.RigidBondLen();
-      if (ff.UseBondPot()) {
-        mc.bond_len = ff.EqBondLen();
-        mc.vary_bond = 1;
-      }
-      mc.seed = ((unsigned long long)rng() << 32) | rng();
-      const auto t0 = std::chrono::steady_clock::now();
-      const plum_mc_stats st = ff.DeviceMoves(mols, mc, pending);
-      seconds_device += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      for (int k = 0; k < 5; k++) {
-        attempted[k] += st.attempted[k];
-        accepted[k] += st.accepted[k];
-      }
-    }
-    pending = 0;
-  };
   for (long long s = 0; s < steps; s++) {
     const int rn = (int)rng();
     if (ff.UseGC() && rn % r.gc_freq == 0) {
-      flush();
+      device_moves(pending);
+      pending = 0;
       const auto t0 = std::chrono::steady_clock::now();
       GCMove();
       seconds_gc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -223,7 +223,198 @@ void Simulation::RunDevice(long long steps) {
     }
     steps_done++;
   }
-  flush();
+  device_moves(pending);
+}
+
+void Simulation::RunWithOutput(long long steps, const std::string& prefix, bool device) {
+  update_counts();
+  std::ofstream stat(prefix + "_stat.dat");
+  std::ofstream tp(prefix + "_traj_p.xyz"), tc(prefix + "_traj_c.xyz"), ta(prefix + "_traj_a.xyz"),
+      tn(prefix + "_traj_n.xyz");
+  if (!stat || !tp || !tc || !ta || !tn) throw std::runtime_error("cannot write the output files of " + prefix);
+  std::ostream* traj[4] = {&tp, &tc, &ta, &tn};
+  if (avg.rho_z.empty()) avg.rho_z.assign(3 * (size_t)std::ceil(box[2] / avg.rho_z_res), 0.0);
+  print_stat_header(stat);
+  long long pending = 0;
+  for (long long s = 0; s < steps; s++) {
+    const long long step = steps_done + 1;
+    const int rn = (int)rng();
+    const bool gc = ff.UseGC() && rn % r.gc_freq == 0;
+    // what Sample or a writer reads at the end of this step must be the
+    // configuration after it: batched moves are flushed before those steps
+    const bool post_eq = step > r.steps_eq;
+    const bool reads = (post_eq && step % r.sample_freq == 0) || step % r.stat_out_freq == 0 ||
+                       (post_eq && step % r.traj_out_freq == 0);
+    if (gc) {
+      device_moves(pending);
+      pending = 0;
+      const long long ga[2] = {gc_attempted[0], gc_attempted[1]}, gk[2] = {gc_accepted[0], gc_accepted[1]};
+      GCMove();
+      if (post_eq)
+        for (int k = 0; k < 2; k++) {
+          avg.gc_attempted[k] += gc_attempted[k] - ga[k];
+          avg.gc_accepted[k] += gc_accepted[k] - gk[k];
+        }
+    } else if (device) {
+      pending++;
+    } else {
+      TranslationalMove();
+    }
+    steps_done++;
+    if (reads) {
+      device_moves(pending);
+      pending = 0;
+    }
+    sample(step);
+    if (step % r.stat_out_freq == 0) print_stat(stat, step);
+    if (post_eq && step % r.traj_out_freq == 0) print_traj(traj, step);
+    if (post_eq && step % r.stat_out_freq == 0) {
+      PrintLastCrd(prefix);
+      PrintLastCrd(prefix, "2");
+      PrintLastTop(prefix);
+      print_last_rho_z(prefix);
+    }
+  }
+  device_moves(pending);
+}
+
+// Simulation::Sample (simulation.cc:723-798).  The wall-force slit pressure
+// (CalcPressureForceLJELSlit, pressure.cc:404-488) is not built: with walls
+// the pressure columns are printed as nan.
+void Simulation::sample(long long step) {
+  if (step > r.steps_eq && step % r.sample_freq == 0) {
+    avg.ff_count++;
+    if (ff.UsePairPot()) avg.pair += ff.TotPairEnergy();
+    if (ff.UseEwaldPot()) avg.ewald += ff.TotEwaldEnergy();
+    if (ff.UseBondPot()) avg.bond += ff.TotBondEnergy();
+    if (ff.UseExtPot()) avg.ext += ff.TotExtEnergy();
+    avg.density += (double)mols.size() / (box[0] * box[1] * box[2]);
+    // Simulation::CalcRhoZ (simulation.cc:1029-1045)
+    const int nb = (int)(avg.rho_z.size() / 3);
+    for (int i = r.phantom; i < (int)mols.size(); i++) {
+      const int id = mols[i].Size() > 1 ? 0 : (mols[i].bds[0].Charge() >= 0 ? 1 : 2);
+      for (const Bead& b : mols[i].bds) {
+        double z = b.GetCrd(0, 2);
+        z -= box[2] * std::floor(z / box[2]);
+        const int bin = (int)std::floor(z / avg.rho_z_res);
+        if (bin >= 0 && bin < nb) avg.rho_z[3 * (size_t)bin + id] += 1.0 / (box[0] * box[1] * avg.rho_z_res);
+      }
+    }
+    if (!ff.UseExtPot() && step % (r.sample_freq * 10) == 0) ff.CalcPressureVolScalingHSELSlit(mols);
+    if (r.calc_chem_pot) avg.chem_pot += ff.CalcChemicalPotentialF(mols, rng);
+  }
+  if (step > r.steps_eq && step % r.sample_freq == 0 && !mols.empty()) {
+    avg.mol_count++;
+    double xyz[3];
+    RadiusOfGyrationXYZ(xyz);
+    avg.rg += RadiusOfGyration();
+    for (int k = 0; k < 3; k++) avg.rg_xyz[k] += xyz[k];
+    avg.ete += EndToEndDistance();
+  }
+}
+
+// Simulation::PrintStatHeader (simulation.cc:685-721)
+void Simulation::print_stat_header(std::ostream& o) const {
+  o << "#Step";
+  if (ff.UsePairPot()) o << " <PairEne>";
+  if (ff.UseEwaldPot()) o << " <EwaldEne>";
+  if (ff.UseBondPot()) o << " <BondEne>";
+  if (ff.UseExtPot()) o << " <ExtPotEne>";
+  o << " <TotalEnergy>";
+  if (ff.UseGC()) o << " NoOfMol <Density>";
+  if (ff.UseExtPot()) o << " <Pzz_LJ_ion> <Pzz_LJ_pol> <Pzz_LJ_wal> <Pzz_EL_ion> <Pzz_EL_pol> <Pzz_EL_wal>";
+  else o << " <P>";
+  if (r.calc_chem_pot) o << " mu";
+  o << " <|Rg|> <|Rgx|> <|Rgy|> <|Rgz|> <ete>";
+  o << " beadtrans comtrans pivot crankshaft reptate";
+  if (ff.UseGC()) o << " insert delete";
+  o << '\n';
+}
+
+// Simulation::PrintStat (simulation.cc:800-867): running averages with the
+// reference's stream precisions (7 digits for the energies, then 4).
+void Simulation::print_stat(std::ostream& o, long long step) const {
+  o << step;
+  const double n = (double)avg.ff_count;
+  double tot = 0;
+  o.precision(7);
+  if (ff.UsePairPot()) {
+    o << ' ' << avg.pair / n;
+    tot += avg.pair / n;
+  }
+  if (ff.UseEwaldPot()) {
+    o << ' ' << avg.ewald / n;
+    tot += avg.ewald / n;
+  }
+  if (ff.UseBondPot()) {
+    o << ' ' << avg.bond / n;
+    tot += avg.bond / n;
+  }
+  if (ff.UseExtPot()) {
+    o << ' ' << avg.ext / n;
+    tot += avg.ext / n;
+  }
+  o << ' ' << tot;
+  if (ff.UseGC()) {
+    o << ' ' << (int)mols.size();
+    o.precision(4);
+    o << ' ' << avg.density / n;
+  }
+  // ForceField::GetPressure (pressure.cc:490-500): nan without walls
+  if (ff.UseExtPot()) o << " nan nan nan nan nan nan";
+  else o << " nan";
+  if (r.calc_chem_pot) {
+    if (avg.chem_pot / n > 0) o << ' ' << -(1.0 / r.beta) * std::log(avg.chem_pot / n);
+    else o << " INF";
+  }
+  const double m = (double)avg.mol_count;
+  o.precision(4);
+  o << ' ' << std::sqrt(avg.rg / m) << ' ' << std::sqrt(avg.rg_xyz[0] / m) << ' ' << std::sqrt(avg.rg_xyz[1] / m)
+    << ' ' << std::sqrt(avg.rg_xyz[2] / m) << ' ' << std::sqrt(avg.ete / m);
+  for (int k = 0; k < 5; k++) o << ' ' << accepted[k] / (double)attempted[k];
+  if (ff.UseGC())
+    o << ' ' << avg.gc_accepted[0] / (double)avg.gc_attempted[0] << ' '
+      << avg.gc_accepted[1] / (double)avg.gc_attempted[1];
+  o << '\n';
+}
+
+// Simulation::PrintTraj (simulation.cc:869-933): xyz frames of the chains,
+// cations, anions and neutral ions, each molecule shifted by whole boxes so
+// that its centre lies in the box.
+void Simulation::print_traj(std::ostream* out[4], long long step) const {
+  int n_chain_beads = 0;
+  for (int i = r.phantom; i < (int)mols.size(); i++)
+    if (mols[i].Size() > 1) n_chain_beads += mols[i].Size();
+  const int counts[4] = {n_chain_beads, n_cion, n_aion, n_nion};
+  const bool have[4] = {r.grafted + n_chain > 0, n_cion > 0, n_aion > 0, n_nion > 0};
+  for (int f = 0; f < 4; f++)
+    if (have[f]) *out[f] << counts[f] << "\nSTEP: " << step << '\n';
+  for (int i = r.phantom; i < (int)mols.size(); i++) {
+    const Molecule& m = mols[i];
+    double c[3] = {0, 0, 0};
+    for (const Bead& b : m.bds)
+      for (int k = 0; k < 3; k++) c[k] += b.GetCrd(0, k);
+    double disp[3];
+    for (int k = 0; k < 3; k++) disp[k] = std::floor(c[k] / m.Size() / box[k]) * box[k];
+    const int f = m.Size() > 1 ? 0 : (m.bds[0].Charge() > 0 ? 1 : (m.bds[0].Charge() < 0 ? 2 : 3));
+    for (const Bead& b : m.bds)
+      *out[f] << b.Symbol() << ' ' << b.GetCrd(0, 0) - disp[0] << ' ' << b.GetCrd(0, 1) - disp[1] << ' '
+              << b.GetCrd(0, 2) - disp[2] << '\n';
+  }
+}
+
+// Simulation::PrintLastRhoZ (simulation.cc:1047-1064)
+void Simulation::print_last_rho_z(const std::string& prefix) const {
+  const char* bck[2] = {"", "2"};
+  const int nb = (int)(avg.rho_z.size() / 3);
+  for (int f = 0; f < 2; f++) {
+    std::ofstream o(prefix + "_lastrhoZ.dat" + bck[f]);
+    if (!o) throw std::runtime_error("cannot write " + prefix + "_lastrhoZ.dat" + bck[f]);
+    for (int bin = 0; bin < nb; bin++)
+      o << (bin + 0.5) * avg.rho_z_res << '\t' << avg.rho_z[3 * (size_t)bin] / (double)avg.ff_count << '\t'
+        << avg.rho_z[3 * (size_t)bin + 1] / (double)avg.ff_count << '\t'
+        << avg.rho_z[3 * (size_t)bin + 2] / (double)avg.ff_count << '\n';
+  }
 }
 
 // Simulation::GCMove (simulation.cc:357-404)
@@ -325,9 +516,9 @@ int Simulation::DensityProfile(double res, std::vector<double>& out) const {
   return nb;
 }
 
-void Simulation::PrintLastCrd(const std::string& prefix) const {
-  std::ofstream f(prefix + "_lastcrd.dat");
-  if (!f) throw std::runtime_error("cannot write " + prefix + "_lastcrd.dat");
+void Simulation::PrintLastCrd(const std::string& prefix, const std::string& suffix) const {
+  std::ofstream f(prefix + "_lastcrd.dat" + suffix);
+  if (!f) throw std::runtime_error("cannot write " + prefix + "_lastcrd.dat" + suffix);
   int n = 0;
   for (const Molecule& m : mols) n += m.Size();
   f << n << "\n \n";
@@ -378,6 +569,25 @@ double Simulation::RadiusOfGyration() const {
     rg += rm / (m.Size() * m.Size());
   }
   return (r.grafted + n_chain) > 0 ? rg / (r.grafted + n_chain) : 0;
+}
+
+// Simulation::RadiusOfGyrationXYZ (simulation.cc:437-481): per-axis
+// absolute separations of the trial coordinates, no periodic folding.
+void Simulation::RadiusOfGyrationXYZ(double out[3]) const {
+  out[0] = out[1] = out[2] = 0;
+  for (const Molecule& m : mols) {
+    if (m.Size() <= 1) continue;
+    double a[3] = {0, 0, 0};
+    for (int j = 0; j < m.Size() - 1; j++)
+      for (int k = j + 1; k < m.Size(); k++)
+        for (int e = 0; e < 3; e++) {
+          const double d = std::fabs(m.bds[j].GetCrd(1, e) - m.bds[k].GetCrd(1, e));
+          a[e] += d * d;
+        }
+    for (int e = 0; e < 3; e++) out[e] += a[e] / (m.Size() * m.Size());
+  }
+  if (r.grafted + n_chain > 0)
+    for (int e = 0; e < 3; e++) out[e] /= (r.grafted + n_chain);
 }
 
 // Simulation::EndToEndDistance (simulation.cc:483-501), mean squared

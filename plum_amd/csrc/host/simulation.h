@@ -44,10 +44,31 @@ class Simulation {
   // per unit volume of the bins of width res; returns the bin count.
   int DensityProfile(double res, std::vector<double>& out) const;
   // Simulation::PrintLastCrd / PrintLastTop (simulation.cc:935-1027): the
-  // configuration as <prefix>_lastcrd.dat and <prefix>_lasttop.dat, each
-  // molecule shifted by whole boxes so that its centre lies in the box.
-  void PrintLastCrd(const std::string& prefix) const;
+  // configuration as <prefix>_lastcrd.dat<suffix> and <prefix>_lasttop.dat,
+  // each molecule shifted by whole boxes so that its centre lies in the box.
+  void PrintLastCrd(const std::string& prefix, const std::string& suffix = "") const;
   void PrintLastTop(const std::string& prefix) const;
+  // Simulation::RadiusOfGyrationXYZ (simulation.cc:437-481): mean squared
+  // radius of gyration of the chains per axis.
+  void RadiusOfGyrationXYZ(double out[3]) const;
+  // The reference's Run loop with its sampling and output
+  // (simulation.cc:216-239): after every step Sample, PrintStat, PrintTraj,
+  // PrintLastCrd, PrintLastTop and PrintLastRhoZ, each on its own schedule,
+  // into <prefix>_stat.dat, <prefix>_traj_{p,c,a,n}.xyz,
+  // <prefix>_lastcrd.dat(2), <prefix>_lasttop.dat and
+  // <prefix>_lastrhoZ.dat(2).  Steps are numbered on from the steps already
+  // done.  With device = true the translational moves between two GC moves
+  // or sampling points run as device batches (as RunDevice does).
+  void RunWithOutput(long long steps, const std::string& prefix, bool device);
+  // Running averages of Sample (simulation.cc:723-798).
+  struct Averages {
+    long long ff_count = 0, mol_count = 0;
+    double pair = 0, ewald = 0, bond = 0, ext = 0, density = 0, chem_pot = 0;
+    double rg = 0, rg_xyz[3] = {0, 0, 0}, ete = 0;
+    std::vector<double> rho_z;   // 3 per bin: chain beads, cations (and neutral), anions
+    double rho_z_res = 0.02;
+    long long gc_attempted[2] = {0, 0}, gc_accepted[2] = {0, 0};   // after equilibration
+  } avg;
 
   RunIn r;
   double box[3] = {0, 0, 0};
@@ -62,6 +83,12 @@ class Simulation {
 
  private:
   void update_counts();
+  void device_moves(long long n);
+  void sample(long long step);
+  void print_stat_header(std::ostream& o) const;
+  void print_stat(std::ostream& o, long long step) const;
+  void print_traj(std::ostream* out[4], long long step) const;
+  void print_last_rho_z(const std::string& prefix) const;
   int n_mol = 0, n_chain = 0, n_cion = 0, n_aion = 0, n_nion = 0;
 };
 

@@ -13,7 +13,8 @@ HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim
                 "plum_sim_positions", "plum_sim_observables", "plum_sim_destroy",
                 "plum_host_last_error", "plum_synth_count", "plum_synth_system",
                 "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device",
-                "plum_sim_density_profile", "plum_sim_write_last", "plum_sim_pressure_sample")
+                "plum_sim_density_profile", "plum_sim_write_last", "plum_sim_pressure_sample",
+                "plum_sim_run_output")
 
 
 class RunInSummary(C.Structure):
@@ -67,6 +68,7 @@ def lib():
         L.plum_sim_density_profile.argtypes = [vp, C.c_double, C.POINTER(C.c_double), C.c_int,
                                                C.POINTER(C.c_int)]
         L.plum_sim_write_last.argtypes = [vp, C.c_char_p]
+        L.plum_sim_run_output.argtypes = [vp, C.c_longlong, C.c_char_p, C.c_int]
         L.plum_sim_run_device.argtypes = [vp, C.c_longlong, C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         L.plum_sim_pressure_sample.argtypes = [vp, C.POINTER(E.VpSample), C.POINTER(C.c_double),
@@ -134,6 +136,12 @@ class HostSimulation:
 
     def write_last(self, prefix):
         _check(lib().plum_sim_write_last(self.h, prefix.encode()))
+
+    def run_output(self, steps, prefix, device=False):
+        """Simulation::Run with the reference's sampling and output files
+        (<prefix>_stat.dat, _traj_{p,c,a,n}.xyz, _lastcrd.dat(2), _lasttop.dat,
+        _lastrhoZ.dat(2))."""
+        _check(lib().plum_sim_run_output(self.h, int(steps), prefix.encode(), 1 if device else 0))
 
     def pressure_sample(self):
         """One volume-perturbation pressure sample: (sample dict, Yethiraj
