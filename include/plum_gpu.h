@@ -234,6 +234,8 @@ typedef struct {
                             the cell compaction period) */
   int pmax;              /* proposals per speculative batch, 1..16 (1..32 screened with the surrogate) */
   double sweep_us;       /* measured time per move of every system (plum_batch_tune) */
+  int cell_div;          /* cells per search radius and dimension of every system, 1..4
+                            (0 keeps each system's grid); a change rebuilds the cell lists */
 } plum_batch_tuning;
 
 typedef struct {
@@ -251,8 +253,10 @@ int plum_batch_destroy(plum_batch* batch);
    directly (no graph). */
 int plum_batch_mc_run(plum_batch* batch, const plum_mc_params* mc, long long moves_per_replica, int use_graph,
                       plum_batch_stats* total, plum_mc_stats* per);
-/* Autotuner: times `moves_per_trial` sweeps for each candidate launch shape
-   and keeps the fastest (the trial sweeps are ordinary moves). */
+/* Autotuner: for each candidate (cell grid, work-groups per CU, query split,
+   proposals per batch) times `moves_per_trial` sweeps three times and keeps
+   the candidate with the lowest median (the trial sweeps are ordinary
+   moves). */
 int plum_batch_tune(plum_batch* batch, const plum_mc_params* mc, int moves_per_trial, plum_batch_tuning* best);
 int plum_batch_get_tuning(const plum_batch* batch, plum_batch_tuning* out);
 int plum_batch_set_tuning(plum_batch* batch, const plum_batch_tuning* tuning);

@@ -42,6 +42,8 @@ struct plum_batch {
   int sx_rows = 0, sx_cap = 0, maxm_b = 0;
   int tab_chunks = 1;     // phase-table chunks per proposal (plan kernel rows y > 0)
   int fold_chunks = 0;    // S(k) fold chunks of kPlanThreads k vectors (plan kernel)
+  std::vector<int> cell_off_h;   // cell offsets of the systems (prefix, R + 1)
+  std::vector<int> grid_gen;     // each system's grid generation at the last upload
   bool profiling = false;
   std::vector<void*> allocs;
 };
@@ -141,7 +143,35 @@ int batch_setup_da(plum_batch* b) {
 
 // Uploads every system's Geo / Dev (a cell rebuild changes both), launch
 // shape and Monte Carlo parameters.
+void batch_drop_graph(plum_batch* b);
+
+// Cell lists on the tuned grid, and the per-system cell offsets of the
+// batch's compaction for the grids the systems have now (a system in two
+// batches may have been re-gridded through the other).
+int batch_regrid(plum_batch* b) {
+  for (plum_ctx* c : b->reps)
+    if (b->tune.cell_div > 0 && b->tune.cell_div != c->cell_div) CHK(regrid_cells(c, b->tune.cell_div));
+  std::vector<int> off(b->R + 1, 0);
+  for (int r = 0; r < b->R; r++) off[r + 1] = off[r] + b->reps[r]->g.ncell;
+  if (off == b->cell_off_h) return PLUM_OK;
+  b->cell_off_h = off;
+  b->total_cells = off[b->R];
+  HIPCHK(hipMemcpy(b->d_cell_off, off.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice));
+  batch_drop_graph(b);   // the compaction grid changed
+  return PLUM_OK;
+}
+
+// True when a system's cell grid changed since the batch uploaded its state.
+bool batch_grid_moved(const plum_batch* b) {
+  for (int r = 0; r < b->R; r++)
+    if (b->grid_gen[r] != b->reps[r]->grid_gen) return true;
+  return false;
+}
+
 int batch_upload_state(plum_batch* b) {
+  CHK(batch_regrid(b));
+  b->grid_gen.assign(b->R, 0);
+  for (int r = 0; r < b->R; r++) b->grid_gen[r] = b->reps[r]->grid_gen;
   std::vector<Geo> G(b->R);
   std::vector<Dev> D(b->R);
   std::vector<RepCfg> RCv(b->R);
@@ -374,6 +404,7 @@ int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   b->tune.batches_per_graph = 32;
   b->tune.pmax = kMaxProp;
   b->tune.sweep_us = 0;
+  b->tune.cell_div = 0;
   int dev = 0;
   int rc = PLUM_OK;
   auto bail = [&](int code) {
@@ -393,6 +424,7 @@ int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   std::vector<int> off(n + 1, 0);
   for (int r = 0; r < n; r++) off[r + 1] = off[r] + replicas[r]->g.ncell;
   b->total_cells = off[n];
+  b->cell_off_h = off;
   for (int r = 0; r < n; r++) {
     const int cr = tab_chunk_rows(replicas[r]->g.tab_w);
     b->tab_chunks = std::max(b->tab_chunks, (2 * replicas[r]->maxm + cr - 1) / cr);
@@ -420,6 +452,7 @@ int batch_run(plum_batch* b, const plum_mc_params* mcp, long long n_moves, bool 
     HIPCHK(hipEventRecord(c->ev_idle, c->st));
     HIPCHK(hipStreamWaitEvent(b->st, c->ev_idle, 0));
   }
+  if (batch_grid_moved(b)) CHK(batch_upload_state(b));
   b->mc.assign(R, McDev{});
   for (int r = 0; r < R; r++) CHK(mc_setup(b->reps[r], &mcp[r], b->st, &b->mc[r]));
   std::vector<unsigned long long> m0(R);
@@ -534,6 +567,7 @@ int check_tuning(const plum_batch_tuning* t) {
   if (t->batches_per_graph < 1 || t->batches_per_graph > 1024)
     return fail(PLUM_E_ARG, "batches_per_graph must be 1..1024");
   if (t->pmax < 1 || t->pmax > kMaxScreen) return fail(PLUM_E_ARG, "pmax must be 1..32");
+  if (t->cell_div < 0 || t->cell_div > 4) return fail(PLUM_E_ARG, "cell_div must be 0..4");
   return PLUM_OK;
 }
 
@@ -725,29 +759,42 @@ int plum_batch_mc_run(plum_batch* b, const plum_mc_params* mcp, long long n_move
 int plum_batch_tune(plum_batch* b, const plum_mc_params* mcp, int moves_per_trial, plum_batch_tuning* best) {
   if (!b || !mcp) return fail(PLUM_E_ARG, "null argument");
   if (moves_per_trial < 1) return fail(PLUM_E_ARG, "moves_per_trial must be >= 1");
+  const int cdiv[] = {1, 2, 3};
   const int bpc[] = {4, 8, 16};
   const int div[] = {192, 384, 768};
   const int pm[] = {1, 2, 4, 8, 16};
+  constexpr int kRepeats = 3;
   const plum_batch_tuning keep = b->tune;
   plum_batch_tuning win = keep;
   double best_s = 1e300;
-  for (int i : bpc)
-    for (int j : div)
-      for (int k : pm) {
-        plum_batch_tuning t = keep;
-        t.blocks_per_cu = i;
-        t.split_div = j;
-        t.pmax = k;
-        t.batches_per_graph = std::max(1, std::min(64, moves_per_trial / k));
-        CHK(plum_batch_set_tuning(b, &t));
-        plum_batch_stats st{};
-        CHK(batch_run(b, mcp, moves_per_trial, true, nullptr, &st, nullptr));
-        if (st.seconds < best_s) {
-          best_s = st.seconds;
-          win = t;
-          win.sweep_us = 1e6 * st.seconds / moves_per_trial;
+  for (int c : cdiv)
+    for (int i : bpc)
+      for (int j : div)
+        for (int k : pm) {
+          plum_batch_tuning t = keep;
+          t.cell_div = c;
+          t.blocks_per_cu = i;
+          t.split_div = j;
+          t.pmax = k;
+          t.batches_per_graph = std::max(1, std::min(64, moves_per_trial / k));
+          CHK(plum_batch_set_tuning(b, &t));
+          // one untimed pass (graph capture), then the median of three
+          plum_batch_stats st{};
+          CHK(batch_run(b, mcp, moves_per_trial, true, nullptr, &st, nullptr));
+          double s3[kRepeats];
+          for (int rep = 0; rep < kRepeats; rep++) {
+            st = plum_batch_stats{};
+            CHK(batch_run(b, mcp, moves_per_trial, true, nullptr, &st, nullptr));
+            s3[rep] = st.seconds;
+          }
+          std::sort(s3, s3 + kRepeats);
+          const double med = s3[kRepeats / 2];
+          if (med < best_s) {
+            best_s = med;
+            win = t;
+            win.sweep_us = 1e6 * med / moves_per_trial;
+          }
         }
-      }
   win.batches_per_graph = keep.batches_per_graph;
   CHK(plum_batch_set_tuning(b, &win));
   b->tune.sweep_us = win.sweep_us;

@@ -58,6 +58,8 @@ struct plum_ctx {
   int nrealb = 1;        // blocks of the full real-space kernel
   int nsplit = 1;        // blocks per real-space query
   double cand = 0;       // expected candidate pair-images per query
+  int cell_div = 2;      // cells per search radius and dimension
+  int grid_gen = 0;      // bumped by every re-grid (batches re-upload their copies)
   int nkb_s = 0;         // k-space blocks of the speculative loop
   int nrun = 0;          // k-vector runs of the speculative k-space (Dev::krun)
   int xcap = 0;          // cross-term partial capacity
@@ -223,6 +225,50 @@ int build_cells(plum_ctx* c, int min_cap) {
   return PLUM_OK;
 }
 
+int split_for(double cand, int split_div) {
+  return std::max(1, std::min(kMaxSplit, (int)std::lround(cand / split_div)));
+}
+
+// Cell grid of edge ~ rs / div in every dimension (at most 256 cells each).
+void set_cell_grid(plum_ctx* c, int div) {
+  Geo& g = c->g;
+  for (int k = 0; k < 3; k++) {
+    int nc = (int)floor(g.L[k] / (g.rs / div));
+    nc = std::max(1, std::min(nc, 256));
+    g.nc[k] = nc;
+    g.h[k] = g.L[k] / nc;
+    g.inv_h[k] = 1.0 / g.h[k];
+    c->info.n_cells[k] = nc;
+  }
+  g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
+  c->cell_div = div;
+}
+
+// Expected candidate slots of a real-space query (the cells within the search
+// radius of a point) and the query split it gives.
+void set_cand(plum_ctx* c) {
+  const Geo& g = c->g;
+  double cand = g.n / c->info.vol_e;
+  for (int k = 0; k < 3; k++) cand *= 2 * g.rs + g.h[k];
+  c->cand = cand;
+  c->nsplit = split_for(cand, kDefaultSplitDiv);
+}
+
+// A new cell grid for the current positions: the cell list rebuilt from
+// scratch (the old arrays stay allocated until the context is destroyed).
+int regrid_cells(plum_ctx* c, int div) {
+  if (div == c->cell_div) return PLUM_OK;
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int old_ncell = c->g.ncell;
+  set_cell_grid(c, div);
+  if (c->g.ncell != old_ncell) CHK(dalloc(c, &c->d.ccount, c->g.ncell));
+  c->d.cbead = nullptr;   // slot arrays sized for the new grid
+  CHK(build_cells(c, 0));
+  set_cand(c);
+  c->grid_gen++;
+  return PLUM_OK;
+}
+
 // From-scratch energy on the device; with set_running the running totals,
 // dipole state and the per-molecule spring cache are reset to it.
 int run_full(plum_ctx* c, int set_running, double out[8]) {
@@ -281,9 +327,6 @@ int validate(const plum_params* p, int n, const double* xyz, const double* q, co
 int nmm_for(const plum_ctx*, int nm) { return nmm_blocks(nm); }
 
 // Blocks per real-space query: about 1.5 candidate slots per thread.
-int split_for(double cand, int split_div) {
-  return std::max(1, std::min(kMaxSplit, (int)std::lround(cand / split_div)));
-}
 
 // k-space and real-space work of one trial move in one launch, once the
 // moved list (and, for the k space, the phase tables) exists on the device.
@@ -416,16 +459,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   g.rs2 = g.rs * g.rs;
 
   // cell grid: edge ~ rs / div
-  const int div = p.cell_div > 0 ? p.cell_div : 2;
-  for (int k = 0; k < 3; k++) {
-    int nc = (int)floor(g.L[k] / (g.rs / div));
-    nc = std::max(1, std::min(nc, 256));
-    g.nc[k] = nc;
-    g.h[k] = g.L[k] / nc;
-    g.inv_h[k] = 1.0 / g.h[k];
-    info.n_cells[k] = nc;
-  }
-  g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
+  set_cell_grid(c, p.cell_div > 0 ? p.cell_div : 2);
 
   // half-space k list (potential_ewald_coul.cc:78-115; k and -k paired)
   std::vector<int4> kl;
@@ -531,14 +565,7 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   c->xcap = kMaxPairs * cross_blocks(maxm, maxm);
   ALLOC(d.part_x, c->xcap);
   ALLOC(d.ctl, 1);
-  {
-    // split each real-space query so a thread sees ~1.5 candidate slots
-    const double rho = n / info.vol_e;
-    double cand = rho;
-    for (int k = 0; k < 3; k++) cand *= 2 * g.rs + g.h[k];
-    c->cand = cand;
-    c->nsplit = split_for(cand, kDefaultSplitDiv);
-  }
+  set_cand(c);
   // room for any split (the batch autotuner may change it)
   ALLOC(d.part_k, c->nkb); ALLOC(d.part_r, (size_t)kMaxProp * 2 * maxm * kMaxQWave);
   ALLOC(d.part_mm, (size_t)kMaxProp * c->nmmb);

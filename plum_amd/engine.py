@@ -88,7 +88,8 @@ class MCStats(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int), ("split_div", C.c_int),
-                ("batches_per_graph", C.c_int), ("pmax", C.c_int), ("sweep_us", C.c_double)]
+                ("batches_per_graph", C.c_int), ("pmax", C.c_int), ("sweep_us", C.c_double),
+                ("cell_div", C.c_int)]
 
 
 class BatchStats(C.Structure):

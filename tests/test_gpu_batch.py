@@ -74,7 +74,9 @@ def test_batch_autotuner_picks_a_candidate_and_stays_exact():
     b = E.Batch(engs)
     mc = [_mc(sd, 100 + r) for r, sd in enumerate(sds)]
     t = b.tune(mc, moves_per_trial=8)
-    print("tuned", t.blocks_per_cu, t.split_div, t.pmax, "%.1f us/sweep" % t.sweep_us)
+    print("tuned", t.cell_div, t.blocks_per_cu, t.split_div, t.pmax, "%.1f us/sweep" % t.sweep_us)
+    assert t.cell_div in (1, 2, 3)
+    assert all(e.info().n_cells[0] == engs[0].info().n_cells[0] for e in engs)
     assert t.blocks_per_cu in (4, 8, 16)
     assert t.split_div in (192, 384, 768)
     assert t.pmax in (1, 2, 4, 8, 16)
@@ -127,6 +129,50 @@ def test_batch_fp32_pair_drift_stays_within_the_stated_bound():
         for k in ("kspace", "bond", "self", "dipole", "wall"):
             den = max(abs(full[k]), 1e-3 * sum(abs(v) for v in full.values()), 1e-2)
             assert abs(run[k] - full[k]) / den < 1e-9, (r, k, run[k], full[k])
+    b.close()
+    for e in engs:
+        e.close()
+
+
+@pytest.mark.parametrize("div", [1, 3])
+def test_cell_grid_change_keeps_every_decision_exact(div):
+    """Another cell grid (cells of r_s / div) rebuilds the cell list from the
+    current positions; the device loop on it still gives every decided
+    proposal the oracle's dE to 1e-10 and ends in the oracle's configuration."""
+    import test_gpu_record as TR
+    sd = systems.config("small")
+    eng = E.Engine(sd)
+    before = list(eng.info().n_cells)
+    eng.set_tuning(cell_div=div)
+    after = list(eng.info().n_cells)
+    assert (after != before) == (div != 2), (before, after)
+    eng.mc_record(1600)
+    st = eng.mc_run(1500, sd["move_prob"], **TR._kw(sd, 41))
+    recs, n_dec = eng.mc_records(1600)
+    assert n_dec == len(recs) >= 500
+    orc, worst, strict, n_cmp = TR.replay(sd, recs, 1e-10)
+    print("[cell_div %d] cells %s, compared %d, worst %s" % (
+        div, after, n_cmp, " ".join("%s=%.1e" % kv for kv in worst.items())), flush=True)
+    assert np.array_equal(orc.positions(), eng.positions())
+    eng.close()
+
+
+def test_regrid_through_a_batch_reaches_the_engines_own_loop():
+    """An engine whose grid a replica batch changed (after its own device
+    loop had uploaded the old grid) runs its own loop on the new grid: the
+    running totals still equal the from-scratch energy."""
+    sds = _replicas(2, seed=21)
+    engs = [E.Engine(sd) for sd in sds]
+    engs[0].mc_run(300, **_mc(sds[0], 1))
+    b = E.Batch(engs)
+    b.set_tuning(cell_div=3)
+    b.mc_run(100, [_mc(sd, 7 + r) for r, sd in enumerate(sds)])
+    engs[0].mc_run(300, **_mc(sds[0], 2))
+    for e in engs:
+        run, full = e.totals(), e.energy_full()
+        for k in E.TERMS:
+            den = max(abs(full[k]), 1e-3 * sum(abs(v) for v in full.values()), 1e-2)
+            assert abs(run[k] - full[k]) / den < 1e-9, (k, run[k], full[k])
     b.close()
     for e in engs:
         e.close()
