@@ -98,6 +98,14 @@ def lib():
             getattr(L, f).restype = C.c_double
         L.orc_wall_bead.argtypes = [C.c_void_p, C.c_int]
         L.orc_wall_bead.restype = C.c_double
+        L.cf_create.restype = C.c_void_p
+        L.cf_create.argtypes = [C.POINTER(OrcParams), C.c_int, dp, dp, ip, C.c_int]
+        L.cf_destroy.argtypes = [C.c_void_p]
+        L.cf_delta.argtypes = [C.c_void_p, C.c_int, ip, dp, dp]
+        L.cf_commit.argtypes = [C.c_void_p]
+        L.cf_replay.restype = C.c_double
+        L.cf_replay.argtypes = [C.c_void_p, C.c_int, ip, ip, dp, ip, C.c_int, dp]
+        L.cf_threads.restype = C.c_int
         L.orc_set_trial.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, dp]
         L.orc_energy_difference.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_energy_difference.restype = C.c_double
@@ -452,3 +460,54 @@ class GCSim:
 
     def close(self):
         self.orc.close()
+
+
+class CpuFast:
+    """CPU twin of the engine's algorithm (cpu_fast.c): cell-list LJ + real
+    space and a running S(k), FP64, OpenMP.  The CPU baseline's workload:
+    it evaluates and commits the trial moves a device run decided."""
+
+    def __init__(self, sysdef, cell_div=2):
+        L = lib()
+        self.params = make_params(sysdef)
+        xyz = np.ascontiguousarray(sysdef["xyz"], dtype=np.float64).reshape(-1)
+        q = np.ascontiguousarray(sysdef["q"], dtype=np.float64)
+        ty = np.ascontiguousarray(sysdef["type"], dtype=np.int32)
+        self.n = len(q)
+        self.h = L.cf_create(C.byref(self.params), self.n, _dp(xyz), _dp(q), _ip(ty), cell_div)
+
+    def delta(self, idx, trial_xyz):
+        """(LJ, real, k-space) dE of moving beads idx to trial_xyz."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        t = np.ascontiguousarray(trial_xyz, dtype=np.float64).reshape(-1)
+        out = np.zeros(3)
+        lib().cf_delta(self.h, len(idx), _ip(idx), _dp(t), _dp(out))
+        return out
+
+    def commit(self):
+        lib().cf_commit(self.h)
+
+    def replay(self, recs, maxm):
+        """Evaluates the decided proposals in order and commits the accepted
+        ones; returns (seconds of the C loop, dE array (n, 3))."""
+        n = len(recs)
+        nm = np.array([len(r["idx"]) for r in recs], dtype=np.int32)
+        idx = np.zeros(n * maxm, dtype=np.int32)
+        xyz = np.zeros(3 * n * maxm)
+        for i, r in enumerate(recs):
+            k = len(r["idx"])
+            idx[i * maxm:i * maxm + k] = r["idx"]
+            xyz[3 * i * maxm:3 * (i * maxm + k)] = np.asarray(r["trial"]).reshape(-1)
+        acc = np.array([1 if r["accepted"] else 0 for r in recs], dtype=np.int32)
+        out = np.zeros(3 * n)
+        sec = lib().cf_replay(self.h, n, _ip(nm), _ip(idx), _dp(xyz), _ip(acc), maxm, _dp(out))
+        return sec, out.reshape(-1, 3)
+
+    @staticmethod
+    def threads():
+        return lib().cf_threads()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cf_destroy(self.h)
+            self.h = None
