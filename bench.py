@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--brush-moves", type=int, default=20000)
     ap.add_argument("--replica-sweeps", type=int, default=512)
     ap.add_argument("--no-gc", action="store_true")
-    ap.add_argument("--gc-steps", type=int, default=3000)
+    ap.add_argument("--gc-steps", type=int, default=20000)
     ap.add_argument("--gc-only", action="store_true",
                     help="run only the grand-canonical leg (for profiling)")
     ap.add_argument("--growth-only", action="store_true",
@@ -431,39 +431,49 @@ def gc_leg(steps):
     deletion (16 trials per grown bead, each growth step's trial energies in
     one GPU pass), one exchange every 10 steps on average and translational
     moves on the device loop in between; starting from 50 chains in a box of
-    60."""
+    60 at the chemical potential where 50 chains are in balance.  Reports the
+    mean chain count with its block error, the drift between the halves, the
+    exchange acceptance both ways and the volume-perturbation pressure
+    (Yethiraj and de Miguel estimators, pressure.cc:187-387) sampled along
+    the run."""
     import tempfile
     from plum_amd import host as H
     sd = systems.config("c4")
     g = sd["gc"]
     path = systems.write_reference_inputs(sd, tempfile.mkdtemp(prefix="plum_gc_"))
     sim = H.HostSimulation(path, 2024)
-    per = 1 + g["chain_len"]
+    per = 2 * g["chain_len"]
     t = time.perf_counter()
     w = [sim.widom() for _ in range(2)]
     widom_s = (time.perf_counter() - t) / 2
     growth_steps = 50 * (g["chain_len"] - 1)
-    sim.run_device(max(steps // 10, 100))           # warm up
-    a0, c0, n0 = sim.gc_stats()[0], sim.gc_stats()[1], None
+    a0, c0 = sim.gc_stats()[0], sim.gc_stats()[1]
     att0, acc0, sec0 = sim.run_device(0)
-    chains = []
+    chains, py, pd = [], None, None
     t = time.perf_counter()
-    chunk = max(steps // 50, 1)
+    nchunk = 50
+    chunk = max(steps // nchunk, 1)
     done = 0
+    t_p = 0.0
     while done < steps:
         att, acc, sec = sim.run_device(chunk)
         done += chunk
-        nb = sim.gc_stats()[2]
-        chains.append(nb / (g["chain_len"] * 2))
-    secs = time.perf_counter() - t
+        chains.append(sim.gc_stats()[2] / per)
+        tp = time.perf_counter()
+        _, py, pd, n_p = sim.pressure_sample()
+        t_p += time.perf_counter() - tp
+    secs = time.perf_counter() - t - t_p
     ga, gacc, nb, nm = sim.gc_stats()
     ins = ga[0] - a0[0]
     dele = ga[1] - a0[1]
     moves = sum(att) - sum(att0)
     ch = np.array(chains)
-    blocks = ch[: len(ch) // 10 * 10].reshape(10, -1).mean(1)
+    nb10 = len(ch) // 10 * 10
+    blocks = ch[:nb10].reshape(10, -1).mean(1)
+    half = len(ch) // 2
     log("gc: %d steps in %.2f s: %d insertions (%d accepted), %d deletions (%d accepted), "
-        "<chains> %.2f" % (done, secs, ins, gacc[0] - c0[0], dele, gacc[1] - c0[1], ch.mean()))
+        "<chains> %.2f +- %.2f, P %.3e / %.3e" % (done, secs, ins, gacc[0] - c0[0], dele, gacc[1] - c0[1],
+                                                ch.mean(), blocks.std(ddof=1) / np.sqrt(10), py, pd))
     res = {"workload": "grand-canonical bulk, 32-bead chains (-1 per bead) + 32 counterions "
                        "exchanged by configurational bias, 16 trials per bead, mu %.1f kT, box 60, "
                        "start 50 chains, exchange every 10 steps" % g["chem_pot"],
@@ -474,6 +484,10 @@ def gc_leg(steps):
            "translational_moves_per_s": moves / max(sec[0] - sec0[0], 1e-9),
            "seconds_in_exchanges": sec[1] - sec0[1], "seconds_in_device_loop": sec[0] - sec0[0],
            "chains_mean": float(ch.mean()), "chains_err": float(blocks.std(ddof=1) / np.sqrt(10)),
+           "chains_std": float(ch.std(ddof=1)),
+           "chains_first_half": float(ch[:half].mean()), "chains_second_half": float(ch[half:].mean()),
+           "pressure_yethiraj": py, "pressure_demiguel": pd, "pressure_samples": n_p,
+           "pressure_unit": "kT per unit volume (volume perturbation dz = 1e-5 along z)",
            "growth_step_us": 1e6 * widom_s / growth_steps,
            "widom_mean_weight": float(np.mean(w))}
     sim.close()
