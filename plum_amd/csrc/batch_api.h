@@ -86,7 +86,7 @@ namespace {
 int sur_train_step(plum_ctx* c, hipStream_t st, int samples, bool gather, bool update, bool sync);
 
 constexpr int kSpecKernels = 7;
-const char* kSpecKernelNames[kSpecKernels] = {"k_propose_spec", "k_descr_spec", "k_mlp_infer", "k_plan_spec",
+const char* kSpecKernelNames[kSpecKernels] = {"k_propose_spec", "k_descr_spec", "k_mlp_spec", "k_plan_spec",
                                               "k_compute_spec", "k_reduce_spec", "k_resolve_spec"};
 
 template <typename T>
@@ -187,7 +187,6 @@ int batch_upload_state(plum_batch* b) {
     q.nkb_s = c->nkb_s;
     q.nrun = c->nrun;
     q.nsplit = c->nsplit;
-    q.qwave = query_waves(c->nsplit, b->tune.pmax * b->R);
     q.ncell = c->g.ncell;
     q.pmax = b->tune.pmax;
     q.maxm = c->maxm;
@@ -225,16 +224,14 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     // surrogate: descriptors of every moved bead (old, new) of every system,
     // then the network on all of them in one launch
     plum_ctx* c = b->reps[0];
-    hipLaunchKernelGGL(k_descr_spec, dim3((2 * b->maxm_b + kDescrWaves - 1) / kDescrWaves, b->tune.pmax, R),
+    hipLaunchKernelGGL(k_descr_spec, dim3(2 * b->maxm_b, b->tune.pmax, R),
                        dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC);
     if (ev) (void)hipEventRecord(ev[2], b->st);
-    // the rows of the proposals in use: the first 2 pmax maxm rows of each
-    // system's slice (R == 1: the surrogate's own rows, maxm of that system)
-    const int per = 2 * b->tune.pmax * (R == 1 ? c->maxm : b->maxm_b);
-    const int stride = R == 1 ? per : b->sx_rows;
-    const int rows = R * per;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(b->n_cu, (rows + 127) / 128))), dim3(kBlock), 0, b->st,
-                       c->sur->net, b->sx, rows, b->sf, per, stride);
+    // the network on the moved beads' rows only: tiles of 32 rows over at
+    // most 2 pmax maxm rows per system, those past a system's rows exit
+    const int tiles = (2 * b->tune.pmax * (R == 1 ? c->maxm : b->maxm_b) + 31) / 32;
+    hipLaunchKernelGGL(k_mlp_spec, dim3(std::max(1, std::min(b->n_cu, R * tiles))), dim3(kBlock), 0, b->st,
+                       c->sur->net, b->sx, b->sf, b->dD, b->dRC, R, b->sx_rows, tiles);
   } else if (ev) {
     (void)hipEventRecord(ev[2], b->st);
   }

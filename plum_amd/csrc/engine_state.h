@@ -56,6 +56,7 @@ struct SpecPlan {
   int pass1[kMaxScreen];         // first stage of delayed acceptance passed (1 without it)
   int pair_p[kMaxPairs], pair_q[kMaxPairs];
   int pair_off[kMaxPairs + 1];   // cross-term blocks per near pair (prefix)
+  int qwave;                     // waves per real-space query this batch (query_waves)
   int q_off[kMaxProp + 1];       // real-space query blocks per exact slot (prefix)
   int mm_off[kMaxProp + 1];      // moved-moved blocks per exact slot (prefix)
   unsigned acc_mask;             // exact slots accepted by the last resolve: S(k) += dS pending

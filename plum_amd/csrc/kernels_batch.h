@@ -65,7 +65,6 @@ struct RepCfg {
   float r_d;    // descriptor radius of the surrogate
   int train;    // this system feeds the surrogate's training ring
   int nrun;     // k-vector runs (Dev::krun)
-  int qwave;    // waves per real-space query (query_waves)
 };
 
 // Real-space query blocks of a move of nm beads: one query per cluster of up
@@ -74,9 +73,11 @@ struct RepCfg {
 // kQWaves waves per block.
 constexpr int kQWaves = kBlock / kWave;
 constexpr int kMaxQWave = 4 * kMaxSplit;
-// One wave per split share; a batch of few proposals (fewer than 16 over all
-// systems) spreads each query over four times as many waves, as its queries
-// alone would leave most of the chip idle.
+// One wave per split share; a batch with few exact proposals (fewer than 16
+// over all systems) spreads each query over four times as many waves, as its
+// queries alone would leave most of the chip idle.  The plan chooses it per
+// batch from the exact slots it assigned (with delayed acceptance most
+// screened proposals never reach the queries).
 __host__ __device__ __forceinline__ int query_waves(int nsplit, int proposals) {
   return nsplit == 1 ? 1 : (proposals >= 16 ? nsplit : kQWaves * nsplit);
 }
@@ -457,11 +458,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
       if (exact >= 0) sm.s_screen[exact] = t;
     }
     wave_sync_lds();
-    // query and moved-moved blocks per exact slot, prefixes over the slots
+    // waves per query for this batch's exact slots (the other systems of a
+    // batch taken to have as many), then query and moved-moved blocks per
+    // exact slot, prefixes over the slots
+    const int qw = query_waves(rc.nsplit, ne * R);
     int nq = 0, nmb = 0;
     if (t < ne) {
       const int n = sm.s_prop[sm.s_screen[t]].n;
-      nq = n > 0 ? query_blocks(n, rc.qwave) : 0;
+      nq = n > 0 ? query_blocks(n, qw) : 0;
       nmb = n > 0 ? nmm_blocks(n) : 0;
       plan->screen_of[t] = sm.s_screen[t];
     }
@@ -479,6 +483,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     }
     const int off = ne > 0 ? __shfl(nq, ne - 1, kWave) + __shfl(nmb, ne - 1, kWave) : 0;
     if (t == 0) {
+      plan->qwave = qw;
       plan->q_off[0] = 0;
       plan->mm_off[0] = 0;
       plan->n_decide = ndec;
@@ -952,7 +957,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       const int local = b - uload(&plan->q_off[e]);
       const Dev v = prop_view(g, d, rc, uload(&plan->screen_of[e]), e);
       const int nm = uload(&v.rec->n_moved);
-      const int qw = rc.qwave;
+      const int qw = uload(&plan->qwave);
       // wave gw: share gw % qw of query q2 = gw / qw = 2 c + on, cluster c
       // (moved beads c kCluster ..) at their old (on = 0) or new positions
       const int w = threadIdx.x / kWave, gw = local * kQWaves + w;
@@ -1044,7 +1049,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_spec(const Geo* __res
     if (e >= ne) return;
     const int nm = d.recs[plan->screen_of[e]].n_moved;
     if (nm == 0) return;
-    const int nreal = 2 * query_clusters(nm) * rc.qwave, nmm = nmm_blocks(nm);
+    const int nreal = 2 * query_clusters(nm) * plan->qwave, nmm = nmm_blocks(nm);
     const double2* pr = d.part_r + (size_t)e * rc.rstride;
     const double2* pm = d.part_mm + (size_t)e * rc.nmmb;
     double lj = 0, re = 0;

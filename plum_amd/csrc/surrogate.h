@@ -43,16 +43,18 @@ struct DescrWaveSmem {
   float s_rel[kWave][3];
 };
 constexpr int kDescrWaves = kBlock / kWave;
+constexpr int kDescrLoads = 4;   // candidate loads in flight per lane
 
-// Descriptor of moved bead m of proposal view v at its old (on = 0) or new
-// (on = 1) position, against every unmoved bead image within r_d (images as
-// the LJ term takes them: z unshifted with two periodic dimensions).  One
-// wave per row, no block barriers: the wave scans the query cells 64 at a
-// time, each lane sums its candidates, and a butterfly over the 64 lanes
-// (32 + 16 + ... + 1 exchanges, fixed order) leaves feature f in lane f.
-__device__ void descr_wave(const Geo& g, const Dev& v, float r_d, int m, int on, float* out, DescrWaveSmem& sm) {
+// Descriptor sums of moved bead m of proposal view v at its old (on = 0) or
+// new (on = 1) position, against every unmoved bead image within r_d (images
+// as the LJ term takes them: z unshifted with two periodic dimensions), over
+// the candidates of wave wv of nw that share the row.  No block barriers: the
+// wave scans the query cells 64 at a time, each lane sums its candidates, and
+// a butterfly over the 64 lanes (32 + 16 + ... + 1 exchanges, fixed order)
+// leaves the wave's sum of feature f in lane f.
+__device__ float descr_wave(const Geo& g, const Dev& v, float r_d, const double4& p, int wv, int nw,
+                            DescrWaveSmem& sm) {
   const int lane = threadIdx.x % kWave;
-  const double4 p = on ? v.mv_new[m] : v.mv_old[m];
   const int contig = v.rec->contig, mlo = v.rec->lo, mhi = v.rec->hi;
   const double pw[3] = {wrap_coord(p.x, g.L[0], g.inv_L[0]), wrap_coord(p.y, g.L[1], g.inv_L[1]),
                         wrap_coord(p.z, g.L[2], g.inv_L[2])};
@@ -97,33 +99,51 @@ __device__ void descr_wave(const Geo& g, const Dev& v, float r_d, int m, int on,
     sm.s_end[lane] = end;
     const int total = __shfl(end, nloc - 1, kWave);
     wave_sync_lds();
-    for (int it = lane; it < total; it += kWave) {
-      int lo_i = 0, hi_i = nloc - 1;
-      while (lo_i < hi_i) {
-        const int mid = (lo_i + hi_i) >> 1;
-        if (sm.s_end[mid] > it) hi_i = mid;
-        else lo_i = mid + 1;
-      }
-      const int t = lo_i;
-      const int slot = it - (t ? sm.s_end[t - 1] : 0);
-      const size_t si = (size_t)sm.s_cell[t] * g.cap + slot;
-      const int j = v.cbead[si];
-      if (j < 0 || (contig ? (j >= mlo && j <= mhi) : v.moved[j])) continue;
-      const float4 pr = v.cpos32[si];
-      const float fx = pr.x - sm.s_rel[t][0], fy = pr.y - sm.s_rel[t][1], fz = pr.z - sm.s_rel[t][2];
-      const float r2 = fx * fx + fy * fy + fz * fz;
-      if (r2 >= rd2) continue;
-      const float r = sqrtf(r2);
-      const float fc = 0.5f * (__cosf(pi_rd * r) + 1.0f);
-      const float qq = qi * pr.w;
-      const int ch = qq > 0.f ? 0 : (qq < 0.f ? 1 : 2);
+    // candidates in rounds of kDescrLoads per lane: every load of a round is
+    // issued before the first candidate of the round is used
+    for (int r0 = 0; r0 < total; r0 += kDescrLoads * kWave * nw) {
+      int tt[kDescrLoads], jj[kDescrLoads];
+      float4 pp[kDescrLoads];
 #pragma unroll
-      for (int k = 0; k < kRbf; k++) {
-        const float dr = r - (kRbfMin + k * dmu);
-        const float ph = __expf(-gam * dr * dr) * fc;
-        acc[k] += ch == 0 ? ph : 0.f;
-        acc[kRbf + k] += ch == 1 ? ph : 0.f;
-        acc[2 * kRbf + k] += ch == 2 ? ph : 0.f;
+      for (int u = 0; u < kDescrLoads; u++) {
+        const int it = r0 + (u * nw + wv) * kWave + lane;
+        tt[u] = 0;
+        jj[u] = -1;
+        pp[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (it < total) {
+          int lo_i = 0, hi_i = nloc - 1;
+          while (lo_i < hi_i) {
+            const int mid = (lo_i + hi_i) >> 1;
+            if (sm.s_end[mid] > it) hi_i = mid;
+            else lo_i = mid + 1;
+          }
+          const int slot = it - (lo_i ? sm.s_end[lo_i - 1] : 0);
+          const size_t si = (size_t)sm.s_cell[lo_i] * g.cap + slot;
+          tt[u] = lo_i;
+          jj[u] = v.cbead[si];
+          pp[u] = v.cpos32[si];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kDescrLoads; u++) {
+        const int j = jj[u], t = tt[u];
+        if (j < 0 || (contig ? (j >= mlo && j <= mhi) : v.moved[j])) continue;
+        const float4 pr = pp[u];
+        const float fx = pr.x - sm.s_rel[t][0], fy = pr.y - sm.s_rel[t][1], fz = pr.z - sm.s_rel[t][2];
+        const float r2 = fx * fx + fy * fy + fz * fz;
+        if (r2 >= rd2) continue;
+        const float r = sqrtf(r2);
+        const float fc = 0.5f * (__cosf(pi_rd * r) + 1.0f);
+        const float qq = qi * pr.w;
+        const int ch = qq > 0.f ? 0 : (qq < 0.f ? 1 : 2);
+#pragma unroll
+        for (int k = 0; k < kRbf; k++) {
+          const float dr = r - (kRbfMin + k * dmu);
+          const float ph = __expf(-gam * dr * dr) * fc;
+          acc[k] += ch == 0 ? ph : 0.f;
+          acc[kRbf + k] += ch == 1 ? ph : 0.f;
+          acc[2 * kRbf + k] += ch == 2 ? ph : 0.f;
+        }
       }
     }
     wave_sync_lds();   // the next chunk rewrites the cell arrays
@@ -140,35 +160,51 @@ __device__ void descr_wave(const Geo& g, const Dev& v, float r_d, int m, int on,
       acc[i] = keep + __shfl_xor(give, o, kWave);
     }
   }
-  float val;
-  if (lane < kRbf * kChan) val = kFeatScale * acc[0];
-  else if (lane == kRbf * kChan) val = qi;
-  else if (g.npbc == 2 && lane == kRbf * kChan + 1) val = __expf(-0.5f * (float)p.z);
-  else if (g.npbc == 2 && lane == kRbf * kChan + 2) val = __expf(-0.5f * (float)(g.Lsim[2] - p.z));
-  else val = 0.f;
-  if (lane < kDescr) out[lane] = val;
+  return acc[0];
+}
+
+// Feature f of the row in lane f from the summed basis values `sum`.
+__device__ __forceinline__ float descr_finish(const Geo& g, const double4& p, float sum) {
+  const int lane = threadIdx.x % kWave;
+  if (lane < kRbf * kChan) return kFeatScale * sum;
+  if (lane == kRbf * kChan) return (float)p.w;
+  if (g.npbc == 2 && lane == kRbf * kChan + 1) return __expf(-0.5f * (float)p.z);
+  if (g.npbc == 2 && lane == kRbf * kChan + 2) return __expf(-0.5f * (float)(g.Lsim[2] - p.z));
+  return 0.f;
 }
 
 static_assert(kRbf * kChan + 3 <= kDescr && kDescr == kWave, "descriptor row = one value per lane");
 
 // Descriptors of every moved bead of every proposal, old and new: grid
-// (2 maxm / 4, pmax, systems), one wave per row; row m2 of system r is bead
-// m2 % maxm at old (m2 < maxm) or new.
+// (2 maxm, pmax, systems), one row per block; row m2 of system r is bead
+// m2 % maxm at old (m2 < maxm) or new.  The block's four waves split the
+// row's candidates (a row holds hundreds to thousands of them, and one wave
+// alone would wait on their loads in turn); the four partial sums are added
+// in wave order.
 __global__ __launch_bounds__(kBlock) void k_descr_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
                                                        const RepCfg* __restrict__ RC) {
   __shared__ DescrWaveSmem sm[kDescrWaves];
+  __shared__ float s_sum[kDescrWaves][kWave];
   const Geo& g = G[blockIdx.z];
   const Dev& d = D[blockIdx.z];
   const RepCfg& rc = RC[blockIdx.z];
-  const int w = threadIdx.x / kWave;
-  const int m2 = blockIdx.x * kDescrWaves + w;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int m2 = blockIdx.x;
   if (m2 >= 2 * rc.maxm) return;
   const int s = blockIdx.y;
   if (s >= d.plan->n_prop) return;
   const int on = m2 >= rc.maxm ? 1 : 0, m = m2 - on * rc.maxm;
   const Dev v = prop_view(g, d, rc, s, 0);
   if (m >= v.rec->n_moved) return;
-  descr_wave(g, v, rc.r_d, m, on, d.sur_x + ((size_t)(2 * s + on) * rc.maxm + m) * kDescr, sm[w]);
+  const double4 p = on ? v.mv_new[m] : v.mv_old[m];
+  s_sum[w][lane] = descr_wave(g, v, rc.r_d, p, w, kDescrWaves, sm[w]);
+  __syncthreads();
+  if (w == 0) {
+    float sum = s_sum[0][lane];
+#pragma unroll
+    for (int k = 1; k < kDescrWaves; k++) sum += s_sum[k][lane];
+    d.sur_x[((size_t)(2 * s + on) * rc.maxm + m) * kDescr + lane] = descr_finish(g, p, sum);
+  }
 }
 
 // ------------------------------------------------------------ inference
@@ -200,25 +236,40 @@ struct NetBF16 {
   float b4;
 };
 
+// LDS rows are padded by 4 BF16 (8 bytes): the 32 rows an MFMA operand load
+// touches then start 2 banks apart instead of all in one bank.
+constexpr int kPad = 4;
+constexpr int kLd1 = kDescr + kPad, kLdH = kHid + kPad;
 struct MlpSmem {
-  __align__(16) unsigned short w1t[kHid * kDescr];
-  __align__(16) unsigned short w2t[kHid * kHid];
-  __align__(16) unsigned short w3t[kHid * kHid];
+  __align__(16) unsigned short w1t[kHid * kLd1];
+  __align__(16) unsigned short w2t[kHid * kLdH];
+  __align__(16) unsigned short w3t[kHid * kLdH];
   float b1[kHid], b2[kHid], b3[kHid], w4[kHid];
-  __align__(16) unsigned short act[kBlock / kWave][2][32 * kHid];
+  __align__(16) unsigned short act[kBlock / kWave][2][32 * kLdH];
 };
+static_assert(sizeof(MlpSmem) <= 160 * 1024, "fits the LDS of one CU");
 
-// One layer for the wave's 32 rows: out = tanh(in[32 x K] Wt^T + b) in BF16.
+// tanh of the inference layers: (1 - e) / (1 + e) with e = exp(-2 |x|), the
+// sign of x restored.  Absolute error below 2e-7 (the rounding of e near
+// x = 0), far under the BF16 rounding (2^-9 relative) applied to the result.
+__device__ __forceinline__ float tanh_act(float x) {
+  const float e = __expf(-2.0f * fabsf(x));
+  return copysignf((1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e), x);
+}
+
+// One layer for the wave's 32 rows: out = tanh(in[32 x K] Wt^T + b) in BF16,
+// column tiles nt0 .. nt0 + NT - 1 of 32 columns (LDS rows of in and Wt
+// K + kPad apart, of out kHid + kPad).
 // v_mfma_f32_32x32x8bf16_1k: lane l supplies A[l % 32][4 (l / 32) .. +3] and
 // B[4 (l / 32) .. +3][l % 32]; it receives D[row][l % 32] for row =
 // 8 (i / 4) + 4 (l / 32) + i % 4, i = 0..15.
-template <int K>
+template <int K, int NT>
 __device__ __forceinline__ void mlp_layer(const unsigned short* in, const unsigned short* wt, const float* b,
-                                          unsigned short* out) {
+                                          unsigned short* out, int nt0) {
   const int lane = threadIdx.x % kWave;
   const int r = lane % 32, hk = (lane / 32) * 4;
 #pragma unroll
-  for (int nt = 0; nt < kHid / 32; nt++) {
+  for (int nt = nt0; nt < nt0 + NT; nt++) {
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.f;
@@ -226,16 +277,54 @@ __device__ __forceinline__ void mlp_layer(const unsigned short* in, const unsign
 #pragma unroll
     for (int kk = 0; kk < K / 8; kk++) {
       bf16x4 a, bb;
-      a = *reinterpret_cast<const bf16x4*>(in + r * K + kk * 8 + hk);
-      bb = *reinterpret_cast<const bf16x4*>(wt + col * K + kk * 8 + hk);
+      a = *reinterpret_cast<const bf16x4*>(in + r * (K + kPad) + kk * 8 + hk);
+      bb = *reinterpret_cast<const bf16x4*>(wt + col * (K + kPad) + kk * 8 + hk);
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bb, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int row = 8 * (i / 4) + hk + (i % 4);
-      out[row * kHid + col] = to_bf16(tanhf(acc[i] + b[col]));
+      out[row * kLdH + col] = to_bf16(tanh_act(acc[i] + b[col]));
     }
   }
+}
+
+// Copy the BF16 network into LDS: every thread issues all its 16-byte loads
+// (4 of W1, 8 each of W2 and W3) before its first LDS store, so the 80 KB
+// arrive in one round trip rather than one per element.
+__device__ __forceinline__ void stage_net(const NetBF16* __restrict__ net, MlpSmem& sm) {
+  constexpr int n1 = kHid * kDescr / 8 / kBlock, n2 = kHid * kHid / 8 / kBlock;
+  static_assert(n1 * 8 * kBlock == kHid * kDescr && n2 * 8 * kBlock == kHid * kHid, "whole 16-byte rows per thread");
+  static_assert(offsetof(NetBF16, w2t) % 16 == 0 && offsetof(NetBF16, w3t) % 16 == 0, "16-byte weight rows");
+  const int t = threadIdx.x;
+  const uint4* g1 = reinterpret_cast<const uint4*>(net->w1t);
+  const uint4* g2 = reinterpret_cast<const uint4*>(net->w2t);
+  const uint4* g3 = reinterpret_cast<const uint4*>(net->w3t);
+  uint4 r1[n1], r2[n2], r3[n2];
+#pragma unroll
+  for (int k = 0; k < n1; k++) r1[k] = g1[t + k * kBlock];
+#pragma unroll
+  for (int k = 0; k < n2; k++) {
+    r2[k] = g2[t + k * kBlock];
+    r3[k] = g3[t + k * kBlock];
+  }
+  float bb[4] = {0.f, 0.f, 0.f, 0.f};
+  if (t < kHid) bb[0] = net->b1[t], bb[1] = net->b2[t], bb[2] = net->b3[t], bb[3] = net->w4[t];
+  // 16 bytes of a global row land as two 8-byte halves of a padded LDS row
+  auto put = [](unsigned short* dst, int ld, int per_row, int c, const uint4& v) {
+    uint2* o = reinterpret_cast<uint2*>(dst + (c / per_row) * ld + (c % per_row) * 8);
+    o[0] = make_uint2(v.x, v.y);
+    o[1] = make_uint2(v.z, v.w);
+  };
+#pragma unroll
+  for (int k = 0; k < n1; k++) put(sm.w1t, kLd1, kDescr / 8, t + k * kBlock, r1[k]);
+#pragma unroll
+  for (int k = 0; k < n2; k++) {
+    put(sm.w2t, kLdH, kHid / 8, t + k * kBlock, r2[k]);
+    put(sm.w3t, kLdH, kHid / 8, t + k * kBlock, r3[k]);
+  }
+  if (t < kHid) sm.b1[t] = bb[0], sm.b2[t] = bb[1], sm.b3[t] = bb[2], sm.w4[t] = bb[3];
+  __syncthreads();
 }
 
 // f(row) for `rows` descriptor rows; each wave takes 32 rows at a time.
@@ -244,44 +333,144 @@ __device__ __forceinline__ void mlp_layer(const unsigned short* in, const unsign
 __global__ __launch_bounds__(kBlock) void k_mlp_infer(const NetBF16* __restrict__ net, const float* __restrict__ x,
                                                       int rows, float* __restrict__ f, int per, int stride) {
   __shared__ MlpSmem sm;
-  for (int i = threadIdx.x; i < kHid * kDescr; i += kBlock) sm.w1t[i] = net->w1t[i];
-  for (int i = threadIdx.x; i < kHid * kHid; i += kBlock) {
-    sm.w2t[i] = net->w2t[i];
-    sm.w3t[i] = net->w3t[i];
-  }
-  for (int i = threadIdx.x; i < kHid; i += kBlock) {
-    sm.b1[i] = net->b1[i];
-    sm.b2[i] = net->b2[i];
-    sm.b3[i] = net->b3[i];
-    sm.w4[i] = net->w4[i];
-  }
+  stage_net(net, sm);
   const float b4 = net->b4;
-  __syncthreads();
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   unsigned short* a0 = sm.act[w][0];
   unsigned short* a1 = sm.act[w][1];
   for (int base = (blockIdx.x * (kBlock / kWave) + w) * 32; base < rows; base += gridDim.x * (kBlock / kWave) * 32) {
-    // input rows (zero past the end) in BF16
-    for (int i = lane; i < 32 * kDescr; i += kWave) {
-      const int rr = base + i / kDescr;
+    // input rows (zero past the end) in BF16, one row per pass; all 32 loads
+    // are issued before the first store
+    static_assert(kDescr == kWave, "one descriptor row per pass");
+    float xv[32];
+#pragma unroll
+    for (int it = 0; it < 32; it++) {
+      const int rr = base + it;
       const size_t pr = (size_t)(rr / per) * stride + rr % per;
-      a0[i] = to_bf16(rr < rows ? x[pr * kDescr + (i % kDescr)] : 0.f);
+      xv[it] = rr < rows ? x[pr * kDescr + lane] : 0.f;
     }
+#pragma unroll
+    for (int it = 0; it < 32; it++) a0[it * kLd1 + lane] = to_bf16(xv[it]);
     __builtin_amdgcn_wave_barrier();
-    mlp_layer<kDescr>(a0, sm.w1t, sm.b1, a1);
+    mlp_layer<kDescr, kHid / 32>(a0, sm.w1t, sm.b1, a1, 0);
     __builtin_amdgcn_wave_barrier();
-    mlp_layer<kHid>(a1, sm.w2t, sm.b2, a0);
+    mlp_layer<kHid, kHid / 32>(a1, sm.w2t, sm.b2, a0, 0);
     __builtin_amdgcn_wave_barrier();
-    mlp_layer<kHid>(a0, sm.w3t, sm.b3, a1);
+    mlp_layer<kHid, kHid / 32>(a0, sm.w3t, sm.b3, a1, 0);
     __builtin_amdgcn_wave_barrier();
     // output row: lanes r and r + 32 each take half of row r
     const int rrow = lane % 32, half = lane / 32;
     float s = 0;
-    for (int j = half * (kHid / 2); j < (half + 1) * (kHid / 2); j++) s += from_bf16(a1[rrow * kHid + j]) * sm.w4[j];
+    for (int j = half * (kHid / 2); j < (half + 1) * (kHid / 2); j++) s += from_bf16(a1[rrow * kLdH + j]) * sm.w4[j];
     s += __shfl_down(s, 32, kWave);
     const int ro = base + rrow;
     if (half == 0 && ro < rows) f[(size_t)(ro / per) * stride + ro % per] = s + b4;
     __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Inference of the device loop over the moved beads' rows only.  System r's
+// descriptor rows are 2 n_s per proposal s (old rows, then new rows) in
+// proposal order: a wave builds the prefix of 2 n_s over the proposals (one
+// lane per proposal, the counts the proposal launch wrote) and maps its tile
+// of 32 rows onto physical rows r sx_rows + (2 s + on) maxm + m; tiles past
+// the system's rows are skipped.  Tile t of all systems goes to block
+// t % gridDim.x, whose four waves share it (a quarter of the input rows, a
+// quarter of each layer's columns and of the output rows each), so the few
+// valid tiles of a batch run on different CUs with a short chain each.
+__device__ __forceinline__ size_t spec_row(int j, int pref, int np, int r, int sx_rows, int maxm) {
+  // s = number of proposals whose prefix ends at or before row j (every
+  // lane runs the same six steps: the shuffles never diverge)
+  int s = 0;
+#pragma unroll
+  for (int step = kWave / 2; step > 0; step >>= 1) {
+    const int cand = s + step;
+    const int pv = __shfl(pref, min(cand, kWave) - 1, kWave);
+    if (cand <= np && pv <= j) s = cand;
+  }
+  const int before = __shfl(pref, max(s - 1, 0), kWave);
+  const int b0 = s ? before : 0;
+  const int ns = (__shfl(pref, s, kWave) - b0) / 2;
+  const int k = j - b0, on = k >= ns ? 1 : 0, m = k - on * ns;
+  return (size_t)r * sx_rows + (size_t)(2 * s + on) * maxm + m;
+}
+
+// Rows of system d this batch (2 n_s summed over its proposals), the same in
+// every lane; with `pref` each lane also gets the inclusive prefix up to
+// proposal `lane` (the total in lanes past the last proposal).
+__device__ __forceinline__ int spec_rows(const Dev& d, int lane, int* pref) {
+  const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
+  int p = lane < np ? 2 * d.recs[lane].n_moved : 0;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int u = __shfl_up(p, o, kWave);
+    if (lane >= o) p += u;
+  }
+  if (pref) *pref = p;
+  return __builtin_amdgcn_readfirstlane(__shfl(p, kWave - 1, kWave));
+}
+
+__global__ __launch_bounds__(kBlock) void k_mlp_spec(const NetBF16* __restrict__ net, const float* __restrict__ x,
+                                                     float* __restrict__ f, const Dev* __restrict__ D,
+                                                     const RepCfg* __restrict__ RC, int R, int sx_rows,
+                                                     int tiles_per_sys) {
+  static_assert(kBlock / kWave == kHid / 32, "one 32-column slice of each layer per wave");
+  __shared__ MlpSmem sm;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int ntile = R * tiles_per_sys;
+  // a block none of whose tiles holds rows exits before staging the network
+  // (every wave reaches the same answer)
+  int busy = 0;
+  for (int tile = blockIdx.x; tile < ntile && !busy; tile += gridDim.x) {
+    const int r = tile / tiles_per_sys, lt = tile - r * tiles_per_sys;
+    busy = lt * 32 < spec_rows(D[r], lane, nullptr);
+  }
+  if (!busy) return;
+  stage_net(net, sm);
+  const float b4 = net->b4;
+  unsigned short* a0 = sm.act[0][0];
+  unsigned short* a1 = sm.act[0][1];
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int r = tile / tiles_per_sys, lt = tile - r * tiles_per_sys;
+    const Dev& d = D[r];
+    const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
+    const int maxm = RC[r].maxm;
+    int pref;
+    const int total = spec_rows(d, lane, &pref);
+    if (lt * 32 >= total) continue;
+    // lane q % 32 holds the physical row of tile row q (rows past the
+    // system's end repeat its last row and are not written back)
+    const int prow = (int)spec_row(min(lt * 32 + lane % 32, total - 1), pref, np, r, sx_rows, maxm);
+    // wave w stages tile rows 8 w .. 8 w + 7 in BF16 (zero past the system's
+    // rows), all eight loads issued before the first store
+    float xv[8];
+#pragma unroll
+    for (int it = 0; it < 8; it++) {
+      const int q = 8 * w + it;
+      const int pr = __shfl(prow, q, kWave);
+      xv[it] = lt * 32 + q < total ? x[(size_t)pr * kDescr + lane] : 0.f;
+    }
+#pragma unroll
+    for (int it = 0; it < 8; it++) a0[(8 * w + it) * kLd1 + lane] = to_bf16(xv[it]);
+    // each layer: wave w computes columns 32 w .. 32 w + 31 of all 32 rows
+    __syncthreads();
+    mlp_layer<kDescr, 1>(a0, sm.w1t, sm.b1, a1, w);
+    __syncthreads();
+    mlp_layer<kHid, 1>(a1, sm.w2t, sm.b2, a0, w);
+    __syncthreads();
+    mlp_layer<kHid, 1>(a0, sm.w3t, sm.b3, a1, w);
+    __syncthreads();
+    // output: wave w takes rows 8 w .. 8 w + 7, eight lanes of 16 columns
+    // per row, summed in a fixed order
+    const int q = 8 * w + lane / 8, part = lane % 8;
+    float sum = 0;
+#pragma unroll
+    for (int j = part * 16; j < part * 16 + 16; j++) sum += from_bf16(a1[q * kLdH + j]) * sm.w4[j];
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) sum += __shfl_down(sum, o, 8);
+    const int pq = __shfl(prow, q, kWave);
+    if (part == 0 && lt * 32 + q < total) f[pq] = sum + b4;
+    __syncthreads();   // the next tile rewrites both activation buffers
   }
 }
 
