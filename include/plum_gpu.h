@@ -362,7 +362,8 @@ typedef struct {
    pass on the GPU: trials = K chain-bead positions then, if charged, K
    counterion positions; grown = the chain beads placed so far then their
    counterions; energies[K] receives each trial's energy (1e8 when LJ or a
-   wall forbids it).  The slab dipole term is not supported. */
+   wall forbids it), with the slab dipole term (DipoleEDiff,
+   potential_ewald_coul.cc:473-530) when the context has it. */
 int plum_cbmc_trials(plum_ctx* ctx, const plum_cbmc_request* req, const double* trials, const double* grown,
                      double* energies);
 

@@ -137,6 +137,9 @@ def lib():
                                             C.c_double, C.c_int, dp, dp, C.c_int, C.c_int,
                                             C.c_int]
         L.orc_cbmc_beads_energy.restype = C.c_double
+        L.orc_cbmc_dipole_ediff.argtypes = [C.c_void_p, dp, C.c_double, dp, C.c_double, C.c_int, dp, dp,
+                                            C.c_int, C.c_int, C.c_int]
+        L.orc_cbmc_dipole_ediff.restype = C.c_double
         L.orc_gc_insertion.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
                                        C.POINTER(OrcGC), dp, dp]
         L.orc_gc_deletion.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
@@ -301,6 +304,22 @@ class Oracle:
         return lib().orc_cbmc_beads_energy(self.h, _dp(p1), int(t1), float(q1), _dp(p2), int(t2),
                                            float(q2), int(ch), _dp(gc), _dp(gi), cur, lo, hi)
 
+    def cbmc_dipole_ediff(self, b1, q1, grown=None, b2=None, q2=0.0, grown_ions=None, exclude_mols=None):
+        """PotentialEwaldCoul::DipoleEDiff (potential_ewald_coul.cc:473-530): the
+        slab dipole energy change when trial bead b1 (and counterion b2) joins
+        the system minus exclude_mols = (lo, hi) plus the chain grown so far."""
+        ch = b2 is not None
+        p1 = np.ascontiguousarray(b1, dtype=np.float64)
+        p2 = np.ascontiguousarray(b2 if ch else np.zeros(3), dtype=np.float64)
+        g = np.zeros((0, 3)) if grown is None else np.asarray(grown, dtype=np.float64).reshape(-1, 3)
+        cur = len(g)
+        gc = np.ascontiguousarray(g) if cur else np.zeros(3)
+        gi = (np.ascontiguousarray(np.asarray(grown_ions, dtype=np.float64).reshape(cur, 3))
+              if ch and cur else np.zeros(max(cur, 1) * 3))
+        lo, hi = exclude_mols if exclude_mols is not None else (0, -1)
+        return lib().orc_cbmc_dipole_ediff(self.h, _dp(p1), float(q1), _dp(p2), float(q2), int(ch), _dp(gc),
+                                           _dp(gi), cur, lo, hi)
+
     def random_bond_lengths(self, rng, gauss, n):
         """n draws of PotentialSpring::RandomBondLen on the MT19937 `rng`
         with gasdev state `gauss` (an OrcGauss)."""
@@ -381,7 +400,7 @@ class GCSim:
                         bead_charge=g["bead_charge"],
                         rigid_bond=sd["r0"] if rigid_bond is None else rigid_bond,
                         chain_len=g["chain_len"], trials=g["trials"], type_c=g.get("type", 0),
-                        bond=1 if sd["use_bond"] else 2, phantom=sd.get("phantom", 0), coion=0,
+                        bond=1 if sd["use_bond"] else 2, phantom=sd.get("phantom", 0), coion=g.get("coion", 0),
                         grafted=sd.get("grafted", 0), grafted_counterion=0)
         self.rng = MT19937(seed)
         self.gauss = OrcGauss()

@@ -976,13 +976,35 @@ double orc_bead_energy_sample(orc_sys* s, const double xyz[3], double q, int t,
   return e;
 }
 
+/* PotentialEwaldCoul::DipoleEDiff (potential_ewald_coul.cc:473-530, called at
+   cbmc.cc:109-112): with M_z the sum of q z over the molecules outside
+   [ex_lo, ex_hi] (CURRENT z), then the grown chain beads, then their
+   counterions, the change 2 pi l_B / V (M_n^2 - M_z^2) when the trial bead
+   (q1 at b1) and, for a charged chain, its counterion (q2 at b2) join:
+   M_n = M_z + q1 z1 (+ q2 z2). */
+double orc_cbmc_dipole_ediff(const orc_sys* s, const double b1[3], double q1, const double b2[3], double q2,
+                             int charged, const double* grown_c, const double* grown_i, int cur, int ex_lo,
+                             int ex_hi) {
+  double mo = 0;
+  for (int m = 0; m < s->n_mols; m++) {
+    if (ex_lo <= ex_hi && m >= ex_lo && m <= ex_hi) continue;
+    for (int j = s->mol_start[m]; j < s->mol_start[m + 1]; j++) mo += s->q[j] * s->cur[3 * j + 2];
+  }
+  for (int i = 0; i < cur; i++) mo += q1 * grown_c[3 * i + 2];
+  if (charged)
+    for (int i = 0; i < cur; i++) mo += q2 * grown_i[3 * i + 2];
+  double mn = mo + q1 * b1[2];
+  if (charged) mn += q2 * b2[2];
+  return s->p.lB * 2 * kPi / (s->e.vol_e / 1.0) * (mn * mn - mo * mo);
+}
+
 /* ForceField::BeadsEnergy (cbmc.cc:5-151): trial chain bead b1 (type t1,
    charge q1) and, for charged chains, its trial counterion b2, against the
    molecules outside [ex_lo, ex_hi] (cbmc.cc:23; none when ex_lo > ex_hi),
    the grown chain beads grown_c[0..cur) and counterions grown_i[0..cur),
-   with the reference's break on LJ >= 1e8.  The slab dipole term
-   (DipoleEDiff) is not restated.  Trial coordinates as the reference reads
-   them (equal to the current ones outside a pending move). */
+   with the reference's break on LJ >= 1e8, and the slab dipole term
+   (orc_cbmc_dipole_ediff) with the self energies.  Trial coordinates as the
+   reference reads them (equal to the current ones outside a pending move). */
 double orc_cbmc_beads_energy(orc_sys* s, const double b1[3], int t1, double q1, const double b2[3],
                              int t2, double q2, int charged, const double* grown_c,
                              const double* grown_i, int cur, int ex_lo, int ex_hi) {
@@ -1052,6 +1074,8 @@ double orc_cbmc_beads_energy(orc_sys* s, const double b1[3], int t1, double q1, 
       ewald_e += 0.5 * real_pair_xyz(s, b2, q2, b2, q2);
       ewald_e += 0.5 * repl_pair_xyz(s, b2, q2, b2, q2);
     }
+    if (s->p.dipole)
+      ewald_e += orc_cbmc_dipole_ediff(s, b1, q1, b2, q2, charged, grown_c, grown_i, cur, ex_lo, ex_hi);
   }
   /* 4. walls */
   if (s->p.use_ext) {

@@ -157,6 +157,9 @@ double orc_end_to_end(const orc_sys* s);
    ex_lo > ex_hi).  Pinned by the GPU trial energies only through this
    restatement of the reference loops (parity unpinned beyond the pair
    functions it calls, which are pinned). */
+double orc_cbmc_dipole_ediff(const orc_sys* s, const double b1[3], double q1, const double b2[3], double q2,
+                             int charged, const double* grown_c, const double* grown_i, int cur, int ex_lo,
+                             int ex_hi);
 double orc_cbmc_beads_energy(orc_sys* s, const double b1[3], int t1, double q1, const double b2[3],
                              int t2, double q2, int charged, const double* grown_c,
                              const double* grown_i, int cur, int ex_lo, int ex_hi);

@@ -15,7 +15,7 @@ struct CbmcHost {
   hipGraphExec_t g_exec = nullptr;
   int cap_k = 0, cap_g = 0, nkb = 0;
   CbmcIn* in = nullptr;
-  double *xyz = nullptr, *grown = nullptr, *kpart = nullptr, *out = nullptr;
+  double *xyz = nullptr, *grown = nullptr, *kpart = nullptr, *out = nullptr, *mz = nullptr;
   double2* cells = nullptr;
   char* h_stage = nullptr;   // pinned: CbmcIn + trials + grown
   double* h_out = nullptr;   // pinned
@@ -46,6 +46,7 @@ int cbmc_reserve(plum_ctx* c, int K, int cur) {
   CHK(dalloc(c, &h->cells, (size_t)2 * ck * kMaxSplit));
   CHK(dalloc(c, &h->kpart, (size_t)ck * h->nkb));
   CHK(dalloc(c, &h->out, ck));
+  CHK(dalloc(c, &h->mz, 1));
   if (h->h_stage) (void)hipHostFree(h->h_stage);
   if (h->h_out) (void)hipHostFree(h->h_out);
   h->h_stage = nullptr;
@@ -73,7 +74,6 @@ extern "C" int plum_cbmc_trials(plum_ctx* c, const plum_cbmc_request* rq, const 
     return fail(PLUM_E_ARG, "bead type out of range");
   const bool ex = rq->exclude_lo <= rq->exclude_hi;
   if (ex && (rq->exclude_lo < 0 || rq->exclude_hi >= c->g.n)) return fail(PLUM_E_ARG, "exclusion range out of range");
-  if (c->g.dipole) return fail(PLUM_E_UNSUPPORTED, "configurational-bias trials with the slab dipole term");
   for (int i = 0; i < 3 * K * (1 + ch); i++)
     if (!std::isfinite(trials[i])) return fail(PLUM_E_ARG, "non-finite trial coordinate");
   CHK(cbmc_reserve(c, K, cur));
@@ -109,8 +109,10 @@ extern "C" int plum_cbmc_trials(plum_ctx* c, const plum_cbmc_request* rq, const 
   const int nkb = (g.use_ewald && g.n_kh > 0) ? h->nkb : 0;
   if (nkb > 0)
     hipLaunchKernelGGL(k_cbmc_kspace, dim3(nkb), dim3(kBlock), 0, c->st, g, c->d, h->in, h->xyz, h->grown, h->kpart);
+  if (g.dipole && g.use_ewald)
+    hipLaunchKernelGGL(k_cbmc_mz, dim3(1), dim3(kBlock), 0, c->st, g, c->d, h->in, h->grown, h->mz);
   hipLaunchKernelGGL(k_cbmc_final, dim3(K), dim3(kBlock), 0, c->st, g, c->d, h->in, h->xyz, h->grown, h->cells, ns,
-                     h->kpart, nkb, h->out);
+                     h->kpart, nkb, h->mz, h->out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(h->h_out, h->out, sizeof(double) * K, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -205,7 +207,6 @@ extern "C" int plum_cbmc_grow(plum_ctx* c, const plum_growth_request* rq, const 
     return fail(PLUM_E_ARG, "bead type out of range");
   const bool ex = rq->exclude_lo <= rq->exclude_hi;
   if (ex && (rq->exclude_lo < 0 || rq->exclude_hi >= c->g.n)) return fail(PLUM_E_ARG, "exclusion range out of range");
-  if (c->g.dipole) return fail(PLUM_E_UNSUPPORTED, "configurational-bias growth with the slab dipole term");
   CHK(grow_reserve(c, K, L));
   CbmcHost* h = c->cbmc;
   GrowIn in{};

@@ -213,6 +213,17 @@ __device__ __forceinline__ double dipole_change(double e_new, double e_old) {
   return e_new - e_old;
 }
 
+// PotentialEwaldCoul::DipoleEDiff (potential_ewald_coul.cc:473-530): the
+// slab dipole energy change coef (M_n^2 - M_o^2) when a bead (q z = a) and
+// its counterion (b; 0 for a neutral chain) join a system of dipole M_o, as
+// the reference evaluates it.
+__device__ __forceinline__ double dipole_ediff(double coef, double mo, double a, double b) {
+#pragma clang fp contract(off)
+  double mn = mo + a;
+  mn += b;
+  return coef * (mn * mn - mo * mo);
+}
+
 // Truncated-shifted LJ / WCA at distance r (potential_truncated_lj.cc:49-85).
 __device__ __forceinline__ double lj_energy(double r, const LJPair& p) {
   if (r <= 0) return kVeryLargeEnergy;

@@ -101,13 +101,33 @@ __global__ __launch_bounds__(kBlock) void k_cbmc_kspace(Geo g, Dev d, const Cbmc
   }
 }
 
-// Parts 2 and 3 explicit and the final sum, one block per trial.
+// Slab dipole of the system less the excluded beads, then the grown chain
+// beads, then their counterions (the M_z of DipoleEDiff before the trial
+// joins, potential_ewald_coul.cc:485-520), one block.
+__global__ __launch_bounds__(kBlock) void k_cbmc_mz(Geo g, Dev d, const CbmcIn* __restrict__ in,
+                                                    const double* __restrict__ grown, double* __restrict__ mz) {
+  __shared__ double sh[kBlock / kWave];
+  double m = 0;
+  for (int j = threadIdx.x; j < g.n; j += kBlock)
+    if (j < in->ex_lo || j > in->ex_hi) m += d.q[j] * d.z[j];
+  m = block_sum<kBlock>(m, sh);
+  if (threadIdx.x == 0) {
+    const int cur = in->cur;
+    for (int i = 0; i < cur; i++) m += in->q_c * grown[3 * (size_t)i + 2];
+    if (in->charged)
+      for (int i = 0; i < cur; i++) m += in->q_i * grown[3 * (size_t)(cur + i) + 2];
+    *mz = m;
+  }
+}
+
+// Parts 2 and 3 explicit and the final sum, one block per trial; `mz` (slab
+// dipole only) is the M_z k_cbmc_mz left.
 __global__ __launch_bounds__(kBlock) void k_cbmc_final(Geo g, Dev d, const CbmcIn* __restrict__ in,
                                                        const double* __restrict__ xyz,
                                                        const double* __restrict__ grown,
                                                        const double2* __restrict__ cells, int nsplit,
                                                        const double* __restrict__ kpart, int nkb,
-                                                       double* __restrict__ out) {
+                                                       const double* __restrict__ mz, double* __restrict__ out) {
   __shared__ double sh[kBlock / kWave];
   const int K = in->K, cur = in->cur, t = blockIdx.x;
   const int ch = in->charged;
@@ -168,6 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_cbmc_final(Geo g, Dev d, const CbmcI
       if (ch)
         ewald_e += g.self_coef * in->q_i * in->q_i + real_pair(r1, in->q_c, r2, in->q_i, g) +
                    0.5 * real_pair(r2, in->q_i, r2, in->q_i, g);
+      if (g.dipole) ewald_e += dipole_ediff(g.dip_coef, *mz, in->q_c * r1[2], ch ? in->q_i * r2[2] : 0.0);
     }
     out[t] = pair_e >= kVeryLargeEnergy ? kVeryLargeEnergy : pair_e + ewald_e;
   }

@@ -31,6 +31,9 @@ struct GrowOut {
   double W;       // Rosenbluth weight
   int stop;       // step at which the weight reached 0 (-1: grown to the end)
   int pad;
+  // slab dipole: sum of q z over the system less the excluded beads plus the
+  // chain beads chosen so far, and over the counterions chosen so far
+  double mz_c, mz_i;
 };
 
 // Random numbers and known positions of one growth, one upload:
@@ -86,6 +89,19 @@ __device__ __forceinline__ void grow_trial(const GrowIn& in, const GrowPtr& p, c
 
 __global__ __launch_bounds__(kBlock) void k_grow_init(Geo g, Dev d, const GrowIn* __restrict__ in,
                                                       double2* __restrict__ Sx, GrowOut* __restrict__ out) {
+  __shared__ double sh[kBlock / kWave];
+  if (blockIdx.x == 0 && g.dipole && g.use_ewald) {
+    // M_z of the system less the excluded beads (DipoleEDiff's molecule
+    // loop, potential_ewald_coul.cc:485-506)
+    double m = 0;
+    for (int j = threadIdx.x; j < g.n; j += kBlock)
+      if (j < in->ex_lo || j > in->ex_hi) m += d.q[j] * d.z[j];
+    m = block_sum<kBlock>(m, sh);
+    if (threadIdx.x == 0) {
+      out->mz_c = m;
+      out->mz_i = 0;
+    }
+  }
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (g.use_ewald && k < g.n_kh) {
     const int4 l = d.kl[k];
@@ -286,6 +302,11 @@ __global__ __launch_bounds__(kSelThreads) void k_grow_select(Geo g, Dev d, const
       if (g.use_ewald && pair_e < kVeryLargeEnergy) {
         ewald_e = ere + x.y + ek + g.self_coef * in.q_c * in.q_c;
         if (ch) ewald_e += g.self_coef * in.q_i * in.q_i;
+        if (g.dipole) {
+          double r1[3], r2[3];
+          grow_trial(in, p, grown, s, t, r1, r2);
+          ewald_e += dipole_ediff(g.dip_coef, out->mz_c + out->mz_i, in.q_c * r1[2], ch ? in.q_i * r2[2] : 0.0);
+        }
       }
       s_e[t] = pair_e >= kVeryLargeEnergy ? kVeryLargeEnergy : pair_e + ewald_e;
     }
@@ -327,6 +348,10 @@ __global__ __launch_bounds__(kSelThreads) void k_grow_select(Geo g, Dev d, const
         grown[3 * (size_t)(L + s) + k] = r2[k];
         s_pos[k] = r1[k];
         s_pos[3 + k] = r2[k];
+      }
+      if (g.dipole && g.use_ewald) {
+        out->mz_c += in.q_c * r1[2];
+        if (ch) out->mz_i += in.q_i * r2[2];
       }
     }
   }

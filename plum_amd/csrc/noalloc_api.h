@@ -20,7 +20,8 @@ constexpr size_t kNaOffGrown = kNaOffXyz + 6 * kNaMaxTrials;            // 6 cur
 constexpr size_t kNaOffCells = kNaOffGrown + 6 * kNaMaxGrown;           // 2 K kMaxSplit double2
 constexpr size_t kNaOffKpart = kNaOffCells + 2 * 2 * kNaMaxTrials * kMaxSplit;  // K nkb doubles
 constexpr size_t kNaOffOut = kNaOffKpart + (size_t)kNaMaxTrials * kNaMaxKBlocks;  // K doubles
-constexpr size_t kNaDoubles = kNaOffOut + kNaMaxTrials;
+constexpr size_t kNaOffMz = kNaOffOut + kNaMaxTrials;                   // slab dipole M_z, 1 double
+constexpr size_t kNaDoubles = kNaOffMz + 1;
 static_assert(sizeof(CbmcIn) <= 8 * sizeof(double), "request slot");
 static_assert(kNaDoubles <= 16384, "scratch stays at 128 KiB");
 
@@ -71,7 +72,6 @@ extern "C" int plum_cbmc_trials_noalloc(plum_ctx* c, const plum_cbmc_request* rq
     return fail(PLUM_E_ARG, "bead type out of range");
   const bool ex = rq->exclude_lo <= rq->exclude_hi;
   if (ex && (rq->exclude_lo < 0 || rq->exclude_hi >= g.n)) return fail(PLUM_E_ARG, "exclusion range out of range");
-  if (g.dipole) return fail(PLUM_E_UNSUPPORTED, "configurational-bias trials with the slab dipole term");
   for (int i = 0; i < 3 * K * (1 + ch); i++)
     if (!std::isfinite(trials[i])) return fail(PLUM_E_ARG, "non-finite trial coordinate");
   double* sp = nullptr;
@@ -92,6 +92,7 @@ extern "C" int plum_cbmc_trials_noalloc(plum_ctx* c, const plum_cbmc_request* rq
   double2* d_cells = reinterpret_cast<double2*>(sp + kNaOffCells);
   double* d_kpart = sp + kNaOffKpart;
   double* d_out = sp + kNaOffOut;
+  double* d_mz = sp + kNaOffMz;
   // work still queued by the device loop comes first
   if (c->self) {
     HIPCHK(hipEventRecord(c->ev_idle, c->self->st));
@@ -108,8 +109,10 @@ extern "C" int plum_cbmc_trials_noalloc(plum_ctx* c, const plum_cbmc_request* rq
     hipLaunchKernelGGL(k_cbmc_cells<0>, dim3(2 * K * ns), dim3(kBlock), 0, c->st, g, c->d, d_in, d_xyz, ns, d_cells);
   if (nkb > 0)
     hipLaunchKernelGGL(k_cbmc_kspace, dim3(nkb), dim3(kBlock), 0, c->st, g, c->d, d_in, d_xyz, d_grown, d_kpart);
+  if (g.dipole && g.use_ewald)
+    hipLaunchKernelGGL(k_cbmc_mz, dim3(1), dim3(kBlock), 0, c->st, g, c->d, d_in, d_grown, d_mz);
   hipLaunchKernelGGL(k_cbmc_final, dim3(K), dim3(kBlock), 0, c->st, g, c->d, d_in, d_xyz, d_grown, d_cells, ns,
-                     d_kpart, nkb, d_out);
+                     d_kpart, nkb, d_mz, d_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(energies, d_out, sizeof(double) * K, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));

@@ -34,9 +34,12 @@ def deck_sysdef(run_in):
     nxt(), nxt()                               # vp_el_res, vp_bead_size
     use_pair, use_ewald, use_bond, use_rigid, _ang, _dih, use_ext, use_gc = (
         int(nxt()) for _ in range(8))
+    gc = None
     if use_gc:
-        for _ in range(7):
-            nxt()
+        # chemical potential, de Broglie prefactor, move frequency, chain
+        # length, bead charge, bead type, CBMC trials (force_field.cc:96-103)
+        gc = dict(chem_pot=float(nxt()), debroglie=float(nxt()), freq=int(nxt()), chain_len=int(nxt()),
+                  bead_charge=float(nxt()), type_name=nxt(), trials=int(nxt()), coion=coion)
     lj, wall = {}, {}
     lj_cutoff = 2.5
     lB = dielectric = alpha = 0.0
@@ -93,6 +96,9 @@ def deck_sysdef(run_in):
               wall_eps_w=wall_eps_w, phantom=phantom, grafted=grafted, move_prob=move_prob,
               move_size=move_size, types=types, box=box, xyz=xyz, q=q, type=ty, mol_start=ms,
               name=os.path.basename(here))
+    if gc is not None:
+        gc["type"] = names.index(gc.pop("type_name"))
+        sd["gc"] = gc
     bond_len = rigid if use_rigid else (r0 if use_bond else 0.0)
     vary = (not use_rigid) and bool(use_bond)
     return sd, dict(phantom=phantom, coion=coion, grafted=grafted, bond_len=bond_len,

@@ -2,7 +2,8 @@
 the oracle's restatement of ForceField::BeadsEnergy (cbmc.cc:5-151), one
 trial at a time: insertion steps with and without a grown chain, charged and
 neutral chains, a deletion step that leaves a chain and its counterions out
-of the system, walls, forbidden trials, and the FP32-pair path."""
+of the system, walls, forbidden trials, the slab dipole term, and the
+FP32-pair path."""
 import numpy as np
 import pytest
 
@@ -84,6 +85,17 @@ def test_walls_and_forbidden_trials():
     assert got[1] == 1e8 and want[1] == 1e8
 
 
+@pytest.mark.parametrize("cur,charged,exclude", [(0, True, None), (3, True, None), (2, False, None),
+                                                 (4, True, (5, 9))])
+def test_slab_dipole_term_matches_oracle(cur, charged, exclude):
+    """Slab Ewald with the dipole correction: each trial's energy carries
+    DipoleEDiff (potential_ewald_coul.cc:473-530) over the system less the
+    excluded molecules, the grown chain and its counterions."""
+    sd = systems.config("slab_small")
+    assert sd["dipole"]
+    _compare(sd, 12, cur, charged, seed=30 + cur, exclude_mols=exclude)
+
+
 def test_fp32_pair_path():
     sd = systems.config("small")
     _compare(sd, 16, 4, True, seed=21, precision=E.PLUM_FP32_PAIR, tol=1e-6)
@@ -138,7 +150,8 @@ def _stepwise(eng, mode, L, K, first, disp, ions, sel, actual, charged, exclude,
 
 @pytest.mark.parametrize("name,mode,charged", [("small", 0, True), ("small", 0, False),
                                                ("gc_small", 1, True), ("small", 2, True),
-                                               ("c4", 0, True), ("c4", 1, True)])
+                                               ("c4", 0, True), ("c4", 1, True),
+                                               ("slab_small", 0, True), ("slab_small", 2, True)])
 def test_whole_growth_matches_the_stepwise_loop(name, mode, charged):
     """plum_cbmc_grow (every step on the device) against the reference's
     growth loop driven step by step through plum_cbmc_trials on the same

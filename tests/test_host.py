@@ -224,6 +224,40 @@ def test_reference_grand_canonical_example_replays_on_the_oracle():
 
 
 @pytest.mark.gpu
+def test_reference_confined_grand_canonical_example_replays_on_the_oracle():
+    """The reference's own confined grand-canonical deck (18 surface sites and
+    their counterions between walls, slab Ewald with the dipole correction,
+    8-bead chains of charge -1 with counterions, 30 trials, mu = -90): every
+    growth step's trial energies carry the slab dipole term (DipoleEDiff,
+    potential_ewald_coul.cc:473-530); the host driver with GPU growths and
+    the oracle take the same exchanges and trial moves and end in the same
+    configuration."""
+    from deck import deck_sysdef
+    from oracle import oracle as O
+    path = os.path.join(EX, "confined_muvt", "run.in")
+    sd, mv = deck_sysdef(path)
+    sd["r0"] = mv["bond_len"]              # rigid bonds: the reptation and growth length
+    steps = 800
+    sim = H.HostSimulation(path, 11)
+    att, acc = sim.run(steps)
+    gatt, gacc, nb, nm = sim.gc_stats()
+    orc = O.GCSim(sd, 11)
+    orc.run(steps)
+    print("confined example deck: host gc %s %s trial %s %s | oracle gc %s %s trial %s %s | beads %d" % (
+        gatt, gacc, att, acc, orc.gc_attempted, orc.gc_accepted, orc.attempted, orc.accepted,
+        nb), flush=True)
+    assert gatt == orc.gc_attempted and gacc == orc.gc_accepted
+    assert att == orc.attempted and acc == orc.accepted
+    assert nb == len(orc.sd["q"]) and sum(gacc) > 0
+    assert np.max(np.abs(sim.positions(nb) - orc.orc.positions())) <= 1e-9
+    run, full = sim.energies()
+    ref = orc.orc.energy_full()
+    assert abs(full["dipole"] - ref["dipole"]) <= 1e-10 * max(abs(ref["dipole"]), 1e-2)
+    orc.close()
+    sim.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["small", "slab_small"])
 def test_last_state_files_resume_the_run(tmp_path, name):
     """PrintLastCrd / PrintLastTop: the written configuration, read back by a

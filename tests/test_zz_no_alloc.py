@@ -29,7 +29,7 @@ def _inputs(sd, K, cur, seed):
     return tri, ions, grown, gions
 
 
-@pytest.mark.parametrize("name", ["small", "c0"])
+@pytest.mark.parametrize("name", ["small", "c0", "slab_small"])
 def test_noalloc_trials_equal_the_allocating_entry(name):
     sd = systems.config(name)
     eng = E.Engine(sd)
