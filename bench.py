@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--replica-sweeps", type=int, default=512)
     ap.add_argument("--no-gc", action="store_true")
     ap.add_argument("--gc-steps", type=int, default=20000)
+    ap.add_argument("--brush-only", action="store_true",
+                    help="run only the brush / delayed-acceptance leg")
     ap.add_argument("--gc-only", action="store_true",
                     help="run only the grand-canonical leg (for profiling)")
     ap.add_argument("--growth-only", action="store_true",
@@ -284,11 +286,46 @@ def fp32_term_errors(sd, moves=20):
     return {k: float("%.3g" % v) for k, v in worst.items()}
 
 
+def integrated_autocorr_time(x):
+    """Integrated autocorrelation time of a series (in samples), summed over
+    lags until the window reaches 5 tau (Sokal's self-consistent window)."""
+    x = np.asarray(x, dtype=float) - np.mean(x)
+    n = len(x)
+    var = np.dot(x, x) / n
+    if n < 8 or var == 0:
+        return 1.0
+    tau = 1.0
+    for lag in range(1, n // 2):
+        tau += 2.0 * np.dot(x[:-lag], x[lag:]) / ((n - lag) * var)
+        if lag >= 5 * tau:
+            break
+    return max(tau, 1.0)
+
+
+def chain_ess(eng, moves, chunks, kw, seed):
+    """Runs `moves` moves in `chunks` pieces, recording the total energy after
+    each; returns (seconds, accepted, moves, ESS of the energy series)."""
+    series, secs, acc, done = [], 0.0, 0, 0
+    for c in range(chunks):
+        st = eng.mc_run(moves // chunks, seed=seed + c, **kw)
+        secs += st.seconds
+        acc += sum(st.accepted)
+        done += st.moves
+        series.append(sum(eng.totals().values()))
+    tau = integrated_autocorr_time(series)
+    return secs, acc, done, len(series) / (2.0 * tau), tau
+
+
 def brush_leg(moves, batch=32):
     """Confined brush (64 x 50 grafted chains + counterions, 400 charged
     surface sites, slab Ewald): the plain speculative path against delayed
-    acceptance with the online-trained surrogate, in accepted moves per
-    second."""
+    acceptance with the surrogate, in accepted moves per second and effective
+    samples of the energy per second.  The surrogate adapts online from
+    scratch (exact dE of proposals evaluated anyway as training data), then
+    is frozen: diminishing adaptation taken to its limit, after which the
+    first stage is a fixed function and the two-stage chain is exactly the
+    Metropolis chain's target.  Frozen, it screens 32 proposals per batch and
+    spends no exact slots on training."""
     sd = systems.config("c3")
     kw = dict(move_prob=sd["move_prob"], move_size=sd["move_size"], bond_len=sd["r0"],
               phantom=sd["phantom"], grafted=sd["grafted"])
@@ -296,37 +333,60 @@ def brush_leg(moves, batch=32):
                        "walls z=0,40, slab Ewald (Lz 190), moves 20/50/30 COM/pivot/crankshaft",
            "particles": int(len(sd["q"])), "moves": moves}
     out = {}
-    for label, pmax in (("plain_p1", 1), ("plain_p8", 8), ("plain_p16", 16)):
+    shapes = {"plain_p1": (1, 1), "plain_p8": (8, 1), "plain_p16": (16, 1), "plain_p8_cells3": (8, 3),
+              "plain_p16_cells3": (16, 3)}
+    for label, (pmax, cdiv) in shapes.items():
         eng = E.Engine(sd)
-        eng.set_tuning(pmax=pmax)
+        eng.set_tuning(pmax=pmax, cell_div=cdiv)
         eng.mc_run(2000, seed=1, **kw)
         st = eng.mc_run(moves, seed=2, **kw)
         acc = sum(st.accepted)
         out[label] = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
         log("brush %s: %.0f moves/s, %.0f accepted/s" % (label, out[label][0], out[label][1]))
         eng.close()
-    # the plain path at its better batch size is the reference point
-    out["plain"] = max(out["plain_p8"], out["plain_p16"], key=lambda v: v[1])
-    plain_pmax = 8 if out["plain"] is out["plain_p8"] else 16
+    # the plain path at its best batch size and cell grid is the reference point
+    best = max((k for k in shapes if k != "plain_p1"), key=lambda k: out[k][1])
+    out["plain"] = out[best]
+    plain_pmax, plain_div = shapes[best]
+    # effective samples of the energy per second, plain
+    eng = E.Engine(sd)
+    eng.set_tuning(pmax=plain_pmax, cell_div=plain_div)
+    eng.mc_run(2000, seed=1, **kw)
+    p_secs, p_acc, p_moves, p_ess, p_tau = chain_ess(eng, 4 * moves, 100, kw, 7)
+    eng.close()
+    # adaptation: online training from scratch
     eng = E.Engine(sd)
     eng.surrogate_enable(screen=32, fill=4, train_every=16, train_samples=128, ring=2048, lr=1e-3,
                          r_d=4.0, seed=1)
-    eng.mc_run(3 * moves, seed=1, **kw)          # online training from scratch
+    eng.set_tuning(pmax=32, cell_div=3)
+    eng.mc_run(3 * moves, seed=1, **kw)
     s0 = eng.surrogate_stats()
     st = eng.mc_run(moves, seed=2, **kw)
     s1 = eng.surrogate_stats()
     acc = sum(st.accepted)
-    da = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
-    log("brush DA: %.0f moves/s, %.0f accepted/s" % (da[0], da[1]))
+    adapt = (st.moves / st.seconds, acc / st.seconds, acc / max(st.moves, 1))
+    log("brush DA adapting: %.0f moves/s, %.0f accepted/s" % (adapt[0], adapt[1]))
+    # frozen: the trained weights, no training, no fill slots
+    w = eng.surrogate_weights()
+    loss, steps = s1["last_loss"], s1["train_steps"]
+    eng.surrogate_enable(screen=32, fill=0, train_every=0, train_samples=1, ring=1, lr=1e-3, r_d=4.0, seed=1)
+    eng.surrogate_weights(w)
+    eng.set_tuning(pmax=32, cell_div=3)
+    f0 = eng.surrogate_stats()
+    f_secs, f_acc, f_moves, f_ess, f_tau = chain_ess(eng, 4 * moves, 100, kw, 11)
+    f1 = eng.surrogate_stats()
+    da = (f_moves / f_secs, f_acc / f_secs, f_acc / max(f_moves, 1))
+    log("brush DA frozen: %.0f moves/s, %.0f accepted/s, ESS/s %.1f (plain %.1f)" % (
+        da[0], da[1], f_ess / f_secs, p_ess / p_secs))
     # the batched form: several brushes advanced together, every one screened
-    # by the network trained above (it stays with the first system)
+    # by the frozen network (it stays with the first system)
     R = batch
     sds = [sd] + [systems.config("c3", seed=1003 + 101 * r) for r in range(1, R)]
     engs = [eng] + [E.Engine(x) for x in sds[1:]]
     mcs = [dict(kw, seed=500 + r) for r in range(R)]
     b = E.Batch(engs)
-    b.set_tuning(pmax=16, batches_per_graph=16)
-    b.mc_run(moves // R, mcs)                  # warm-up (training continues)
+    b.set_tuning(pmax=32, batches_per_graph=16, cell_div=3)
+    b.mc_run(moves // R, mcs)
     stb = b.mc_run(moves // R * 2, mcs)
     accb = sum(stb.accepted)
     log("brush DA batched x%d: %.0f moves/s, %.0f accepted/s" % (R, stb.moves / stb.seconds,
@@ -334,7 +394,7 @@ def brush_leg(moves, batch=32):
     b.close()
     eng.surrogate_disable()
     b = E.Batch(engs)
-    b.set_tuning(pmax=16)
+    b.set_tuning(pmax=plain_pmax, cell_div=plain_div)
     b.mc_run(moves // R, mcs)
     stp = b.mc_run(moves // R * 2, mcs)
     accp = sum(stp.accepted)
@@ -350,22 +410,31 @@ def brush_leg(moves, batch=32):
         "plain_batched_accepted_per_s": accp / stp.seconds,
         "surrogate_batched_vs_plain": (accb / stb.seconds) / out["plain"][1],
         "surrogate_batched_vs_plain_batched": (accb / stb.seconds) / (accp / stp.seconds),
-    })
-    res.update({
         "plain_fp64_moves_per_s": out["plain"][0], "plain_fp64_accepted_per_s": out["plain"][1],
         "plain_fp64_accept_ratio": out["plain"][2], "plain_fp64_proposals_per_batch": plain_pmax,
-        "plain_p8_accepted_per_s": out["plain_p8"][1], "plain_p16_accepted_per_s": out["plain_p16"][1],
+        "plain_fp64_cell_div": plain_div,
+        "plain_accepted_per_s_by_shape": {k: out[k][1] for k in shapes},
         "plain_one_proposal_moves_per_s": out["plain_p1"][0],
         "plain_one_proposal_accepted_per_s": out["plain_p1"][1],
         "surrogate_moves_per_s": da[0], "surrogate_accepted_per_s": da[1],
         "surrogate_accept_ratio": da[2],
         "accepted_per_s_vs_plain": da[1] / out["plain"][1],
         "accepted_per_s_vs_one_proposal_plain": da[1] / out["plain_p1"][1],
-        "first_stage_pass_fraction": (s1["s1_passed"] - s0["s1_passed"]) / max(st.moves, 1),
-        "exact_evals_per_accepted": ((s1["s1_passed"] - s0["s1_passed"]) +
-                                     (s1["train_evals"] - s0["train_evals"])) / max(acc, 1),
-        "stale_cuts": s1["stale_cuts"] - s0["stale_cuts"],
-        "training_loss_kT2": s1["last_loss"], "training_steps": s1["train_steps"],
+        "ess_per_s_energy_plain": p_ess / p_secs, "ess_per_s_energy_surrogate": f_ess / f_secs,
+        "ess_per_s_vs_plain": (f_ess / f_secs) / (p_ess / p_secs),
+        "autocorr_time_energy_chunks": {"plain": p_tau, "surrogate": f_tau,
+                                        "chunk_moves": 4 * moves // 100},
+        "first_stage_pass_fraction": (f1["s1_passed"] - f0["s1_passed"]) / max(f_moves, 1),
+        "exact_evals_per_accepted": ((f1["s1_passed"] - f0["s1_passed"]) +
+                                     (f1["train_evals"] - f0["train_evals"])) / max(f_acc, 1),
+        "stale_cuts": f1["stale_cuts"] - f0["stale_cuts"],
+        "adapting_moves_per_s": adapt[0], "adapting_accepted_per_s": adapt[1],
+        "adapting_exact_evals_per_accepted": ((s1["s1_passed"] - s0["s1_passed"]) +
+                                              (s1["train_evals"] - s0["train_evals"])) / max(acc, 1),
+        "training_loss_kT2": loss, "training_steps": steps,
+        "schedule": "adapt online for 3 x moves (fill 4, a training step every 16 batches), then "
+                    "freeze (no training, no fill slots, 32 proposals screened per batch); cells of "
+                    "a third of the search radius for the surrogate runs",
     })
     return res
 
@@ -533,6 +602,9 @@ def main():
         return
     if a.gc_only:
         print(json.dumps({"grand_canonical": gc_leg(a.gc_steps)}), flush=True)
+        return
+    if a.brush_only:
+        print(json.dumps({"brush_delayed_acceptance": brush_leg(a.brush_moves)}), flush=True)
         return
     if a.growth_only:
         print(json.dumps({"growth": growth_leg()}), flush=True)
