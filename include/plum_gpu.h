@@ -234,7 +234,7 @@ typedef struct {
                             the cell compaction period) */
   int pmax;              /* proposals per speculative batch, 1..16 (1..32 screened with the surrogate) */
   double sweep_us;       /* measured time per move of every system (plum_batch_tune) */
-  int cell_div;          /* cells per search radius and dimension of every system, 1..4
+  int cell_div;          /* cells per search radius and dimension of every system, 1..8
                             (0 keeps each system's grid); a change rebuilds the cell lists */
 } plum_batch_tuning;
 

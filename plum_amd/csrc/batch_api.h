@@ -26,6 +26,7 @@ struct plum_batch {
   int total_cells = 0;
   int max_nkb_full = 1;
   int n_cu = 0;
+  int descr_occ = 1;    // resident k_descr_spec blocks per CU
   plum_batch_tuning tune{};
   std::vector<McDev> mc;
   hipGraph_t graph = nullptr;
@@ -224,14 +225,25 @@ void launch_batch(plum_batch* b, hipEvent_t* ev = nullptr) {
     // surrogate: descriptors of every moved bead (old, new) of every system,
     // then the network on all of them in one launch
     plum_ctx* c = b->reps[0];
-    hipLaunchKernelGGL(k_descr_spec, dim3(2 * b->maxm_b, b->tune.pmax, R),
-                       dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC);
+    // descriptor rows: four waves per row for a lone system (few rows, a
+    // long candidate scan each), one per row when many systems fill the chip;
+    // as many blocks over all systems as the chip holds at once (a second
+    // round of blocks would start only as the first finishes its rows),
+    // never more than the rows
+    const int wpr = R == 1 ? 4 : (R < 8 ? 2 : 1);
+    const int rows_max = 2 * b->tune.pmax * (R == 1 ? c->maxm : b->maxm_b);
+    const int per_block = kDescrWaves / wpr;
+    const int dblocks =
+        std::max(1, std::min((rows_max + per_block - 1) / per_block, b->descr_occ * b->n_cu / R));
+    hipLaunchKernelGGL(k_descr_spec, dim3(dblocks * R), dim3(kBlock), 0, b->st, b->dG, b->dD, b->dRC, wpr, R,
+                       dblocks);
     if (ev) (void)hipEventRecord(ev[2], b->st);
-    // the network on the moved beads' rows only: tiles of 32 rows over at
-    // most 2 pmax maxm rows per system, those past a system's rows exit
-    const int tiles = (2 * b->tune.pmax * (R == 1 ? c->maxm : b->maxm_b) + 31) / 32;
-    hipLaunchKernelGGL(k_mlp_spec, dim3(std::max(1, std::min(b->n_cu, R * tiles))), dim3(kBlock), 0, b->st,
-                       c->sur->net, b->sx, b->sf, b->dD, b->dRC, R, b->sx_rows, tiles);
+    // the network on the moved beads' rows only: tiles of 32 rows, at most
+    // 2 pmax maxm rows per system, blocks past a system's tiles exit
+    const int tiles = (rows_max + 31) / 32;
+    const int mblocks = std::max(1, std::min(tiles, (b->n_cu + R - 1) / R));
+    hipLaunchKernelGGL(k_mlp_spec, dim3(mblocks, R), dim3(kBlock), 0, b->st, c->sur->net, b->sx, b->sf, b->dD,
+                       b->dRC, b->sx_rows);
   } else if (ev) {
     (void)hipEventRecord(ev[2], b->st);
   }
@@ -411,6 +423,9 @@ int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return bail(fail(PLUM_E_HIP, "cannot query the device"));
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->descr_occ, k_descr_spec, kBlock, 0) != hipSuccess)
+    return bail(fail(PLUM_E_HIP, "cannot query the occupancy of the descriptor kernel"));
+  b->descr_occ = std::max(1, b->descr_occ);
   if (hipStreamCreate(&b->st) != hipSuccess) return bail(fail(PLUM_E_HIP, "hipStreamCreate failed"));
   if ((rc = balloc(b, &b->dG, n)) || (rc = balloc(b, &b->dD, n)) || (rc = balloc(b, &b->dMC, n)) ||
       (rc = balloc(b, &b->dRC, n)) || (rc = balloc(b, &b->d_vb, n)) || (rc = balloc(b, &b->d_cell_off, n + 1)) ||
@@ -564,7 +579,7 @@ int check_tuning(const plum_batch_tuning* t) {
   if (t->batches_per_graph < 1 || t->batches_per_graph > 1024)
     return fail(PLUM_E_ARG, "batches_per_graph must be 1..1024");
   if (t->pmax < 1 || t->pmax > kMaxScreen) return fail(PLUM_E_ARG, "pmax must be 1..32");
-  if (t->cell_div < 0 || t->cell_div > 4) return fail(PLUM_E_ARG, "cell_div must be 0..4");
+  if (t->cell_div < 0 || t->cell_div > 8) return fail(PLUM_E_ARG, "cell_div must be 0..8");
   return PLUM_OK;
 }
 

@@ -15,6 +15,7 @@ constexpr double k216 = 1.12246204831;
 constexpr int kDiCorrection = 3;
 
 constexpr int kBlock = 256;      // work-group size of the wide kernels
+constexpr int kXcds = 8;         // XCDs of an MI355X; workgroup i is dispatched to XCD i % 8
 constexpr int kWave = 64;        // CDNA wavefront
 constexpr int kMaxQueryCells = 1024;
 

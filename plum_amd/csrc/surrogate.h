@@ -35,23 +35,31 @@ __host__ __device__ __forceinline__ float rbf_gamma(float r_d) {
 }
 
 // ------------------------------------------------------------ descriptors
-// Per-wave scratch of descr_wave: one chunk of up to 64 query cells.
+constexpr int kDescrWaves = kBlock / kWave;
+constexpr int kDescrLoads = 4;                       // candidate loads in flight per lane
+constexpr int kDescrQueue = kDescrLoads * kWave;     // in-range candidates of one round
+
+// Per-wave scratch of descr_wave: one chunk of up to 64 query cells, and the
+// in-range candidates of one round of the scan.
 struct DescrWaveSmem {
   int s_cell[kWave];
   int s_img[kWave];
   int s_end[kWave];
   float s_rel[kWave][3];
+  float2 s_q[kDescrQueue];   // (r, smooth cutoff) of an in-range candidate
+  int s_qc[kDescrQueue];     // its channel
 };
-constexpr int kDescrWaves = kBlock / kWave;
-constexpr int kDescrLoads = 4;   // candidate loads in flight per lane
 
 // Descriptor sums of moved bead m of proposal view v at its old (on = 0) or
 // new (on = 1) position, against every unmoved bead image within r_d (images
 // as the LJ term takes them: z unshifted with two periodic dimensions), over
-// the candidates of wave wv of nw that share the row.  No block barriers: the
-// wave scans the query cells 64 at a time, each lane sums its candidates, and
-// a butterfly over the 64 lanes (32 + 16 + ... + 1 exchanges, fixed order)
-// leaves the wave's sum of feature f in lane f.
+// the candidates of wave wv of nw that share the row.  No block barriers.
+// The wave scans the query cells 64 at a time, kDescrLoads candidates per
+// lane per round; the candidates within r_d (a small fraction of those
+// scanned) go to a queue, and lane f then adds basis function f of every
+// queued candidate of its channel, in queue order: the sum of feature f ends
+// in lane f with one exponential per candidate and lane, and no lane runs
+// the basis of a candidate it does not need.
 __device__ float descr_wave(const Geo& g, const Dev& v, float r_d, const double4& p, int wv, int nw,
                             DescrWaveSmem& sm) {
   const int lane = threadIdx.x % kWave;
@@ -68,9 +76,11 @@ __device__ float descr_wave(const Geo& g, const Dev& v, float r_d, const double4
   const float qi = (float)p.w;
   const float rd2 = r_d * r_d, gam = rbf_gamma(r_d), dmu = (r_d - kRbfMin) / (kRbf - 1);
   const float pi_rd = 3.14159265358979f / r_d;
-  float acc[kWave];
-#pragma unroll
-  for (int f = 0; f < kWave; f++) acc[f] = 0.f;
+  // this lane's feature: channel and basis centre (lanes past the basis
+  // features add nothing)
+  const int my_ch = lane < kRbf * kChan ? lane / kRbf : -1;
+  const float my_mu = kRbfMin + (lane % kRbf) * dmu;
+  float acc = 0.f;
   for (int base = 0; base < ncq; base += kWave) {
     const int nloc = min(kWave, ncq - base);
     int end = 0;
@@ -124,43 +134,37 @@ __device__ float descr_wave(const Geo& g, const Dev& v, float r_d, const double4
           pp[u] = v.cpos32[si];
         }
       }
+      // the round's candidates within r_d join the queue in (u, lane) order
+      int qn = 0;
 #pragma unroll
       for (int u = 0; u < kDescrLoads; u++) {
         const int j = jj[u], t = tt[u];
-        if (j < 0 || (contig ? (j >= mlo && j <= mhi) : v.moved[j])) continue;
         const float4 pr = pp[u];
         const float fx = pr.x - sm.s_rel[t][0], fy = pr.y - sm.s_rel[t][1], fz = pr.z - sm.s_rel[t][2];
         const float r2 = fx * fx + fy * fy + fz * fz;
-        if (r2 >= rd2) continue;
-        const float r = sqrtf(r2);
-        const float fc = 0.5f * (__cosf(pi_rd * r) + 1.0f);
-        const float qq = qi * pr.w;
-        const int ch = qq > 0.f ? 0 : (qq < 0.f ? 1 : 2);
-#pragma unroll
-        for (int k = 0; k < kRbf; k++) {
-          const float dr = r - (kRbfMin + k * dmu);
-          const float ph = __expf(-gam * dr * dr) * fc;
-          acc[k] += ch == 0 ? ph : 0.f;
-          acc[kRbf + k] += ch == 1 ? ph : 0.f;
-          acc[2 * kRbf + k] += ch == 2 ? ph : 0.f;
+        const bool in = j >= 0 && !(contig ? (j >= mlo && j <= mhi) : v.moved[j]) && r2 < rd2;
+        const unsigned long long mask = __ballot(in);
+        if (in) {
+          const int pos = qn + __popcll(mask & ((1ull << lane) - 1ull));
+          const float r = sqrtf(r2);
+          const float qq = qi * pr.w;
+          sm.s_q[pos] = make_float2(r, 0.5f * (__cosf(pi_rd * r) + 1.0f));
+          sm.s_qc[pos] = qq > 0.f ? 0 : (qq < 0.f ? 1 : 2);
         }
+        qn += __popcll(mask);
       }
-    }
-    wave_sync_lds();   // the next chunk rewrites the cell arrays
-  }
-  // butterfly: at offset o each lane keeps the half of its 2o values that its
-  // bit o selects and adds its partner's copy of that half
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    const bool up = (lane & o) != 0;
-#pragma unroll
-    for (int i = 0; i < o; i++) {
-      const float keep = up ? acc[i + o] : acc[i];
-      const float give = up ? acc[i] : acc[i + o];
-      acc[i] = keep + __shfl_xor(give, o, kWave);
+      wave_sync_lds();
+#pragma unroll 4
+      for (int e = 0; e < qn; e++) {
+        const float2 q = sm.s_q[e];
+        const float dr = q.x - my_mu;
+        const float ph = __expf(-gam * dr * dr) * q.y;
+        acc += sm.s_qc[e] == my_ch ? ph : 0.f;
+      }
+      wave_sync_lds();   // the next round rewrites the queue
     }
   }
-  return acc[0];
+  return acc;
 }
 
 // Feature f of the row in lane f from the summed basis values `sum`.
@@ -175,35 +179,103 @@ __device__ __forceinline__ float descr_finish(const Geo& g, const double4& p, fl
 
 static_assert(kRbf * kChan + 3 <= kDescr && kDescr == kWave, "descriptor row = one value per lane");
 
-// Descriptors of every moved bead of every proposal, old and new: grid
-// (2 maxm, pmax, systems), one row per block; row m2 of system r is bead
-// m2 % maxm at old (m2 < maxm) or new.  The block's four waves split the
-// row's candidates (a row holds hundreds to thousands of them, and one wave
-// alone would wait on their loads in turn); the four partial sums are added
-// in wave order.
+// The moved beads' rows of the device loop.  System r's descriptor rows are
+// 2 n_s per proposal s (old rows, then new rows) in proposal order; row j of
+// them is bead m of proposal s at old (on = 0) or new position, physical row
+// r sx_rows + (2 s + on) maxm + m.  A wave finds s from the inclusive prefix
+// of 2 n_s over the proposals (one lane per proposal, the counts the proposal
+// launch wrote: spec_rows).
+__device__ __forceinline__ void spec_slot(int j, int pref, int np, int& s, int& on, int& m) {
+  // s = number of proposals whose prefix ends at or before row j (every
+  // lane runs the same six steps: the shuffles never diverge)
+  s = 0;
+#pragma unroll
+  for (int step = kWave / 2; step > 0; step >>= 1) {
+    const int cand = s + step;
+    const int pv = __shfl(pref, min(cand, kWave) - 1, kWave);
+    if (cand <= np && pv <= j) s = cand;
+  }
+  const int before = __shfl(pref, max(s - 1, 0), kWave);
+  const int b0 = s ? before : 0;
+  const int ns = (__shfl(pref, s, kWave) - b0) / 2;
+  const int k = j - b0;
+  on = k >= ns ? 1 : 0;
+  m = k - on * ns;
+}
+
+__device__ __forceinline__ size_t spec_row(int j, int pref, int np, int r, int sx_rows, int maxm) {
+  int s, on, m;
+  spec_slot(j, pref, np, s, on, m);
+  return (size_t)r * sx_rows + (size_t)(2 * s + on) * maxm + m;
+}
+
+// Rows of system d this batch (2 n_s summed over its proposals), the same in
+// every lane; with `pref` each lane also gets the inclusive prefix up to
+// proposal `lane` (the total in lanes past the last proposal).
+__device__ __forceinline__ int spec_rows(const Dev& d, int lane, int* pref) {
+  const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
+  int p = lane < np ? 2 * d.recs[lane].n_moved : 0;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int u = __shfl_up(p, o, kWave);
+    if (lane >= o) p += u;
+  }
+  if (pref) *pref = p;
+  return __builtin_amdgcn_readfirstlane(__shfl(p, kWave - 1, kWave));
+}
+
+// Descriptors of every moved bead of every proposal, old and new: nbx blocks
+// per system.  Each block takes kDescrWaves / wpr rows at a
+// time, wpr waves per row splitting the row's candidates (a row holds
+// hundreds to thousands of them: with few rows in a batch, one wave alone
+// would wait on their loads in turn; with many, one wave per row saves the
+// repeated cell setup); the partial sums of a row are added in wave order.
+// Rows past the system's count are never visited, so a batch's grid does not
+// grow with the padded row count.
 __global__ __launch_bounds__(kBlock) void k_descr_spec(const Geo* __restrict__ G, const Dev* __restrict__ D,
-                                                       const RepCfg* __restrict__ RC) {
+                                                       const RepCfg* __restrict__ RC, int wpr, int R, int nbx) {
   __shared__ DescrWaveSmem sm[kDescrWaves];
   __shared__ float s_sum[kDescrWaves][kWave];
-  const Geo& g = G[blockIdx.z];
-  const Dev& d = D[blockIdx.z];
-  const RepCfg& rc = RC[blockIdx.z];
+  // block (bx, r) of a grid of nbx blocks per system; when the systems split
+  // evenly over the eight XCDs (workgroups are dispatched to XCD id % 8),
+  // every block of system r runs on XCD r % 8, so each XCD's L2 holds the
+  // cell lists of its own systems only
+  int r, bx;
+  if (R % kXcds == 0) {
+    const int xcd = blockIdx.x % kXcds, k = blockIdx.x / kXcds, spx = R / kXcds;
+    r = (k % spx) * kXcds + xcd;
+    bx = k / spx;
+  } else {
+    r = blockIdx.x / nbx;
+    bx = blockIdx.x - r * nbx;
+  }
+  const Geo& g = G[r];
+  const Dev& d = D[r];
+  const RepCfg& rc = RC[r];
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int m2 = blockIdx.x;
-  if (m2 >= 2 * rc.maxm) return;
-  const int s = blockIdx.y;
-  if (s >= d.plan->n_prop) return;
-  const int on = m2 >= rc.maxm ? 1 : 0, m = m2 - on * rc.maxm;
-  const Dev v = prop_view(g, d, rc, s, 0);
-  if (m >= v.rec->n_moved) return;
-  const double4 p = on ? v.mv_new[m] : v.mv_old[m];
-  s_sum[w][lane] = descr_wave(g, v, rc.r_d, p, w, kDescrWaves, sm[w]);
-  __syncthreads();
-  if (w == 0) {
-    float sum = s_sum[0][lane];
-#pragma unroll
-    for (int k = 1; k < kDescrWaves; k++) sum += s_sum[k][lane];
-    d.sur_x[((size_t)(2 * s + on) * rc.maxm + m) * kDescr + lane] = descr_finish(g, p, sum);
+  const int per = kDescrWaves / wpr, slot = w / wpr, part = w - slot * wpr;
+  const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
+  int pref;
+  const int total = spec_rows(d, lane, &pref);
+  for (int base = bx * per; base < total; base += nbx * per) {
+    const int j = base + slot;
+    float sum = 0.f;
+    int s = 0, on = 0, m = 0;
+    double4 p = make_double4(0, 0, 0, 0);
+    if (j < total) {
+      spec_slot(j, pref, np, s, on, m);
+      const Dev v = prop_view(g, d, rc, s, 0);
+      p = on ? v.mv_new[m] : v.mv_old[m];
+      sum = descr_wave(g, v, rc.r_d, p, part, wpr, sm[w]);
+    }
+    s_sum[w][lane] = sum;
+    __syncthreads();
+    if (j < total && part == 0) {
+#pragma unroll 1
+      for (int k = 1; k < wpr; k++) sum += s_sum[w + k][lane];
+      d.sur_x[((size_t)(2 * s + on) * rc.maxm + m) * kDescr + lane] = descr_finish(g, p, sum);
+    }
+    __syncthreads();   // the next rows rewrite s_sum
   }
 }
 
@@ -369,75 +441,31 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(const NetBF16* __restrict_
   }
 }
 
-// Inference of the device loop over the moved beads' rows only.  System r's
-// descriptor rows are 2 n_s per proposal s (old rows, then new rows) in
-// proposal order: a wave builds the prefix of 2 n_s over the proposals (one
-// lane per proposal, the counts the proposal launch wrote) and maps its tile
-// of 32 rows onto physical rows r sx_rows + (2 s + on) maxm + m; tiles past
-// the system's rows are skipped.  Tile t of all systems goes to block
-// t % gridDim.x, whose four waves share it (a quarter of the input rows, a
-// quarter of each layer's columns and of the output rows each), so the few
-// valid tiles of a batch run on different CUs with a short chain each.
-__device__ __forceinline__ size_t spec_row(int j, int pref, int np, int r, int sx_rows, int maxm) {
-  // s = number of proposals whose prefix ends at or before row j (every
-  // lane runs the same six steps: the shuffles never diverge)
-  int s = 0;
-#pragma unroll
-  for (int step = kWave / 2; step > 0; step >>= 1) {
-    const int cand = s + step;
-    const int pv = __shfl(pref, min(cand, kWave) - 1, kWave);
-    if (cand <= np && pv <= j) s = cand;
-  }
-  const int before = __shfl(pref, max(s - 1, 0), kWave);
-  const int b0 = s ? before : 0;
-  const int ns = (__shfl(pref, s, kWave) - b0) / 2;
-  const int k = j - b0, on = k >= ns ? 1 : 0, m = k - on * ns;
-  return (size_t)r * sx_rows + (size_t)(2 * s + on) * maxm + m;
-}
-
-// Rows of system d this batch (2 n_s summed over its proposals), the same in
-// every lane; with `pref` each lane also gets the inclusive prefix up to
-// proposal `lane` (the total in lanes past the last proposal).
-__device__ __forceinline__ int spec_rows(const Dev& d, int lane, int* pref) {
-  const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
-  int p = lane < np ? 2 * d.recs[lane].n_moved : 0;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const int u = __shfl_up(p, o, kWave);
-    if (lane >= o) p += u;
-  }
-  if (pref) *pref = p;
-  return __builtin_amdgcn_readfirstlane(__shfl(p, kWave - 1, kWave));
-}
-
+// Inference of the device loop over the moved beads' rows only: grid
+// (blocks per system, systems); tile t of 32 rows (spec_row) of system r goes
+// to block (t % gridDim.x, r), whose four waves share it (a quarter of the
+// input rows, a quarter of each layer's columns and of the output rows each),
+// so the few tiles of a batch run on different CUs with a short chain each;
+// blocks past a system's tiles exit at once.
 __global__ __launch_bounds__(kBlock) void k_mlp_spec(const NetBF16* __restrict__ net, const float* __restrict__ x,
                                                      float* __restrict__ f, const Dev* __restrict__ D,
-                                                     const RepCfg* __restrict__ RC, int R, int sx_rows,
-                                                     int tiles_per_sys) {
+                                                     const RepCfg* __restrict__ RC, int sx_rows) {
   static_assert(kBlock / kWave == kHid / 32, "one 32-column slice of each layer per wave");
   __shared__ MlpSmem sm;
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int ntile = R * tiles_per_sys;
-  // a block none of whose tiles holds rows exits before staging the network
-  // (every wave reaches the same answer)
-  int busy = 0;
-  for (int tile = blockIdx.x; tile < ntile && !busy; tile += gridDim.x) {
-    const int r = tile / tiles_per_sys, lt = tile - r * tiles_per_sys;
-    busy = lt * 32 < spec_rows(D[r], lane, nullptr);
-  }
-  if (!busy) return;
+  const int r = blockIdx.y;
+  const Dev& d = D[r];
+  const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
+  const int maxm = RC[r].maxm;
+  int pref;
+  const int total = spec_rows(d, lane, &pref);
+  // a block with no tile of rows exits before staging the network
+  if ((int)blockIdx.x * 32 >= total) return;
   stage_net(net, sm);
   const float b4 = net->b4;
   unsigned short* a0 = sm.act[0][0];
   unsigned short* a1 = sm.act[0][1];
-  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-    const int r = tile / tiles_per_sys, lt = tile - r * tiles_per_sys;
-    const Dev& d = D[r];
-    const int np = __builtin_amdgcn_readfirstlane(d.plan->n_prop);
-    const int maxm = RC[r].maxm;
-    int pref;
-    const int total = spec_rows(d, lane, &pref);
-    if (lt * 32 >= total) continue;
+  for (int lt = blockIdx.x; lt * 32 < total; lt += gridDim.x) {
     // lane q % 32 holds the physical row of tile row q (rows past the
     // system's end repeat its last row and are not written back)
     const int prow = (int)spec_row(min(lt * 32 + lane % 32, total - 1), pref, np, r, sx_rows, maxm);
