@@ -29,6 +29,8 @@ from plum_amd import systems  # noqa: E402
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0            # dense MFMA
+MLP_FLOP_PER_ROW = 2.0 * (64 * 128 + 2 * 128 * 128 + 128)   # 64-128-128-128-1 network
 # algorithmic FLOP per pair-image evaluation (minimum image, r, erfc, LJ)
 FLOP_PER_PAIR = 45.0
 
@@ -378,6 +380,29 @@ def brush_leg(moves, batch=32):
     da = (f_moves / f_secs, f_acc / f_secs, f_acc / max(f_moves, 1))
     log("brush DA frozen: %.0f moves/s, %.0f accepted/s, ESS/s %.1f (plain %.1f)" % (
         da[0], da[1], f_ess / f_secs, p_ess / p_secs))
+    # the surrogate kernels' rates: device durations (HIP events) of one batch's
+    # descriptor and inference launches, and the rows a launch carries (2 rows
+    # per moved bead of each screened proposal; the mean moved count from a
+    # record of the decided proposals)
+    eng.mc_record(4000)
+    eng.set_profiling(True)
+    eng.mc_run(3000, seed=13, **kw)
+    kt = eng.kernel_times()
+    eng.set_profiling(False)
+    recs, _ = eng.mc_records(4000)
+    eng.mc_record(0)
+    n_bar = float(np.mean([r["n"] for r in recs])) if recs else 0.0
+    rows = 2.0 * n_bar * 32
+    d_us, m_us = kt["k_descr_spec"][0], kt["k_mlp_spec"][0]
+    mlp_tf = rows * MLP_FLOP_PER_ROW / (m_us * 1e-6) / 1e12 if m_us > 0 else 0.0
+    sur_k = {"k_descr_spec_us": d_us, "k_mlp_spec_us": m_us, "rows_per_launch": rows,
+             "mean_moved_beads": n_bar, "descriptor_rows_per_s": rows / (d_us * 1e-6) if d_us > 0 else 0.0,
+             "mlp": {"bound": "mfma", "achieved": mlp_tf, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": mlp_tf / PEAK_BF16_TFLOPS,
+                     "basis": "%.0f FLOP per row (64-128-128-128-1) x %.0f rows per launch" % (
+                         MLP_FLOP_PER_ROW, rows)}}
+    log("brush surrogate kernels: descriptors %.1f us, inference %.1f us for %.0f rows (%.2f TFLOP/s)" % (
+        d_us, m_us, rows, mlp_tf))
     # the batched form: several brushes advanced together, every one screened
     # by the frozen network (it stays with the first system)
     R = batch
@@ -428,6 +453,7 @@ def brush_leg(moves, batch=32):
         "exact_evals_per_accepted": ((f1["s1_passed"] - f0["s1_passed"]) +
                                      (f1["train_evals"] - f0["train_evals"])) / max(f_acc, 1),
         "stale_cuts": f1["stale_cuts"] - f0["stale_cuts"],
+        "surrogate_kernels": sur_k,
         "adapting_moves_per_s": adapt[0], "adapting_accepted_per_s": adapt[1],
         "adapting_exact_evals_per_accepted": ((s1["s1_passed"] - s0["s1_passed"]) +
                                               (s1["train_evals"] - s0["train_evals"])) / max(acc, 1),

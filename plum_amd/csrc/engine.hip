@@ -560,6 +560,11 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
   double2* erfc_d = nullptr;
   ALLOC(erfc_d, kErfcNodes);
   g.erfc_tab = erfc_d;
+  std::vector<float2> erfc32_h(kErfcNodes);
+  for (int i = 0; i < kErfcNodes; i++) erfc32_h[i] = make_float2((float)erfc_h[i].x, (float)erfc_h[i].y);
+  float2* erfc32_d = nullptr;
+  ALLOC(erfc32_d, kErfcNodes);
+  g.erfc_tab32 = erfc32_d;
   c->nkb_s = (g.use_ewald && g.n_kh > 0) ? (c->nrun + kKSpec - 1) / kKSpec : 0;
   ALLOC(d.part_ks, (size_t)kKVals * std::max(1, c->nkb_s));
   c->xcap = kMaxPairs * cross_blocks(maxm, maxm);
@@ -602,7 +607,8 @@ int plum_create(const plum_params* params, int n, const double* xyz, const doubl
       (rc = up(d.lj, lj.data(), sizeof(LJPair) * lj.size())) ||
       (rc = up(d.lj32, lj32.data(), sizeof(LJPair32) * lj32.size())) ||
       (rc = up(d.wall, wall.data(), sizeof(WallType) * wall.size())) ||
-      (rc = up(erfc_d, erfc_h.data(), sizeof(double2) * kErfcNodes)))
+      (rc = up(erfc_d, erfc_h.data(), sizeof(double2) * kErfcNodes)) ||
+      (rc = up(erfc32_d, erfc32_h.data(), sizeof(float2) * kErfcNodes)))
     return bail(rc);
   if (hipMemsetAsync(d.moved, 0, n, c->st) != hipSuccess ||
       hipMemsetAsync(d.mv_slot, 0xff, sizeof(int) * n, c->st) != hipSuccess ||

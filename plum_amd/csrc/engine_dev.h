@@ -53,6 +53,7 @@ struct Geo {
   double fp64_below;    // FP32 path: pairs closer than this use FP64 math
   int single_image;     // only the minimum image can lie within rc (real_pair)
   const double2* erfc_tab;  // erfc nodes (erfc_nodes), kErfcNodes entries
+  const float2* erfc_tab32;  // the same nodes rounded to FP32 (erfc_nodes32)
 };
 
 // FP64 erfc(x), x >= 0, by the Taylor series about the nearest tabulated
@@ -94,6 +95,25 @@ __device__ __forceinline__ double erfc_nodes(const Geo& g, double x) {
     w *= t * (1.0 / (n + 2));
     s += q * w;
   }
+  return node.x - node.y * s;
+}
+
+// FP32 erfc(x) of the FP32 pair path: the same series about the same nodes
+// (FP32-rounded), four terms: the remainder, about (2 x_i |t|)^4 / 5! of the
+// first term, is below 1/16^4 / 120 < 1.3e-7 of it, and the first term E_i t
+// is at most 2 x_i |t| <= 1/16 of erfc(x_i) below x = 8, so the result
+// carries the FP32 rounding of the nodes and of the sum only (a few ulp).
+__device__ __forceinline__ float erfc_nodes32(const Geo& g, float x) {
+  if (!(x < (float)kErfcMax)) return 0.f;
+  const int i = (int)(x * kErfcPerUnit + 0.5f);
+  const float xi = (float)i * (1.0f / kErfcPerUnit);
+  const float t = x - xi;
+  const float2 node = g.erfc_tab32[i];
+  const float m2x = -2.0f * xi;
+  // q_0 = 1, q_1 = -2 x_i, q_2 = 4 x_i^2 - 2, q_3 = -2 x_i q_2 - 4 q_1
+  const float q2 = m2x * m2x - 2.0f, q3 = m2x * q2 - 4.0f * m2x;
+  const float t2 = t * t;
+  const float s = t + t2 * (0.5f * m2x + t * ((1.0f / 6.0f) * q2 + t * ((1.0f / 24.0f) * q3)));
   return node.x - node.y * s;
 }
 

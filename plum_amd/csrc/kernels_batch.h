@@ -660,6 +660,7 @@ __device__ void kspace_spec_wave(const Geo& g, const Dev& d, const RepCfg& rc, c
 constexpr int kQueue = 2 * kWave;   // pair queue of one wave (holds < 2 fills)
 static_assert(kCluster <= kWave, "one lane per query point");
 
+template <int P>
 struct QWaveSmem {
   int s_cell[kWave];
   int s_img[kWave];
@@ -669,36 +670,37 @@ struct QWaveSmem {
   double s_r2[kQueue];     // squared distance of a queued pair
   double s_qq[kQueue];     // its charge product
   int s_t[kQueue];         // its LJ pair index | 1 << 24 when LJ applies (no z image in a slab)
+  // P = 1: the queue of pairs at or beyond fp64_below, evaluated in FP32
+  double s_fr2[P ? kQueue : 1];
+  float s_fqq[P ? kQueue : 1];
+  int s_ft[P ? kQueue : 1];
 };
 
 // LJ + real-space terms of one queued pair (squared distance r2 <= rs^2,
-// charge product qq, LJ pair index lji, whether LJ applies).  P = 1: FP32 pair
-// math beyond fp64_below (the FP64 distance and cutoff tests are kept).
-template <int P>
+// charge product qq, LJ pair index lji, whether LJ applies), FP64.
 __device__ __forceinline__ void pair_terms(const Geo& g, const Dev& d, double r2, double qq, int lji, bool lj_ok,
                                            double& elj, double& ere) {
   const double r = sqrt(r2);
-  if (P == 0 || r < g.fp64_below) {
-    // FP64 (the FP32 path keeps the steep, large close-contact terms here)
-    if (lj_ok) elj += lj_energy(r, d.lj[lji]);
-    if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc_nodes(g, g.sqrt_alpha * r) / r;
-    return;
-  }
+  if (lj_ok) elj += lj_energy(r, d.lj[lji]);
+  if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc_nodes(g, g.sqrt_alpha * r) / r;
+}
+
+// The same in FP32 (the FP32 pair path, pairs at or beyond fp64_below; the
+// cutoff tests compare the FP64 squared distance), added to the FP64 sums.
+__device__ __forceinline__ void pair_terms32(const Geo& g, const Dev& d, double r2, float qq, int lji, bool lj_ok,
+                                             double& elj, double& ere) {
   const float r2f = (float)r2;
   if (lj_ok) {
     const LJPair32 lp = d.lj32[lji];
-    if (r < lp.cut) {
+    if (r2 < (double)lp.cut * (double)lp.cut) {
       const float s2 = lp.sigma2 / r2f;
       const float r6 = s2 * s2 * s2;
       elj += (double)(lp.eps4 * (r6 * r6 - r6) - lp.shift);
     }
   }
-  if (g.use_ewald) {
-    const float qqf = (float)qq;
-    if (qqf != 0.f && r <= g.rc) {
-      const float rf = (float)r;
-      ere += (double)((float)g.lB * qqf * erfcf((float)g.sqrt_alpha * rf) / rf);
-    }
+  if (g.use_ewald && qq != 0.f && r2 <= g.rc * g.rc) {
+    const float rf = sqrtf(r2f);
+    ere += (double)((float)g.lB * qq * erfc_nodes32(g, (float)g.sqrt_alpha * rf) / rf);
   }
 }
 
@@ -710,7 +712,7 @@ __device__ __forceinline__ void pair_terms(const Geo& g, const Dev& d, double r2
 template <int P>
 __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4* __restrict__ pts,
                                       const int* __restrict__ ptype, int np, int contig, int mlo, int mhi, int part,
-                                      int nparts, QWaveSmem& sm) {
+                                      int nparts, QWaveSmem<P>& sm) {
   const int lane = threadIdx.x % kWave;
   // each point as the image nearest the first point's wrapped position (an
   // image of a point has the same pair images within the radius), and the
@@ -764,16 +766,29 @@ __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4*
   const int nt = g.n_types;
   double elj = 0, ere = 0;
   int qhead = 0, qn = 0;   // queue start and fill (wave-uniform)
+  int fhead = 0, fn = 0;   // the same for the FP32 queue (P = 1)
+  const double fp64_b2 = P ? g.fp64_below * g.fp64_below : 0.0;
   // the pairs queued at qhead .. qhead + count - 1, one per lane
   auto drain = [&](int count) {
     wave_sync_lds();
     if (lane < count) {
       const int k = (qhead + lane) % kQueue;
       const int t = sm.s_t[k];
-      pair_terms<P>(g, d, sm.s_r2[k], sm.s_qq[k], t & 0xffffff, (t >> 24) != 0, elj, ere);
+      pair_terms(g, d, sm.s_r2[k], sm.s_qq[k], t & 0xffffff, (t >> 24) != 0, elj, ere);
     }
     qhead = (qhead + count) % kQueue;
     qn -= count;
+    wave_sync_lds();
+  };
+  auto drain32 = [&](int count) {
+    wave_sync_lds();
+    if (lane < count) {
+      const int k = (fhead + lane) % kQueue;
+      const int t = sm.s_ft[k];
+      pair_terms32(g, d, sm.s_fr2[k], sm.s_fqq[k], t & 0xffffff, (t >> 24) != 0, elj, ere);
+    }
+    fhead = (fhead + count) % kQueue;
+    fn -= count;
     wave_sync_lds();
   };
   wave_sync_lds();   // the points are in LDS
@@ -867,9 +882,11 @@ __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4*
         const double dx = cx - qp.x, dy = cy - qp.y, dz = cz - qp.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const bool in = ok && r2 <= g.rs2;
-        const unsigned long long m = __ballot(in);
+        // FP64 queue: every pair (P = 0), the close ones (P = 1)
+        const bool near = in && (P == 0 || r2 < fp64_b2);
+        const unsigned long long m = __ballot(near);
         if (m) {
-          if (in) {
+          if (near) {
             const int k = (qhead + qn + lanes_below(m)) % kQueue;
             sm.s_r2[k] = r2;
             sm.s_qq[k] = qp.w * pj.w;
@@ -877,6 +894,22 @@ __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4*
           }
           qn += __popcll(m);
           if (qn >= kWave) drain(kWave);
+        }
+        if (P == 1) {
+          // each queue drains 64 pairs of one precision at a time, so the
+          // lanes of a drain never split between the two
+          const bool far = in && !near;
+          const unsigned long long mf = __ballot(far);
+          if (mf) {
+            if (far) {
+              const int k = (fhead + fn + lanes_below(mf)) % kQueue;
+              sm.s_fr2[k] = r2;
+              sm.s_fqq[k] = (float)(qp.w * pj.w);
+              sm.s_ft[k] = (sm.s_ptt[i] * nt + tj) | tflag;
+            }
+            fn += __popcll(mf);
+            if (fn >= kWave) drain32(kWave);
+          }
         }
       }
       it = it2;
@@ -888,6 +921,7 @@ __device__ double2 real_query_cluster(const Geo& g, const Dev& d, const double4*
     wave_sync_lds();   // the next chunk rewrites the cell arrays
   }
   if (qn > 0) drain(qn);
+  if (P == 1 && fn > 0) drain32(fn);
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     elj += __shfl_down(elj, off, kWave);
@@ -933,7 +967,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int* __restrict__ vb_count, int R) {
   __shared__ int s_pre[kMaxReplicas];
   __shared__ struct U {
-    QWaveSmem q[kQWaves];
+    QWaveSmem<P> q[kQWaves];
   } sm;
   __shared__ CSpecSmem cs;
   for (int r = threadIdx.x; r < R; r += kBlock) s_pre[r] = vb_count[r];

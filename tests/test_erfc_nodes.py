@@ -6,7 +6,8 @@ The same steps in double precision are checked here against erfc at 40
 digits (mpmath): the method's error over [0, 8) is a few ulp, so the device
 real-space terms carry FP64 rounding only.  The device code itself is pinned
 end to end by the GPU parity tests (every real-space dE within 1e-10 of the
-oracle's libm erfc).
+oracle's libm erfc).  The FP32 pair path's erfc_nodes32 (the same nodes in
+float32, four terms) is checked the same way at float32 precision.
 """
 import math
 
@@ -69,3 +70,38 @@ def test_erfc_nodes_exact_at_the_nodes_and_zero_beyond_range():
         assert _erfc_nodes(x, nodes) == nodes[i][0]
     assert _erfc_nodes(8.0, nodes) == 0.0
     assert math.erfc(8.0) < 1.2e-29
+
+
+def _erfc_nodes32(x, nodes32):
+    """erfc_nodes32 (engine_dev.h) step by step in float32."""
+    f = np.float32
+    x = f(x)
+    if not x < f(X_MAX):
+        return f(0.0)
+    i = int(x * f(PER_UNIT) + f(0.5))
+    xi = f(i) * f(1.0 / PER_UNIT)
+    t = x - xi
+    a, e = nodes32[i]
+    m2x = f(-2.0) * xi
+    q2 = m2x * m2x - f(2.0)
+    q3 = m2x * q2 - f(4.0) * m2x
+    t2 = t * t
+    s = t + t2 * (f(0.5) * m2x + t * (f(1.0 / 6.0) * q2 + t * (f(1.0 / 24.0) * q3)))
+    return a - e * s
+
+
+def test_fp32_erfc_nodes_within_a_few_fp32_ulp_on_0_to_8():
+    """The FP32 pair path's erfc: the same nodes rounded to float32, four
+    terms of the series, evaluated in float32, against erfc at 40 digits of
+    the float32 argument."""
+    nodes32 = [(np.float32(a), np.float32(e)) for a, e in _nodes()]
+    rng = np.random.default_rng(8)
+    xs = np.concatenate([rng.uniform(0, X_MAX, 6000), np.linspace(0, X_MAX - 1e-3, 2001),
+                         rng.uniform(0, 1 / 64, 500)]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        ref = float(mpmath.erfc(mpmath.mpf(float(x))))
+        got = float(_erfc_nodes32(x, nodes32))
+        worst = max(worst, abs(got - ref) / ref)
+    print("FP32 erfc nodes: worst relative error %.2e (%.1f float32 ulp)" % (worst, worst / 2.0 ** -24))
+    assert worst <= 8 * 2.0 ** -24, worst
