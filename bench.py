@@ -61,6 +61,9 @@ def parse():
                     help="run only the grand-canonical leg (for profiling)")
     ap.add_argument("--growth-only", action="store_true",
                     help="only whole-chain growths of the grand-canonical workload (for profiling)")
+    ap.add_argument("--brush-batched-only", action="store_true",
+                    help="only the batched delayed-acceptance brushes, network at its initial "
+                         "weights (one launch graph in the process, for kernel traces)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only device copies of known size (scale of the HBM counters)")
     return ap.parse_args()
@@ -465,6 +468,34 @@ def brush_leg(moves, batch=32):
     return res
 
 
+def brush_batched_leg(moves, R=32):
+    """32 brushes advanced together with delayed acceptance, screened by the
+    first system's network at its initial weights (no training: one launch
+    graph in the process, which a kernel trace needs); the descriptor and
+    inference kernels' work does not depend on the weights."""
+    sd = systems.config("c3")
+    kw = dict(move_prob=sd["move_prob"], move_size=sd["move_size"], bond_len=sd["r0"],
+              phantom=sd["phantom"], grafted=sd["grafted"])
+    sds = [sd] + [systems.config("c3", seed=1003 + 101 * r) for r in range(1, R)]
+    engs = [E.Engine(x) for x in sds]
+    engs[0].surrogate_enable(screen=32, fill=0, train_every=0, train_samples=1, ring=1, lr=1e-3, r_d=4.0,
+                             seed=1)
+    mcs = [dict(kw, seed=500 + r) for r in range(R)]
+    b = E.Batch(engs)
+    b.set_tuning(pmax=32, batches_per_graph=16, cell_div=3)
+    b.mc_run(max(1, moves // R // 4), mcs)
+    st = b.mc_run(max(1, moves // R), mcs)
+    s = engs[0].surrogate_stats()
+    res = {"systems": R, "moves": st.moves, "moves_per_s": st.moves / st.seconds,
+           "accepted_per_s": sum(st.accepted) / st.seconds,
+           "first_stage_pass_fraction_system0": s["s1_passed"] / max(s["s1_passed"] + s["s1_rejected"], 1)}
+    log("brush batched x%d (initial weights): %.0f moves/s" % (R, res["moves_per_s"]))
+    b.close()
+    for e in engs:
+        e.close()
+    return res
+
+
 def replica_leg(sweeps):
     """256 independent bulk systems (chain length 10..200, 5..100 chains,
     0..2000 salt pairs, half of them charged, density 0.02..0.2), FP32 pair
@@ -634,6 +665,9 @@ def main():
         return
     if a.growth_only:
         print(json.dumps({"growth": growth_leg()}), flush=True)
+        return
+    if a.brush_batched_only:
+        print(json.dumps({"brush_batched": brush_batched_leg(a.brush_moves)}), flush=True)
         return
     t0 = time.perf_counter()
     sd = systems.config(a.config)
