@@ -218,3 +218,95 @@ def test_delayed_acceptance_over_a_batch_of_systems():
     b.close()
     for e in engs:
         e.close()
+
+
+def test_brush_adapted_then_frozen_surrogate_saves_exact_work():
+    """The schedule the bench runs on the confined brush (6,800 particles,
+    slab Ewald, walls): the network adapts online from scratch, then is
+    frozen.  From the same configuration the frozen two-stage chain and the
+    plain chain each make 40,000 moves: the surrogate spends fewer than half
+    the exact evaluations per accepted move that the plain chain does (every
+    plain proposal is evaluated exactly), and its running totals stay equal
+    to the from-scratch energy.  (The brush built by the workload generator
+    is still relaxing after 3,000,000 plain moves, its energy falling
+    steadily, so an ensemble comparison at test length would compare
+    relaxation speeds; the ensemble check of the same schedule runs on the
+    small slab.)"""
+    sd = systems.config("c3")
+    tr = E.Engine(sd)
+    tr.surrogate_enable(screen=32, fill=4, train_every=16, train_samples=128, ring=2048, lr=1e-3, r_d=4.0,
+                        seed=1)
+    tr.set_tuning(pmax=32, cell_div=3)
+    tr.mc_run(40000, **_mc(sd, 62))
+    weights = tr.surrogate_weights()
+    tr.close()
+    per_acc = {}
+    for mode in ("plain", "da"):
+        eng = E.Engine(sd)
+        if mode == "da":
+            eng.surrogate_enable(screen=32, fill=0, train_every=0, train_samples=1, ring=1, r_d=4.0, seed=1)
+            eng.surrogate_weights(weights)
+            eng.set_tuning(pmax=32, cell_div=3)
+        else:
+            eng.set_tuning(pmax=16, cell_div=3)
+        st = eng.mc_run(40000, **_mc(sd, 63))
+        acc = sum(st.accepted)
+        if mode == "da":
+            s = eng.surrogate_stats()
+            exact = s["s1_passed"] + s["train_evals"]
+            run, full = eng.totals(), eng.energy_full()
+            for k in E.TERMS:
+                if k != "dipole":
+                    assert abs(run[k] - full[k]) <= 1e-9 * max(1.0, abs(full[k])), (k, run[k], full[k])
+        else:
+            exact = sum(st.attempted)
+        per_acc[mode] = exact / max(acc, 1)
+        print("brush %s: %d accepted of %d moves, %.2f exact evaluations per accepted move" % (
+            mode, acc, st.moves, per_acc[mode]), flush=True)
+        eng.close()
+    assert per_acc["da"] < 0.5 * per_acc["plain"]
+
+
+def test_slab_adapted_then_frozen_surrogate_samples_the_plain_ensemble():
+    """The same schedule (online adaptation, then a frozen network) on the
+    slab with walls and the slab dipole term: energy and Rg^2 of twelve
+    independent frozen delayed-acceptance chains agree with twelve plain
+    chains within 2 sigma, all from one configuration equilibrated by a long
+    plain run.  The dipole dE is the exact one here (dipole_exact = 1): with
+    the reference's lag (potential_ewald.cc:528-531) a move's own dipole
+    change is charged to the next proposal evaluated exactly, so a chain
+    that evaluates fewer proposals exactly (the first stage rejects most)
+    carries a different sequence of lagged terms and is not the same chain
+    in distribution."""
+    sd0 = systems.config("slab_small")
+    eq = E.Engine(sd0, dipole_exact=1)
+    eq.mc_run(100000, **_mc(sd0, 71))
+    sd = dict(sd0, xyz=eq.positions())
+    eq.close()
+    tr = E.Engine(sd, dipole_exact=1)
+    tr.surrogate_enable(screen=16, fill=2, train_every=8, train_samples=32, seed=4)
+    tr.mc_run(6000, **_mc(sd, 72))
+    weights = tr.surrogate_weights()
+    tr.close()
+    n_chains, samples, chunk, skip = 12, 200, 150, 40
+    res = {"plain": [], "da": []}
+    for mode in ("plain", "da"):
+        for c in range(n_chains):
+            eng = E.Engine(sd, dipole_exact=1)
+            if mode == "da":
+                eng.surrogate_enable(screen=16, fill=0, train_every=0, seed=4)
+                eng.surrogate_weights(weights)
+            obs = []
+            for i in range(samples):
+                eng.mc_run(chunk, **_mc(sd, 6151 * (c + 1) + i))
+                if i >= skip:
+                    obs.append(_observables(eng, sd))
+            res[mode].append(np.mean(obs, 0))
+            eng.close()
+    p, d = np.array(res["plain"]), np.array(res["da"])
+    err = lambda a: a.std(0, ddof=1) / np.sqrt(len(a))
+    z = np.abs(p.mean(0) - d.mean(0)) / np.hypot(err(p), err(d))
+    for j, label in enumerate(("energy", "Rg^2")):
+        print("slab %s: plain %.4f +- %.4f, DA %.4f +- %.4f, |z| = %.2f" % (
+            label, p.mean(0)[j], err(p)[j], d.mean(0)[j], err(d)[j], z[j]), flush=True)
+    assert z.max() < 2.0
