@@ -591,6 +591,10 @@ def gc_leg(steps):
     g = sd["gc"]
     path = systems.write_reference_inputs(sd, tempfile.mkdtemp(prefix="plum_gc_"))
     sim = H.HostSimulation(path, 2024)
+    # the reference's exchange, recording ln(W / w_1) of every insertion
+    # growth for the two-stage run below (one extra first-bead trial energy
+    # per growth; no random draw, the chain is the reference's)
+    sim.gc_prescreen(1)
     per = 2 * g["chain_len"]
     t = time.perf_counter()
     w = [sim.widom() for _ in range(2)]
@@ -639,7 +643,82 @@ def gc_leg(steps):
            "pressure_unit": "kT per unit volume (volume perturbation dz = 1e-5 along z)",
            "growth_step_us": 1e6 * widom_s / growth_steps,
            "widom_mean_weight": float(np.mean(w))}
+    _, _, mean_log_rest, _, n_rest = sim.gc_prescreen_stats()
     sim.close()
+    res["two_stage"] = gc_two_stage(path, steps, mean_log_rest, n_rest, per, sec[1] - sec0[1], ins + dele)
+    res["two_stage_reference_deck"] = gc_two_stage_deck(os.path.join(ROOT, "tests", "golden", "examples",
+                                                                      "bulk_muvt", "run.in"), 4000)
+    return res
+
+
+def gc_two_stage_deck(path, steps, seed=7):
+    """The reference's own grand-canonical deck (8-bead chains of charge -1
+    with counterions, 30 trials, mu = -90, box 100): the reference's exchange
+    recording ln(W / w_1), then the two-stage exchange with ln theta their
+    mean, the same number of steps from the same start; growths run and
+    exchanges accepted by each."""
+    from plum_amd import host as H
+    out = {}
+    log_theta = 0.0
+    for mode in (1, 2):
+        sim = H.HostSimulation(path, seed)
+        sim.gc_prescreen(mode, log_theta)
+        att, acc, sec = sim.run_device(steps)
+        ga, gacc, nb, nm = sim.gc_stats()
+        rej, pas, mean_log, _, n_rest = sim.gc_prescreen_stats()
+        growths = sum(ga) if mode == 1 else sum(pas)
+        out["reference" if mode == 1 else "two_stage"] = {
+            "exchanges": ga, "accepted": gacc, "growths": growths, "first_stage_rejected": rej,
+            "seconds_in_exchanges": sec[1], "beads_at_end": nb}
+        if mode == 1:
+            log_theta = mean_log
+        sim.close()
+    out["log_theta"] = log_theta
+    r, t = out["reference"], out["two_stage"]
+    log("gc reference deck: reference %d growths, accepted %s, %.2f s in exchanges; two-stage (log theta "
+        "%.2f) %d growths, accepted %s, %.2f s" % (r["growths"], r["accepted"], r["seconds_in_exchanges"],
+                                                  log_theta, t["growths"], t["accepted"],
+                                                  t["seconds_in_exchanges"]))
+    return out
+
+
+def gc_two_stage(path, steps, log_theta, n_rest, per, plain_exchange_s, plain_exchanges):
+    """The same run with the two-stage exchange (plum_sim_gc_prescreen mode
+    2): first stage on the first bead's exact weight times theta, ln theta
+    the mean of ln(W / w_1) over the growths the reference's run recorded,
+    the growth and the exact Rosenbluth weight only past it."""
+    from plum_amd import host as H
+    sim = H.HostSimulation(path, 2024)
+    sim.gc_prescreen(2, log_theta)
+    att0, acc0, sec0 = sim.run_device(0)
+    a0, c0 = sim.gc_stats()[0], sim.gc_stats()[1]
+    chains = []
+    nchunk = 50
+    chunk = max(steps // nchunk, 1)
+    done = 0
+    while done < steps:
+        att, acc, sec = sim.run_device(chunk)
+        done += chunk
+        chains.append(sim.gc_stats()[2] / per)
+    ga, gacc, nb, nm = sim.gc_stats()
+    rej, pas, _, _, _ = sim.gc_prescreen_stats()
+    sim.close()
+    ch = np.array(chains)
+    nb10 = len(ch) // 10 * 10
+    blocks = ch[:nb10].reshape(10, -1).mean(1)
+    ex_s = sec[1] - sec0[1]
+    exchanges = (ga[0] - a0[0]) + (ga[1] - a0[1])
+    res = {"log_theta": log_theta, "growths_fitted": n_rest, "steps": done,
+           "insertions": ga[0] - a0[0], "insertions_accepted": gacc[0] - c0[0],
+           "deletions": ga[1] - a0[1], "deletions_accepted": gacc[1] - c0[1],
+           "first_stage_rejected": rej, "first_stage_passed": pas,
+           "seconds_in_exchanges": ex_s, "exchanges_per_s_in_exchanges": exchanges / max(ex_s, 1e-9),
+           "plain_exchanges_per_s_in_exchanges": plain_exchanges / max(plain_exchange_s, 1e-9),
+           "chains_mean": float(ch.mean()), "chains_err": float(blocks.std(ddof=1) / np.sqrt(10))}
+    log("gc two-stage (log theta %.2f): %d exchanges in %.2f s (plain %d in %.2f s), first stage "
+        "rejected %s passed %s, accepted %s, <chains> %.2f +- %.2f" % (
+            log_theta, exchanges, ex_s, plain_exchanges, plain_exchange_s, rej, pas,
+            [gacc[0] - c0[0], gacc[1] - c0[1]], res["chains_mean"], res["chains_err"]))
     return res
 
 

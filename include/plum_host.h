@@ -49,6 +49,28 @@ int plum_sim_run_device(plum_sim* sim, long long steps, long long attempted[5], 
 /* Grand-canonical counters: [0] insertions, [1] deletions; and the current
    number of beads and molecules (they change with accepted exchanges). */
 int plum_sim_gc_stats(plum_sim* sim, long long attempted[2], long long accepted[2], int* n_beads, int* n_mols);
+/* Two-stage grand-canonical exchange (delayed acceptance; no counterpart in
+   the reference, whose exchange is mode 0).  Mode 2: an insertion draws its
+   first chain bead and counterion as the reference's growth does, then
+   accepts a first stage with min(1, r_1), r_1 = e^{beta mu} C w_1 theta
+   (w_1: the first bead's exact Boltzmann factor, theta = e^{log_theta}: a
+   fixed estimate of the rest of the Rosenbluth weight, C: the reference's
+   insertion factor, cbmc.cc:268-300); only then grows the chain (the
+   reference's draws and trial energies) and accepts with min(1, r / r_1),
+   r = e^{beta mu} C W the reference's ratio.  A deletion's first stage is
+   the inverse, C' / (e^{beta mu} w_1 theta) with the chain's own first bead.
+   The chain sampled is the reference's ensemble; a first-stage rejection
+   costs one trial energy instead of a growth.  Mode 1: the reference's
+   exchange, recording ln(W / w_1) of every insertion growth, the statistic
+   log_theta is fitted to.  Each stage draws its own uniform, so the stream
+   is not the reference's in mode 2. */
+int plum_sim_gc_prescreen(plum_sim* sim, int mode, double log_theta);
+/* Counters of the two stages ([0] insertions, [1] deletions), the mean of
+   ln(W / w_1) and the ln of the mean of W / w_1 over the recorded insertion
+   growths (modes 1 and 2).  The acceptance is proportional to W, so the
+   second, which the largest weights dominate, is the scale for theta. */
+int plum_sim_gc_prescreen_stats(plum_sim* sim, long long s1_rejected[2], long long s1_passed[2],
+                                double* mean_log_rest, double* log_mean_rest, long long* n_rest);
 /* ForceField::CalcChemicalPotentialF (cbmc.cc:444-533): the mean weight of
    50 Widom test insertions of the exchanged chain species (or, without
    grand-canonical moves, of the first chain species, 30 trials per bead). */

@@ -146,6 +146,10 @@ def lib():
                                       C.POINTER(OrcGC), dp]
         L.orc_gc_widom.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
                                    C.POINTER(OrcGC), C.c_int]
+        L.orc_gc_insertion2.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
+                                        C.POINTER(OrcGC), C.c_double, dp, dp, C.POINTER(C.c_int)]
+        L.orc_gc_deletion2.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss),
+                                       C.POINTER(OrcGC), C.c_double, dp, C.POINTER(C.c_int)]
         L.orc_gc_widom.restype = C.c_double
         L.orc_gasdev.argtypes = [C.c_double, C.c_double, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
         L.orc_gasdev.restype = C.c_double
@@ -392,8 +396,13 @@ class GCSim:
     accepted exchange.  The exchanged species follows the reference: chain
     beads and counterions both of the exchanged bead type."""
 
-    def __init__(self, sd, seed, rigid_bond=None):
+    def __init__(self, sd, seed, rigid_bond=None, log_theta=None):
+        """log_theta: the two-stage exchange (orc_gc_insertion2 /
+        orc_gc_deletion2) with theta = e^log_theta; None: the reference's."""
         self.sd = dict(sd)
+        self.log_theta = log_theta
+        self.s1_rejected = [0, 0]
+        self.s1_passed = [0, 0]
         g = sd["gc"]
         self.freq = g["freq"]
         self.gc = OrcGC(chem_pot=g["chem_pot"], debroglie=g["debroglie"],
@@ -421,8 +430,19 @@ class GCSim:
             self.gc_attempted[0] += 1
             chain = np.zeros(6 * L)
             w = C.c_double()
-            if lib().orc_gc_insertion(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
-                                      C.byref(self.gc), _dp(chain), C.byref(w)):
+            if self.log_theta is None:
+                ok = lib().orc_gc_insertion(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                            C.byref(self.gc), _dp(chain), C.byref(w))
+            else:
+                stage = C.c_int()
+                ok = lib().orc_gc_insertion2(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                             C.byref(self.gc), self.log_theta, _dp(chain), C.byref(w),
+                                             C.byref(stage))
+                if stage.value:
+                    self.s1_passed[0] += 1
+                else:
+                    self.s1_rejected[0] += 1
+            if ok:
                 self.gc_accepted[0] += 1
                 chain = chain.reshape(-1, 3)
                 ch = self.gc.bead_charge != 0
@@ -443,8 +463,17 @@ class GCSim:
             if n_mol - self.gc.phantom - self.gc.coion <= 0:
                 return
             w = C.c_double()
-            d = lib().orc_gc_deletion(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
-                                      C.byref(self.gc), C.byref(w))
+            if self.log_theta is None:
+                d = lib().orc_gc_deletion(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                          C.byref(self.gc), C.byref(w))
+            else:
+                stage = C.c_int()
+                d = lib().orc_gc_deletion2(self.orc.h, C.byref(self.rng.s), C.byref(self.gauss),
+                                           C.byref(self.gc), self.log_theta, C.byref(w), C.byref(stage))
+                if stage.value == 1:
+                    self.s1_passed[1] += 1
+                elif stage.value == 0:
+                    self.s1_rejected[1] += 1
             if d >= self.gc.phantom + self.gc.coion:
                 self.gc_accepted[1] += 1
                 ms = self.sd["mol_start"]

@@ -1219,13 +1219,15 @@ static double gen_trials(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc
    chosen among the trials by their weights.  Returns the Rosenbluth weight;
    widom selects the chemical-potential variant (no early return after the
    first bead, cbmc.cc:489-500). */
-static double grow_chain(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain, int widom) {
+static double grow_chain(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain, int widom,
+                         int first_given) {
   const int K = gc->trials, L = gc->chain_len, ch = gc->bead_charge != 0;
   double* tri = (double*)malloc(sizeof(double) * 6 * K);
   double* wts = (double*)malloc(sizeof(double) * K);
   double weight = 1.0;
-  for (int i = 0; i < (ch ? 2 : 1); i++)
-    for (int j = 0; j < 3; j++) chain[3 * (i * L) + j] = u01(r) * s->p.box[j];
+  if (!first_given)
+    for (int i = 0; i < (ch ? 2 : 1); i++)
+      for (int j = 0; j < 3; j++) chain[3 * (i * L) + j] = u01(r) * s->p.box[j];
   const double e0 = orc_cbmc_beads_energy(s, chain, gc->type_c, gc->bead_charge, ch ? &chain[3 * L] : chain,
                                           gc->type_c, -gc->bead_charge, ch, chain, chain + 3 * L, 0, 0, -1);
   if (widom) {
@@ -1256,13 +1258,9 @@ done:
   return weight;
 }
 
-int orc_gc_insertion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain_out,
-                     double* weight_out) {
+/* The insertion's prefactor C of cbmc.cc:268-300. */
+static double insertion_factor(orc_sys* s, const orc_gc* gc) {
   const int L = gc->chain_len;
-  memset(chain_out, 0, sizeof(double) * 6 * L);
-  const double weight = grow_chain(s, r, gs, gc, chain_out, 0);
-  if (weight_out) *weight_out = weight;
-  if (weight <= 0) return 0;
   int n_chain, n_cion, n_aion;
   gc_counts(s, gc, &n_chain, &n_cion, &n_aion);
   const int spc1 = gc_spc1(gc, n_chain, n_cion, n_aion), spc2 = gc_spc2(gc, n_cion, n_aion);
@@ -1274,13 +1272,80 @@ int orc_gc_insertion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc
   double vol_over_lam = 1;
   vol_over_lam *= pow(vol / (gc->debroglie / pow((double)L, 1.5)), spc1_add);
   vol_over_lam *= pow(vol / (gc->debroglie / pow(1.0, 1.5)), spc2_add);
-  const double C = vol_over_lam * (1.0 / factorial);
+  return vol_over_lam * (1.0 / factorial);
+}
+
+/* The deletion's prefactor C of cbmc.cc:403-441 (counts of the current
+   system, before the deletion). */
+static double deletion_factor(orc_sys* s, const orc_gc* gc) {
+  const int L = gc->chain_len;
+  int n_chain, n_cion, n_aion;
+  gc_counts(s, gc, &n_chain, &n_cion, &n_aion);
+  const int spc1 = gc_spc1(gc, n_chain, n_cion, n_aion), spc2 = gc_spc2(gc, n_cion, n_aion);
+  const int spc1_del = 1, spc2_del = gc->bead_charge == 0 ? 0 : L;
+  double factorial = 1;
+  for (int i = spc1; i > spc1 - spc1_del; i--) factorial *= i;
+  for (int i = spc2; i > spc2 - spc2_del; i--) factorial *= i;
+  const double vol = s->p.box[0] * s->p.box[1] * s->p.box[2];
+  double lam_over_vol = 1;
+  lam_over_vol *= pow((gc->debroglie / pow((double)L, 1.5)) / vol, spc1_del);
+  lam_over_vol *= pow((gc->debroglie / pow(1.0, 1.5)) / vol, spc2_del);
+  return lam_over_vol * factorial;
+}
+
+int orc_gc_insertion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* chain_out,
+                     double* weight_out) {
+  const int L = gc->chain_len;
+  memset(chain_out, 0, sizeof(double) * 6 * L);
+  const double weight = grow_chain(s, r, gs, gc, chain_out, 0, 0);
+  if (weight_out) *weight_out = weight;
+  if (weight <= 0) return 0;
+  const double C = insertion_factor(s, gc);
   const double rn = u01(r);
   return rn < (exp(s->p.beta * gc->chem_pot) * weight) * C;
 }
 
-int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* weight_out) {
-  const int K = gc->trials, L = gc->chain_len, ch = gc->bead_charge != 0;
+/* Boltzmann factor of a first chain bead b1 and counterion b2 against the
+   molecules outside [ex_lo, ex_hi] (BeadsEnergy with nothing grown). */
+static double first_bead_weight(orc_sys* s, const orc_gc* gc, const double* b1, const double* b2, int ex_lo,
+                                int ex_hi) {
+  const int ch = gc->bead_charge != 0;
+  static const double none[3] = {0, 0, 0};
+  const double e = orc_cbmc_beads_energy(s, b1, gc->type_c, gc->bead_charge, ch ? b2 : b1, gc->type_c,
+                                         -gc->bead_charge, ch, none, none, 0, ex_lo, ex_hi);
+  return exp(-s->p.beta * e);
+}
+
+/* Two-stage insertion (no reference counterpart; the engine's
+   plum_sim_gc_prescreen mode 2): first bead and counterion drawn as the
+   growth draws them; a first stage min(1, r_1), r_1 = e^{beta mu} C w_1
+   theta; on passing the rest of the growth and a second stage
+   min(1, W / (w_1 theta)).  *stage: 0 rejected at the first stage, 1 past
+   it.  Returns 1 when accepted. */
+int orc_gc_insertion2(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double log_theta,
+                      double* chain_out, double* weight_out, int* stage) {
+  const int L = gc->chain_len, ch = gc->bead_charge != 0;
+  memset(chain_out, 0, sizeof(double) * 6 * L);
+  *stage = 0;
+  if (weight_out) *weight_out = 0;
+  for (int i = 0; i < (ch ? 2 : 1); i++)
+    for (int j = 0; j < 3; j++) chain_out[3 * (i * L) + j] = u01(r) * s->p.box[j];
+  const double w1 = first_bead_weight(s, gc, chain_out, chain_out + 3 * L, 0, -1);
+  if (w1 <= 0) return 0;
+  const double C = insertion_factor(s, gc);
+  const double r1 = exp(s->p.beta * gc->chem_pot) * C * w1 * exp(log_theta);
+  if (!(u01(r) < r1)) return 0;
+  *stage = 1;
+  const double weight = grow_chain(s, r, gs, gc, chain_out, 0, 1);
+  if (weight_out) *weight_out = weight;
+  if (weight <= 0) return 0;
+  return u01(r) < weight / (w1 * exp(log_theta));
+}
+
+/* The chain to delete, chosen by the reference's draw (cbmc.cc:361-372), or
+   -1 (nothing to delete, or not the exchange layout). */
+static int deletion_pick(orc_sys* s, orc_mt19937* r, const orc_gc* gc) {
+  const int L = gc->chain_len, ch = gc->bead_charge != 0;
   int n_chain, n_cion, n_aion;
   gc_counts(s, gc, &n_chain, &n_cion, &n_aion);
   const int spc1 = gc_spc1(gc, n_chain, n_cion, n_aion);
@@ -1291,6 +1356,14 @@ int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc,
   del = gc->phantom + gc->coion + gc->grafted + gc->grafted_counterion + del * (ch ? 1 + L : 1);
   const int counterion = ch ? L : 0;
   if (del + counterion >= s->n_mols || s->mol_start[del + 1] - s->mol_start[del] != L) return -1;
+  return del;
+}
+
+/* The Rosenbluth weight of chain `del` regrown along itself (cbmc.cc:373-400):
+   trial 0 of each step is the existing bead. */
+static double deletion_weight(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, int del) {
+  const int K = gc->trials, L = gc->chain_len, ch = gc->bead_charge != 0;
+  const int counterion = ch ? L : 0;
   const int second = ch ? del + 1 : del;
   double* chain = (double*)calloc(6 * (size_t)L, sizeof(double));
   double* tri = (double*)malloc(sizeof(double) * 6 * K);
@@ -1318,19 +1391,42 @@ int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc,
   free(chain);
   free(tri);
   free(wts);
+  return weight;
+}
+
+int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* weight_out) {
+  const int del = deletion_pick(s, r, gc);
+  if (del < 0) return -1;
+  const double C = deletion_factor(s, gc);
+  const double weight = deletion_weight(s, r, gs, gc, del);
   if (weight_out) *weight_out = weight;
-  const int spc2 = gc_spc2(gc, n_cion, n_aion);
-  const int spc1_del = 1, spc2_del = gc->bead_charge == 0 ? 0 : L;
-  double factorial = 1;
-  for (int i = spc1; i > spc1 - spc1_del; i--) factorial *= i;
-  for (int i = spc2; i > spc2 - spc2_del; i--) factorial *= i;
-  const double vol = s->p.box[0] * s->p.box[1] * s->p.box[2];
-  double lam_over_vol = 1;
-  lam_over_vol *= pow((gc->debroglie / pow((double)L, 1.5)) / vol, spc1_del);
-  lam_over_vol *= pow((gc->debroglie / pow(1.0, 1.5)) / vol, spc2_del);
-  const double C = lam_over_vol * factorial;
   const double rn = u01(r);
   return rn < C / (exp(s->p.beta * gc->chem_pot) * weight) ? del : -1;
+}
+
+/* Two-stage deletion (the inverse of orc_gc_insertion2): the chain chosen as
+   the reference chooses it, a first stage min(1, C / (e^{beta mu} w_1
+   theta)) on its own first bead and counterion against the rest of the
+   system, then the regrowth and a second stage min(1, w_1 theta / W).
+   *stage: -1 no chain to delete, 0 rejected at the first stage, 1 past it. */
+int orc_gc_deletion2(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double log_theta,
+                     double* weight_out, int* stage) {
+  const int L = gc->chain_len, ch = gc->bead_charge != 0;
+  *stage = -1;   /* no chain to screen */
+  if (weight_out) *weight_out = 0;
+  const int del = deletion_pick(s, r, gc);
+  if (del < 0) return -1;
+  *stage = 0;
+  const int second = ch ? del + 1 : del;
+  const double w1 = first_bead_weight(s, gc, &s->cur[3 * s->mol_start[del]], &s->cur[3 * s->mol_start[second]],
+                                      del, del + (ch ? L : 0));
+  const double C = deletion_factor(s, gc);
+  const double r1 = C / (exp(s->p.beta * gc->chem_pot) * w1 * exp(log_theta));
+  if (!(u01(r) < r1)) return -1;
+  *stage = 1;
+  const double weight = deletion_weight(s, r, gs, gc, del);
+  if (weight_out) *weight_out = weight;
+  return u01(r) < w1 * exp(log_theta) / weight ? del : -1;
 }
 
 double orc_gc_widom(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, int n_ins) {
@@ -1349,7 +1445,7 @@ double orc_gc_widom(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc,
   const double C = vol_over_lam * (1.0 / factorial);
   double* chain = (double*)calloc(6 * (size_t)L, sizeof(double));
   double total = 0;
-  for (int c = 0; c < n_ins; c++) total += grow_chain(s, r, gs, gc, chain, 1) * C;
+  for (int c = 0; c < n_ins; c++) total += grow_chain(s, r, gs, gc, chain, 1, 0) * C;
   free(chain);
   return total / (double)n_ins;
 }

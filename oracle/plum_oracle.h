@@ -185,6 +185,10 @@ int orc_gc_insertion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc
                      double* weight);
 /* ForceField::CBMCFChainDeletion (cbmc.cc:337-441).  Returns the molecule
    index of the deleted chain (its counterions follow it) or -1. */
+int orc_gc_insertion2(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double log_theta,
+                      double* chain_out, double* weight_out, int* stage);
+int orc_gc_deletion2(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double log_theta,
+                     double* weight_out, int* stage);
 int orc_gc_deletion(orc_sys* s, orc_mt19937* r, orc_gauss* gs, const orc_gc* gc, double* weight);
 /* ForceField::CalcChemicalPotentialF (cbmc.cc:444-533) with n_ins test
    insertions (mu_tot_ins, 50 in the reference). */

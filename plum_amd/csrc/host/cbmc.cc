@@ -4,6 +4,7 @@
 // order of draws.  A whole growth runs on the GPU in one call
 // (plum_cbmc_grow); CBMCFGenTrialBeads, the one-step surface, evaluates its
 // K trials in one call (plum_cbmc_trials).
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -126,14 +127,14 @@ double ForceField::CBMCFGenTrialBeads(const Bead& end_bead, std::vector<Molecule
 // K trials (bond length, direction, counterion; CBMCFGenTrialBeads,
 // cbmc.cc:161-212) and, for an insertion, the uniform of the choice among
 // them (cbmc.cc:241) unless s == last and !sel_last.
-void ForceField::draw_growth(std::mt19937& g, int mode, int last, bool sel_last) {
+void ForceField::draw_growth(std::mt19937& g, int mode, int last, bool sel_last, bool draw_first) {
   const int K = gc_trials_, L = gc_len_;
   const bool ch = q_gc_ != 0;
-  first_.assign(6, 0.0);
+  if (draw_first) first_.assign(6, 0.0);
   disp_.assign(3 * (size_t)L * K, 0.0);
   ion_.assign(3 * (size_t)L * K, 0.0);
   sel_.assign(L, 0.0);
-  if (mode != 1)
+  if (mode != 1 && draw_first)
     for (int i = 0; i < (ch ? 2 : 1); i++)
       for (int j = 0; j < 3; j++) first_[3 * i + j] = u01(g) * box_[j];
   for (int s = 1; s <= last; s++) {
@@ -156,14 +157,15 @@ void ForceField::draw_growth(std::mt19937& g, int mode, int last, bool sel_last)
 // weight.  The draws are made up front; when the weight reaches 0 at step s
 // the reference stops drawing there, so the stream is rewound and redrawn up
 // to that point.
-double ForceField::grow_device(std::vector<Molecule>& mols, std::mt19937& g, int mode, int delete_id) {
+double ForceField::grow_device(std::vector<Molecule>& mols, std::mt19937& g, int mode, int delete_id,
+                               bool draw_first) {
   sync(mols);
   const int L = gc_len_, K = gc_trials_;
   const bool ch = q_gc_ != 0;
   const std::mt19937 g0 = g;
   const int iset0 = gauss_iset_;
   const double gset0 = gauss_gset_;
-  draw_growth(g, mode, L - 1, true);
+  draw_growth(g, mode, L - 1, true, draw_first);
   plum_growth_request rq{};
   rq.mode = mode;
   rq.chain_len = L;
@@ -195,7 +197,7 @@ double ForceField::grow_device(std::vector<Molecule>& mols, std::mt19937& g, int
     g = g0;
     gauss_iset_ = iset0;
     gauss_gset_ = gset0;
-    draw_growth(g, mode, stop, false);
+    draw_growth(g, mode, stop, false, draw_first);
   }
   return W;
 }
@@ -212,14 +214,10 @@ void ForceField::gc_counts(std::vector<Molecule>& mols, int& spc1, int& spc2) {
   else spc2 = n_cion_;
 }
 
-bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& g) {
-  if (!r_.use_gc) throw std::runtime_error("CBMCFChainInsertion without grand-canonical moves");
-  sync(mols);
+// cbmc.cc:268-300: the insertion's factor (V / Lambda)^n / (N + 1)! ...
+double ForceField::insertion_factor(std::vector<Molecule>& mols) {
   const int L = gc_len_;
   const bool ch = q_gc_ != 0;
-  const double weight = grow_device(mols, g, 0, -1);
-  if (weight <= 0) return false;
-  // cbmc.cc:268-300
   int spc1, spc2;
   gc_counts(mols, spc1, spc2);
   const int spc1_add = 1, spc2_add = ch ? L : 0;
@@ -230,10 +228,31 @@ bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& 
   double vol_over_lam = 1;
   vol_over_lam *= std::pow(vol / (r_.debroglie / std::pow((double)L, 1.5)), spc1_add);
   vol_over_lam *= std::pow(vol / (r_.debroglie / std::pow(1.0, 1.5)), spc2_add);
-  const double C = vol_over_lam * (1.0 / factorial);
-  const double rn = u01(g);
-  if (!(rn < (std::exp(r_.beta * r_.mu) * weight) * C)) return false;
-  // cbmc.cc:309-330: the chain (linear bonds) then one molecule per counterion
+  return vol_over_lam * (1.0 / factorial);
+}
+
+// cbmc.cc:403-441: the deletion's factor (Lambda / V)^n N! ...
+double ForceField::deletion_factor(std::vector<Molecule>& mols) {
+  const int L = gc_len_;
+  const bool ch = q_gc_ != 0;
+  int spc1, spc2;
+  gc_counts(mols, spc1, spc2);
+  const int spc1_del = 1, spc2_del = ch ? L : 0;
+  double factorial = 1;
+  for (int i = spc1; i > spc1 - spc1_del; i--) factorial *= i;
+  for (int i = spc2; i > spc2 - spc2_del; i--) factorial *= i;
+  const double vol = box_[0] * box_[1] * box_[2];
+  double lam_over_vol = 1;
+  lam_over_vol *= std::pow((r_.debroglie / std::pow((double)L, 1.5)) / vol, spc1_del);
+  lam_over_vol *= std::pow((r_.debroglie / std::pow(1.0, 1.5)) / vol, spc2_del);
+  return lam_over_vol * factorial;
+}
+
+// cbmc.cc:309-330: the grown chain (linear bonds) then one molecule per
+// counterion, appended to mols.
+void ForceField::append_chain(std::vector<Molecule>& mols) {
+  const int L = gc_len_;
+  const bool ch = q_gc_ != 0;
   Molecule m;
   for (int i = 0; i < L; i++) {
     Bead b(r_.gc_bead_symbol, -1, -1, q_gc_, chain_[3 * i], chain_[3 * i + 1], chain_[3 * i + 2]);
@@ -250,6 +269,80 @@ bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& 
       c.AddBead(b);
       mols.push_back(c);
     }
+}
+
+// Boltzmann factor of a first chain bead and its counterion (first[0..5])
+// against the system less molecules [delete_id, delete_id + counterions]
+// (BeadsEnergy with nothing grown yet, one plum_cbmc_trials trial).
+double ForceField::first_bead_weight(const double first[6], int delete_id) {
+  double e = 0;
+  trial_energies(1, 0, delete_id, first, &e);
+  return std::exp(-r_.beta * e);
+}
+
+// One growth's ln(W / w_1): its sum and the running ln of the sum of W / w_1.
+void ForceField::GCPrescreenStats::record(double log_rest) {
+  sum_log_rest += log_rest;
+  if (n_rest == 0) lse_rest = log_rest;
+  else {
+    const double hi = std::max(lse_rest, log_rest), lo = std::min(lse_rest, log_rest);
+    lse_rest = hi + std::log1p(std::exp(lo - hi));
+  }
+  n_rest++;
+}
+
+void ForceField::SetGCPrescreen(int mode, double log_theta) {
+  if (mode < 0 || mode > 2) throw std::runtime_error("SetGCPrescreen: mode must be 0, 1 or 2");
+  gc_pre_mode_ = mode;
+  gc_log_theta_ = log_theta;
+}
+
+bool ForceField::CBMCFChainInsertion(std::vector<Molecule>& mols, std::mt19937& g) {
+  if (!r_.use_gc) throw std::runtime_error("CBMCFChainInsertion without grand-canonical moves");
+  if (gc_pre_mode_ == 2) return insertion_two_stage(mols, g);
+  sync(mols);
+  const double weight = grow_device(mols, g, 0, -1);
+  if (weight <= 0) return false;
+  if (gc_pre_mode_ == 1) {
+    // the statistic of the two-stage first stage (no draw: the chain is the
+    // reference's)
+    const double w1 = first_bead_weight(first_.data(), -1);
+    pre_.record(std::log(weight / w1));
+  }
+  const double C = insertion_factor(mols);
+  const double rn = u01(g);
+  if (!(rn < (std::exp(r_.beta * r_.mu) * weight) * C)) return false;
+  append_chain(mols);
+  return true;
+}
+
+// Two-stage insertion: the first bead and counterion drawn as the growth
+// draws them, then the first-stage uniform, then (on passing) the rest of the
+// growth's draws and the second-stage uniform.
+bool ForceField::insertion_two_stage(std::vector<Molecule>& mols, std::mt19937& g) {
+  sync(mols);
+  const bool ch = q_gc_ != 0;
+  first_.assign(6, 0.0);
+  for (int i = 0; i < (ch ? 2 : 1); i++)
+    for (int j = 0; j < 3; j++) first_[3 * i + j] = u01(g) * box_[j];
+  const double w1 = first_bead_weight(first_.data(), -1);
+  if (w1 <= 0) {   // the reference's growth ends here as well
+    pre_.s1_rejected[0]++;
+    return false;
+  }
+  const double C = insertion_factor(mols);
+  const double r1 = std::exp(r_.beta * r_.mu) * C * w1 * std::exp(gc_log_theta_);
+  if (!(u01(g) < r1)) {
+    pre_.s1_rejected[0]++;
+    return false;
+  }
+  pre_.s1_passed[0]++;
+  const double weight = grow_device(mols, g, 0, -1, false);
+  if (weight <= 0) return false;
+  pre_.record(std::log(weight / w1));
+  // min(1, r / r_1) with r = e^{beta mu} C W
+  if (!(u01(g) < weight / (w1 * std::exp(gc_log_theta_)))) return false;
+  append_chain(mols);
   return true;
 }
 
@@ -267,19 +360,34 @@ int ForceField::CBMCFChainDeletion(std::vector<Molecule>& mols, std::mt19937& g)
   const int counterion = ch ? L : 0;
   if (del + counterion >= (int)mols.size() || mols[del].Size() != L)
     throw std::runtime_error("CBMCFChainDeletion: molecules are not in the exchange layout (chain, then its counterions)");
+  if (gc_pre_mode_ == 2) return deletion_two_stage(mols, g, del);
   const double weight = grow_device(mols, g, 1, del);
-  // cbmc.cc:403-441
-  const int spc1_del = 1, spc2_del = ch ? L : 0;
-  double factorial = 1;
-  for (int i = spc1; i > spc1 - spc1_del; i--) factorial *= i;
-  for (int i = spc2; i > spc2 - spc2_del; i--) factorial *= i;
-  const double vol = box_[0] * box_[1] * box_[2];
-  double lam_over_vol = 1;
-  lam_over_vol *= std::pow((r_.debroglie / std::pow((double)L, 1.5)) / vol, spc1_del);
-  lam_over_vol *= std::pow((r_.debroglie / std::pow(1.0, 1.5)) / vol, spc2_del);
-  const double C = lam_over_vol * factorial;
+  const double C = deletion_factor(mols);
   const double rn = u01(g);
   return rn < C / (std::exp(r_.beta * r_.mu) * weight) ? del : -1;
+}
+
+// Two-stage deletion of chain `del` (chosen by the reference's draw): the
+// first-stage uniform on the inverse of the insertion's first stage, the
+// regrowth along the chain on passing, then the second-stage uniform.
+int ForceField::deletion_two_stage(std::vector<Molecule>& mols, std::mt19937& g, int del) {
+  const bool ch = q_gc_ != 0;
+  double f[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < 3; j++) {
+    f[j] = mols[del].bds[0].GetCrd(0, j);
+    if (ch) f[3 + j] = mols[del + 1].bds[0].GetCrd(0, j);
+  }
+  const double w1 = first_bead_weight(f, del);
+  const double C = deletion_factor(mols);
+  const double r1 = C / (std::exp(r_.beta * r_.mu) * w1 * std::exp(gc_log_theta_));
+  if (!(u01(g) < r1)) {
+    pre_.s1_rejected[1]++;
+    return -1;
+  }
+  pre_.s1_passed[1]++;
+  const double weight = grow_device(mols, g, 1, del);
+  // min(1, r / r_1) with r = C / (e^{beta mu} W)
+  return u01(g) < w1 * std::exp(gc_log_theta_) / weight ? del : -1;
 }
 
 double ForceField::CalcChemicalPotentialF(std::vector<Molecule>& mols, std::mt19937& g) {

@@ -62,6 +62,23 @@ class ForceField {
   // cbmc.cc:444: Widom test insertions of the exchanged (or, without GC
   // moves, the first) chain species.
   double CalcChemicalPotentialF(std::vector<Molecule>& mols, std::mt19937& rand_gen);
+  // Two-stage exchange (no counterpart in the reference; mode 2): the first
+  // stage accepts on the first bead's exact Boltzmann factor w_1 times a fixed
+  // estimate theta = e^{log_theta} of the rest of the growth's Rosenbluth
+  // weight, r_1 = e^{beta mu} C w_1 theta (deletion: its inverse), so a
+  // rejected exchange costs one trial energy instead of a growth; the second
+  // stage takes the exact weight, min(1, r / r_1).  Mode 1 keeps the
+  // reference's exchange and only records ln(W / w_1) of every insertion
+  // growth (the statistic theta is fitted to); mode 0 is the reference's.
+  void SetGCPrescreen(int mode, double log_theta);
+  struct GCPrescreenStats {
+    long long s1_rejected[2] = {0, 0}, s1_passed[2] = {0, 0};
+    double sum_log_rest = 0;   // sum of ln(W / w_1) over recorded insertion growths
+    double lse_rest = 0;       // ln sum of W / w_1 over them (running log-sum-exp)
+    long long n_rest = 0;
+    void record(double log_rest);
+  };
+  const GCPrescreenStats& PrescreenStats() const { return pre_; }
   // force_field.cc:1088: rebuilds the energy state with the appended molecules.
   void EnergyInitForAddedMolecule(std::vector<Molecule>& mols);
   const std::vector<double>& TrialWeights() const { return wts_; }
@@ -105,8 +122,15 @@ class ForceField {
   double random_bond_len(std::mt19937& g);
   double gasdev(double mean, double stdev, std::mt19937& g);
   void trial_energies(int K, int cur, int delete_id, const double* tri, double* e);
-  void draw_growth(std::mt19937& g, int mode, int last, bool sel_last);
-  double grow_device(std::vector<Molecule>& mols, std::mt19937& g, int mode, int delete_id);
+  void draw_growth(std::mt19937& g, int mode, int last, bool sel_last, bool draw_first = true);
+  double grow_device(std::vector<Molecule>& mols, std::mt19937& g, int mode, int delete_id,
+                     bool draw_first = true);
+  double insertion_factor(std::vector<Molecule>& mols);   // C of cbmc.cc:268-300
+  double deletion_factor(std::vector<Molecule>& mols);    // C of cbmc.cc:403-441
+  double first_bead_weight(const double first[6], int delete_id);
+  bool insertion_two_stage(std::vector<Molecule>& mols, std::mt19937& g);
+  int deletion_two_stage(std::vector<Molecule>& mols, std::mt19937& g, int del);
+  void append_chain(std::vector<Molecule>& mols);
   void gc_counts(std::vector<Molecule>& mols, int& spc1, int& spc2);
 
   RunIn r_;
@@ -127,6 +151,9 @@ class ForceField {
   double gauss_gset_ = 0;
   std::vector<double> chain_, tri_, wts_, en_;
   std::vector<double> first_, disp_, ion_, sel_, act_;  // draws of one growth
+  int gc_pre_mode_ = 0;      // SetGCPrescreen
+  double gc_log_theta_ = 0;
+  GCPrescreenStats pre_;
   // pressure accumulators (pressure.cc:187-387): samples, sum of N / V, sum
   // of dU, sum of (1 + dz/Lz)^N e^{-beta dU}, and the last A dz
   long long vp_z_ = 0;

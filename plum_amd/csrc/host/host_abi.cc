@@ -1,5 +1,6 @@
 // extern "C" surface of the host layer (include/plum_host.h).
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -142,6 +143,28 @@ int plum_sim_gc_stats(plum_sim* s, long long attempted[2], long long accepted[2]
   for (auto& m : s->s->mols) n += m.Size();
   if (n_beads) *n_beads = n;
   if (n_mols) *n_mols = (int)s->s->mols.size();
+  return PLUM_OK;
+}
+
+int plum_sim_gc_prescreen(plum_sim* s, int mode, double log_theta) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  if (mode < 0 || mode > 2) return fail(PLUM_E_ARG, "mode must be 0, 1 or 2");
+  if (!std::isfinite(log_theta)) return fail(PLUM_E_ARG, "non-finite log_theta");
+  s->s->ff.SetGCPrescreen(mode, log_theta);
+  return PLUM_OK;
+}
+
+int plum_sim_gc_prescreen_stats(plum_sim* s, long long s1_rejected[2], long long s1_passed[2],
+                                double* mean_log_rest, double* log_mean_rest, long long* n_rest) {
+  if (!s || !s->s) return fail(PLUM_E_ARG, "null simulation");
+  const auto& p = s->s->ff.PrescreenStats();
+  for (int k = 0; k < 2; k++) {
+    if (s1_rejected) s1_rejected[k] = p.s1_rejected[k];
+    if (s1_passed) s1_passed[k] = p.s1_passed[k];
+  }
+  if (mean_log_rest) *mean_log_rest = p.n_rest ? p.sum_log_rest / (double)p.n_rest : 0.0;
+  if (log_mean_rest) *log_mean_rest = p.n_rest ? p.lse_rest - std::log((double)p.n_rest) : 0.0;
+  if (n_rest) *n_rest = p.n_rest;
   return PLUM_OK;
 }
 

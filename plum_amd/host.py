@@ -14,7 +14,7 @@ HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim
                 "plum_host_last_error", "plum_synth_count", "plum_synth_system",
                 "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device",
                 "plum_sim_density_profile", "plum_sim_write_last", "plum_sim_pressure_sample",
-                "plum_sim_run_output")
+                "plum_sim_run_output", "plum_sim_gc_prescreen", "plum_sim_gc_prescreen_stats")
 
 
 class RunInSummary(C.Structure):
@@ -65,6 +65,10 @@ def lib():
         L.plum_sim_gc_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.plum_sim_widom.argtypes = [vp, C.POINTER(C.c_double)]
+        L.plum_sim_gc_prescreen.argtypes = [vp, C.c_int, C.c_double]
+        L.plum_sim_gc_prescreen_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_longlong)]
         L.plum_sim_density_profile.argtypes = [vp, C.c_double, C.POINTER(C.c_double), C.c_int,
                                                C.POINTER(C.c_int)]
         L.plum_sim_write_last.argtypes = [vp, C.c_char_p]
@@ -159,6 +163,23 @@ class HostSimulation:
         nb, nm = C.c_int(), C.c_int()
         _check(lib().plum_sim_gc_stats(self.h, att, acc, C.byref(nb), C.byref(nm)))
         return list(att), list(acc), nb.value, nm.value
+
+    def gc_prescreen(self, mode, log_theta=0.0):
+        """Two-stage exchange (plum_sim_gc_prescreen): 0 the reference's, 1 the
+        reference's recording ln(W / w_1), 2 two-stage with theta = e^log_theta."""
+        _check(lib().plum_sim_gc_prescreen(self.h, int(mode), float(log_theta)))
+
+    def gc_prescreen_stats(self):
+        """(first-stage rejections [ins, del], passes [ins, del], mean ln(W / w_1),
+        ln mean W / w_1, growths recorded)."""
+        rej = (C.c_longlong * 2)()
+        pas = (C.c_longlong * 2)()
+        mean = C.c_double()
+        lmean = C.c_double()
+        n = C.c_longlong()
+        _check(lib().plum_sim_gc_prescreen_stats(self.h, rej, pas, C.byref(mean), C.byref(lmean),
+                                                 C.byref(n)))
+        return list(rej), list(pas), mean.value, lmean.value, n.value
 
     def widom(self):
         w = C.c_double()

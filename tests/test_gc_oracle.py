@@ -96,3 +96,39 @@ def test_gasdev_pairs_and_moments():
     state = bytes(r1.s)
     O.lib().orc_gasdev(0.0, 1.0, O.C.byref(r1.s), O.C.byref(g1))
     assert bytes(r1.s) == state
+
+
+@pytest.mark.parametrize("log_theta", [-1.0, 1.5])
+def test_two_stage_exchange_of_ideal_chains_is_poisson(log_theta):
+    """The two-stage exchange (orc_gc_insertion2 / orc_gc_deletion2: a first
+    stage on the first bead's weight times theta, the second on the whole
+    Rosenbluth weight) with every interaction off: whatever theta, the ideal
+    chains' count stays Poisson with the same mean as the reference's
+    exchange (a first stage that is not the exact inverse between insertion
+    and deletion, or a second stage that does not divide it out, shifts it),
+    and theta moves the work between the stages: exchanges stopped at the
+    first stage appear exactly when theta makes r_1 < 1."""
+    L, V, lam = 4, 1000.0, 4.0
+    mu = math.log(lam / (V * L ** 1.5))
+    sd = _ideal(L, False, mu)
+    g = O.GCSim(sd, 19, log_theta=log_theta)
+    ns = []
+    steps = 40000
+    for _ in range(steps):
+        g.run(1)
+        ns.append(len(g.sd["mol_start"]) - 1)
+    rej, pas, acc = g.s1_rejected, g.s1_passed, g.gc_accepted
+    g.close()
+    ns = np.array(ns[steps // 10:], dtype=float)
+    blocks = ns[: len(ns) // 20 * 20].reshape(20, -1).mean(1)
+    mean, err = ns.mean(), blocks.std(ddof=1) / math.sqrt(20)
+    print("two-stage, log theta %.1f: <N> = %.3f +- %.3f (exact %.3f), first stage rejected %s "
+          "passed %s, accepted %s" % (log_theta, mean, err, lam, rej, pas, acc))
+    assert min(acc) > 1000
+    assert abs(mean - lam) < 4 * err + 0.02
+    # theta < 1 stops insertions at the first stage (r_1 < 1 when N is near
+    # its mean) and lets deletions through; theta > 1 the other way round
+    if log_theta < 0:
+        assert rej[0] > 0 and pas[0] > 0
+    else:
+        assert rej[1] > 0 and pas[1] > 0

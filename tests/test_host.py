@@ -224,6 +224,44 @@ def test_reference_grand_canonical_example_replays_on_the_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode,log_theta", [(1, 0.0), (2, -16.0), (2, -6.0)])
+def test_two_stage_exchange_replays_on_the_oracle(mode, log_theta):
+    """The reference's grand-canonical deck with the two-stage exchange
+    (plum_sim_gc_prescreen mode 2: first stage on the first bead's exact
+    weight times theta, growth and second stage only past it) against the
+    oracle's restatement (orc_gc_insertion2 / orc_gc_deletion2) on the same
+    seed: the same first-stage outcomes, exchanges, trial moves and final
+    configuration.  Mode 1 (the reference's exchange, recording ln(W / w_1)
+    of each growth) must leave the reference's chain untouched."""
+    from oracle import oracle as O
+    path = os.path.join(EX, "bulk_muvt", "run.in")
+    sd = _deck_sysdef(path)
+    steps = 800
+    sim = H.HostSimulation(path, 7)
+    sim.gc_prescreen(mode, log_theta)
+    att, acc = sim.run(steps)
+    gatt, gacc, nb, nm = sim.gc_stats()
+    rej, pas, mean_rest, _, n_rest = sim.gc_prescreen_stats()
+    orc = O.GCSim(sd, 7, log_theta=log_theta if mode == 2 else None)
+    orc.run(steps)
+    print("mode %d log theta %.1f: host gc %s %s, first stage rejected %s passed %s, mean ln(W/w1) %.3f "
+          "over %d | oracle gc %s %s, first stage rejected %s passed %s" % (
+              mode, log_theta, gatt, gacc, rej, pas, mean_rest, n_rest, orc.gc_attempted,
+              orc.gc_accepted, orc.s1_rejected, orc.s1_passed), flush=True)
+    assert gatt == orc.gc_attempted and gacc == orc.gc_accepted
+    assert att == orc.attempted and acc == orc.accepted
+    assert nb == len(orc.sd["q"])
+    if mode == 2:
+        assert rej == orc.s1_rejected and pas == orc.s1_passed and sum(pas) > 0
+    else:
+        assert rej == [0, 0] and pas == [0, 0] and n_rest > 0
+    if nb:
+        assert np.max(np.abs(sim.positions(nb) - orc.orc.positions())) <= 1e-9
+    orc.close()
+    sim.close()
+
+
+@pytest.mark.gpu
 def test_reference_confined_grand_canonical_example_replays_on_the_oracle():
     """The reference's own confined grand-canonical deck (18 surface sites and
     their counterions between walls, slab Ewald with the dipole correction,
