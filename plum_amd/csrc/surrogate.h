@@ -110,7 +110,12 @@ __device__ float descr_wave(const Geo& g, const Dev& v, float r_d, const double4
     const int total = __shfl(end, nloc - 1, kWave);
     wave_sync_lds();
     // candidates in rounds of kDescrLoads per lane: every load of a round is
-    // issued before the first candidate of the round is used
+    // issued before the first candidate of the round is used.  A lane's
+    // candidates only grow, so its cell is found by walking on from the cell
+    // of its previous candidate (a step or two: a cell holds tens of
+    // candidates and the lane's next one is 64 nw further on) rather than by
+    // a search over the chunk's prefix.
+    int cur = 0;
     for (int r0 = 0; r0 < total; r0 += kDescrLoads * kWave * nw) {
       int tt[kDescrLoads], jj[kDescrLoads];
       float4 pp[kDescrLoads];
@@ -121,15 +126,10 @@ __device__ float descr_wave(const Geo& g, const Dev& v, float r_d, const double4
         jj[u] = -1;
         pp[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (it < total) {
-          int lo_i = 0, hi_i = nloc - 1;
-          while (lo_i < hi_i) {
-            const int mid = (lo_i + hi_i) >> 1;
-            if (sm.s_end[mid] > it) hi_i = mid;
-            else lo_i = mid + 1;
-          }
-          const int slot = it - (lo_i ? sm.s_end[lo_i - 1] : 0);
-          const size_t si = (size_t)sm.s_cell[lo_i] * g.cap + slot;
-          tt[u] = lo_i;
+          while (sm.s_end[cur] <= it) cur++;   // it < total = s_end[nloc - 1]: stops in the chunk
+          const int slot = it - (cur ? sm.s_end[cur - 1] : 0);
+          const size_t si = (size_t)sm.s_cell[cur] * g.cap + slot;
+          tt[u] = cur;
           jj[u] = v.cbead[si];
           pp[u] = v.cpos32[si];
         }
