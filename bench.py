@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--brush-batched-only", action="store_true",
                     help="only the batched delayed-acceptance brushes, network at its initial "
                          "weights (one launch graph in the process, for kernel traces)")
+    ap.add_argument("--replicas-only", action="store_true",
+                    help="only the 256 replicas at a fixed launch shape (one launch graph, for kernel traces)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only device copies of known size (scale of the HBM counters)")
     return ap.parse_args()
@@ -519,28 +521,57 @@ def brush_batched_leg(moves, R=32):
     return res
 
 
-def replica_leg(sweeps):
+def move_work(sd, info):
+    """Algorithmic FLOP of one trial move: the k-space update 16 n N_k + 8 N_k
+    and the real-space pair sum 45 FLOP per candidate pair-image for the old
+    and new position of each of the n moved beads (n: the move mix's mean;
+    COM, pivot and reptation move the whole chain, crankshaft a third)."""
+    ms = sd["mol_start"]
+    clen = int(max(np.diff(ms)))
+    mp = sd["move_prob"]
+    n_moved = mp[0] * 1 + (mp[1] + mp[2] + mp[4]) * clen + mp[3] * max(clen - 2, 0) / 3.0
+    kflop = 16.0 * n_moved * info.n_k_half + 8.0 * info.n_k_half
+    return kflop + FLOP_PER_PAIR * 2 * n_moved * real_space_work(sd, info)
+
+
+def replica_leg(sweeps, tuned=True):
     """256 independent bulk systems (chain length 10..200, 5..100 chains,
     0..2000 salt pairs, half of them charged, density 0.02..0.2), FP32 pair
     math, advanced together by the batched kernels after the launch-shape
-    autotuner."""
+    autotuner (tuned=False: the shape the autotuner picks here, fixed, so a
+    kernel trace sees one launch graph).  The work rate is the algorithmic
+    FLOP of the moves made over the wall time of the run, a lower bound on the
+    kernels' rate."""
     t0 = time.perf_counter()
     sds = systems.replica_set(256)
     engs = [E.Engine(sd, precision=E.PLUM_FP32_PAIR) for sd in sds]
     log("replicas: %d particles ready in %.1f s" % (sum(len(s["q"]) for s in sds),
                                                     time.perf_counter() - t0))
+    work = [move_work(sd, e.info()) for sd, e in zip(sds, engs)]
     b = E.Batch(engs)
     mc = [dict(move_prob=sd["move_prob"], move_size=1.0, seed=1000 + r, bond_len=1.0)
           for r, sd in enumerate(sds)]
-    t = b.tune(mc, moves_per_trial=16)
-    st = b.mc_run(sweeps, mc)
-    log("replicas: %d moves in %.2f s" % (st.moves, st.seconds))
+    if tuned:
+        t = b.tune(mc, moves_per_trial=16)
+    else:
+        b.set_tuning(blocks_per_cu=16, split_div=768, pmax=4)
+        t = b.tuning()
+        b.mc_run(max(1, sweeps // 8), mc)
+    st, per = b.mc_run(sweeps, mc, per_replica=True)
+    flop = sum(p.moves * w for p, w in zip(per, work))
+    rate = flop / st.seconds / 1e12
+    log("replicas: %d moves in %.2f s, %.2f TFLOP/s of algorithmic work (wall clock)" % (
+        st.moves, st.seconds, rate))
     res = {"workload": "256 bulk replicas, chain length U[10,200] x U[5,100] chains, salt "
                        "U[0,2000] pairs, charged with p=0.5, density U[0.02,0.2]; FP32 pairs",
            "particles": int(sum(len(s["q"]) for s in sds)), "moves_per_replica": sweeps,
            "moves_per_s": st.moves / st.seconds,
            "accept_ratio": sum(st.accepted) / max(sum(st.attempted), 1),
-           "tuning": {"blocks_per_cu": t.blocks_per_cu, "split_div": t.split_div, "pmax": t.pmax}}
+           "tuning": {"blocks_per_cu": t.blocks_per_cu, "split_div": t.split_div, "pmax": t.pmax},
+           "work_rate": {"achieved": rate, "unit": "TFLOP/s", "peak_fp64": PEAK_FP64_TFLOPS,
+                         "frac_of_fp64_peak": rate / PEAK_FP64_TFLOPS,
+                         "basis": "sum over replicas of moves x (16 n N_k + 8 N_k + 45 x 2 n candidates) "
+                                  "FLOP over the run's wall time (kernel time is shorter)"}}
     b.close()
     for e in engs:
         e.close()
@@ -767,6 +798,9 @@ def main():
         return
     if a.growth_only:
         print(json.dumps({"growth": growth_leg()}), flush=True)
+        return
+    if a.replicas_only:
+        print(json.dumps({"replicas": replica_leg(a.replica_sweeps, tuned=False)}), flush=True)
         return
     if a.brush_batched_only:
         print(json.dumps({"brush_batched": brush_batched_leg(a.brush_moves)}), flush=True)
