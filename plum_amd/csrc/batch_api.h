@@ -27,6 +27,7 @@ struct plum_batch {
   int max_nkb_full = 1;
   int n_cu = 0;
   int descr_occ = 1;    // resident k_descr_spec blocks per CU
+  int comp_occ = 1;     // resident k_compute_spec blocks per CU
   plum_batch_tuning tune{};
   std::vector<McDev> mc;
   hipGraph_t graph = nullptr;
@@ -187,6 +188,7 @@ int batch_upload_state(plum_batch* b) {
     RepCfg& q = RCv[r];
     q.nkb_s = c->nkb_s;
     q.nrun = c->nrun;
+    q.qfill = b->n_cu * std::min(b->comp_occ, b->tune.blocks_per_cu) * kQWaves;
     q.nsplit = c->nsplit;
     q.ncell = c->g.ncell;
     q.pmax = b->tune.pmax;
@@ -426,6 +428,11 @@ int batch_new(plum_ctx* const* replicas, int n, plum_batch** out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->descr_occ, k_descr_spec, kBlock, 0) != hipSuccess)
     return bail(fail(PLUM_E_HIP, "cannot query the occupancy of the descriptor kernel"));
   b->descr_occ = std::max(1, b->descr_occ);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->comp_occ,
+                                                   b->precision == PLUM_FP32_PAIR ? k_compute_spec<1> : k_compute_spec<0>,
+                                                   kBlock, 0) != hipSuccess)
+    return bail(fail(PLUM_E_HIP, "cannot query the occupancy of the compute kernel"));
+  b->comp_occ = std::max(1, b->comp_occ);
   if (hipStreamCreate(&b->st) != hipSuccess) return bail(fail(PLUM_E_HIP, "hipStreamCreate failed"));
   if ((rc = balloc(b, &b->dG, n)) || (rc = balloc(b, &b->dD, n)) || (rc = balloc(b, &b->dMC, n)) ||
       (rc = balloc(b, &b->dRC, n)) || (rc = balloc(b, &b->d_vb, n)) || (rc = balloc(b, &b->d_cell_off, n + 1)) ||

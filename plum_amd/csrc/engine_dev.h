@@ -73,12 +73,18 @@ constexpr int kErfcPerUnit = 128;
 constexpr double kErfcMax = 8.0;
 constexpr int kErfcNodes = 8 * kErfcPerUnit + 1;
 
-__device__ __forceinline__ double erfc_nodes(const Geo& g, double x) {
+// Node of x for erfc_nodes (0 where the node is not used: x at or beyond
+// kErfcMax, or not a number), so that a kernel can issue the node's load
+// before it needs the value.
+__device__ __forceinline__ int erfc_node_index(double x) {
+  return x < kErfcMax ? max(0, (int)(x * kErfcPerUnit + 0.5)) : 0;
+}
+
+// erfc(x) from its node i = erfc_node_index(x) and the node's table entry.
+__device__ __forceinline__ double erfc_from_node(double x, int i, double2 node) {
   if (!(x < kErfcMax)) return 0.0;
-  const int i = (int)(x * kErfcPerUnit + 0.5);
   const double xi = (double)i * (1.0 / kErfcPerUnit);
   const double t = x - xi;   // exact: x and x_i are within a factor of 2 (or x_i = 0)
-  const double2 node = g.erfc_tab[i];
   // q_n = (-1)^n H_n(x_i): q_0 = 1, q_1 = -2 x_i, q_{n+1} = -2 x_i q_n - 2 n q_{n-1};
   // w_n = t^{n+1} / (n+1)!
   const double m2x = -2.0 * xi;
@@ -98,23 +104,35 @@ __device__ __forceinline__ double erfc_nodes(const Geo& g, double x) {
   return node.x - node.y * s;
 }
 
+__device__ __forceinline__ double erfc_nodes(const Geo& g, double x) {
+  const int i = erfc_node_index(x);
+  return erfc_from_node(x, i, g.erfc_tab[i]);
+}
+
 // FP32 erfc(x) of the FP32 pair path: the same series about the same nodes
 // (FP32-rounded), four terms: the remainder, about (2 x_i |t|)^4 / 5! of the
 // first term, is below 1/16^4 / 120 < 1.3e-7 of it, and the first term E_i t
 // is at most 2 x_i |t| <= 1/16 of erfc(x_i) below x = 8, so the result
 // carries the FP32 rounding of the nodes and of the sum only (a few ulp).
-__device__ __forceinline__ float erfc_nodes32(const Geo& g, float x) {
+__device__ __forceinline__ int erfc_node_index32(float x) {
+  return x < (float)kErfcMax ? max(0, (int)(x * kErfcPerUnit + 0.5f)) : 0;
+}
+
+__device__ __forceinline__ float erfc_from_node32(float x, int i, float2 node) {
   if (!(x < (float)kErfcMax)) return 0.f;
-  const int i = (int)(x * kErfcPerUnit + 0.5f);
   const float xi = (float)i * (1.0f / kErfcPerUnit);
   const float t = x - xi;
-  const float2 node = g.erfc_tab32[i];
   const float m2x = -2.0f * xi;
   // q_0 = 1, q_1 = -2 x_i, q_2 = 4 x_i^2 - 2, q_3 = -2 x_i q_2 - 4 q_1
   const float q2 = m2x * m2x - 2.0f, q3 = m2x * q2 - 4.0f * m2x;
   const float t2 = t * t;
   const float s = t + t2 * (0.5f * m2x + t * ((1.0f / 6.0f) * q2 + t * ((1.0f / 24.0f) * q3)));
   return node.x - node.y * s;
+}
+
+__device__ __forceinline__ float erfc_nodes32(const Geo& g, float x) {
+  const int i = erfc_node_index32(x);
+  return erfc_from_node32(x, i, g.erfc_tab32[i]);
 }
 
 // LJ pair table entry for (ti, tj): sigma, epsilon, cutoff distance (strict

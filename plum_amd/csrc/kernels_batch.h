@@ -65,6 +65,7 @@ struct RepCfg {
   float r_d;    // descriptor radius of the surrogate
   int train;    // this system feeds the surrogate's training ring
   int nrun;     // k-vector runs (Dev::krun)
+  int qfill;    // waves of k_compute_spec resident on the chip at once
 };
 
 // Real-space query blocks of a move of nm beads: one query per cluster of up
@@ -75,11 +76,14 @@ constexpr int kQWaves = kBlock / kWave;
 constexpr int kMaxQWave = 4 * kMaxSplit;
 // One wave per split share; a batch with few exact proposals (fewer than 16
 // over all systems) spreads each query over four times as many waves, as its
-// queries alone would leave most of the chip idle.  The plan chooses it per
-// batch from the exact slots it assigned (with delayed acceptance most
-// screened proposals never reach the queries).
-__host__ __device__ __forceinline__ int query_waves(int nsplit, int proposals) {
-  return nsplit == 1 ? 1 : (proposals >= 16 ? nsplit : kQWaves * nsplit);
+// queries alone would leave most of the chip idle, unless the system's own
+// query waves (its exact slots' clusters at old and new positions, times the
+// shares) would then outnumber the waves the chip holds at once (fill).  The
+// plan chooses it per batch from the exact slots it assigned (with delayed
+// acceptance most screened proposals never reach the queries).
+__host__ __device__ __forceinline__ int query_waves(int nsplit, int proposals, int clusters, int fill) {
+  if (nsplit == 1 || proposals >= 16) return nsplit;
+  return (long long)clusters * nsplit * kQWaves <= fill ? kQWaves * nsplit : nsplit;
 }
 constexpr int kCluster = 8;   // moved beads per real-space query (cluster)
 __host__ __device__ __forceinline__ int query_clusters(int nm) { return (nm + kCluster - 1) / kCluster; }
@@ -461,7 +465,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_spec(const Geo* __restric
     // waves per query for this batch's exact slots (the other systems of a
     // batch taken to have as many), then query and moved-moved blocks per
     // exact slot, prefixes over the slots
-    const int qw = query_waves(rc.nsplit, ne * R);
+    int ncl = t < ne ? 2 * query_clusters(sm.s_prop[sm.s_screen[t]].n) : 0;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) ncl += __shfl_xor(ncl, o, kWave);
+    const int qw = query_waves(rc.nsplit, ne * R, ncl, rc.qfill);
     int nq = 0, nmb = 0;
     if (t < ne) {
       const int n = sm.s_prop[sm.s_screen[t]].n;
@@ -677,12 +684,20 @@ struct QWaveSmem {
 };
 
 // LJ + real-space terms of one queued pair (squared distance r2 <= rs^2,
-// charge product qq, LJ pair index lji, whether LJ applies), FP64.
+// charge product qq, LJ pair index lji, whether LJ applies), FP64.  Both table
+// entries (LJ pair, erfc node) are loaded before either is used, so that a
+// queue drain waits for memory once rather than twice in a row: the empty asm
+// takes them as inputs, which keeps the loads ahead of both branches.
 __device__ __forceinline__ void pair_terms(const Geo& g, const Dev& d, double r2, double qq, int lji, bool lj_ok,
                                            double& elj, double& ere) {
   const double r = sqrt(r2);
-  if (lj_ok) elj += lj_energy(r, d.lj[lji]);
-  if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc_nodes(g, g.sqrt_alpha * r) / r;
+  const double x = g.sqrt_alpha * r;
+  const int ni = erfc_node_index(x);
+  const LJPair lp = d.lj[lji];
+  const double2 node = g.erfc_tab[ni];
+  asm volatile("" ::"v"(lp.sigma), "v"(lp.cut), "v"(node.x));
+  if (lj_ok) elj += lj_energy(r, lp);
+  if (g.use_ewald && qq != 0 && r > 0 && r <= g.rc) ere += g.lB * qq * erfc_from_node(x, ni, node) / r;
 }
 
 // The same in FP32 (the FP32 pair path, pairs at or beyond fp64_below; the
@@ -690,18 +705,20 @@ __device__ __forceinline__ void pair_terms(const Geo& g, const Dev& d, double r2
 __device__ __forceinline__ void pair_terms32(const Geo& g, const Dev& d, double r2, float qq, int lji, bool lj_ok,
                                              double& elj, double& ere) {
   const float r2f = (float)r2;
+  const float rf = sqrtf(r2f);
+  const float x = (float)g.sqrt_alpha * rf;
+  const int ni = erfc_node_index32(x);
+  const LJPair32 lp = d.lj32[lji];
+  const float2 node = g.erfc_tab32[ni];
+  asm volatile("" ::"v"(lp.sigma2), "v"(lp.cut), "v"(node.x));
   if (lj_ok) {
-    const LJPair32 lp = d.lj32[lji];
     if (r2 < (double)lp.cut * (double)lp.cut) {
       const float s2 = lp.sigma2 / r2f;
       const float r6 = s2 * s2 * s2;
       elj += (double)(lp.eps4 * (r6 * r6 - r6) - lp.shift);
     }
   }
-  if (g.use_ewald && qq != 0.f && r2 <= g.rc * g.rc) {
-    const float rf = sqrtf(r2f);
-    ere += (double)((float)g.lB * qq * erfc_nodes32(g, (float)g.sqrt_alpha * rf) / rf);
-  }
+  if (g.use_ewald && qq != 0.f && r2 <= g.rc * g.rc) ere += (double)((float)g.lB * qq * erfc_from_node32(x, ni, node) / rf);
 }
 
 // LJ + real-space energy of the np points pts[0 .. np) (charges in .w, types
