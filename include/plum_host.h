@@ -75,6 +75,20 @@ int plum_sim_gc_prescreen_stats(plum_sim* sim, long long s1_rejected[2], long lo
    50 Widom test insertions of the exchanged chain species (or, without
    grand-canonical moves, of the first chain species, 30 trials per bead). */
 int plum_sim_widom(plum_sim* sim, double* mean_weight);
+/* Replaces the force sums of one call of ForceField::CalcPressureForceLJELSlit
+   (pressure.cc:404-470) for the configuration given (phantom wall sites are
+   molecules 0 .. phantom-1, the first half on the z = 0 wall): f[0..2] the
+   LJ z forces on ions / chains / the other wall (PairForce,
+   potential_truncated_lj.cc:87-121, plus half of BeadForceOnWall,
+   potential_truncated_lj_wall.cc:136-214, on the system's beads), f[3..5]
+   the Ewald real + reciprocal z forces likewise (PairForceZReal / Repl,
+   potential_ewald_coul.cc:261-324), with the coefficients -1/2, +1/2 and -1
+   of pressure.cc:425-429.  `ewald` is the engine's plum_info for the same
+   parameters (k grid, cutoffs, images).  Host only; the caller does the
+   area and call averaging of pressure.cc:472-487. */
+int plum_slit_forces(const plum_params* p, const plum_info* ewald, int n_beads, const double* xyz,
+                     const double* q, const int* type, int n_mols, const int* mol_start, int phantom,
+                     double f[6]);
 int plum_sim_energy(plum_sim* sim, plum_terms* running, plum_terms* from_scratch);
 int plum_sim_positions(plum_sim* sim, double* xyz, int n_beads);
 /* Simulation::RadiusOfGyration / EndToEndDistance (simulation.cc:406-501). */

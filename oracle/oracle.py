@@ -154,6 +154,12 @@ def lib():
         L.orc_gasdev.argtypes = [C.c_double, C.c_double, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
         L.orc_gasdev.restype = C.c_double
         L.orc_vp_sample.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_slit_forces.argtypes = [C.c_void_p, C.c_int, dp]
+        for f in ("orc_lj_force", "orc_real_force_z", "orc_repl_force_z"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int, C.c_int]
+            getattr(L, f).restype = C.c_double
+        L.orc_wall_force.argtypes = [C.c_void_p, C.c_int]
+        L.orc_wall_force.restype = C.c_double
         L.orc_random_bond_len.argtypes = [C.c_void_p, C.POINTER(OrcMT), C.POINTER(OrcGauss)]
         L.orc_random_bond_len.restype = C.c_double
         L.orc_translational_move.argtypes = [C.c_void_p, C.POINTER(OrcMT), dp, C.c_double,
@@ -338,6 +344,23 @@ class Oracle:
         lib().orc_vp_sample(self.h, int(phantom), _dp(out))
         keys = ("dU", "el", "lj", "wall", "bond", "dipole", "n_mol", "vol", "a_dz")
         return dict(zip(keys, out))
+
+    def slit_forces(self, phantom):
+        """Force sums of one call of the wall-force slit pressure
+        (pressure.cc:404-470): LJ on ions / chains / the other wall, then the
+        Ewald z forces likewise."""
+        out = np.zeros(6)
+        lib().orc_slit_forces(self.h, int(phantom), _dp(out))
+        return out
+
+    def force(self, kind, a, b=None):
+        """One of the pair / wall forces the slit pressure sums: kind 'lj'
+        (PairForce of beads a, b), 'real' or 'repl' (z force on wall site a
+        from bead b), 'wall' (BeadForceOnWall of bead a)."""
+        if kind == "wall":
+            return lib().orc_wall_force(self.h, a)
+        name = {"lj": "orc_lj_force", "real": "orc_real_force_z", "repl": "orc_repl_force_z"}[kind]
+        return getattr(lib(), name)(self.h, a, b)
 
     def rg(self):
         return lib().orc_radius_of_gyration(self.h)

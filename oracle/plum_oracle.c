@@ -1611,3 +1611,191 @@ void orc_vp_sample(orc_sys* s, int phantom, double out[9]) {
   out[7] = box[0] * box[1] * box[2];
   out[8] = box[0] * box[1] * kDz;
 }
+
+/* ------------------------------------------------------------------ */
+/* wall-force slit pressure                                            */
+/* ------------------------------------------------------------------ */
+
+/* PotentialTruncatedLJ::PairForce (potential_truncated_lj.cc:87-121); r6
+   starts at 0 as in lj_pair_xyz (the reference leaves it unset when r > 0).
+   Beyond a full LJ cutoff it keeps the reference's expression with
+   r6 = (sigma / lj_cutoff)^6. */
+static double lj_force_xyz(const orc_sys* s, const double* a, int ta, const double* b, int tb) {
+  double force = 0;
+  double r = bb_dist(a, b, s->p.box, s->p.npbc);
+  double sigma = (s->p.lj_sigma[ta] + s->p.lj_sigma[tb]) / 2;
+  double epsilon = sqrt(s->p.lj_eps[ta] * s->p.lj_eps[tb]);
+  double r6 = 0;
+  if (r <= 0) r6 = -1;
+  if (r6 == -1) {
+    force = kVeryLargeEnergy;
+  } else if (s->p.lj_cutoff < 0) {
+    if (r < k216 * sigma) {
+      r6 = pow(sigma / r, 6);
+      force = 4 * epsilon * (r6 * r6 * 12 / r - r6 * 6 / r);
+    }
+  } else if (r < s->p.lj_cutoff) {
+    r6 = pow(sigma / r, 6);
+    force = 4 * epsilon * (r6 * r6 * 12 / r - r6 * 6 / r);
+  } else {
+    r6 = pow(sigma / s->p.lj_cutoff, 6);
+    force = 4 * epsilon * (r6 * r6 * 12 / r - r6 * 6 / r);
+  }
+  return force;
+}
+
+/* PotentialEwaldCoul::PairForceZReal (potential_ewald_coul.cc:261-293): z
+   force on the wall site a from bead b, over the real-space images. */
+static double real_force_z_xyz(const orc_sys* s, const double* a, double qa, const double* b, double qb) {
+  double force_z = 0;
+  if (qa * qb == 0) return force_z;
+  double r[3];
+  get_dist_vector(b, a, s->e.box_e, s->p.npbc, r);
+  double prefactor = s->p.lB * qa * qb;
+  double prefactor2 = 2 * sqrt(s->p.alpha / kPi);
+  const int* c = s->e.real_cell;
+  const double* L = s->e.box_e;
+  for (int i = -c[0]; i <= c[0]; i++)
+    for (int j = -c[1]; j <= c[1]; j++)
+      for (int k = -c[2]; k <= c[2]; k++) {
+        double rv0 = r[0] + i * L[0];
+        double rv1 = r[1] + j * L[1];
+        double rv2 = r[2] + k * L[2];
+        double d = sqrt(rv0 * rv0 + rv1 * rv1 + rv2 * rv2);
+        if (d > 0 && d <= s->e.real_cutoff)
+          force_z += prefactor * ((prefactor2 * exp(-s->p.alpha * d * d)) + (erfc(sqrt(s->p.alpha) * d) / d)) *
+                     rv2 / (d * d);
+      }
+  return force_z;
+}
+
+/* PotentialEwaldCoul::PairForceZRepl (potential_ewald_coul.cc:297-324). */
+static double repl_force_z_xyz(const orc_sys* s, const double* a, double qa, const double* b, double qb) {
+  double force_z = 0;
+  if (qa * qb == 0) return force_z;
+  double r[3];
+  get_dist_vector(b, a, s->e.box_e, s->p.npbc, r);
+  double prefactor = s->p.lB * 4 * kPi * qa * qb / s->e.vol_e;
+  for (int lx = 0; lx < s->ceto[0]; lx++)
+    for (int ly = 0; ly < s->ceto[1]; ly++)
+      for (int lz = 0; lz < s->ceto[2]; lz++) {
+        int idx = s->ceto[1] * s->ceto[2] * lx + s->ceto[2] * ly + lz;
+        if (s->k2[idx] > 0 && s->k2[idx] <= s->e.repl_cutoff)
+          force_z += prefactor * s->kz[lz] * s->ek2[idx] *
+                     sin(s->kx[lx] * r[0] + s->ky[ly] * r[1] + s->kz[lz] * r[2]);
+      }
+  return force_z;
+}
+
+/* PotentialTruncatedLJWall::BeadForceOnWall (potential_truncated_lj_wall.cc:
+   136-214) at z (the coordinates are settled when the pressure is sampled),
+   branch for branch; the full-LJ branch keeps the reference's /z in its last
+   term. */
+static double wall_force_z(const orc_sys* s, double z, int t) {
+  double force = 0;
+  const double Lz = s->p.box[2];
+  double sigma = s->p.wall_sigma[t];
+  double epsilon = s->p.wall_eps[t];
+  int kind = s->p.type_kind[t];
+  double R0 = 3 * k213 * sigma;
+  double K = 1;
+  double m_cut = s->p.wall_cut;
+  if (epsilon == 0) {
+    force = 0;
+  } else if (z <= 0 || z >= Lz) {
+    return kVeryLargeEnergy;
+  } else {
+    if (m_cut < 0 && kind != ORC_KIND_R && kind != ORC_KIND_L) {
+      if (z < k213 * sigma) {
+        double r3 = pow(sigma / z, 3);
+        force += 2.59807621135 * epsilon * (r3 * r3 * 6 / z - r3 * 3 / z);
+      }
+      if (Lz - z < k213 * sigma) {
+        double r3 = pow(sigma / (Lz - z), 3);
+        force += 2.59807621135 * epsilon * (r3 * r3 * 6 / (Lz - z) - r3 * 3 / (Lz - z));
+      }
+    } else if (kind == ORC_KIND_L) {
+      double r3 = pow(sigma / z, 3);
+      force += 2.59807621135 * epsilon * (r3 * r3 * 6 / z - r3 * 3 / z);
+      force += -K * R0 * z / (1 - pow(z / R0, 2));
+      if (Lz - z < k213 * sigma) {
+        double r3b = pow(sigma / (Lz - z), 3);
+        force += 2.59807621135 * epsilon * (r3b * r3b * 6 / (Lz - z) - r3b * 3 / (Lz - z));
+      }
+    } else if (kind == ORC_KIND_R) {
+      double r3 = pow(sigma / (Lz - z), 3);
+      force += 2.59807621135 * epsilon * (r3 * r3 * 6 / (Lz - z) - r3 * 3 / (Lz - z));
+      force += -K * R0 * (Lz - z) / (1 - pow((Lz - z) / R0, 2));
+      if (z < k213 * sigma) {
+        double r3b = pow(sigma / z, 3);
+        force += 2.59807621135 * epsilon * (r3b * r3b * 6 / z - r3b * 3 / z);
+      }
+    } else {
+      if (z < m_cut) {
+        double r3 = pow(sigma / z, 3);
+        force += 2.59807621135 * epsilon * (r3 * r3 * 6 / z - r3 * 3 / z);
+      } else {
+        double r3 = pow(sigma / m_cut, 3);
+        force += 2.59807621135 * epsilon * (r3 * r3 * 6 / z - r3 * 3 / z);
+      }
+      if (Lz - z < m_cut) {
+        double r3 = pow(sigma / (Lz - z), 3);
+        force += 2.59807621135 * epsilon * (r3 * r3 * 6 / (Lz - z) - r3 * 3 / (Lz - z));
+      } else {
+        double r3 = pow(sigma / m_cut, 3);
+        force += 2.59807621135 * epsilon * (r3 * r3 * 6 / z - r3 * 3 / z);
+      }
+    }
+  }
+  return force;
+}
+
+double orc_lj_force(orc_sys* s, int a, int b) {
+  return lj_force_xyz(s, &s->cur[3 * a], s->type[a], &s->cur[3 * b], s->type[b]);
+}
+double orc_real_force_z(orc_sys* s, int a, int b) {
+  return real_force_z_xyz(s, &s->cur[3 * a], s->q[a], &s->cur[3 * b], s->q[b]);
+}
+double orc_repl_force_z(orc_sys* s, int a, int b) {
+  return repl_force_z_xyz(s, &s->cur[3 * a], s->q[a], &s->cur[3 * b], s->q[b]);
+}
+double orc_wall_force(orc_sys* s, int a) { return wall_force_z(s, s->cur[3 * a + 2], s->type[a]); }
+
+/* ForceField::CalcPressureForceLJELSlit (pressure.cc:404-470), the sums of
+   one call: the loops, coefficients and classes as the reference has them. */
+void orc_slit_forces(orc_sys* s, int phantom, double f[6]) {
+  for (int k = 0; k < 6; k++) f[k] = 0;
+  for (int i = 0; i < phantom; i++) {
+    const int wi = s->mol_start[i];
+    const double* pw = &s->cur[3 * wi];
+    for (int j = phantom / 2; j < s->n_mols; j++) {
+      const int jlen = s->mol_start[j + 1] - s->mol_start[j];
+      for (int k = s->mol_start[j]; k < s->mol_start[j + 1]; k++) {
+        double C = -1;
+        if (i < phantom / 2 && j >= phantom) C = -0.5;
+        else if (i < phantom && j >= phantom) C = 0.5;
+        else C = -1;
+        int k_id = 2;
+        if (j >= phantom) k_id = jlen > 1 ? 1 : 0;
+        const double* pb = &s->cur[3 * k];
+        if (s->p.use_pair && (i < phantom / 2 || j >= phantom)) {
+          double r[3];
+          get_dist_vector(pb, pw, s->p.box, s->p.npbc, r);
+          double z_component = r[2] / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+          f[k_id] += C * z_component * lj_force_xyz(s, pb, s->type[k], pw, s->type[wi]);
+        }
+        if (s->p.use_ewald && (i < phantom / 2 || j >= phantom)) {
+          double force = C * real_force_z_xyz(s, pw, s->q[wi], pb, s->q[k]);
+          force += C * repl_force_z_xyz(s, pw, s->q[wi], pb, s->q[k]);
+          f[3 + k_id] += force;
+        }
+      }
+    }
+  }
+  if (s->p.use_ext)
+    for (int i = phantom; i < s->n_mols; i++) {
+      const int ilen = s->mol_start[i + 1] - s->mol_start[i];
+      for (int j = s->mol_start[i]; j < s->mol_start[i + 1]; j++)
+        f[ilen > 1 ? 1 : 0] += 0.5 * wall_force_z(s, s->cur[3 * j + 2], s->type[j]);
+    }
+}

@@ -201,6 +201,19 @@ double orc_random_bond_len(orc_sys* s, orc_mt19937* r, orc_gauss* gs);
    (pressure.cc:187-387): out[0] dU, [1..5] its Ewald-pair, LJ-pair, wall,
    spring and dipole parts, [6] n_mol - phantom, [7] Lx Ly Lz, [8] Lx Ly kDz. */
 void orc_vp_sample(orc_sys* s, int phantom, double out[9]);
+/* The force sums of one call of ForceField::CalcPressureForceLJELSlit
+   (pressure.cc:404-470) on the current coordinates, phantom wall sites first
+   (half on each wall): f[0..2] LJ on ions / chains / the other wall, f[3..5]
+   the Ewald real + reciprocal z forces likewise (before the area and call
+   averaging of pressure.cc:472-487). */
+void orc_slit_forces(orc_sys* s, int phantom, double f[6]);
+/* PotentialTruncatedLJ::PairForce, PotentialEwaldCoul::PairForceZReal /
+   PairForceZRepl (wall site a first) and PotentialTruncatedLJWall::
+   BeadForceOnWall on the current coordinates of beads a, b. */
+double orc_lj_force(orc_sys* s, int a, int b);
+double orc_real_force_z(orc_sys* s, int a, int b);
+double orc_repl_force_z(orc_sys* s, int a, int b);
+double orc_wall_force(orc_sys* s, int a);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 
 ENGINE_SRC = ["engine.hip"]
 HOST_SRC = ["host/runin.cc", "host/force_field.cc", "host/cbmc.cc", "host/simulation.cc",
-            "host/host_abi.cc", "host/synth.cc"]
+            "host/host_abi.cc", "host/synth.cc", "host/slit_pressure.cc"]
 
 
 def run(cmd):

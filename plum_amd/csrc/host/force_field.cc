@@ -1,4 +1,5 @@
 #include "force_field.h"
+#include "plum_host.h"
 
 #include <cmath>
 #include <sstream>
@@ -245,6 +246,60 @@ double ForceField::PressureYethiraj() const {
 double ForceField::PressureDeMiguel() const {
   if (vp_z_ == 0) return 0;
   return r_.beta / vp_adz_ * std::log(vp_dm_ / vp_z_);
+}
+
+void ForceField::CalcPressureForceLJELSlit(std::vector<Molecule>& mols) {
+  slit_z_++;
+  std::vector<double> xyz;
+  std::vector<double> q;
+  std::vector<int> type, ms(1, 0);
+  for (const Molecule& m : mols) {
+    for (int k = 0; k < m.Size(); k++) {
+      const Bead& b = m.bds[k];
+      for (int e = 0; e < 3; e++) xyz.push_back(b.GetCrd(0, e));
+      q.push_back(b.Charge());
+      type.push_back(b.Type());
+    }
+    ms.push_back((int)q.size());
+  }
+  plum_params p{};
+  double box[3] = {box_[0], box_[1], box_[2]};
+  build_params(p, box);
+  plum_info e{};
+  check(plum_get_info(ctx_, &e), "CalcPressureForceLJELSlit");
+  double f[6];
+  check(plum_slit_forces(&p, &e, (int)q.size(), xyz.data(), q.data(), type.data(), (int)mols.size(), ms.data(),
+                         r_.phantom, f),
+        "CalcPressureForceLJELSlit");
+  // pressure.cc:438-487: the wall-wall parts once (first call), the others
+  // cumulative and averaged over the calls
+  p_tensor_[6] += f[0];
+  p_tensor_[7] += f[1];
+  p_tensor_[9] += f[3];
+  p_tensor_[10] += f[4];
+  if (slit_z_ == 1) {
+    p_tensor_[8] += f[2];
+    p_tensor_[11] += f[5];
+  }
+  const double area = box_[0] * box_[1];
+  if (slit_z_ == 1) {
+    p_tensor_[2] = p_tensor_[8] / area;
+    p_tensor_[5] = p_tensor_[11] / area;
+  }
+  p_tensor_[0] = p_tensor_[6] / (slit_z_ * area);
+  p_tensor_[1] = p_tensor_[7] / (slit_z_ * area);
+  p_tensor_[3] = p_tensor_[9] / (slit_z_ * area);
+  p_tensor_[4] = p_tensor_[10] / (slit_z_ * area);
+}
+
+std::string ForceField::GetPressure() const {
+  std::ostringstream foo;
+  if (r_.use_ext)
+    foo << p_tensor_[0] << " " << p_tensor_[1] << " " << p_tensor_[2] << " " << p_tensor_[3] << " " << p_tensor_[4]
+        << " " << p_tensor_[5];
+  else
+    foo << "nan";
+  return foo.str();
 }
 
 double ForceField::TotPairEnergy() { return Totals().lj; }

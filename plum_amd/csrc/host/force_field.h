@@ -94,6 +94,12 @@ class ForceField {
   plum_vp CalcPressureVolScalingHSELSlit(std::vector<Molecule>& mols);
   double PressureYethiraj() const;
   double PressureDeMiguel() const;
+  // ForceField::CalcPressureForceLJELSlit (pressure.cc:404-487): one sample
+  // of the wall-force slit pressure from the current molecules (host,
+  // plum_slit_forces), accumulated as the reference accumulates p_tensor.
+  void CalcPressureForceLJELSlit(std::vector<Molecule>& mols);
+  // ForceField::GetPressure (pressure.cc:490-500).
+  std::string GetPressure() const;
   long long PressureSamples() const { return vp_z_; }
 
   bool UseGC() const { return r_.use_gc; }
@@ -158,6 +164,8 @@ class ForceField {
   // of dU, sum of (1 + dz/Lz)^N e^{-beta dU}, and the last A dz
   long long vp_z_ = 0;
   double vp_ideal_ = 0, vp_du_ = 0, vp_dm_ = 0, vp_adz_ = 0;
+  long long slit_z_ = 0;        // calls of CalcPressureForceLJELSlit (the reference's vp_z there)
+  double p_tensor_[12] = {0};   // pressure.cc:389-403
 };
 
 }  // namespace plum

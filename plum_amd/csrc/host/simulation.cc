@@ -278,9 +278,9 @@ void Simulation::RunWithOutput(long long steps, const std::string& prefix, bool 
   device_moves(pending);
 }
 
-// Simulation::Sample (simulation.cc:723-798).  The wall-force slit pressure
-// (CalcPressureForceLJELSlit, pressure.cc:404-488) is not built: with walls
-// the pressure columns are printed as nan.
+// Simulation::Sample (simulation.cc:723-798); with walls the wall-force slit
+// pressure (CalcPressureForceLJELSlit, pressure.cc:404-487) every
+// sample_freq * 10 steps, without them the volume-perturbation pressure.
 void Simulation::sample(long long step) {
   if (step > r.steps_eq && step % r.sample_freq == 0) {
     avg.ff_count++;
@@ -300,7 +300,8 @@ void Simulation::sample(long long step) {
         if (bin >= 0 && bin < nb) avg.rho_z[3 * (size_t)bin + id] += 1.0 / (box[0] * box[1] * avg.rho_z_res);
       }
     }
-    if (!ff.UseExtPot() && step % (r.sample_freq * 10) == 0) ff.CalcPressureVolScalingHSELSlit(mols);
+    if (ff.UseExtPot() && step % (r.sample_freq * 10) == 0) ff.CalcPressureForceLJELSlit(mols);
+    else if (step % (r.sample_freq * 10) == 0) ff.CalcPressureVolScalingHSELSlit(mols);
     if (r.calc_chem_pot) avg.chem_pot += ff.CalcChemicalPotentialF(mols, rng);
   }
   if (step > r.steps_eq && step % r.sample_freq == 0 && !mols.empty()) {
@@ -360,9 +361,9 @@ void Simulation::print_stat(std::ostream& o, long long step) const {
     o.precision(4);
     o << ' ' << avg.density / n;
   }
-  // ForceField::GetPressure (pressure.cc:490-500): nan without walls
-  if (ff.UseExtPot()) o << " nan nan nan nan nan nan";
-  else o << " nan";
+  // ForceField::GetPressure (pressure.cc:490-500): the six slit-pressure
+  // averages with walls, nan without them
+  o << ' ' << ff.GetPressure();
   if (r.calc_chem_pot) {
     if (avg.chem_pot / n > 0) o << ' ' << -(1.0 / r.beta) * std::log(avg.chem_pot / n);
     else o << " INF";

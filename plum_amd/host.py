@@ -14,7 +14,8 @@ HOST_SYMBOLS = ("plum_runin_parse", "plum_sim_create", "plum_sim_run", "plum_sim
                 "plum_host_last_error", "plum_synth_count", "plum_synth_system",
                 "plum_sim_gc_stats", "plum_sim_widom", "plum_sim_run_device",
                 "plum_sim_density_profile", "plum_sim_write_last", "plum_sim_pressure_sample",
-                "plum_sim_run_output", "plum_sim_gc_prescreen", "plum_sim_gc_prescreen_stats")
+                "plum_sim_run_output", "plum_sim_gc_prescreen", "plum_sim_gc_prescreen_stats",
+                "plum_slit_forces")
 
 
 class RunInSummary(C.Structure):
@@ -89,6 +90,36 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise HostError("plum host error %d: %s" % (rc, lib().plum_host_last_error().decode()))
+
+
+def slit_forces(sysdef, info, phantom):
+    """Force sums of one call of the wall-force slit pressure
+    (ForceField::CalcPressureForceLJELSlit, pressure.cc:404-470) for the
+    configuration of `sysdef`, phantom wall sites first: LJ on ions / chains /
+    the other wall, then the Ewald z forces likewise.  `info` carries the
+    Ewald box, cutoffs and images (plum_info fields: box_e, vol_e,
+    real_cutoff, real_cell, repl_cutoff, repl_cell)."""
+    import numpy as np
+    L = lib()
+    if not getattr(L, "_slit_bound", False):
+        L.plum_slit_forces.argtypes = [C.POINTER(E.Params), C.POINTER(E.Info), C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L._slit_bound = True
+    p = E.make_params(sysdef)
+    e = E.Info()
+    for k in ("box_e", "real_cell", "repl_cell"):
+        for d in range(3):
+            getattr(e, k)[d] = getattr(info, k)[d]
+    for k in ("vol_e", "real_cutoff", "repl_cutoff"):
+        setattr(e, k, getattr(info, k))
+    xyz = np.ascontiguousarray(sysdef["xyz"], dtype=np.float64).reshape(-1)
+    q = np.ascontiguousarray(sysdef["q"], dtype=np.float64)
+    ty = np.ascontiguousarray(sysdef["type"], dtype=np.int32)
+    ms = np.ascontiguousarray(sysdef["mol_start"], dtype=np.int32)
+    f = np.zeros(6)
+    _check(L.plum_slit_forces(C.byref(p), C.byref(e), len(q), xyz.ctypes.data, q.ctypes.data, ty.ctypes.data,
+                              len(ms) - 1, ms.ctypes.data, int(phantom), f.ctypes.data))
+    return f
 
 
 def parse_runin(path):

@@ -155,3 +155,47 @@ def test_device_batches_between_output_steps_keep_the_files_consistent(tmp_path)
     shift = (pos - xyz) / box
     assert np.allclose(shift, np.round(shift), atol=1e-9)
     sim.close()
+
+
+def test_slit_pressure_columns_on_the_reference_confined_deck(tmp_path):
+    """With walls the stat file carries the six wall-force slit pressure
+    averages (ForceField::CalcPressureForceLJELSlit every sample_freq * 10
+    steps, pressure.cc:404-487; GetPressure :490-500): ion,
+    chain and wall-wall LJ, then the same for the Ewald forces (the header's
+    <Pzz_LJ_ion> .. <Pzz_EL_wal>), each force sum over the area (the
+    wall-wall ones from the first sample, the others averaged over the
+    samples).  The oracle replays the reference's chain from
+    the same seed and sums the same forces at the sample steps (1000, 2000);
+    the columns agree to the 6 significant digits they are printed with."""
+    path = os.path.join(EX, "confined_nvt", "run.in")
+    sd, mv = deck_sysdef(path)
+    steps, seed = 2000, 2017
+    prefix = str(tmp_path / "out")
+    sim = H.HostSimulation(path, seed)
+    sim.run_output(steps, prefix)
+    orc = O.Oracle(sd)
+    rng = O.MT19937(seed)
+    fs = []
+    for step in range(1, steps + 1):
+        O.mc_step(orc, rng, sd["move_prob"], sd["move_size"], mv["phantom"], mv["grafted"],
+                  mv["bond_len"], mv["vary_bond"])
+        if step > 100 and step % 1000 == 0:
+            fs.append(orc.slit_forces(mv["phantom"]))
+    area = sd["box"][0] * sd["box"][1]
+    fs = np.array(fs)
+    want = [fs[:, 0].mean() / area, fs[:, 1].mean() / area, fs[0, 2] / area,
+            fs[:, 3].mean() / area, fs[:, 4].mean() / area, fs[0, 5] / area]
+    lines = open(prefix + "_stat.dat").read().splitlines()
+    head = lines[0].split()
+    rows = [r.split() for r in lines[1:]]
+    last = [r for r in rows if r[0] == str(steps)][0]
+    assert len(last) == len(head)
+    c = head.index("<Pzz_LJ_ion>")
+    assert head[c:c + 6] == ["<Pzz_LJ_ion>", "<Pzz_LJ_pol>", "<Pzz_LJ_wal>", "<Pzz_EL_ion>", "<Pzz_EL_pol>",
+                             "<Pzz_EL_wal>"]
+    got = last[c:c + 6]
+    print("stat", got, "oracle", want, flush=True)
+    assert any(abs(w) > 0 for w in want)
+    for text, ref in zip(got, want):
+        assert _close(text, ref, 6), (got, want)
+    sim.close()
