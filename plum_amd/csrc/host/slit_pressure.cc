@@ -151,6 +151,12 @@ extern "C" int plum_slit_forces(const plum_params* pp, const plum_info* ee, int 
     return PLUM_E_ARG;
   const plum_params& p = *pp;
   const plum_info& e = *ee;
+  if (p.n_types < 1 || p.n_types > PLUM_MAX_TYPES || mol_start[0] != 0 || mol_start[n_mols] != n_beads)
+    return PLUM_E_ARG;
+  for (int m = 0; m < n_mols; m++)
+    if (mol_start[m + 1] <= mol_start[m]) return PLUM_E_ARG;
+  for (int i = 0; i < n_beads; i++)
+    if (type[i] < 0 || type[i] >= p.n_types) return PLUM_E_ARG;
   for (int k = 0; k < 6; k++) f[k] = 0;
   const int half = phantom / 2;
   // partner class of molecule j (pressure.cc:430-434): 0 ion, 1 chain, 2 the
